@@ -1,0 +1,256 @@
+"""ctypes mirror of include/fts.h and a thin Python handle over it.
+
+The engine itself is C++/HIP (libfts_hip.so); this module only declares the
+C-ABI structures and wraps the calls, so Python callers (tests, bench.py) drive
+exactly what a C or C++ caller would.  `Engine` takes the loaded library and the
+symbol prefix, so the same wrapper serves any library that exports this ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+FTS_OK, FTS_NONFINITE, FTS_INVALID, FTS_DEVICE_ERROR = 0, 1, 2, 3
+SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT = 0, 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
+MAX_SPECIES = MAX_IONS = MAX_TYPES = 8
+STRESS_ALL = 0x3F
+STRESS_PISO = 0x40
+
+FIELD_OMEGA, FIELD_PSI, FIELD_RHO_TYPE, FIELD_RHO_P, FIELD_RHO_C = 0, 1, 2, 3, 4
+FIELD_RHO_C_POLY, FIELD_RHO_C_ION, FIELD_ION_RHO, FIELD_QF, FIELD_QB = 5, 6, 7, 8, 9
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [("num_chains", C.c_longlong), ("N", C.c_int),
+                ("types", C.POINTER(C.c_int)), ("charges", C.POINTER(C.c_double)),
+                ("bonds", C.POINTER(C.c_double))]
+
+
+class CounterionDesc(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("auto_number", C.c_int), ("number", C.c_longlong),
+                ("valence", C.c_double), ("smearing_length", C.c_double)]
+
+
+class SaltDesc(C.Structure):
+    _fields_ = [("enabled", C.c_int), ("num_salt", C.c_longlong), ("cation_valence", C.c_double),
+                ("anion_valence", C.c_double), ("cation_stoich", C.c_int), ("anion_stoich", C.c_int),
+                ("smearing_length", C.c_double), ("combine_identical_species", C.c_int)]
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("m1", C.c_int), ("m2", C.c_int), ("m3", C.c_int), ("dL", C.c_double), ("a", C.c_double),
+                ("monomer_types", C.c_int), ("xi", C.c_double), ("chi", C.POINTER(C.c_double)),
+                ("eigen_tol_rel", C.c_double), ("lB", C.c_double), ("n_species", C.c_int),
+                ("species", C.POINTER(SpeciesDesc)), ("ions_enabled", C.c_int),
+                ("default_smearing_length", C.c_double), ("counterions", CounterionDesc), ("salt", SaltDesc),
+                ("dt", C.c_double), ("n_lambda_omega", C.c_int), ("lambda_omega", C.POINTER(C.c_double)),
+                ("lambda_psi", C.c_double), ("noise", C.c_int), ("scheme", C.c_int), ("precision", C.c_int),
+                ("seed", C.c_ulonglong), ("stress_mask", C.c_int)]
+
+
+class Observables(C.Structure):
+    _fields_ = [("step", C.c_longlong), ("n_species", C.c_int), ("n_ions", C.c_int), ("stress_mask", C.c_int),
+                ("pad_", C.c_int),
+                ("Q_poly", (C.c_double * 2) * MAX_SPECIES), ("Q_ion", (C.c_double * 2) * MAX_IONS),
+                ("mu_ex_poly", (C.c_double * 2) * MAX_SPECIES), ("mu_id_poly", C.c_double * MAX_SPECIES),
+                ("mu_self_poly", C.c_double * MAX_SPECIES),
+                ("mu_ex_ion", (C.c_double * 2) * MAX_IONS), ("mu_id_ion", C.c_double * MAX_IONS),
+                ("mu_self_ion", C.c_double * MAX_IONS),
+                ("H", C.c_double * 2), ("stress", (C.c_double * 2) * 6), ("p_iso", C.c_double * 2)]
+
+    def as_dict(self):
+        ns, ni = self.n_species, self.n_ions
+        c = lambda arr, n: np.array([complex(arr[i][0], arr[i][1]) for i in range(n)])
+        return {
+            "step": self.step,
+            "Q_poly": c(self.Q_poly, ns), "Q_ion": c(self.Q_ion, ni),
+            "mu_ex_poly": c(self.mu_ex_poly, ns), "mu_id_poly": np.array(self.mu_id_poly[:ns]),
+            "mu_self_poly": np.array(self.mu_self_poly[:ns]),
+            "mu_ex_ion": c(self.mu_ex_ion, ni), "mu_id_ion": np.array(self.mu_id_ion[:ni]),
+            "mu_self_ion": np.array(self.mu_self_ion[:ni]),
+            "H": complex(self.H[0], self.H[1]),
+            "stress": c(self.stress, 6), "p_iso": complex(self.p_iso[0], self.p_iso[1]),
+        }
+
+
+class Info(C.Structure):
+    _fields_ = [("grid_size", C.c_longlong), ("step", C.c_longlong), ("n_omega", C.c_int),
+                ("monomer_types", C.c_int), ("n_species", C.c_int), ("n_ions", C.c_int),
+                ("n_segment_classes", C.c_int), ("precision", C.c_int), ("scheme", C.c_int), ("pad_", C.c_int),
+                ("volume", C.c_double), ("rho0", C.c_double), ("short_range_self_pressure", C.c_double),
+                ("mode_eigenvalue", C.c_double * MAX_TYPES), ("omega_coefficient", C.c_double * MAX_TYPES),
+                ("gamma_is_imag", C.c_double * MAX_TYPES), ("mode_O", (C.c_double * MAX_TYPES) * MAX_TYPES),
+                ("ion_valence", C.c_double * MAX_IONS), ("ion_count", C.c_longlong * MAX_IONS),
+                ("ion_smearing", C.c_double * MAX_IONS)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("sweep_ms", C.c_double), ("bead_steps", C.c_longlong), ("sweeps", C.c_longlong),
+                ("algorithmic_bytes", C.c_double)]
+
+
+class FtsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fts error {code}: {msg}")
+        self.code = code
+
+
+def _bind(lib, prefix):
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "create": (C.c_int, [P(SystemDesc), P(vp)]),
+        "destroy": (None, [vp]),
+        "last_error": (C.c_char_p, []),
+        "get_info": (C.c_int, [vp, P(Info)]),
+        "set_fields": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
+        "get_field": (C.c_int, [vp, C.c_int, P(C.c_double), C.c_size_t]),
+        "compute_densities": (C.c_int, [vp, P(C.c_double)]),
+        "measure": (C.c_int, [vp, P(Observables)]),
+        "step": (C.c_int, [vp, C.c_int, C.c_int, P(Observables), C.c_int, P(C.c_int)]),
+        "set_step": (C.c_int, [vp, C.c_longlong]),
+        "fft3d": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_int]),
+        "bead_step": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)]),
+        "set_timing": (C.c_int, [vp, C.c_int]),
+        "get_timing": (C.c_int, [vp, P(Timing)]),
+    }
+    fns = {}
+    for name, (res, args) in sig.items():
+        sym = prefix + name
+        if not hasattr(lib, sym):
+            continue
+        f = getattr(lib, sym)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    return fns
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Engine:
+    """One engine instance over a library exporting the fts C ABI."""
+
+    def __init__(self, lib, prefix, system):
+        self._fns = _bind(lib, prefix)
+        self._keep = system  # keeps the ctypes arrays of the description alive
+        h = C.c_void_p()
+        rc = self._fns["create"](C.byref(system.desc), C.byref(h))
+        self._check(rc)
+        self._h = h
+        self.info = self.get_info()
+        self.M = int(self.info.grid_size)
+        self.shape = (system.desc.m1, system.desc.m2, system.desc.m3)
+
+    def _check(self, rc):
+        if rc != FTS_OK:
+            msg = self._fns["last_error"]()
+            raise FtsError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._fns["destroy"](self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_info(self):
+        info = Info()
+        self._check(self._fns["get_info"](self._h, C.byref(info)))
+        return info
+
+    def set_fields(self, omega, psi=None):
+        om = np.ascontiguousarray(np.asarray(omega, dtype=np.complex128)).view(np.float64)
+        ps = None if psi is None else np.ascontiguousarray(np.asarray(psi, dtype=np.complex128)).view(np.float64)
+        self._check(self._fns["set_fields"](self._h, _dptr(om), None if ps is None else _dptr(ps)))
+
+    def field_count(self, fid):
+        i = self.info
+        return {FIELD_OMEGA: i.n_omega, FIELD_RHO_TYPE: i.monomer_types, FIELD_ION_RHO: i.n_ions}.get(fid, 1)
+
+    def get_field(self, fid, count=None):
+        if count is None:
+            count = self.field_count(fid)
+        out = np.zeros(count * self.M, dtype=np.complex128)
+        v = out.view(np.float64)
+        self._check(self._fns["get_field"](self._h, fid, _dptr(v), v.size))
+        return out.reshape(count, *self.shape)
+
+    def compute_densities(self):
+        q = np.zeros(2 * MAX_SPECIES)
+        self._check(self._fns["compute_densities"](self._h, _dptr(q)))
+        n = self.info.n_species
+        return q[:2 * n:2] + 1j * q[1:2 * n:2]
+
+    def measure(self):
+        rec = Observables()
+        self._check(self._fns["measure"](self._h, C.byref(rec)))
+        return rec.as_dict()
+
+    def step(self, n, record_every=0, capacity=None):
+        if capacity is None:
+            capacity = n // record_every if record_every > 0 else 0
+        recs = (Observables * max(1, capacity))()
+        nrec = C.c_int(0)
+        rc = self._fns["step"](self._h, n, record_every, recs, capacity, C.byref(nrec))
+        out = [recs[i].as_dict() for i in range(nrec.value)]
+        self._check(rc)
+        return out
+
+    def set_step(self, step):
+        self._check(self._fns["set_step"](self._h, step))
+
+    def fft3d(self, x, sign):
+        xin = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).reshape(-1)
+        out = np.zeros_like(xin)
+        self._check(self._fns["fft3d"](self._h, _dptr(xin.view(np.float64)), _dptr(out.view(np.float64)), sign))
+        return out.reshape(self.shape)
+
+    def bead_step(self, q, exp_minus_w, bond=1.0):
+        qi = np.ascontiguousarray(np.asarray(q, dtype=np.complex128)).reshape(-1)
+        ei = np.ascontiguousarray(np.asarray(exp_minus_w, dtype=np.complex128)).reshape(-1)
+        out = np.zeros_like(qi)
+        self._check(self._fns["bead_step"](self._h, _dptr(qi.view(np.float64)), _dptr(ei.view(np.float64)),
+                                           float(bond), _dptr(out.view(np.float64))))
+        return out.reshape(self.shape)
+
+    def set_timing(self, enabled):
+        self._check(self._fns["set_timing"](self._h, int(enabled)))
+
+    def get_timing(self):
+        t = Timing()
+        self._check(self._fns["get_timing"](self._h, C.byref(t)))
+        return {"sweep_ms": t.sweep_ms, "bead_steps": t.bead_steps, "sweeps": t.sweeps,
+                "algorithmic_bytes": t.algorithmic_bytes}
+
+
+_HIP_LIB = None
+
+
+def hip_library_path():
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfts_hip.so")
+
+
+def load_hip_library():
+    """Load the in-tree HIP engine; raise if it has not been built."""
+    global _HIP_LIB
+    if _HIP_LIB is None:
+        path = hip_library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"HIP engine library missing: {path} (run __graft_entry__.build())")
+        _HIP_LIB = C.CDLL(path)
+    return _HIP_LIB
+
+
+def hip_engine(system):
+    """An engine on the GPU (libfts_hip.so, symbols fts_*)."""
+    return Engine(load_hip_library(), "fts_", system)

@@ -1,0 +1,67 @@
+// fft.h -- host interface of the engine's in-house 3D FFT passes (fft.hip).
+//
+// A 3D transform of an m1 x m2 x m3 grid (x slowest) is split into two kinds of
+// pass, each one HBM read and one HBM write of the field:
+//   plane pass   one workgroup per x-plane: the whole m2 x m3 plane is staged in
+//                LDS and transformed along z and y (2D FFT);
+//   pencil pass  one workgroup per (y, z-chunk) tile of m1 x ZC points:
+//                transform along x, optionally multiply by a k-space kernel and
+//                transform back.
+// The propagator bead step q' = e^{-W} IFFT[phi FFT[q] / M] is then
+//   pencil(FWD, x phi/M, INV)   then   plane(INV, x e^{-W}, store q', FWD)
+// with the forward plane pass of the next bead folded into the current one.
+// Each pass can carry two independent jobs (blockIdx.y), so the forward and
+// backward propagator sweeps share every launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "fts_device.h"
+
+namespace fts {
+
+// k-space kernel applied in the pencil pass:
+//   K(k) = [sep ? fx[kx] fy[ky] fz[kz] : 1] * [table ? table[idx] : 1] * (c0 + c1 k^2 + c2 k_a k_b)
+// sep holds m1 + m2 + m3 doubles (x, y, z factors); table holds M doubles.
+struct KOp {
+    const double* sep;
+    const double* table;
+    double c0, c1, c2;
+    int ka, kb;
+};
+
+enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, PLANE_LAST = 4 };
+enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3 };
+
+template <typename T> struct PlaneJob {
+    const V2<T>* src;
+    V2<T>* dst;
+    V2<T>* qout;
+    const V2<T>* emw;
+};
+
+template <typename T> struct PencilJob {
+    V2<T>* data;
+    KOp op;
+};
+
+struct FftGeom {
+    int m1, m2, m3;
+    const double* kvec;   // device: kx[m1], ky[m2], kz[m3]
+    const void* tw_x;     // device twiddles e^{-2 pi i t / m1} (T complex)
+    const void* tw_yz;    // device twiddles e^{-2 pi i t / m2}, m2 == m3
+};
+
+// True when the plane and pencil kernels exist for this grid and precision.
+bool fft_supported(int m1, int m2, int m3, bool fp32);
+
+// Launch one plane pass over all x-planes for njobs (1 or 2) jobs.
+template <typename T>
+hipError_t launch_plane(const FftGeom& g, int mode, const PlaneJob<T>* jobs, int njobs, hipStream_t s);
+
+// Launch one pencil pass for njobs (1 or 2) jobs.
+template <typename T>
+hipError_t launch_pencil(const FftGeom& g, int mode, const PencilJob<T>* jobs, int njobs, hipStream_t s);
+
+} // namespace fts

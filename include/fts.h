@@ -1,0 +1,228 @@
+/*
+ * fts.h -- C-ABI boundary of the MI355X complex-Langevin field-theoretic
+ * simulation engine (libfts_hip.so).
+ *
+ * The reference exposes its hot path only as in-process C++ class methods that
+ * take a cufftHandle and a deviceState& (no plugin or FFI layer).  Each entry
+ * point below replaces one of those methods; the reference interface it stands
+ * for is cited beside it.  Conventions:
+ *   - every call returns an int status: FTS_OK, FTS_NONFINITE, FTS_INVALID or
+ *     FTS_DEVICE_ERROR; fts_last_error() gives the text of the last failure
+ *     (the reference prints "error: ..." and returns false, errHandle.cu:5-9;
+ *     it calls exit() on CUDA errors, errHandle.cu:65-84 -- this library never
+ *     exits);
+ *   - complex arrays are interleaved doubles (re, im), grid index
+ *     i = (x*m2 + y)*m3 + z (common.h:54-58), multi-field arrays field-major
+ *     [count][M] (deviceState.h:7-13);
+ *   - the engine owns device memory; the caller owns host arrays, which are
+ *     copied synchronously at the boundary;
+ *   - one engine per host thread; an engine handle is not thread-safe.
+ */
+#ifndef FTS_H
+#define FTS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FTS_OK 0
+#define FTS_NONFINITE 1
+#define FTS_INVALID 2
+#define FTS_DEVICE_ERROR 3
+
+#define FTS_SCHEME_PC_REFERENCE 0 /* predictor-corrector, complexLangevin.cu:328-368 */
+#define FTS_SCHEME_EM 1           /* plain explicit Euler-Maruyama (no reference counterpart) */
+#define FTS_SCHEME_SEMI_IMPLICIT 2 /* k-space Debye-preconditioned semi-implicit EM (no reference counterpart) */
+
+#define FTS_PREC_FP64 0 /* complex FP64 everywhere, as the reference (common.h:23) */
+#define FTS_PREC_FP32 1 /* complex FP32 propagators/FFTs, per-bead log-renormalisation, FP64 reductions */
+
+#define FTS_MAX_SPECIES 8
+#define FTS_MAX_IONS 8
+#define FTS_MAX_TYPES 8
+
+/* stress_mask bits: components in the reference's fixed order (stress.cu:56-61) */
+#define FTS_STRESS_XX 0x01
+#define FTS_STRESS_YY 0x02
+#define FTS_STRESS_ZZ 0x04
+#define FTS_STRESS_XY 0x08
+#define FTS_STRESS_XZ 0x10
+#define FTS_STRESS_YZ 0x20
+#define FTS_STRESS_PISO 0x40 /* isotropic pressure, observable.cu:297-317 */
+
+/* field ids for fts_get_field (complexLangevin.cu:236-253 plus the propagators) */
+#define FTS_FIELD_OMEGA 0      /* [n_omega][M] */
+#define FTS_FIELD_PSI 1        /* [M] */
+#define FTS_FIELD_RHO_TYPE 2   /* [monomer_types][M] */
+#define FTS_FIELD_RHO_P 3      /* [M] */
+#define FTS_FIELD_RHO_C 4      /* [M] */
+#define FTS_FIELD_RHO_C_POLY 5 /* [M] */
+#define FTS_FIELD_RHO_C_ION 6  /* [M] */
+#define FTS_FIELD_ION_RHO 7    /* [n_ions][M] */
+#define FTS_FIELD_QF 8         /* [N][M] forward propagator of the last species solved */
+#define FTS_FIELD_QB 9         /* [N][M] backward propagator of the last species solved */
+
+/* One polymer species: param.h:8-17 plus its sequence block (polymer.cu:123-160). */
+typedef struct fts_species_desc {
+    long long num_chains;  /* n_p */
+    int N;                 /* beads per chain */
+    const int* types;      /* [N] monomer type of each bead */
+    const double* charges; /* [N] valence of each bead */
+    const double* bonds;   /* [N-1] bond lengths b_s, or NULL for all 1.0 (polymer.cu:151) */
+} fts_species_desc;
+
+/* [ions.counterions] (param.h:19-26) */
+typedef struct fts_counterion_desc {
+    int enabled;
+    int auto_number; /* neutralise the polymer charge (ion.cu:11-19) */
+    long long number;
+    double valence;
+    double smearing_length; /* <= 0: default smearing length */
+} fts_counterion_desc;
+
+/* [ions.salt] (param.h:28-39) */
+typedef struct fts_salt_desc {
+    int enabled;
+    long long num_salt;
+    double cation_valence;
+    double anion_valence;
+    int cation_stoich;
+    int anion_stoich;
+    double smearing_length; /* <= 0: default smearing length */
+    int combine_identical_species; /* merge equal (z, a_j) species (ion.cu:117-127) */
+} fts_salt_desc;
+
+/* Everything the reference derives from its parameter and sequence files that the
+ * hot path reads (param.cu:342-436, interactionMode.cu:9-298). */
+typedef struct fts_system {
+    int m1, m2, m3;  /* grid */
+    double dL;       /* grid spacing; L_d = dL * m_d, dV = dL^3 (param.cu:348-351) */
+    double a;        /* Gaussian smearing length */
+    int monomer_types;  /* M */
+    double xi;          /* K = xi + chi (1 - delta) */
+    const double* chi;  /* [M*M] Flory matrix or NULL (all zero) */
+    double eigen_tol_rel; /* active-mode tolerance; < 0 selects the reference default 1e-12 */
+    double lB;          /* Bjerrum length; 0 disables electrostatics */
+    int n_species;
+    const fts_species_desc* species;
+    int ions_enabled;
+    double default_smearing_length; /* <= 0: a (param.cu:353) */
+    fts_counterion_desc counterions;
+    fts_salt_desc salt;
+    double dt;
+    int n_lambda_omega;          /* 1 (broadcast) or n_omega values (param.cu:390-410) */
+    const double* lambda_omega;
+    double lambda_psi;
+    int noise;                   /* 0 disables the CL noise (param.h:100) */
+    int scheme;                  /* FTS_SCHEME_* */
+    int precision;               /* FTS_PREC_* */
+    unsigned long long seed;     /* Philox4x32-10 key */
+    int stress_mask;             /* FTS_STRESS_* bits; 0 = no stress observable */
+} fts_system;
+
+/* observableRecord (observable.h:7-13), with the partition functions beside it. */
+typedef struct fts_observables {
+    long long step;
+    int n_species;
+    int n_ions;
+    int stress_mask;
+    int pad_;
+    double Q_poly[FTS_MAX_SPECIES][2];
+    double Q_ion[FTS_MAX_IONS][2];
+    double mu_ex_poly[FTS_MAX_SPECIES][2]; /* -ln Q, principal branch (common.h:197-201) */
+    double mu_id_poly[FTS_MAX_SPECIES];
+    double mu_self_poly[FTS_MAX_SPECIES];
+    double mu_ex_ion[FTS_MAX_IONS][2];
+    double mu_id_ion[FTS_MAX_IONS];
+    double mu_self_ion[FTS_MAX_IONS];
+    double H[2];          /* mean-field Hamiltonian density H/V (observable.cu:160-182) */
+    double stress[6][2];  /* xx yy zz xy xz yz (stress.cu:161-196); zero if not in mask */
+    double p_iso[2];      /* observable.cu:297-317; zero if not in mask */
+} fts_observables;
+
+typedef struct fts_info {
+    long long grid_size;
+    long long step;     /* CL steps taken (noise counter) */
+    int n_omega;        /* active interaction modes */
+    int monomer_types;
+    int n_species;
+    int n_ions;
+    int n_segment_classes; /* distinct (type, charge) bead classes: e^{-W} fields kept per species */
+    int precision;
+    int scheme;
+    int pad_;
+    double volume;
+    double rho0;
+    double short_range_self_pressure; /* param.cu:135-149 */
+    double mode_eigenvalue[FTS_MAX_TYPES];   /* active modes, descending (interactionMode.cu:214-263) */
+    double omega_coefficient[FTS_MAX_TYPES]; /* rho0/|d_l| (param.cu:115-118) */
+    double gamma_is_imag[FTS_MAX_TYPES];     /* 1 if gamma_l = i else 0 (interactionMode.cu:328-332) */
+    double mode_O[FTS_MAX_TYPES][FTS_MAX_TYPES]; /* O[l][alpha] of the active modes */
+    double ion_valence[FTS_MAX_IONS];
+    long long ion_count[FTS_MAX_IONS];
+    double ion_smearing[FTS_MAX_IONS];
+} fts_info;
+
+/* Accumulated device timing of the propagator sweeps (HIP events on the engine's
+ * stream), enabled by fts_set_timing. */
+typedef struct fts_timing {
+    double sweep_ms;          /* total event time of all propagator sweeps */
+    long long bead_steps;     /* bead steps (q_s -> q_{s+1}) executed inside them, both directions */
+    long long sweeps;         /* number of timed sweep launches */
+    double algorithmic_bytes; /* 2*c*M per bead step, summed */
+} fts_timing;
+
+typedef struct fts_engine fts_engine;
+
+/* Build an engine: param::derive + interactionMode::build + ion::build +
+ * spectral::build_kernels + complexLangevin::initialize (param.cu:342-436,
+ * ion.cu:3-46, spectral.cu:3-35, complexLangevin.cu:166-191).  Fields start at 0. */
+int fts_create(const fts_system* sys, fts_engine** out);
+void fts_destroy(fts_engine* h);
+const char* fts_last_error(void);
+int fts_get_info(fts_engine* h, fts_info* out);
+
+/* Set omega [n_omega][M] and psi [M] (interleaved complex); removes the psi zero
+ * mode as complexLangevin.cu:183 does on load.  psi may be NULL (zero). */
+int fts_set_fields(fts_engine* h, const double* omega, const double* psi);
+
+/* Copy a field out: n_doubles must equal the field's size in doubles. */
+int fts_get_field(fts_engine* h, int field_id, double* out, size_t n_doubles);
+
+/* complexLangevin::compute_densities at the current fields (complexLangevin.cu:255-260):
+ * polymerSpecies::solve_and_compute_densities (polymer.cu:449-478) then
+ * ion::compute_densities (ion.cu:54-72).  Q_out: [n_species][2] or NULL. */
+int fts_compute_densities(fts_engine* h, double* Q_out);
+
+/* compute_densities then observable::measure (observable.cu:53-67) at the
+ * current fields. */
+int fts_measure(fts_engine* h, fts_observables* out);
+
+/* Run n CL steps (complexLangevin.cu:669-698 for PC_REFERENCE).  When
+ * record_every > 0, every step with step % record_every == 0 stores a record
+ * (PC: at the predictor state, as complexLangevin.cu:684; EM/SI: at the state the
+ * step starts from) into records[0..capacity); *n_records receives the count.
+ * Returns FTS_NONFINITE (fields left at the failing step) on a non-finite Q or field. */
+int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, int capacity, int* n_records);
+
+/* Noise counter: the next step draws Philox counter `step`. */
+int fts_set_step(fts_engine* h, long long step);
+
+/* Test hooks on the hot path. */
+/* 3D DFT of one field on the engine's FFT, unnormalised; sign -1 forward (e^{-ikx}), +1 inverse
+ * (the cuFFT convention of common.h:1063-1077). */
+int fts_fft3d(fts_engine* h, const double* in, double* out, int sign);
+/* One bead step q' = e^{-W} * IFFT[phi_b(k) FFT[q]/M] (polymer.cu:396-400) with bond length b. */
+int fts_bead_step(fts_engine* h, const double* q, const double* exp_minus_w, double bond, double* q_out);
+
+/* Device timing of the propagator sweeps (engine only; the oracle ignores it). */
+int fts_set_timing(fts_engine* h, int enabled);
+int fts_get_timing(fts_engine* h, fts_timing* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FTS_H */
