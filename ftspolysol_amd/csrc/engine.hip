@@ -1,0 +1,1597 @@
+// engine.hip -- host engine and C ABI of the MI355X complex-Langevin engine.
+//
+// One fts_engine owns every device buffer (the role of deviceState,
+// deviceState.cu:8-105) and one HIP stream (default priority).  The hot path
+// per species and state is
+//   smoothing           3-pass FFT applies (plane FWD, pencil x K/M, plane INV)
+//   Boltzmann factors   one e^{-W} field per distinct (type, charge) bead class
+//   two sweeps          forward and backward propagators advance together:
+//                       pencil(FWD, x phi/M, INV) + plane(INV, x e^{-W}, store q, FWD)
+//                       per bead step, both sweeps in each launch
+//   Q                   FP64 reduction of qF_{N-1}
+//   contraction         one pass over the stored qF, qB -> unsmoothed densities
+// followed by Gaussian smoothing, ions, drifts and the fused CL update kernels.
+// Reductions stay on the device; the host reads a scalar only where it needs
+// one (finiteness checks, records).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fts.h"
+#include "fft.h"
+#include "kernels.h"
+
+using namespace fts;
+
+namespace {
+
+constexpr double Pi = 3.141592653589793238462643383279502884;
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return fail(FTS_DEVICE_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline int ew_blocks(size_t n) { return static_cast<int>((n + EW_THREADS - 1) / EW_THREADS); }
+
+template <typename X> struct DevBuf {
+    X* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(X) * count);
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct HostSpecies {
+    long long n = 0;
+    int N = 0;
+    std::vector<int> type;
+    std::vector<double> charge, bond;
+    std::vector<int> bond_cls;   // sep-table class of each bond
+    // bead classes: distinct (type, charge)
+    std::vector<int> cls_of_bead, cls_type, cls_start, cls_beads;
+    std::vector<double> cls_z;
+    DevBuf<int> d_cls_type, d_cls_start, d_cls_beads;
+    DevBuf<double> d_cls_z, d_cls_shift;
+    std::vector<double> cls_shift; // per class Re<W>, FP32 only
+    double charge_square_sum() const
+    {
+        double t = 0;
+        for (double z : charge) t += z * z;
+        return t;
+    }
+    double net_charge() const
+    {
+        double t = 0;
+        for (double z : charge) t += z;
+        return t;
+    }
+};
+
+struct HostIon {
+    double z = 0, a = 0;
+    long long n = 0;
+};
+
+// scalar slots in the device scalar buffer
+enum {
+    SC_Q = 0,        // [8] polymer partition functions (of the stored, possibly shifted, propagators)
+    SC_QION = 8,     // [8] ion partition functions
+    SC_PSI_MEAN = 16,
+    SC_ETA_MEAN = 17,
+    SC_DRIFT_MEAN = 18,
+    SC_TMP = 19,
+    SC_CONTRACT = 24, // [7]
+    SC_H = 32,
+    SC_SHIFT = 40,    // [MAX_CLASSES]
+    SC_COUNT = 40 + MAX_CLASSES
+};
+
+} // namespace
+
+struct fts_engine {
+    // ---- parameters ---------------------------------------------------------
+    int m1 = 0, m2 = 0, m3 = 0, M = 0;
+    double L1 = 0, L2 = 0, L3 = 0, V = 0, dV = 0, a = 0, lB = 0, dt = 0, lambda_psi = 0;
+    bool noise = true;
+    int scheme = 0, precision = 0, stress_mask = 0;
+    unsigned long long seed = 0;
+    long long step_counter = 0;
+    int nT = 0, n_omega = 0;
+    double xi = 0, rho0 = 0, P_sr = 0;
+    std::vector<double> K, eig, O;
+    std::vector<int> active;
+    std::vector<double> lambda_omega;
+    long long total_segments = 0;
+    std::vector<long long> total_type_segments;
+    std::vector<HostSpecies> species;
+    std::vector<HostIon> ions;
+    int Nmax = 0, max_cls = 0, last_species = 0;
+    double mode_term[6] = {0, 0, 0, 0, 0, 0};
+    ModeCoefs mc{};
+    std::vector<double2> mode_coef_host; // gamma O [nO][nT]
+
+    // ---- device ---------------------------------------------------------------
+    hipStream_t stream = nullptr;
+    FftGeom g64{}, g32{};
+    DevBuf<double> kvec, sep_gamma, sep_bond, sep_ion, si_table_omega, si_table_psi;
+    DevBuf<double2> tw64_x, tw64_yz;
+    DevBuf<float2> tw32_x, tw32_yz;
+    int n_bond_cls = 0;
+    std::vector<double> bond_cls_len;
+    DevBuf<double2> omega, omega_bar, omega_s, drift, psi, psi_bar, psi_s, dpsi, dpsi_bar;
+    DevBuf<double2> W_type_s, rho_hat, rho_type, rho_c_hat, rho_c, rho_c_poly, rho_c_ion, rho_p;
+    DevBuf<double2> ion_psi, ion_b, ion_rho, t1, t2, t3, mode_coef;
+    DevBuf<double2> partials, scal;
+    DevBuf<int> flag;
+    // propagators (double or float complex)
+    void* qF = nullptr;
+    void* qB = nullptr;
+    void* E = nullptr;
+    void* tmpF = nullptr;
+    void* tmpB = nullptr;
+    void* kA = nullptr;
+    void* kB = nullptr;
+    std::vector<double> q_shift_total; // per species: sum of class shifts over the chain
+
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> event_pool;
+    fts_timing tstat{};
+
+    ~fts_engine() { release(); }
+    void release();
+
+    size_t csize() const { return precision == FTS_PREC_FP32 ? sizeof(float2) : sizeof(double2); }
+    double2* f(DevBuf<double2>& b, size_t k) { return b.p + k * static_cast<size_t>(M); }
+    bool electro() const { return lB > 0.0; }
+    double omega_coefficient(int l) const { return rho0 * (1.0 / std::fabs(eig[static_cast<size_t>(active[static_cast<size_t>(l)])])); }
+    bool gamma_imag(int l) const { return eig[static_cast<size_t>(active[static_cast<size_t>(l)])] > 0.0; }
+    double Ocoef(int l, int al) const { return O[static_cast<size_t>(active[static_cast<size_t>(l)]) * nT + al]; }
+
+    // setup
+    int build_modes(const fts_system* s);
+    int build_ions(const fts_system* s);
+    int setup_device();
+    int build_si_tables();
+
+    // primitives
+    KOp kop_sep(const double* sep, double c0) const { return KOp{sep, nullptr, c0, 0.0, 0.0, 0, 0}; }
+    int fft_apply(const double2* src, double2* dst, const KOp& op);
+    int fft_forward(const double2* src, double2* dst);
+    template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst);
+    int reduce_finalize(int K, int slot, double scale);
+    int sum_field(const double2* a, int slot, double scale);
+    int remove_mean(double2* a);
+    int read_scal(int slot, int count, double2* out);
+
+    // hot path
+    template <typename T> int solve_species(int p, bool accumulate = true);
+    int compute_densities(const double2* om, const double2* ps);
+    int psi_drift(const double2* ps, const double2* rho_src, double2* out);
+    int check_finite(bool& ok);
+    int measure(const double2* om, const double2* ps, fts_observables* rec);
+    template <typename T> int stress_chain(int p, std::vector<std::array<double2, 7>>& acc);
+    int contract(const void* A, const void* B, bool fp32, const double* sep, double2 out[7]);
+    int step_pc(bool record, fts_observables* rec);
+    int step_em(bool record, fts_observables* rec, bool si);
+
+    // timing
+    hipEvent_t get_event();
+    int resolve_timing();
+};
+
+void fts_engine::release()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& pr : pending) {
+        event_pool.push_back(pr.first);
+        event_pool.push_back(pr.second);
+    }
+    pending.clear();
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    event_pool.clear();
+    for (DevBuf<double>* b : {&kvec, &sep_gamma, &sep_bond, &sep_ion, &si_table_omega, &si_table_psi}) b->release();
+    tw64_x.release();
+    tw64_yz.release();
+    tw32_x.release();
+    tw32_yz.release();
+    for (DevBuf<double2>* b : {&omega, &omega_bar, &omega_s, &drift, &psi, &psi_bar, &psi_s, &dpsi, &dpsi_bar, &W_type_s, &rho_hat,
+                               &rho_type, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &ion_psi, &ion_b, &ion_rho, &t1, &t2,
+                               &t3, &mode_coef, &partials, &scal})
+        b->release();
+    flag.release();
+    for (HostSpecies& sp : species) {
+        sp.d_cls_type.release();
+        sp.d_cls_start.release();
+        sp.d_cls_beads.release();
+        sp.d_cls_z.release();
+        sp.d_cls_shift.release();
+    }
+    for (void** p : {&qF, &qB, &E, &tmpF, &tmpB, &kA, &kB}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Setup: interaction modes (interactionMode.cu:9-298), ions (ion.cu:3-46, 117-134)
+// ---------------------------------------------------------------------------
+namespace {
+
+double frob(const std::vector<double>& x)
+{
+    long double s = 0.0L;
+    for (double v : x) s += static_cast<long double>(v) * v;
+    return std::sqrt(static_cast<double>(s));
+}
+
+bool jacobi_eigen(const std::vector<double>& in, int n, std::vector<double>& ev, std::vector<double>& vecs)
+{
+    std::vector<double> A = in;
+    vecs.assign(static_cast<size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) vecs[static_cast<size_t>(i) * n + i] = 1.0;
+    ev.assign(static_cast<size_t>(n), 0.0);
+    if (n == 1) {
+        ev[0] = A[0];
+        return true;
+    }
+    auto off = [&]() {
+        long double s = 0.0L;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                if (i != j) s += static_cast<long double>(A[static_cast<size_t>(i) * n + j]) * A[static_cast<size_t>(i) * n + j];
+        return std::sqrt(static_cast<double>(s));
+    };
+    const double scale = std::max(1.0, frob(in));
+    const int max_sweeps = std::max(50, 100 * n * n);
+    bool converged = false;
+    for (int sweep = 0; sweep < max_sweeps && !converged; ++sweep) {
+        if (off() <= 1.0e-14 * scale) {
+            converged = true;
+            break;
+        }
+        bool rotated = false;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[static_cast<size_t>(p) * n + q];
+                if (std::fabs(apq) <= 2.220446049250313e-16 * scale) continue;
+                const double app = A[static_cast<size_t>(p) * n + p], aqq = A[static_cast<size_t>(q) * n + q];
+                const double tau = (aqq - app) / (2.0 * apq);
+                const double t = tau >= 0.0 ? 1.0 / (tau + std::sqrt(1.0 + tau * tau)) : -1.0 / (-tau + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    if (k == p || k == q) continue;
+                    const double akp = A[static_cast<size_t>(k) * n + p], akq = A[static_cast<size_t>(k) * n + q];
+                    const double nkp = c * akp - s * akq, nkq = s * akp + c * akq;
+                    A[static_cast<size_t>(k) * n + p] = A[static_cast<size_t>(p) * n + k] = nkp;
+                    A[static_cast<size_t>(k) * n + q] = A[static_cast<size_t>(q) * n + k] = nkq;
+                }
+                A[static_cast<size_t>(p) * n + p] = c * c * app - 2.0 * s * c * apq + s * s * aqq;
+                A[static_cast<size_t>(q) * n + q] = s * s * app + 2.0 * s * c * apq + c * c * aqq;
+                A[static_cast<size_t>(p) * n + q] = A[static_cast<size_t>(q) * n + p] = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = vecs[static_cast<size_t>(k) * n + p], vkq = vecs[static_cast<size_t>(k) * n + q];
+                    vecs[static_cast<size_t>(k) * n + p] = c * vkp - s * vkq;
+                    vecs[static_cast<size_t>(k) * n + q] = s * vkp + c * vkq;
+                }
+                rotated = true;
+            }
+        if (!rotated) converged = true;
+    }
+    if (!converged && off() > 1.0e-10 * scale) return false;
+    for (int i = 0; i < n; ++i) ev[static_cast<size_t>(i)] = A[static_cast<size_t>(i) * n + i];
+    return true;
+}
+
+inline int spectral_mode(int i, int m) { return i > m / 2 ? i - m : i; }
+inline double wave_number(int i, int m, double L) { return 2.0 * Pi * spectral_mode(i, m) / L; }
+
+} // namespace
+
+int fts_engine::build_modes(const fts_system* s)
+{
+    nT = s->monomer_types;
+    if (nT <= 0 || nT > FTS_MAX_TYPES) return fail(FTS_INVALID, "monomer_types out of range");
+    xi = s->xi;
+    if (!std::isfinite(xi) || xi < 0.0) return fail(FTS_INVALID, "xi");
+    const double tol_rel = s->eigen_tol_rel < 0.0 ? 1.0e-12 : s->eigen_tol_rel;
+    std::vector<double> chi(static_cast<size_t>(nT) * nT, 0.0);
+    if (s->chi) chi.assign(s->chi, s->chi + static_cast<size_t>(nT) * nT);
+    double max_chi = 0.0;
+    for (double x : chi) {
+        if (!std::isfinite(x)) return fail(FTS_INVALID, "chi matrix entry");
+        max_chi = std::max(max_chi, std::fabs(x));
+    }
+    const double sym_tol = 1.0e-12 * std::max(1.0, max_chi);
+    for (int al = 0; al < nT; ++al) {
+        if (std::fabs(chi[static_cast<size_t>(al) * nT + al]) > sym_tol) return fail(FTS_INVALID, "chi diagonal entries must be zero");
+        chi[static_cast<size_t>(al) * nT + al] = 0.0;
+        for (int be = al + 1; be < nT; ++be) {
+            double& ab = chi[static_cast<size_t>(al) * nT + be];
+            double& ba = chi[static_cast<size_t>(be) * nT + al];
+            if (std::fabs(ab - ba) > sym_tol) return fail(FTS_INVALID, "chi matrix must be symmetric");
+            ab = ba = 0.5 * (ab + ba);
+        }
+    }
+    K.assign(static_cast<size_t>(nT) * nT, 0.0);
+    for (int al = 0; al < nT; ++al)
+        for (int be = 0; be < nT; ++be) K[static_cast<size_t>(al) * nT + be] = al == be ? xi : xi + chi[static_cast<size_t>(al) * nT + be];
+    std::vector<double> rev, rvec;
+    if (!jacobi_eigen(K, nT, rev, rvec)) return fail(FTS_INVALID, "Jacobi diagonalization of interaction matrix did not converge");
+    std::vector<int> order(static_cast<size_t>(nT));
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) {
+        if (rev[static_cast<size_t>(x)] != rev[static_cast<size_t>(y)]) return rev[static_cast<size_t>(x)] > rev[static_cast<size_t>(y)];
+        return x < y;
+    });
+    eig.assign(static_cast<size_t>(nT), 0.0);
+    O.assign(static_cast<size_t>(nT) * nT, 0.0);
+    for (int l = 0; l < nT; ++l) {
+        const int raw = order[static_cast<size_t>(l)];
+        eig[static_cast<size_t>(l)] = rev[static_cast<size_t>(raw)];
+        double* row = &O[static_cast<size_t>(l) * nT];
+        for (int al = 0; al < nT; ++al) row[al] = rvec[static_cast<size_t>(al) * nT + raw];
+        int piv = 0;
+        for (int al = 1; al < nT; ++al)
+            if (std::fabs(row[al]) > std::fabs(row[piv])) piv = al;
+        if (row[piv] < 0.0)
+            for (int al = 0; al < nT; ++al) row[al] *= -1.0;
+    }
+    double max_abs = 0.0;
+    for (double d : eig) max_abs = std::max(max_abs, std::fabs(d));
+    const double tol_abs = tol_rel * std::max(1.0, max_abs);
+    active.clear();
+    for (int l = 0; l < nT; ++l)
+        if (std::fabs(eig[static_cast<size_t>(l)]) > tol_abs) active.push_back(l);
+    n_omega = static_cast<int>(active.size());
+    std::vector<double> res(static_cast<size_t>(nT) * nT), orth(static_cast<size_t>(nT) * nT);
+    for (int al = 0; al < nT; ++al)
+        for (int be = 0; be < nT; ++be) {
+            double v = 0.0, d = 0.0;
+            for (int l = 0; l < nT; ++l) {
+                v += O[static_cast<size_t>(l) * nT + al] * eig[static_cast<size_t>(l)] * O[static_cast<size_t>(l) * nT + be];
+                d += O[static_cast<size_t>(al) * nT + l] * O[static_cast<size_t>(be) * nT + l];
+            }
+            res[static_cast<size_t>(al) * nT + be] = K[static_cast<size_t>(al) * nT + be] - v;
+            orth[static_cast<size_t>(al) * nT + be] = d - (al == be ? 1.0 : 0.0);
+        }
+    if (frob(res) / std::max(1.0, frob(K)) > 1.0e-10) return fail(FTS_INVALID, "interaction eigenmode reconstruction check failed");
+    if (frob(orth) > 1.0e-10) return fail(FTS_INVALID, "interaction eigenvector orthogonality check failed");
+    return FTS_OK;
+}
+
+int fts_engine::build_ions(const fts_system* s)
+{
+    ions.clear();
+    double polymer_charge = 0.0;
+    for (const HostSpecies& sp : species) polymer_charge += static_cast<double>(sp.n) * sp.net_charge();
+    const double def_a = s->default_smearing_length > 0.0 ? s->default_smearing_length : a;
+    const bool combine = s->salt.combine_identical_species != 0;
+    auto append = [&](double z, long long n, double aj) {
+        if (n == 0) return;
+        if (combine)
+            for (HostIon& ion : ions)
+                if (std::fabs(ion.z - z) <= 1.0e-12 && std::fabs(ion.a - aj) <= 1.0e-12) {
+                    ion.n += n;
+                    return;
+                }
+        ions.push_back(HostIon{z, aj, n});
+    };
+    if (s->ions_enabled && s->counterions.enabled) {
+        const double z = s->counterions.valence;
+        const double aj = s->counterions.smearing_length > 0.0 ? s->counterions.smearing_length : def_a;
+        if (electro() && z == 0.0) return fail(FTS_INVALID, "counterion valence");
+        long long n = s->counterions.number;
+        if (s->counterions.auto_number) {
+            if (electro()) {
+                const double need = -polymer_charge / z;
+                if (need < -1.0e-10) return fail(FTS_INVALID, "counterion sign");
+                const double r = std::round(need);
+                if (std::fabs(need - r) > 1.0e-8 * std::max(1.0, std::fabs(need))) return fail(FTS_INVALID, "counterion number");
+                n = static_cast<long long>(r);
+            } else {
+                n = 0;
+            }
+        }
+        if (n < 0 || aj <= 0.0) return fail(FTS_INVALID, "counterion");
+        append(z, n, aj);
+    }
+    if (s->ions_enabled && s->salt.enabled) {
+        const fts_salt_desc& sl = s->salt;
+        const double aj = sl.smearing_length > 0.0 ? sl.smearing_length : def_a;
+        if (sl.num_salt < 0 || sl.cation_stoich < 0 || sl.anion_stoich < 0) return fail(FTS_INVALID, "salt counts");
+        if (electro()) {
+            if (sl.cation_valence == 0.0 || sl.anion_valence == 0.0) return fail(FTS_INVALID, "salt");
+            if (std::fabs(sl.cation_stoich * sl.cation_valence + sl.anion_stoich * sl.anion_valence) > 1.0e-10)
+                return fail(FTS_INVALID, "salt is not neutral");
+        }
+        append(sl.cation_valence, sl.num_salt * static_cast<long long>(sl.cation_stoich), aj);
+        append(sl.anion_valence, sl.num_salt * static_cast<long long>(sl.anion_stoich), aj);
+    }
+    double total = polymer_charge, scale = std::fabs(polymer_charge);
+    for (const HostIon& ion : ions) {
+        if (ion.n < 0 || (electro() && ion.z == 0.0) || ion.a <= 0.0) return fail(FTS_INVALID, "ion species");
+        total += ion.z * static_cast<double>(ion.n);
+        scale += std::fabs(ion.z * static_cast<double>(ion.n));
+    }
+    if (electro() && std::fabs(total) > 1.0e-8 * std::max(1.0, scale)) return fail(FTS_INVALID, "electroneutrality failed");
+    if (ions.size() > FTS_MAX_IONS) return fail(FTS_INVALID, "too many ion species");
+    return FTS_OK;
+}
+
+// Device buffers, FFT tables and per-species bead classes.
+int fts_engine::setup_device()
+{
+    HIP_TRY(hipStreamCreate(&stream));
+    const size_t Ms = static_cast<size_t>(M);
+    // wave numbers and twiddles
+    std::vector<double> kv;
+    for (int x = 0; x < m1; ++x) kv.push_back(wave_number(x, m1, L1));
+    for (int y = 0; y < m2; ++y) kv.push_back(wave_number(y, m2, L2));
+    for (int z = 0; z < m3; ++z) kv.push_back(wave_number(z, m3, L3));
+    HIP_TRY(kvec.alloc(kv.size()));
+    HIP_TRY(hipMemcpy(kvec.p, kv.data(), sizeof(double) * kv.size(), hipMemcpyHostToDevice));
+    auto twiddles = [](int n, std::vector<double2>& t64, std::vector<float2>& t32) {
+        const long double two_pi = 6.283185307179586476925286766559L;
+        for (int t = 0; t < n; ++t) {
+            const long double ang = -two_pi * t / n;
+            double2 v;
+            v.x = static_cast<double>(cosl(ang));
+            v.y = static_cast<double>(sinl(ang));
+            t64.push_back(v);
+            float2 w;
+            w.x = static_cast<float>(v.x);
+            w.y = static_cast<float>(v.y);
+            t32.push_back(w);
+        }
+    };
+    std::vector<double2> tx64, tyz64;
+    std::vector<float2> tx32, tyz32;
+    twiddles(m1, tx64, tx32);
+    twiddles(m2, tyz64, tyz32);
+    HIP_TRY(tw64_x.alloc(m1));
+    HIP_TRY(tw64_yz.alloc(m2));
+    HIP_TRY(tw32_x.alloc(m1));
+    HIP_TRY(tw32_yz.alloc(m2));
+    HIP_TRY(hipMemcpy(tw64_x.p, tx64.data(), sizeof(double2) * m1, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tw64_yz.p, tyz64.data(), sizeof(double2) * m2, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tw32_x.p, tx32.data(), sizeof(float2) * m1, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tw32_yz.p, tyz32.data(), sizeof(float2) * m2, hipMemcpyHostToDevice));
+    g64 = FftGeom{m1, m2, m3, kvec.p, tw64_x.p, tw64_yz.p};
+    g32 = FftGeom{m1, m2, m3, kvec.p, tw32_x.p, tw32_yz.p};
+    // separable kernel tables: factor per dimension of exp(-c k^2) = prod_d exp(-c k_d^2)
+    auto sep_table = [&](double c, std::vector<double>& out) {
+        for (double k : kv) out.push_back(std::exp(-c * k * k));
+    };
+    std::vector<double> sg;
+    sep_table(0.5 * a * a, sg); // Gamma = exp(-a^2 k^2 / 2)  (common.h:290-293)
+    HIP_TRY(sep_gamma.alloc(sg.size()));
+    HIP_TRY(hipMemcpy(sep_gamma.p, sg.data(), sizeof(double) * sg.size(), hipMemcpyHostToDevice));
+    std::vector<double> sb;
+    for (double b : bond_cls_len) sep_table(b * b / 6.0, sb); // phi = exp(-b^2 k^2 / 6)  (common.h:295-298)
+    HIP_TRY(sep_bond.alloc(std::max<size_t>(1, sb.size())));
+    if (!sb.empty()) HIP_TRY(hipMemcpy(sep_bond.p, sb.data(), sizeof(double) * sb.size(), hipMemcpyHostToDevice));
+    std::vector<double> si;
+    for (const HostIon& ion : ions) sep_table(0.5 * ion.a * ion.a, si);
+    HIP_TRY(sep_ion.alloc(std::max<size_t>(1, si.size())));
+    if (!si.empty()) HIP_TRY(hipMemcpy(sep_ion.p, si.data(), sizeof(double) * si.size(), hipMemcpyHostToDevice));
+    // stress mode term (stress.cu:106-149)
+    {
+        double sum_over_k2[6] = {0, 0, 0, 0, 0, 0};
+        long long modes = 0;
+        for (int x = 0; x < m1; ++x)
+            for (int y = 0; y < m2; ++y)
+                for (int z = 0; z < m3; ++z) {
+                    const double kx = kv[static_cast<size_t>(x)], ky = kv[static_cast<size_t>(m1 + y)], kz = kv[static_cast<size_t>(m1 + m2 + z)];
+                    const double k2 = kx * kx + ky * ky + kz * kz;
+                    if (k2 > 0.0) {
+                        ++modes;
+                        const double v[6] = {kx * kx, ky * ky, kz * kz, kx * ky, kx * kz, ky * kz};
+                        for (int c = 0; c < 6; ++c) sum_over_k2[c] += v[c] / k2;
+                    }
+                }
+        for (int c = 0; c < 6; ++c)
+            mode_term[c] = electro() ? ((1.0 / 3.0) * static_cast<double>(modes) * (c < 3 ? 1.0 : 0.0) - sum_over_k2[c]) / V : 0.0;
+    }
+    // fields
+    const size_t nO = static_cast<size_t>(n_omega), nTs = static_cast<size_t>(nT), nI = std::max<size_t>(1, ions.size());
+    for (DevBuf<double2>* b : {&omega, &omega_bar, &omega_s, &drift}) HIP_TRY(b->alloc(std::max<size_t>(1, nO) * Ms));
+    for (DevBuf<double2>* b : {&psi, &psi_bar, &psi_s, &dpsi, &dpsi_bar, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &t1, &t2, &t3})
+        HIP_TRY(b->alloc(Ms));
+    for (DevBuf<double2>* b : {&W_type_s, &rho_hat, &rho_type}) HIP_TRY(b->alloc(nTs * Ms));
+    for (DevBuf<double2>* b : {&ion_psi, &ion_b, &ion_rho}) HIP_TRY(b->alloc(nI * Ms));
+    HIP_TRY(partials.alloc(8 * RED_BLOCKS));
+    HIP_TRY(scal.alloc(SC_COUNT));
+    HIP_TRY(flag.alloc(1));
+    HIP_TRY(mode_coef.alloc(std::max<size_t>(1, nO * nTs)));
+    mode_coef_host.assign(std::max<size_t>(1, nO * nTs), double2{0.0, 0.0});
+    for (int l = 0; l < n_omega; ++l)
+        for (int al = 0; al < nT; ++al) {
+            // gamma_l * O[l][alpha]  (param.cu:125-133, complex_scale(gamma, O))
+            const double o = Ocoef(l, al);
+            mode_coef_host[static_cast<size_t>(l) * nT + al] = gamma_imag(l) ? double2{0.0 * o, 1.0 * o} : double2{1.0 * o, 0.0 * o};
+        }
+    HIP_TRY(hipMemcpy(mode_coef.p, mode_coef_host.data(), sizeof(double2) * mode_coef_host.size(), hipMemcpyHostToDevice));
+    for (DevBuf<double2>* b : {&omega, &omega_bar, &psi, &psi_bar, &scal}) HIP_TRY(hipMemset(b->p, 0, sizeof(double2) * b->n));
+    // propagators
+    const size_t cs = csize();
+    for (void** p : {&qF, &qB}) HIP_TRY(hipMalloc(p, cs * static_cast<size_t>(Nmax) * Ms));
+    HIP_TRY(hipMalloc(&E, cs * static_cast<size_t>(max_cls) * Ms));
+    for (void** p : {&tmpF, &tmpB, &kA, &kB}) HIP_TRY(hipMalloc(p, cs * Ms));
+    // bead classes
+    for (HostSpecies& sp : species) {
+        const int nc = static_cast<int>(sp.cls_type.size());
+        HIP_TRY(sp.d_cls_type.alloc(nc));
+        HIP_TRY(sp.d_cls_z.alloc(nc));
+        HIP_TRY(sp.d_cls_shift.alloc(nc));
+        HIP_TRY(sp.d_cls_start.alloc(nc + 1));
+        HIP_TRY(sp.d_cls_beads.alloc(sp.N));
+        HIP_TRY(hipMemcpy(sp.d_cls_type.p, sp.cls_type.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(sp.d_cls_z.p, sp.cls_z.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(sp.d_cls_shift.p, 0, sizeof(double) * nc));
+        HIP_TRY(hipMemcpy(sp.d_cls_start.p, sp.cls_start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(sp.d_cls_beads.p, sp.cls_beads.data(), sizeof(int) * sp.N, hipMemcpyHostToDevice));
+        sp.cls_shift.assign(static_cast<size_t>(nc), 0.0);
+    }
+    q_shift_total.assign(species.size(), 0.0);
+    // mode coefficients for the update kernels
+    mc.nO = n_omega;
+    mc.nT = nT;
+    for (int l = 0; l < n_omega; ++l) {
+        mc.c[l] = omega_coefficient(l);
+        mc.lam_dt[l] = dt * lambda_omega[static_cast<size_t>(l)];
+        mc.sigma[l] = (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_omega[static_cast<size_t>(l)] * dt / dV);
+        for (int al = 0; al < nT; ++al) mc.g[l * nT + al] = mode_coef_host[static_cast<size_t>(l) * nT + al];
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return FTS_OK;
+}
+
+// Linear-response kernels of the semi-implicit scheme (same definition as the CPU
+// oracle's; no reference counterpart):
+//   A_l(k)   = c_l + s_l Gamma^2 sum_p (n_p/V) sum_{s,s'} O_{l a(s)} O_{l a(s')} phi_{ss'}(k)
+//   A_psi(k) = k^2/(4 pi lB) + Gamma^2 sum_p (n_p/V) sum_{s,s'} z_s z_s' phi_{ss'}(k) + sum_j z_j^2 (n_j/V) Gamma_j^2
+// The tables hold P(k) = 1 / (1 + dt lam A(k)).
+int fts_engine::build_si_tables()
+{
+    std::vector<double> kv(static_cast<size_t>(m1 + m2 + m3));
+    HIP_TRY(hipMemcpy(kv.data(), kvec.p, sizeof(double) * kv.size(), hipMemcpyDeviceToHost));
+    std::vector<double> Pw(static_cast<size_t>(n_omega) * M), Pp(static_cast<size_t>(M), 1.0);
+    auto pair_sum = [&](const HostSpecies& sp, const std::vector<double>& w, double ksq) {
+        double total = 0.0, h = 0.0;
+        for (int s = 0; s < sp.N; ++s) {
+            total += w[static_cast<size_t>(s)] * w[static_cast<size_t>(s)] + 2.0 * w[static_cast<size_t>(s)] * h;
+            if (s < sp.N - 1) {
+                const double b = sp.bond[static_cast<size_t>(s)];
+                h = (h + w[static_cast<size_t>(s)]) * std::exp(-b * b * ksq / 6.0);
+            }
+        }
+        return total;
+    };
+    std::vector<std::vector<std::vector<double>>> wmode(static_cast<size_t>(n_omega));
+    for (int l = 0; l < n_omega; ++l)
+        for (const HostSpecies& sp : species) {
+            std::vector<double> w(static_cast<size_t>(sp.N));
+            for (int s = 0; s < sp.N; ++s) w[static_cast<size_t>(s)] = Ocoef(l, sp.type[static_cast<size_t>(s)]);
+            wmode[static_cast<size_t>(l)].push_back(w);
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < M; ++i) {
+        const int z = i % m3, y = (i / m3) % m2, x = i / (m2 * m3);
+        const double kx = kv[static_cast<size_t>(x)], ky = kv[static_cast<size_t>(m1 + y)], kz = kv[static_cast<size_t>(m1 + m2 + z)];
+        const double ksq = kx * kx + ky * ky + kz * kz;
+        const double G2 = std::exp(-a * a * ksq);
+        for (int l = 0; l < n_omega; ++l) {
+            double resp = 0.0;
+            for (size_t p = 0; p < species.size(); ++p)
+                resp += static_cast<double>(species[p].n) / V * pair_sum(species[p], wmode[static_cast<size_t>(l)][p], ksq);
+            const double A = omega_coefficient(l) + (gamma_imag(l) ? 1.0 : -1.0) * G2 * resp;
+            Pw[static_cast<size_t>(l) * M + i] = 1.0 / (1.0 + dt * lambda_omega[static_cast<size_t>(l)] * A);
+        }
+        if (electro()) {
+            double resp = 0.0;
+            for (const HostSpecies& sp : species) resp += static_cast<double>(sp.n) / V * pair_sum(sp, sp.charge, ksq);
+            double ion_resp = 0.0;
+            for (const HostIon& ion : ions) {
+                const double g = std::exp(-0.5 * ion.a * ion.a * ksq);
+                ion_resp += ion.z * ion.z * static_cast<double>(ion.n) / V * g * g;
+            }
+            const double A = ksq / (4.0 * Pi * lB) + G2 * resp + ion_resp;
+            Pp[static_cast<size_t>(i)] = 1.0 / (1.0 + dt * lambda_psi * A);
+        }
+    }
+    for (double v : Pw)
+        if (!(v > 0.0) || !std::isfinite(v)) return fail(FTS_INVALID, "semi-implicit kernel 1 + dt lam A must be positive");
+    HIP_TRY(si_table_omega.alloc(Pw.size()));
+    HIP_TRY(si_table_psi.alloc(Pp.size()));
+    HIP_TRY(hipMemcpy(si_table_omega.p, Pw.data(), sizeof(double) * Pw.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(si_table_psi.p, Pp.data(), sizeof(double) * Pp.size(), hipMemcpyHostToDevice));
+    return FTS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Primitives
+// ---------------------------------------------------------------------------
+// dst = IFFT[K(k) FFT[src]]; callers fold 1/M into K (fft_apply_real_kernel, common.h:1079-1085)
+int fts_engine::fft_apply(const double2* src, double2* dst, const KOp& op)
+{
+    PlaneJob<double> pj{src, dst, nullptr, nullptr};
+    HIP_TRY(launch_plane<double>(g64, PLANE_FWD, &pj, 1, stream));
+    PencilJob<double> xj{dst, op};
+    HIP_TRY(launch_pencil<double>(g64, PENCIL_FWD_MUL_INV, &xj, 1, stream));
+    PlaneJob<double> ij{dst, dst, nullptr, nullptr};
+    HIP_TRY(launch_plane<double>(g64, PLANE_INV, &ij, 1, stream));
+    return FTS_OK;
+}
+
+template <typename T> int fts_engine::fft_forward_t(const V2<T>* src, V2<T>* dst)
+{
+    const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
+    PlaneJob<T> pj{src, dst, nullptr, nullptr};
+    HIP_TRY(launch_plane<T>(g, PLANE_FWD, &pj, 1, stream));
+    PencilJob<T> xj{dst, KOp{}};
+    HIP_TRY(launch_pencil<T>(g, PENCIL_FWD, &xj, 1, stream));
+    return FTS_OK;
+}
+
+int fts_engine::fft_forward(const double2* src, double2* dst) { return fft_forward_t<double>(src, dst); }
+
+int fts_engine::reduce_finalize(int Kc, int slot, double scale)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, partials.p, Kc, scal.p + slot, scale);
+    HIP_TRY(hipGetLastError());
+    return FTS_OK;
+}
+
+int fts_engine::sum_field(const double2* x, int slot, double scale)
+{
+    hipLaunchKernelGGL(k_sum<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, x, M, partials.p);
+    HIP_TRY(hipGetLastError());
+    return reduce_finalize(1, slot, scale);
+}
+
+// a -= mean(a)  (remove_complex_mean, common.h:1008-1012)
+int fts_engine::remove_mean(double2* x)
+{
+    int rc = sum_field(x, SC_TMP, 1.0 / static_cast<double>(M));
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_add_scalar, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, x, scal.p + SC_TMP, -1.0, M);
+    HIP_TRY(hipGetLastError());
+    return FTS_OK;
+}
+
+int fts_engine::read_scal(int slot, int count, double2* out)
+{
+    HIP_TRY(hipMemcpyAsync(out, scal.p + slot, sizeof(double2) * count, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return FTS_OK;
+}
+
+hipEvent_t fts_engine::get_event()
+{
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+int fts_engine::resolve_timing()
+{
+    if (pending.empty()) return FTS_OK;
+    HIP_TRY(hipEventSynchronize(pending.back().second));
+    for (auto& pr : pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+        tstat.sweep_ms += ms;
+        event_pool.push_back(pr.first);
+        event_pool.push_back(pr.second);
+    }
+    pending.clear();
+    return FTS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Propagator solve for species p (polymer.cu:418-426 + accumulate, 434-447)
+// ---------------------------------------------------------------------------
+template <typename T> int fts_engine::solve_species(int p, bool accumulate)
+{
+    using Cv = V2<T>;
+    HostSpecies& sp = species[static_cast<size_t>(p)];
+    last_species = p;
+    const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
+    const size_t Ms = static_cast<size_t>(M);
+    Cv* qf = static_cast<Cv*>(qF);
+    Cv* qb = static_cast<Cv*>(qB);
+    Cv* e = static_cast<Cv*>(E);
+    Cv* tf = static_cast<Cv*>(tmpF);
+    Cv* tb = static_cast<Cv*>(tmpB);
+    const int nc = static_cast<int>(sp.cls_type.size());
+    // FP32: shift each class's potential by Re<W_c> (log-renormalisation kept in FP64)
+    if (sizeof(T) == 4) {
+        for (int c = 0; c < nc; ++c) {
+            hipLaunchKernelGGL(k_sum_class_w, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, f(W_type_s, sp.cls_type[static_cast<size_t>(c)]),
+                               psi_s.p, electro() ? sp.cls_z[static_cast<size_t>(c)] : 0.0, M, partials.p);
+            HIP_TRY(hipGetLastError());
+            int rc = reduce_finalize(1, SC_SHIFT + c, 1.0 / static_cast<double>(M));
+            if (rc) return rc;
+        }
+        std::vector<double2> sh(static_cast<size_t>(nc));
+        int rc = read_scal(SC_SHIFT, nc, sh.data());
+        if (rc) return rc;
+        double total = 0.0;
+        for (int c = 0; c < nc; ++c) sp.cls_shift[static_cast<size_t>(c)] = sh[static_cast<size_t>(c)].x;
+        for (int s = 0; s < sp.N; ++s) total += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)])];
+        q_shift_total[static_cast<size_t>(p)] = total;
+        HIP_TRY(hipMemcpyAsync(sp.d_cls_shift.p, sp.cls_shift.data(), sizeof(double) * nc, hipMemcpyHostToDevice, stream));
+    }
+    hipLaunchKernelGGL(k_boltzmann<T>, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, W_type_s.p, psi_s.p, sp.d_cls_type.p, sp.d_cls_z.p,
+                       sp.d_cls_shift.p, nc, M, e);
+    HIP_TRY(hipGetLastError());
+    auto E_of = [&](int s) { return e + static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)]) * Ms; };
+    const int N = sp.N;
+    if (N == 1) {
+        HIP_TRY(hipMemcpyAsync(qf, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(qb, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
+    } else {
+        PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0)}, {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1)}};
+        HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, 2, stream));
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (timing) {
+            ev0 = get_event();
+            HIP_TRY(hipEventRecord(ev0, stream));
+        }
+        const double invM = 1.0 / static_cast<double>(M);
+        for (int s = 0; s < N - 1; ++s) {
+            const int bf = sp.bond_cls[static_cast<size_t>(s)];
+            const int bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
+            PencilJob<T> pj[2] = {{tf, kop_sep(sep_bond.p + static_cast<size_t>(bf) * (m1 + m2 + m3), invM)},
+                                  {tb, kop_sep(sep_bond.p + static_cast<size_t>(bb) * (m1 + m2 + m3), invM)}};
+            HIP_TRY(launch_pencil<T>(g, PENCIL_FWD_MUL_INV, pj, 2, stream));
+            const int sf = s + 1, sb = N - 2 - s;
+            PlaneJob<T> st[2] = {{tf, tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf)}, {tb, tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb)}};
+            HIP_TRY(launch_plane<T>(g, s < N - 2 ? PLANE_STEP : PLANE_LAST, st, 2, stream));
+        }
+        if (timing) {
+            ev1 = get_event();
+            HIP_TRY(hipEventRecord(ev1, stream));
+            pending.emplace_back(ev0, ev1);
+            tstat.bead_steps += 2LL * (N - 1);
+            tstat.sweeps += 1;
+            tstat.algorithmic_bytes += 2.0 * (N - 1) * 2.0 * static_cast<double>(sizeof(Cv)) * static_cast<double>(M);
+        }
+    }
+    // Q = <qF_{N-1}> (polymer.cu:425)
+    hipLaunchKernelGGL(k_sum<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, qf + static_cast<size_t>(N - 1) * Ms, M, partials.p);
+    HIP_TRY(hipGetLastError());
+    int rc = reduce_finalize(1, SC_Q + p, 1.0 / static_cast<double>(M));
+    if (rc) return rc;
+    if (!accumulate) return FTS_OK;
+    // density contraction
+    hipLaunchKernelGGL(k_contract_density<T>, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, qf, qb, sp.d_cls_start.p, sp.d_cls_beads.p,
+                       sp.d_cls_type.p, sp.d_cls_z.p, sp.d_cls_shift.p, nc, W_type_s.p, psi_s.p, scal.p + SC_Q + p,
+                       static_cast<double>(sp.n) / V, electro() ? 1 : 0, M, rho_hat.p, rho_c_hat.p);
+    HIP_TRY(hipGetLastError());
+    return FTS_OK;
+}
+
+// complexLangevin::compute_densities (complexLangevin.cu:255-260)
+int fts_engine::compute_densities(const double2* om, const double2* ps)
+{
+    const size_t Ms = static_cast<size_t>(M);
+    const double invM = 1.0 / static_cast<double>(M);
+    int rc;
+    // build_smoothed_sources (polymer.cu:341-363)
+    for (int l = 0; l < n_omega; ++l)
+        if ((rc = fft_apply(om + l * Ms, f(omega_s, l), kop_sep(sep_gamma.p, invM)))) return rc;
+    hipLaunchKernelGGL(k_combine_modes, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, omega_s.p, mode_coef.p, n_omega, nT, M, W_type_s.p);
+    HIP_TRY(hipGetLastError());
+    if (electro()) {
+        if ((rc = fft_apply(ps, psi_s.p, kop_sep(sep_gamma.p, invM)))) return rc;
+    } else {
+        HIP_TRY(hipMemsetAsync(psi_s.p, 0, sizeof(double2) * Ms, stream));
+    }
+    HIP_TRY(hipMemsetAsync(rho_hat.p, 0, sizeof(double2) * rho_hat.n, stream));
+    HIP_TRY(hipMemsetAsync(rho_c_hat.p, 0, sizeof(double2) * Ms, stream));
+    for (int p = 0; p < static_cast<int>(species.size()); ++p) {
+        rc = precision == FTS_PREC_FP32 ? solve_species<float>(p) : solve_species<double>(p);
+        if (rc) return rc;
+    }
+    // smoothing of the densities (polymer.cu:467-476)
+    for (int al = 0; al < nT; ++al)
+        if ((rc = fft_apply(f(rho_hat, al), f(rho_type, al), kop_sep(sep_gamma.p, invM)))) return rc;
+    hipLaunchKernelGGL(k_sum_fields, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho_type.p, nT, M, rho_p.p);
+    HIP_TRY(hipGetLastError());
+    if (electro()) {
+        if ((rc = fft_apply(rho_c_hat.p, rho_c_poly.p, kop_sep(sep_gamma.p, invM)))) return rc;
+        HIP_TRY(hipMemcpyAsync(rho_c.p, rho_c_poly.p, sizeof(double2) * Ms, hipMemcpyDeviceToDevice, stream));
+    } else {
+        HIP_TRY(hipMemsetAsync(rho_c_poly.p, 0, sizeof(double2) * Ms, stream));
+        HIP_TRY(hipMemsetAsync(rho_c.p, 0, sizeof(double2) * Ms, stream));
+    }
+    HIP_TRY(hipMemsetAsync(rho_c_ion.p, 0, sizeof(double2) * Ms, stream));
+    // ion::compute_densities (ion.cu:54-72)
+    for (size_t j = 0; j < ions.size(); ++j) {
+        const double nd = static_cast<double>(ions[j].n) / V;
+        double2* rho = f(ion_rho, j);
+        if (!electro()) {
+            hipLaunchKernelGGL(k_fill, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho, double2{nd, 0.0}, M);
+            HIP_TRY(hipGetLastError());
+            continue;
+        }
+        const double* sepj = sep_ion.p + j * static_cast<size_t>(m1 + m2 + m3);
+        if ((rc = fft_apply(ps, f(ion_psi, j), kop_sep(sepj, invM)))) return rc;
+        hipLaunchKernelGGL(k_ion_boltzmann, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, f(ion_psi, j), ions[j].z, M, f(ion_b, j), partials.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = reduce_finalize(1, SC_QION + static_cast<int>(j), invM))) return rc;
+        hipLaunchKernelGGL(k_ion_density, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, f(ion_b, j), scal.p + SC_QION + j, nd, M, rho);
+        HIP_TRY(hipGetLastError());
+        // ion charge z S_{Gamma_j} rho_j (fft_apply_real_kernel_scaled, common.h:1087-1091)
+        if ((rc = fft_apply(rho, t1.p, kop_sep(sepj, ions[j].z * invM)))) return rc;
+        hipLaunchKernelGGL(k_add2, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho_c_ion.p, rho_c.p, t1.p, M);
+        HIP_TRY(hipGetLastError());
+    }
+    return FTS_OK;
+}
+
+// u_psi = S_{k^2} psi / (4 pi lB) + i rho, minus its mean (complexLangevin.cu:299-311)
+int fts_engine::psi_drift(const double2* ps, const double2* rho_src, double2* out)
+{
+    if (!electro()) {
+        HIP_TRY(hipMemsetAsync(out, 0, sizeof(double2) * M, stream));
+        return FTS_OK;
+    }
+    int rc = fft_apply(ps, t2.p, KOp{nullptr, nullptr, 0.0, 1.0 / static_cast<double>(M), 0.0, 0, 0});
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_psi_drift, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, t2.p, rho_src, 1.0 / (4.0 * Pi * lB), M, out);
+    HIP_TRY(hipGetLastError());
+    return remove_mean(out);
+}
+
+int fts_engine::check_finite(bool& ok)
+{
+    const int one = 1;
+    HIP_TRY(hipMemcpyAsync(flag.p, &one, sizeof(int), hipMemcpyHostToDevice, stream));
+    const size_t no = static_cast<size_t>(n_omega) * M;
+    if (no) hipLaunchKernelGGL(k_finite, dim3(ew_blocks(no)), dim3(EW_THREADS), 0, stream, omega.p, no, flag.p);
+    hipLaunchKernelGGL(k_finite, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi.p, static_cast<size_t>(M), flag.p);
+    HIP_TRY(hipGetLastError());
+    int h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    ok = h != 0;
+    return FTS_OK;
+}
+
+// out_c = sum_k w_c(k) S(k) A(k) B(-k)
+int fts_engine::contract(const void* A, const void* B, bool fp32, const double* sep, double2 out[7])
+{
+    if (fp32)
+        hipLaunchKernelGGL(k_contract_kspace<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, static_cast<const float2*>(A),
+                           static_cast<const float2*>(B), kvec.p, sep, m1, m2, m3, partials.p);
+    else
+        hipLaunchKernelGGL(k_contract_kspace<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, static_cast<const double2*>(A),
+                           static_cast<const double2*>(B), kvec.p, sep, m1, m2, m3, partials.p);
+    HIP_TRY(hipGetLastError());
+    int rc = reduce_finalize(7, SC_CONTRACT, 1.0);
+    if (rc) return rc;
+    return read_scal(SC_CONTRACT, 7, out);
+}
+
+// Chain-connectivity sums of species p: acc[s][c] = sum_k w_c phi_s(k) FFT[qB_{s+1}](k) FFT[qF_s](-k),
+// the k-space form of <qF_s S_{-k_a k_b}[e^{W_s} qB_s]> with FFT[e^{W_s} qB_s] = phi_s FFT[qB_{s+1}].
+template <typename T> int fts_engine::stress_chain(int p, std::vector<std::array<double2, 7>>& acc)
+{
+    using Cv = V2<T>;
+    const HostSpecies& sp = species[static_cast<size_t>(p)];
+    const size_t Ms = static_cast<size_t>(M);
+    Cv* qf = static_cast<Cv*>(qF);
+    Cv* qb = static_cast<Cv*>(qB);
+    for (int s = 0; s + 1 < sp.N; ++s) {
+        int rc = fft_forward_t<T>(qf + static_cast<size_t>(s) * Ms, static_cast<Cv*>(kA));
+        if (rc) return rc;
+        rc = fft_forward_t<T>(qb + static_cast<size_t>(s + 1) * Ms, static_cast<Cv*>(kB));
+        if (rc) return rc;
+        const double* sep = sep_bond.p + static_cast<size_t>(sp.bond_cls[static_cast<size_t>(s)]) * (m1 + m2 + m3);
+        rc = contract(kB, kA, sizeof(T) == 4, sep, acc[static_cast<size_t>(s)].data());
+        if (rc) return rc;
+    }
+    return FTS_OK;
+}
+
+// observable::measure (observable.cu:53-67) at (om, ps), after compute_densities at the same state.
+int fts_engine::measure(const double2* om, const double2* ps, fts_observables* rec)
+{
+    std::memset(rec, 0, sizeof(*rec));
+    rec->step = step_counter;
+    rec->n_species = static_cast<int>(species.size());
+    rec->n_ions = static_cast<int>(ions.size());
+    rec->stress_mask = stress_mask;
+    const double invM = 1.0 / static_cast<double>(M);
+    const double invM2 = invM * invM;
+    int rc;
+    double2 Q[FTS_MAX_SPECIES], Qi[FTS_MAX_IONS];
+    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
+    if (!ions.empty() && (rc = read_scal(SC_QION, static_cast<int>(ions.size()), Qi))) return rc;
+    auto neglog = [](double2 z) { return double2{-std::log(std::sqrt(z.x * z.x + z.y * z.y)), -std::atan2(z.y, z.x)}; };
+    double2 h{0.0, 0.0};
+    for (size_t p = 0; p < species.size(); ++p) {
+        double2 mu = neglog(Q[p]);
+        mu.x += q_shift_total[p]; // Q = Q~ e^{-sum of shifts}
+        const double scale_q = std::exp(-q_shift_total[p]);
+        rec->Q_poly[p][0] = Q[p].x * scale_q;
+        rec->Q_poly[p][1] = Q[p].y * scale_q;
+        rec->mu_ex_poly[p][0] = mu.x;
+        rec->mu_ex_poly[p][1] = mu.y;
+        rec->mu_id_poly[p] = std::log(static_cast<double>(species[p].n) / V);
+        rec->mu_self_poly[p] = -lB * species[p].charge_square_sum() / (2.0 * std::sqrt(Pi) * a);
+        const double w = static_cast<double>(species[p].n) / V;
+        h.x += w * mu.x;
+        h.y += w * mu.y;
+    }
+    for (size_t j = 0; j < ions.size(); ++j) {
+        double2 mu{0.0, 0.0}, Qj{1.0, 0.0};
+        if (electro()) {
+            Qj = Qi[j];
+            mu = neglog(Qj);
+        }
+        rec->Q_ion[j][0] = Qj.x;
+        rec->Q_ion[j][1] = Qj.y;
+        rec->mu_ex_ion[j][0] = mu.x;
+        rec->mu_ex_ion[j][1] = mu.y;
+        rec->mu_id_ion[j] = std::log(static_cast<double>(ions[j].n) / V);
+        rec->mu_self_ion[j] = -lB * ions[j].z * ions[j].z / (2.0 * std::sqrt(Pi) * ions[j].a);
+        const double w = static_cast<double>(ions[j].n) / V;
+        h.x += w * mu.x;
+        h.y += w * mu.y;
+    }
+    // H/V: sum_l (c_l/2) <omega_l^2> + (1/(8 pi lB)) <psi S_{k^2} psi> + sum n mu / V  (observable.cu:160-182)
+    for (int l = 0; l < n_omega; ++l) {
+        const double2* w = om + static_cast<size_t>(l) * M;
+        hipLaunchKernelGGL(k_sum_product, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, w, w, M, partials.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = reduce_finalize(1, SC_H, 0.5 * omega_coefficient(l) * invM))) return rc;
+        double2 v;
+        if ((rc = read_scal(SC_H, 1, &v))) return rc;
+        h.x += v.x;
+        h.y += v.y;
+    }
+    double2 S_psi[7] = {};
+    if (electro()) {
+        if ((rc = fft_forward(ps, t3.p))) return rc;
+        if ((rc = contract(t3.p, t3.p, false, nullptr, S_psi))) return rc;
+        const double s = 1.0 / (8.0 * Pi * lB) * invM2;
+        h.x += s * (S_psi[0].x + S_psi[1].x + S_psi[2].x);
+        h.y += s * (S_psi[0].y + S_psi[1].y + S_psi[2].y);
+    }
+    rec->H[0] = h.x;
+    rec->H[1] = h.y;
+    if (!(stress_mask & 0x7F)) return FTS_OK;
+
+    // stress tensor (stress.cu:161-321) in k-space
+    double2 sig[6];
+    long long ion_count = 0, chains = 0;
+    for (const HostIon& ion : ions) ion_count += ion.n;
+    for (const HostSpecies& sp : species) chains += sp.n;
+    const double ideal = (static_cast<double>(chains) + static_cast<double>(ion_count)) / V;
+    for (int c = 0; c < 6; ++c) sig[c] = double2{(ideal + P_sr) * (c < 3 ? 1.0 : 0.0) + mode_term[c], 0.0};
+    if (electro()) {
+        const double s = 1.0 / (4.0 * Pi * lB) * invM2;
+        const double2 k2s{S_psi[0].x + S_psi[1].x + S_psi[2].x, S_psi[0].y + S_psi[1].y + S_psi[2].y};
+        for (int c = 0; c < 6; ++c) {
+            const double d = c < 3 ? 1.0 / 3.0 : 0.0;
+            sig[c].x += s * (S_psi[c].x - d * k2s.x);
+            sig[c].y += s * (S_psi[c].y - d * k2s.y);
+        }
+    }
+    // chain connectivity: + (n/(V Q M^2)) sum_s (b_s^2/3) sum_k k_a k_b phi_s qB^_{s+1}(k) qF^_s(-k)
+    for (size_t p = 0; p < species.size(); ++p) {
+        if (species.size() > 1) {
+            // the propagator store holds one species: re-solve species p at the same state
+            rc = precision == FTS_PREC_FP32 ? solve_species<float>(static_cast<int>(p), false) : solve_species<double>(static_cast<int>(p), false);
+            if (rc) return rc;
+        }
+        const HostSpecies& sp = species[p];
+        std::vector<std::array<double2, 7>> acc(static_cast<size_t>(std::max(1, sp.N - 1)));
+        rc = precision == FTS_PREC_FP32 ? stress_chain<float>(static_cast<int>(p), acc) : stress_chain<double>(static_cast<int>(p), acc);
+        if (rc) return rc;
+        double2 tot[6] = {};
+        for (int s = 0; s + 1 < sp.N; ++s) {
+            const double w = sp.bond[static_cast<size_t>(s)] * sp.bond[static_cast<size_t>(s)] / 3.0;
+            for (int c = 0; c < 6; ++c) {
+                tot[c].x += w * acc[static_cast<size_t>(s)][c].x;
+                tot[c].y += w * acc[static_cast<size_t>(s)][c].y;
+            }
+        }
+        // prefactor (n / V) / Q~ (the shifts cancel between qF_s qB_{s+1} and Q~)
+        const double2 pref = [&]() {
+            const double2 q = Q[p];
+            const double ax = std::fabs(q.x), ay = std::fabs(q.y);
+            double2 inv;
+            if (ax >= ay) {
+                const double r = q.y / q.x, d = q.x * (1.0 + r * r);
+                inv = double2{1.0 / d, -r / d};
+            } else {
+                const double r = q.x / q.y, d = q.y * (1.0 + r * r);
+                inv = double2{r / d, -1.0 / d};
+            }
+            const double sc = static_cast<double>(sp.n) / V * invM2;
+            return double2{inv.x * sc, inv.y * sc};
+        }();
+        for (int c = 0; c < 6; ++c) {
+            sig[c].x += pref.x * tot[c].x - pref.y * tot[c].y;
+            sig[c].y += pref.x * tot[c].y + pref.y * tot[c].x;
+        }
+    }
+    // polymer field coupling: sum_alpha [ -a^2/M^2 S_ab(W^_a, P^_a) + (delta/2) S_1/M^2 ] (+ psi terms)
+    const double a2 = a * a;
+    for (int al = 0; al < nT; ++al) {
+        if ((rc = fft_forward(f(W_type_s, al), t1.p))) return rc;
+        if ((rc = fft_forward(f(rho_hat, al), t2.p))) return rc;
+        double2 S[7];
+        if ((rc = contract(t1.p, t2.p, false, nullptr, S))) return rc;
+        for (int c = 0; c < 6; ++c) {
+            const double d = c < 3 ? 0.5 : 0.0;
+            sig[c].x += invM2 * (-a2 * S[c].x + d * S[6].x);
+            sig[c].y += invM2 * (-a2 * S[c].y + d * S[6].y);
+        }
+    }
+    if (electro()) {
+        if ((rc = fft_forward(psi_s.p, t1.p))) return rc;
+        if ((rc = fft_forward(rho_c_hat.p, t2.p))) return rc;
+        double2 S[7];
+        if ((rc = contract(t1.p, t2.p, false, nullptr, S))) return rc;
+        for (int c = 0; c < 6; ++c) {
+            const double d = c < 3 ? 1.0 / 6.0 : 0.0;
+            const double2 v{invM2 * (-a2 * S[c].x + d * S[6].x), invM2 * (-a2 * S[c].y + d * S[6].y)};
+            sig[c].x += -v.y; // i * v
+            sig[c].y += v.x;
+        }
+        // ions: i (n_j z_j / (V Q_j)) [ -a_j^2/M^2 S_ab(psi~^_j, b^_j) + (delta/6) S_1/M^2 ]
+        for (size_t j = 0; j < ions.size(); ++j) {
+            if ((rc = fft_forward(f(ion_psi, j), t1.p))) return rc;
+            if ((rc = fft_forward(f(ion_b, j), t2.p))) return rc;
+            double2 S[7];
+            if ((rc = contract(t1.p, t2.p, false, nullptr, S))) return rc;
+            const double2 q = Qi[j];
+            const double ax = std::fabs(q.x), ay = std::fabs(q.y);
+            double2 inv;
+            if (ax >= ay) {
+                const double r = q.y / q.x, d = q.x * (1.0 + r * r);
+                inv = double2{1.0 / d, -r / d};
+            } else {
+                const double r = q.x / q.y, d = q.y * (1.0 + r * r);
+                inv = double2{r / d, -1.0 / d};
+            }
+            const double sc = static_cast<double>(ions[j].n) * ions[j].z / V;
+            const double2 pref{-inv.y * sc, inv.x * sc}; // i * inv * sc
+            const double aj2 = ions[j].a * ions[j].a;
+            for (int c = 0; c < 6; ++c) {
+                const double d = c < 3 ? 1.0 / 6.0 : 0.0;
+                const double2 v{invM2 * (-aj2 * S[c].x + d * S[6].x), invM2 * (-aj2 * S[c].y + d * S[6].y)};
+                sig[c].x += pref.x * v.x - pref.y * v.y;
+                sig[c].y += pref.x * v.y + pref.y * v.x;
+            }
+        }
+    }
+    for (int c = 0; c < 6; ++c)
+        if (stress_mask & (1 << c)) {
+            rec->stress[c][0] = sig[c].x;
+            rec->stress[c][1] = sig[c].y;
+        }
+    if (stress_mask & FTS_STRESS_PISO) {
+        // p_iso = tr(sigma)/3 (observable.cu:297-317 is the same sum with k_a k_b -> k^2/3)
+        rec->p_iso[0] = (sig[0].x + sig[1].x + sig[2].x) / 3.0;
+        rec->p_iso[1] = (sig[0].y + sig[1].y + sig[2].y) / 3.0;
+    }
+    return FTS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CL steps
+// ---------------------------------------------------------------------------
+namespace {
+bool finite2(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
+} // namespace
+
+// run_step (complexLangevin.cu:669-698): predictor-corrector with the same noise in both halves
+int fts_engine::step_pc(bool record, fts_observables* rec)
+{
+    const unsigned long long t = static_cast<unsigned long long>(step_counter);
+    const size_t Ms = static_cast<size_t>(M);
+    double2 Q[FTS_MAX_SPECIES];
+    int rc = compute_densities(omega.p, psi.p);
+    if (rc) return rc;
+    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
+    for (size_t p = 0; p < species.size(); ++p)
+        if (!finite2(Q[p])) return fail(FTS_NONFINITE, "non-finite Q at step " + std::to_string(step_counter + 1));
+    const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
+    if (electro()) {
+        hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, seed, t, 0x100u, sigma_psi, M, partials.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = reduce_finalize(1, SC_ETA_MEAN, 1.0 / static_cast<double>(M)))) return rc;
+    }
+    if (n_omega > 0) {
+        hipLaunchKernelGGL(k_omega_predict, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, omega.p, rho_type.p, seed, t, M, drift.p, omega_bar.p);
+        HIP_TRY(hipGetLastError());
+    }
+    // predictor psi drift uses rho_c of the current fields (complexLangevin.cu:331, 674)
+    if ((rc = psi_drift(psi.p, rho_c.p, dpsi.p))) return rc;
+    if (electro()) {
+        hipLaunchKernelGGL(k_psi_update, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, -dt * lambda_psi,
+                           static_cast<const double2*>(nullptr), 0.0, sigma_psi, scal.p + SC_ETA_MEAN, seed, t, M, psi_bar.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = remove_mean(psi_bar.p))) return rc;
+    } else {
+        HIP_TRY(hipMemsetAsync(psi_bar.p, 0, sizeof(double2) * Ms, stream));
+    }
+    if ((rc = compute_densities(omega_bar.p, psi_bar.p))) return rc;
+    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
+    for (size_t p = 0; p < species.size(); ++p)
+        if (!finite2(Q[p])) return fail(FTS_NONFINITE, "non-finite Qbar at step " + std::to_string(step_counter + 1));
+    if ((rc = psi_drift(psi_bar.p, rho_c.p, dpsi_bar.p))) return rc;
+    if (record && (rc = measure(omega_bar.p, psi_bar.p, rec))) return rc;
+    if (n_omega > 0) {
+        hipLaunchKernelGGL(k_omega_correct, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, drift.p, omega_bar.p, rho_type.p, seed, t, M, omega.p);
+        HIP_TRY(hipGetLastError());
+    }
+    if (electro()) {
+        hipLaunchKernelGGL(k_psi_update, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, -0.5 * dt * lambda_psi,
+                           static_cast<const double2*>(dpsi_bar.p), -0.5 * dt * lambda_psi, sigma_psi, scal.p + SC_ETA_MEAN, seed, t, M, psi.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = remove_mean(psi.p))) return rc;
+    } else {
+        HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));
+    }
+    ++step_counter;
+    bool ok = true;
+    if ((rc = check_finite(ok))) return rc;
+    if (!ok) return fail(FTS_NONFINITE, "updated fields have non-finite values at step " + std::to_string(step_counter));
+    return FTS_OK;
+}
+
+// Plain Euler-Maruyama, and the semi-implicit variant w <- w + S_P[-dt lam u + eta],
+// P(k) = 1/(1 + dt lam A(k)).  Records are taken at the state the step starts from.
+int fts_engine::step_em(bool record, fts_observables* rec, bool si)
+{
+    const unsigned long long t = static_cast<unsigned long long>(step_counter);
+    const size_t Ms = static_cast<size_t>(M);
+    const double invM = 1.0 / static_cast<double>(M);
+    double2 Q[FTS_MAX_SPECIES];
+    int rc = compute_densities(omega.p, psi.p);
+    if (rc) return rc;
+    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
+    for (size_t p = 0; p < species.size(); ++p)
+        if (!finite2(Q[p])) return fail(FTS_NONFINITE, "non-finite Q at step " + std::to_string(step_counter + 1));
+    if (record && (rc = measure(omega.p, psi.p, rec))) return rc;
+    const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
+    if (electro()) {
+        hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, seed, t, 0x100u, sigma_psi, M, partials.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = reduce_finalize(1, SC_ETA_MEAN, invM))) return rc;
+    }
+    if ((rc = psi_drift(psi.p, rho_c.p, dpsi.p))) return rc;
+    if (n_omega > 0) {
+        hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, seed, t, M, si ? 0 : 1, omega.p, drift.p);
+        HIP_TRY(hipGetLastError());
+        if (si) {
+            for (int l = 0; l < n_omega; ++l)
+                if ((rc = fft_apply(f(drift, l), f(drift, l), KOp{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0}))) return rc;
+            const size_t n = static_cast<size_t>(n_omega) * Ms;
+            hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream, omega.p, drift.p, n);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    if (electro()) {
+        hipLaunchKernelGGL(k_psi_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, dpsi.p, -dt * lambda_psi, sigma_psi,
+                           scal.p + SC_ETA_MEAN, seed, t, M, dpsi.p);
+        HIP_TRY(hipGetLastError());
+        if (si && (rc = fft_apply(dpsi.p, dpsi.p, KOp{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0}))) return rc;
+        hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, Ms);
+        HIP_TRY(hipGetLastError());
+        if ((rc = remove_mean(psi.p))) return rc;
+    } else {
+        HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));
+    }
+    ++step_counter;
+    bool ok = true;
+    if ((rc = check_finite(ok))) return rc;
+    if (!ok) return fail(FTS_NONFINITE, "updated fields have non-finite values at step " + std::to_string(step_counter));
+    return FTS_OK;
+}
+
+template <typename S, typename D>
+__global__ void k_cast(const V2<S>* __restrict__ src, V2<D>* __restrict__ dst, double scale, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * EW_THREADS + threadIdx.x;
+    if (i < n) dst[i] = cmk<V2<D>>(static_cast<D>(scale * src[i].x), static_cast<D>(scale * src[i].y));
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* fts_last_error(void) { return g_error.c_str(); }
+
+int fts_create(const fts_system* s, fts_engine** out)
+{
+    if (!s || !out) return fail(FTS_INVALID, "null argument");
+    *out = nullptr;
+    if (s->m1 <= 0 || s->m2 <= 0 || s->m3 <= 0) return fail(FTS_INVALID, "mesh");
+    const long long msize = static_cast<long long>(s->m1) * s->m2 * s->m3;
+    if (msize > 2147483647LL) return fail(FTS_INVALID, "mesh size");
+    if (s->precision != FTS_PREC_FP64 && s->precision != FTS_PREC_FP32) return fail(FTS_INVALID, "precision");
+    if (!fft_supported(s->m1, s->m2, s->m3, s->precision == FTS_PREC_FP32))
+        return fail(FTS_INVALID, "mesh not supported by the engine FFT: need m2 == m3, lengths in {8,12,16,24,32,48,64,96,128}, "
+                                 "and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
+    if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
+    if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
+    if (s->n_species <= 0 || s->n_species > FTS_MAX_SPECIES || !s->species) return fail(FTS_INVALID, "polymer species_count");
+    if (s->scheme < 0 || s->scheme > 2) return fail(FTS_INVALID, "scheme");
+    fts_engine* e = new fts_engine();
+    auto bail = [&](int rc) {
+        delete e;
+        return rc;
+    };
+    e->m1 = s->m1;
+    e->m2 = s->m2;
+    e->m3 = s->m3;
+    e->M = static_cast<int>(msize);
+    e->L1 = s->dL * s->m1;
+    e->L2 = s->dL * s->m2;
+    e->L3 = s->dL * s->m3;
+    e->V = e->L1 * e->L2 * e->L3;
+    e->dV = s->dL * s->dL * s->dL;
+    e->a = s->a;
+    e->lB = s->lB;
+    e->dt = s->dt;
+    e->lambda_psi = s->lambda_psi;
+    e->noise = s->noise != 0;
+    e->scheme = s->scheme;
+    e->precision = s->precision;
+    e->seed = s->seed;
+    e->stress_mask = s->stress_mask;
+    int rc = e->build_modes(s);
+    if (rc) return bail(rc);
+    e->total_type_segments.assign(static_cast<size_t>(e->nT), 0);
+    for (int p = 0; p < s->n_species; ++p) {
+        const fts_species_desc& d = s->species[p];
+        if (d.N <= 0 || d.num_chains <= 0 || !d.types || !d.charges) return bail(fail(FTS_INVALID, "polymer species"));
+        HostSpecies sp;
+        sp.n = d.num_chains;
+        sp.N = d.N;
+        sp.type.assign(d.types, d.types + d.N);
+        sp.charge.assign(d.charges, d.charges + d.N);
+        sp.bond.assign(static_cast<size_t>(std::max(0, d.N - 1)), 1.0);
+        for (int b = 0; b + 1 < d.N; ++b) {
+            if (d.bonds) sp.bond[static_cast<size_t>(b)] = d.bonds[b];
+            if (!(sp.bond[static_cast<size_t>(b)] > 0.0)) return bail(fail(FTS_INVALID, "sequence bond length"));
+        }
+        for (int t : sp.type)
+            if (t < 0 || t >= e->nT) return bail(fail(FTS_INVALID, "sequence monomer type"));
+        // bond classes (distinct lengths)
+        for (double b : sp.bond) {
+            int cls = -1;
+            for (size_t k = 0; k < e->bond_cls_len.size(); ++k)
+                if (e->bond_cls_len[k] == b) cls = static_cast<int>(k);
+            if (cls < 0) {
+                cls = static_cast<int>(e->bond_cls_len.size());
+                e->bond_cls_len.push_back(b);
+            }
+            sp.bond_cls.push_back(cls);
+        }
+        // bead classes: distinct (type, charge seen by the field)
+        for (int s2 = 0; s2 < d.N; ++s2) {
+            const int t = sp.type[static_cast<size_t>(s2)];
+            const double z = s->lB > 0.0 ? sp.charge[static_cast<size_t>(s2)] : 0.0;
+            int cls = -1;
+            for (size_t k = 0; k < sp.cls_type.size(); ++k)
+                if (sp.cls_type[k] == t && sp.cls_z[k] == z) cls = static_cast<int>(k);
+            if (cls < 0) {
+                cls = static_cast<int>(sp.cls_type.size());
+                sp.cls_type.push_back(t);
+                sp.cls_z.push_back(z);
+            }
+            sp.cls_of_bead.push_back(cls);
+        }
+        const int nc = static_cast<int>(sp.cls_type.size());
+        if (nc > MAX_CLASSES) return bail(fail(FTS_INVALID, "too many distinct (type, charge) bead classes"));
+        sp.cls_start.assign(static_cast<size_t>(nc + 1), 0);
+        for (int c = 0; c < nc; ++c) {
+            for (int s2 = 0; s2 < d.N; ++s2)
+                if (sp.cls_of_bead[static_cast<size_t>(s2)] == c) sp.cls_beads.push_back(s2);
+            sp.cls_start[static_cast<size_t>(c + 1)] = static_cast<int>(sp.cls_beads.size());
+        }
+        e->max_cls = std::max(e->max_cls, nc);
+        e->Nmax = std::max(e->Nmax, d.N);
+        e->total_segments += sp.n * d.N;
+        for (int t : sp.type) e->total_type_segments[static_cast<size_t>(t)] += sp.n;
+        e->species.push_back(std::move(sp));
+    }
+    e->rho0 = static_cast<double>(e->total_segments) / e->V;
+    {
+        // short_range_self_pressure (param.cu:135-149)
+        long double wd = 0.0L;
+        for (int al = 0; al < e->nT; ++al) wd += static_cast<long double>(e->total_type_segments[static_cast<size_t>(al)]) * e->K[static_cast<size_t>(al) * e->nT + al];
+        const long double md = wd / static_cast<long double>(e->total_segments);
+        e->P_sr = static_cast<double>(md / (16.0L * static_cast<long double>(Pi) * std::sqrt(static_cast<long double>(Pi)) * e->a * e->a * e->a));
+    }
+    e->lambda_omega.assign(static_cast<size_t>(e->n_omega), 0.0);
+    if (s->n_lambda_omega == 1 && s->lambda_omega) e->lambda_omega.assign(static_cast<size_t>(e->n_omega), s->lambda_omega[0]);
+    else if (s->n_lambda_omega == e->n_omega && s->lambda_omega) e->lambda_omega.assign(s->lambda_omega, s->lambda_omega + e->n_omega);
+    else if (s->n_lambda_omega != 0) return bail(fail(FTS_INVALID, "lambda_omega length must be 1 or the number of active interaction modes"));
+    for (double x : e->lambda_omega)
+        if (!std::isfinite(x) || x < 0.0) return bail(fail(FTS_INVALID, "lambda_omega"));
+    if ((rc = e->build_ions(s))) return bail(rc);
+    if ((rc = e->setup_device())) return bail(rc);
+    if (e->scheme == FTS_SCHEME_SEMI_IMPLICIT && (rc = e->build_si_tables())) return bail(rc);
+    *out = e;
+    return FTS_OK;
+}
+
+void fts_destroy(fts_engine* h) { delete h; }
+
+int fts_get_info(fts_engine* h, fts_info* o)
+{
+    if (!h || !o) return fail(FTS_INVALID, "null argument");
+    std::memset(o, 0, sizeof(*o));
+    o->grid_size = h->M;
+    o->step = h->step_counter;
+    o->n_omega = h->n_omega;
+    o->monomer_types = h->nT;
+    o->n_species = static_cast<int>(h->species.size());
+    o->n_ions = static_cast<int>(h->ions.size());
+    o->n_segment_classes = h->max_cls;
+    o->precision = h->precision;
+    o->scheme = h->scheme;
+    o->volume = h->V;
+    o->rho0 = h->rho0;
+    o->short_range_self_pressure = h->P_sr;
+    for (int l = 0; l < h->n_omega; ++l) {
+        o->mode_eigenvalue[l] = h->eig[static_cast<size_t>(h->active[static_cast<size_t>(l)])];
+        o->omega_coefficient[l] = h->omega_coefficient(l);
+        o->gamma_is_imag[l] = h->gamma_imag(l) ? 1.0 : 0.0;
+        for (int al = 0; al < h->nT; ++al) o->mode_O[l][al] = h->Ocoef(l, al);
+    }
+    for (size_t j = 0; j < h->ions.size(); ++j) {
+        o->ion_valence[j] = h->ions[j].z;
+        o->ion_count[j] = h->ions[j].n;
+        o->ion_smearing[j] = h->ions[j].a;
+    }
+    return FTS_OK;
+}
+
+int fts_set_fields(fts_engine* h, const double* om, const double* ps)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    const size_t Ms = static_cast<size_t>(h->M);
+    const size_t no = static_cast<size_t>(h->n_omega) * Ms;
+    if (om) HIP_TRY(hipMemcpyAsync(h->omega.p, om, sizeof(double2) * no, hipMemcpyHostToDevice, h->stream));
+    else HIP_TRY(hipMemsetAsync(h->omega.p, 0, sizeof(double2) * no, h->stream));
+    if (ps) HIP_TRY(hipMemcpyAsync(h->psi.p, ps, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
+    else HIP_TRY(hipMemsetAsync(h->psi.p, 0, sizeof(double2) * Ms, h->stream));
+    // remove_psi_zero_mode on load (complexLangevin.cu:183, 313-317)
+    int rc = h->electro() ? h->remove_mean(h->psi.p) : FTS_OK;
+    if (rc) return rc;
+    if (!h->electro()) HIP_TRY(hipMemsetAsync(h->psi.p, 0, sizeof(double2) * Ms, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FTS_OK;
+}
+
+int fts_get_field(fts_engine* h, int id, double* out, size_t n_doubles)
+{
+    if (!h || !out) return fail(FTS_INVALID, "null argument");
+    const size_t Ms = static_cast<size_t>(h->M);
+    const double2* src = nullptr;
+    size_t count = 0;
+    switch (id) {
+    case FTS_FIELD_OMEGA: src = h->omega.p; count = static_cast<size_t>(h->n_omega) * Ms; break;
+    case FTS_FIELD_PSI: src = h->psi.p; count = Ms; break;
+    case FTS_FIELD_RHO_TYPE: src = h->rho_type.p; count = static_cast<size_t>(h->nT) * Ms; break;
+    case FTS_FIELD_RHO_P: src = h->rho_p.p; count = Ms; break;
+    case FTS_FIELD_RHO_C: src = h->rho_c.p; count = Ms; break;
+    case FTS_FIELD_RHO_C_POLY: src = h->rho_c_poly.p; count = Ms; break;
+    case FTS_FIELD_RHO_C_ION: src = h->rho_c_ion.p; count = Ms; break;
+    case FTS_FIELD_ION_RHO: src = h->ion_rho.p; count = h->ions.size() * Ms; break;
+    case FTS_FIELD_QF:
+    case FTS_FIELD_QB: {
+        const HostSpecies& sp = h->species[static_cast<size_t>(h->last_species)];
+        count = static_cast<size_t>(sp.N) * Ms;
+        if (n_doubles != 2 * count) return fail(FTS_INVALID, "field size mismatch");
+        const void* q = id == FTS_FIELD_QF ? h->qF : h->qB;
+        // undo the FP32 potential shifts: qF_s = q~F_s e^{-sum_{t<=s} shift}, qB_s = q~B_s e^{-sum_{t>=s} shift}
+        for (int s = 0; s < sp.N; ++s) {
+            double shift = 0.0;
+            if (h->precision == FTS_PREC_FP32) {
+                for (int t = 0; t < sp.N; ++t)
+                    if ((id == FTS_FIELD_QF && t <= s) || (id == FTS_FIELD_QB && t >= s))
+                        shift += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(t)])];
+            }
+            double2* dst = h->t1.p;
+            if (h->precision == FTS_PREC_FP32)
+                hipLaunchKernelGGL((k_cast<float, double>), dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, h->stream,
+                                   static_cast<const float2*>(q) + static_cast<size_t>(s) * Ms, dst, std::exp(-shift), Ms);
+            else
+                hipLaunchKernelGGL((k_cast<double, double>), dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, h->stream,
+                                   static_cast<const double2*>(q) + static_cast<size_t>(s) * Ms, dst, 1.0, Ms);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(out + 2 * static_cast<size_t>(s) * Ms, dst, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        return FTS_OK;
+    }
+    default: return fail(FTS_INVALID, "field id");
+    }
+    if (n_doubles != 2 * count) return fail(FTS_INVALID, "field size mismatch");
+    HIP_TRY(hipMemcpyAsync(out, src, sizeof(double) * n_doubles, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FTS_OK;
+}
+
+int fts_compute_densities(fts_engine* h, double* Q_out)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    int rc = h->compute_densities(h->omega.p, h->psi.p);
+    if (rc) return rc;
+    double2 Q[FTS_MAX_SPECIES];
+    if ((rc = h->read_scal(SC_Q, static_cast<int>(h->species.size()), Q))) return rc;
+    bool ok = true;
+    for (size_t p = 0; p < h->species.size(); ++p) {
+        const double sc = std::exp(-h->q_shift_total[p]);
+        if (Q_out) {
+            Q_out[2 * p] = Q[p].x * sc;
+            Q_out[2 * p + 1] = Q[p].y * sc;
+        }
+        ok = ok && finite2(Q[p]);
+    }
+    return ok ? FTS_OK : fail(FTS_NONFINITE, "non-finite Q");
+}
+
+int fts_measure(fts_engine* h, fts_observables* out)
+{
+    if (!h || !out) return fail(FTS_INVALID, "null argument");
+    int rc = h->compute_densities(h->omega.p, h->psi.p);
+    if (rc) return rc;
+    if ((rc = h->measure(h->omega.p, h->psi.p, out))) return rc;
+    for (int p = 0; p < out->n_species; ++p)
+        if (!std::isfinite(out->Q_poly[p][0]) || !std::isfinite(out->Q_poly[p][1])) return fail(FTS_NONFINITE, "non-finite Q");
+    return FTS_OK;
+}
+
+int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, int capacity, int* n_records)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    int nrec = 0;
+    if (n_records) *n_records = 0;
+    int rc = FTS_OK;
+    for (int k = 0; k < n; ++k) {
+        const long long step = h->step_counter + 1; // 1-based step label (complexLangevin.cu:206)
+        const bool record = record_every > 0 && step % record_every == 0 && records && nrec < capacity;
+        fts_observables* rec = record ? &records[nrec] : nullptr;
+        rc = h->scheme == FTS_SCHEME_PC_REFERENCE ? h->step_pc(record, rec) : h->step_em(record, rec, h->scheme == FTS_SCHEME_SEMI_IMPLICIT);
+        if (record) {
+            rec->step = step;
+            ++nrec;
+        }
+        if (n_records) *n_records = nrec;
+        if (rc != FTS_OK) break;
+        if (h->timing && h->pending.size() > 256 && (rc = h->resolve_timing())) break;
+    }
+    return rc;
+}
+
+int fts_set_step(fts_engine* h, long long step)
+{
+    if (!h || step < 0) return fail(FTS_INVALID, "bad step");
+    h->step_counter = step;
+    return FTS_OK;
+}
+
+int fts_fft3d(fts_engine* h, const double* in, double* out, int sign)
+{
+    if (!h || !in || !out || (sign != 1 && sign != -1)) return fail(FTS_INVALID, "bad argument");
+    const size_t Ms = static_cast<size_t>(h->M);
+    HIP_TRY(hipMemcpyAsync(h->t1.p, in, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
+    PlaneJob<double> pj{h->t1.p, h->t2.p, nullptr, nullptr};
+    HIP_TRY(launch_plane<double>(h->g64, sign < 0 ? PLANE_FWD : PLANE_INV, &pj, 1, h->stream));
+    PencilJob<double> xj{h->t2.p, KOp{}};
+    HIP_TRY(launch_pencil<double>(h->g64, sign < 0 ? PENCIL_FWD : PENCIL_INV, &xj, 1, h->stream));
+    HIP_TRY(hipMemcpyAsync(out, h->t2.p, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return FTS_OK;
+}
+
+int fts_bead_step(fts_engine* h, const double* q, const double* emw, double bond, double* q_out)
+{
+    if (!h || !q || !emw || !q_out || !(bond > 0.0)) return fail(FTS_INVALID, "bad argument");
+    const size_t Ms = static_cast<size_t>(h->M);
+    std::vector<double> sep;
+    for (int x = 0; x < h->m1; ++x) sep.push_back(std::exp(-bond * bond * std::pow(wave_number(x, h->m1, h->L1), 2) / 6.0));
+    for (int y = 0; y < h->m2; ++y) sep.push_back(std::exp(-bond * bond * std::pow(wave_number(y, h->m2, h->L2), 2) / 6.0));
+    for (int z = 0; z < h->m3; ++z) sep.push_back(std::exp(-bond * bond * std::pow(wave_number(z, h->m3, h->L3), 2) / 6.0));
+    DevBuf<double> dsep;
+    HIP_TRY(dsep.alloc(sep.size()));
+    HIP_TRY(hipMemcpyAsync(dsep.p, sep.data(), sizeof(double) * sep.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->t1.p, q, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->t2.p, emw, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
+    const KOp op = h->kop_sep(dsep.p, 1.0 / static_cast<double>(h->M));
+    const int nb = ew_blocks(Ms);
+    hipError_t err;
+    if (h->precision == FTS_PREC_FP32) {
+        float2* a = static_cast<float2*>(h->tmpF);
+        float2* e = static_cast<float2*>(h->E);
+        float2* o = static_cast<float2*>(h->qF);
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, h->stream, h->t1.p, a, 1.0, Ms);
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, h->stream, h->t2.p, e, 1.0, Ms);
+        PlaneJob<float> j1{a, a, nullptr, nullptr};
+        if ((err = launch_plane<float>(h->g32, PLANE_FWD, &j1, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+        PencilJob<float> j2{a, op};
+        if ((err = launch_pencil<float>(h->g32, PENCIL_FWD_MUL_INV, &j2, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+        PlaneJob<float> j3{a, nullptr, o, e};
+        if ((err = launch_plane<float>(h->g32, PLANE_LAST, &j3, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+        hipLaunchKernelGGL((k_cast<float, double>), dim3(nb), dim3(EW_THREADS), 0, h->stream, o, h->t3.p, 1.0, Ms);
+    } else {
+        PlaneJob<double> j1{h->t1.p, h->t1.p, nullptr, nullptr};
+        if ((err = launch_plane<double>(h->g64, PLANE_FWD, &j1, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+        PencilJob<double> j2{h->t1.p, op};
+        if ((err = launch_pencil<double>(h->g64, PENCIL_FWD_MUL_INV, &j2, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+        PlaneJob<double> j3{h->t1.p, nullptr, h->t3.p, h->t2.p};
+        if ((err = launch_plane<double>(h->g64, PLANE_LAST, &j3, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(q_out, h->t3.p, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    dsep.release();
+    return FTS_OK;
+}
+
+int fts_set_timing(fts_engine* h, int enabled)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    int rc = h->resolve_timing();
+    if (rc) return rc;
+    h->timing = enabled != 0;
+    h->tstat = fts_timing{};
+    return FTS_OK;
+}
+
+int fts_get_timing(fts_engine* h, fts_timing* out)
+{
+    if (!h || !out) return fail(FTS_INVALID, "null argument");
+    int rc = h->resolve_timing();
+    if (rc) return rc;
+    *out = h->tstat;
+    return FTS_OK;
+}
+
+} // extern "C"

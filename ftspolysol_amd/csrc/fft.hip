@@ -1,0 +1,231 @@
+// fft.hip -- plane and pencil FFT passes for gfx950 (see fft.h).
+//
+// Both passes stage their tile in LDS and run the Stockham line FFT of
+// fts_device.h on it, so a pass costs one HBM read and one HBM write of the
+// field.  The plane pass holds a whole m2 x m3 plane (96 x 97 complex doubles
+// = 149 KB of gfx950's 160 KB LDS at 96^3 FP64); the pencil pass holds m1 rows
+// of 16 consecutive points of the flattened (y, z) plane, so each of its row
+// reads is one contiguous 256 B (FP64) / 128 B (FP32) segment.
+#include "fft.h"
+
+namespace fts {
+namespace {
+
+constexpr int PENCIL_ZC = 16;
+
+template <int N> constexpr int plane_threads()
+{
+    return (N * N / plan_e(N)) > 1024 ? 1024 : (N * N / plan_e(N));
+}
+template <int N1> constexpr int pencil_threads() { return PENCIL_ZC * (N1 / plan_e(N1)); }
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j0, PlaneJob<T> j1, const V2<T>* __restrict__ twg)
+{
+    using V = V2<T>;
+    constexpr int NT = plane_threads<N>();
+    constexpr int LS = N + 1;
+    __shared__ V lds[N * LS];
+    __shared__ V tw[N];
+    const int tid = threadIdx.x;
+    const PlaneJob<T> job = blockIdx.y == 0 ? j0 : j1;
+    const size_t plane = static_cast<size_t>(blockIdx.x) * (N * N);
+    for (int t = tid; t < N; t += NT) tw[t] = twg[t];
+
+    const V* src = (MODE == PLANE_INIT) ? job.emw : job.src;
+#pragma unroll 4
+    for (int i = tid; i < N * N; i += NT) {
+        const V v = src[plane + i];
+        lds[(i / N) * LS + (i % N)] = v;
+        if (MODE == PLANE_INIT) job.qout[plane + i] = v;
+    }
+    __syncthreads();
+
+    if constexpr (MODE == PLANE_INV || MODE == PLANE_STEP || MODE == PLANE_LAST) {
+        lds_fft<T, N, +1, N, LS, 1, false, NT>(lds, tw, tid); // along z
+        lds_fft<T, N, +1, N, 1, LS, true, NT>(lds, tw, tid);  // along y
+    }
+    if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST) {
+#pragma unroll 4
+        for (int i = tid; i < N * N; i += NT) {
+            V* p = &lds[(i / N) * LS + (i % N)];
+            const V v = cmul(*p, job.emw[plane + i]);
+            *p = v;
+            job.qout[plane + i] = v;
+        }
+        __syncthreads();
+    }
+    if constexpr (MODE == PLANE_FWD || MODE == PLANE_INIT || MODE == PLANE_STEP) {
+        lds_fft<T, N, -1, N, LS, 1, false, NT>(lds, tw, tid);
+        lds_fft<T, N, -1, N, 1, LS, true, NT>(lds, tw, tid);
+    }
+    if constexpr (MODE != PLANE_LAST) {
+#pragma unroll 4
+        for (int i = tid; i < N * N; i += NT) job.dst[plane + i] = lds[(i / N) * LS + (i % N)];
+    }
+}
+
+__device__ __forceinline__ double kop_value(const KOp& op, const double* kvec, int m1, int m2, int x, int y, int z, size_t idx)
+{
+    double v = 1.0;
+    if (op.sep) v = op.sep[x] * op.sep[m1 + y] * op.sep[m1 + m2 + z];
+    if (op.table) v *= op.table[idx];
+    if (op.c1 != 0.0 || op.c2 != 0.0) {
+        const double k[3] = {kvec[x], kvec[m1 + y], kvec[m1 + m2 + z]};
+        double poly = op.c0;
+        if (op.c1 != 0.0) poly += op.c1 * (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+        if (op.c2 != 0.0) poly += op.c2 * (k[op.ka] * k[op.kb]);
+        v *= poly;
+    } else {
+        v *= op.c0;
+    }
+    return v;
+}
+
+template <typename T, int N1, int MODE>
+__global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<T> j0, PencilJob<T> j1, FftGeom g)
+{
+    using V = V2<T>;
+    constexpr int NT = pencil_threads<N1>();
+    constexpr int ZC = PENCIL_ZC;
+    constexpr int LS = ZC + 1;
+    __shared__ V lds[N1 * LS];
+    __shared__ V tw[N1];
+    const int tid = threadIdx.x;
+    const PencilJob<T> job = blockIdx.y == 0 ? j0 : j1;
+    const size_t sx = static_cast<size_t>(g.m2) * g.m3;
+    const size_t jbase = static_cast<size_t>(blockIdx.x) * ZC;
+    const V* twg = static_cast<const V*>(g.tw_x);
+    for (int t = tid; t < N1; t += NT) tw[t] = twg[t];
+    V* data = job.data;
+#pragma unroll 4
+    for (int i = tid; i < N1 * ZC; i += NT) {
+        const int x = i / ZC, c = i % ZC;
+        lds[x * LS + c] = data[x * sx + jbase + c];
+    }
+    __syncthreads();
+    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_MUL_INV) lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
+    if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL) {
+        const KOp op = job.op;
+#pragma unroll 4
+        for (int i = tid; i < N1 * ZC; i += NT) {
+            const int x = i / ZC, c = i % ZC;
+            const size_t j = jbase + c;
+            const int y = static_cast<int>(j / g.m3), z = static_cast<int>(j % g.m3);
+            const double kv = kop_value(op, g.kvec, g.m1, g.m2, x, y, z, x * sx + j);
+            lds[x * LS + c] = cscale(lds[x * LS + c], static_cast<T>(kv));
+        }
+        __syncthreads();
+    }
+    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV) lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
+#pragma unroll 4
+    for (int i = tid; i < N1 * ZC; i += NT) {
+        const int x = i / ZC, c = i % ZC;
+        data[x * sx + jbase + c] = lds[x * LS + c];
+    }
+}
+
+template <typename T, int N, int MODE>
+hipError_t plane_go(const FftGeom& g, const PlaneJob<T>* jobs, int njobs, hipStream_t s)
+{
+    dim3 grid(g.m1, njobs);
+    hipLaunchKernelGGL((plane_kernel<T, N, MODE>), grid, dim3(plane_threads<N>()), 0, s, jobs[0], jobs[njobs > 1 ? 1 : 0],
+                       static_cast<const V2<T>*>(g.tw_yz));
+    return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t plane_mode(const FftGeom& g, int mode, const PlaneJob<T>* jobs, int njobs, hipStream_t s)
+{
+    switch (mode) {
+    case PLANE_FWD: return plane_go<T, N, PLANE_FWD>(g, jobs, njobs, s);
+    case PLANE_INV: return plane_go<T, N, PLANE_INV>(g, jobs, njobs, s);
+    case PLANE_INIT: return plane_go<T, N, PLANE_INIT>(g, jobs, njobs, s);
+    case PLANE_STEP: return plane_go<T, N, PLANE_STEP>(g, jobs, njobs, s);
+    case PLANE_LAST: return plane_go<T, N, PLANE_LAST>(g, jobs, njobs, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, int N1, int MODE>
+hipError_t pencil_go(const FftGeom& g, const PencilJob<T>* jobs, int njobs, hipStream_t s)
+{
+    const size_t tiles = static_cast<size_t>(g.m2) * g.m3 / PENCIL_ZC;
+    dim3 grid(static_cast<unsigned>(tiles), njobs);
+    hipLaunchKernelGGL((pencil_kernel<T, N1, MODE>), grid, dim3(pencil_threads<N1>()), 0, s, jobs[0], jobs[njobs > 1 ? 1 : 0], g);
+    return hipGetLastError();
+}
+
+template <typename T, int N1>
+hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, int njobs, hipStream_t s)
+{
+    switch (mode) {
+    case PENCIL_FWD_MUL_INV: return pencil_go<T, N1, PENCIL_FWD_MUL_INV>(g, jobs, njobs, s);
+    case PENCIL_FWD: return pencil_go<T, N1, PENCIL_FWD>(g, jobs, njobs, s);
+    case PENCIL_INV: return pencil_go<T, N1, PENCIL_INV>(g, jobs, njobs, s);
+    case PENCIL_MUL: return pencil_go<T, N1, PENCIL_MUL>(g, jobs, njobs, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+constexpr int kSizes[] = {8, 12, 16, 24, 32, 48, 64, 96, 128};
+
+bool in_sizes(int n)
+{
+    for (int s : kSizes)
+        if (s == n) return true;
+    return false;
+}
+
+} // namespace
+
+bool fft_supported(int m1, int m2, int m3, bool fp32)
+{
+    if (!in_sizes(m1) || !in_sizes(m2) || m2 != m3) return false;
+    const size_t elem = fp32 ? 8 : 16;
+    const size_t lds = static_cast<size_t>(m2) * (m2 + 1) * elem + m2 * elem;
+    return lds <= 160 * 1024;
+}
+
+template <typename T>
+hipError_t launch_plane(const FftGeom& g, int mode, const PlaneJob<T>* jobs, int njobs, hipStream_t s)
+{
+    switch (g.m2) {
+    case 8: return plane_mode<T, 8>(g, mode, jobs, njobs, s);
+    case 12: return plane_mode<T, 12>(g, mode, jobs, njobs, s);
+    case 16: return plane_mode<T, 16>(g, mode, jobs, njobs, s);
+    case 24: return plane_mode<T, 24>(g, mode, jobs, njobs, s);
+    case 32: return plane_mode<T, 32>(g, mode, jobs, njobs, s);
+    case 48: return plane_mode<T, 48>(g, mode, jobs, njobs, s);
+    case 64: return plane_mode<T, 64>(g, mode, jobs, njobs, s);
+    case 96: return plane_mode<T, 96>(g, mode, jobs, njobs, s);
+    case 128:
+        if constexpr (sizeof(T) == 4) return plane_mode<T, 128>(g, mode, jobs, njobs, s);
+        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_pencil(const FftGeom& g, int mode, const PencilJob<T>* jobs, int njobs, hipStream_t s)
+{
+    switch (g.m1) {
+    case 8: return pencil_mode<T, 8>(g, mode, jobs, njobs, s);
+    case 12: return pencil_mode<T, 12>(g, mode, jobs, njobs, s);
+    case 16: return pencil_mode<T, 16>(g, mode, jobs, njobs, s);
+    case 24: return pencil_mode<T, 24>(g, mode, jobs, njobs, s);
+    case 32: return pencil_mode<T, 32>(g, mode, jobs, njobs, s);
+    case 48: return pencil_mode<T, 48>(g, mode, jobs, njobs, s);
+    case 64: return pencil_mode<T, 64>(g, mode, jobs, njobs, s);
+    case 96: return pencil_mode<T, 96>(g, mode, jobs, njobs, s);
+    case 128: return pencil_mode<T, 128>(g, mode, jobs, njobs, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_plane<double>(const FftGeom&, int, const PlaneJob<double>*, int, hipStream_t);
+template hipError_t launch_plane<float>(const FftGeom&, int, const PlaneJob<float>*, int, hipStream_t);
+template hipError_t launch_pencil<double>(const FftGeom&, int, const PencilJob<double>*, int, hipStream_t);
+template hipError_t launch_pencil<float>(const FftGeom&, int, const PencilJob<float>*, int, hipStream_t);
+
+} // namespace fts

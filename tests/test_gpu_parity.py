@@ -1,0 +1,173 @@
+"""GPU parity: the HIP engine (libfts_hip.so, through the C ABI) against the FP64
+CPU oracle on the same inputs and the same Philox noise.
+
+Tolerances: FP64 engine vs oracle 1e-10 relative for Q, H, mu, densities and the
+stress tensor (the FP64 parity bar); FFTs and a single bead step 1e-13; the FP32
+engine 1e-4 relative to FP64 for H, mu and stress, 2e-5 for a bead step.
+"""
+import numpy as np
+import pytest
+
+from ftspolysol_amd.api import (FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
+                                FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
+                                SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
+from ftspolysol_amd.system import System, initial_fields, make_system
+from oracle_lib import oracle_engine
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def bare_system(m, **kw):
+    return System(m=m, dL=0.5, species=[dict(num_chains=1, types=[0, 0], charges=[0.0, 0.0])], xi=0.1, **kw)
+
+
+@pytest.mark.parametrize("m", [(8, 8, 8), (16, 12, 12), (24, 16, 16), (32, 32, 32), (48, 24, 24), (96, 96, 96), (128, 64, 64)])
+def test_fft_matches_numpy(m):
+    print(f"fft {m}", flush=True)
+    e = hip_engine(bare_system(m))
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    assert rel(e.fft3d(x, -1), np.fft.fftn(x)) < 1e-13
+    assert rel(e.fft3d(x, +1), np.fft.ifftn(x) * x.size) < 1e-13
+
+
+@pytest.mark.parametrize("m,bond", [((32, 32, 32), 1.3), ((96, 96, 96), 1.0), ((16, 24, 24), 0.7)])
+def test_bead_step_matches_oracle(m, bond):
+    print(f"bead step {m} b={bond}", flush=True)
+    s = bare_system(m)
+    g, o = hip_engine(s), oracle_engine(s)
+    rng = np.random.default_rng(5)
+    q = 1 + 0.1 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    w = 0.2 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    emw = np.exp(-w)
+    ref = o.bead_step(q, emw, bond)
+    assert rel(g.bead_step(q, emw, bond), ref) < 1e-13
+    s32 = bare_system(m, precision=PREC_FP32)
+    assert rel(hip_engine(s32).bead_step(q, emw, bond), ref) < 2e-5
+
+
+def c1_like(n=32, N=20, **kw):
+    return make_system(n, N, rho=0.2, salt=0.05, B=0.5, lB=1.0, dt=0.01, seed=1234, **kw)
+
+
+def copolymer_system(**kw):
+    """Two species, two monomer types with chi, non-uniform charges and bonds."""
+    N1, N2 = 12, 8
+    sp1 = dict(num_chains=6, types=[0] * 6 + [1] * 6, charges=[-1.0, 0.0] * 6, bonds=np.linspace(0.8, 1.2, N1 - 1))
+    sp2 = dict(num_chains=9, types=[1, 0] * 4, charges=[0.5] * N2)
+    rho0 = (6 * N1 + 9 * N2) / (8.0 ** 3)
+    return System(m=(16, 16, 16), dL=0.5, a=1.0, species=[sp1, sp2], monomer_types=2, xi=0.5 * rho0,
+                  chi=[[0.0, 0.3], [0.3, 0.0]], lB=0.8, counterion=dict(auto=True, valence=1.0),
+                  salt=dict(num_salt=10, smearing_length=0.7), dt=0.005, lambda_omega=(1.0,), seed=99, **kw)
+
+
+def set_random_fields(engines, n_omega, m, sigma, seed):
+    rng = np.random.default_rng(seed)
+    om = sigma * (rng.standard_normal((n_omega,) + m) + 1j * rng.standard_normal((n_omega,) + m))
+    ps = sigma * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    for e in engines:
+        e.set_fields(om, ps)
+
+
+FIELDS = [FIELD_OMEGA, FIELD_PSI, FIELD_RHO_TYPE, FIELD_RHO_P, FIELD_RHO_C, FIELD_RHO_C_POLY, FIELD_RHO_C_ION, FIELD_ION_RHO]
+
+
+@pytest.mark.parametrize("which", ["c1", "copolymer"])
+def test_densities_match_oracle(which):
+    print(f"densities {which}", flush=True)
+    s = c1_like() if which == "c1" else copolymer_system()
+    g, o = hip_engine(s), oracle_engine(s)
+    set_random_fields([g, o], g.info.n_omega, s.m, 0.1, 17)
+    Qg, Qo = g.compute_densities(), o.compute_densities()
+    assert rel(Qg, Qo) < 1e-11
+    for fid in FIELDS:
+        assert rel(g.get_field(fid), o.get_field(fid)) < 1e-10, fid
+    if which == "c1":
+        assert rel(g.get_field(FIELD_QF, 20), o.get_field(FIELD_QF, 20)) < 1e-11
+        assert rel(g.get_field(FIELD_QB, 20), o.get_field(FIELD_QB, 20)) < 1e-11
+
+
+def obs_rel(a, b):
+    """Relative differences of the observables.  mu_ex = -ln Q is compared with the
+    error scale of Q itself: an absolute error dQ/Q in mu is all a last-bit error
+    in Q can cause, and mu near 0 (Q near 1, the ions) makes a relative measure of
+    mu meaningless, so mu uses |d mu| / max(|mu|, 1)."""
+    out = {}
+    for key in ("H", "p_iso"):
+        out[key] = abs(a[key] - b[key]) / abs(b[key])
+    for key in ("stress", "Q_poly", "Q_ion"):
+        out[key] = rel(a[key], b[key]) if len(b[key]) else 0.0
+    for key in ("mu_ex_poly", "mu_ex_ion"):
+        if len(b[key]):
+            out[key] = float(np.abs(a[key] - b[key]).max() / max(np.abs(b[key]).max(), 1.0))
+    return out
+
+
+@pytest.mark.parametrize("which", ["c1", "copolymer"])
+def test_measure_matches_oracle(which):
+    print(f"measure {which}", flush=True)
+    s = c1_like(stress_mask=0x7F) if which == "c1" else copolymer_system(stress_mask=0x7F)
+    g, o = hip_engine(s), oracle_engine(s)
+    set_random_fields([g, o], g.info.n_omega, s.m, 0.1, 23)
+    r = obs_rel(g.measure(), o.measure())
+    assert max(r.values()) < 1e-10, r
+
+
+@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT])
+def test_trajectory_matches_oracle(scheme):
+    print(f"trajectory scheme={scheme}", flush=True)
+    s = c1_like(stress_mask=0x7F, scheme=scheme)
+    g, o = hip_engine(s), oracle_engine(s)
+    om, ps = initial_fields(s, sigma=1e-3, cfg=1)
+    for e in (g, o):
+        e.set_fields(om, ps)
+    rg, ro = g.step(6, record_every=2), o.step(6, record_every=2)
+    assert len(rg) == len(ro) == 3
+    for a, b in zip(rg, ro):
+        assert a["step"] == b["step"]
+        r = obs_rel(a, b)
+        assert max(r.values()) < 1e-10, r
+    assert rel(g.get_field(FIELD_OMEGA), o.get_field(FIELD_OMEGA)) < 1e-10
+    assert rel(g.get_field(FIELD_PSI), o.get_field(FIELD_PSI)) < 1e-10
+
+
+def test_copolymer_pc_trajectory():
+    print("copolymer pc", flush=True)
+    s = copolymer_system(stress_mask=0x7F)
+    g, o = hip_engine(s), oracle_engine(s)
+    set_random_fields([g, o], g.info.n_omega, s.m, 0.05, 31)
+    rg, ro = g.step(3, record_every=1), o.step(3, record_every=1)
+    for a, b in zip(rg, ro):
+        r = obs_rel(a, b)
+        assert max(r.values()) < 1e-10, r
+
+
+def test_fp32_engine_tracks_fp64_at_96():
+    """FP32 propagators (log-renormalised) vs the FP64 engine at the headline size."""
+    print("fp32 vs fp64 96^3 N=100", flush=True)
+    s64 = make_system(96, 100, stress_mask=0x7F, seed=5)
+    s32 = make_system(96, 100, stress_mask=0x7F, seed=5, precision=PREC_FP32)
+    g64, g32 = hip_engine(s64), hip_engine(s32)
+    set_random_fields([g64, g32], 1, s64.m, 0.05, 41)
+    a, b = g32.measure(), g64.measure()
+    r = obs_rel(a, b)
+    assert max(r.values()) < 1e-4, r
+    # density means are exact: <rho_p> = n N / V and <rho_c> = 0
+    rho_p = g64.get_field(FIELD_RHO_P)
+    nN_V = s64.desc.species[0].num_chains * 100 / g64.info.volume
+    assert abs(rho_p.mean() - nN_V) < 1e-12 * nN_V
+    assert abs(g64.get_field(FIELD_RHO_C).mean()) < 1e-12
+
+
+def test_headline_measure_matches_oracle_64():
+    print("measure 64^3 N=50 vs oracle", flush=True)
+    s = make_system(64, 50, salt=0.1, stress_mask=0x7F, seed=8)
+    g, o = hip_engine(s), oracle_engine(s)
+    set_random_fields([g, o], 1, s.m, 0.05, 43)
+    r = obs_rel(g.measure(), o.measure())
+    assert max(r.values()) < 1e-10, r
