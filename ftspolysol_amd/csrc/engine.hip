@@ -105,6 +105,7 @@ enum {
     SC_TMP = 19,
     SC_CONTRACT = 24, // [7]
     SC_H = 32,
+    SC_LF = 33,       // [8] per species: log-gain LF(N-1) of the renormalised forward sweep
     SC_SHIFT = 40,    // [MAX_CLASSES]
     SC_COUNT = 40 + MAX_CLASSES
 };
@@ -155,6 +156,8 @@ struct fts_engine {
     void* kA = nullptr;
     void* kB = nullptr;
     std::vector<double> q_shift_total; // per species: sum of class shifts over the chain
+    DevBuf<double> norms, logc, rfac, rfac2; // FP32 per-bead renormalisation (see k_scale_prefix)
+    std::vector<double> q_log_gain;          // per species: LF(N-1) of the last solve (host copy)
 
     // timing
     bool timing = false;
@@ -187,6 +190,7 @@ struct fts_engine {
     int sum_field(const double2* a, int slot, double scale);
     int remove_mean(double2* a);
     int read_scal(int slot, int count, double2* out);
+    int read_log_gain();
 
     // hot path
     template <typename T> int solve_species(int p, bool accumulate = true);
@@ -214,7 +218,7 @@ void fts_engine::release()
     pending.clear();
     for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
     event_pool.clear();
-    for (DevBuf<double>* b : {&kvec, &sep_gamma, &sep_bond, &sep_ion, &si_table_omega, &si_table_psi}) b->release();
+    for (DevBuf<double>* b : {&kvec, &sep_gamma, &sep_bond, &sep_ion, &si_table_omega, &si_table_psi, &norms, &logc, &rfac, &rfac2}) b->release();
     tw64_x.release();
     tw64_yz.release();
     tw32_x.release();
@@ -560,6 +564,12 @@ int fts_engine::setup_device()
         sp.cls_shift.assign(static_cast<size_t>(nc), 0.0);
     }
     q_shift_total.assign(species.size(), 0.0);
+    q_log_gain.assign(species.size(), 0.0);
+    HIP_TRY(norms.alloc(4 * static_cast<size_t>(m1)));
+    HIP_TRY(logc.alloc(2 * static_cast<size_t>(Nmax)));
+    HIP_TRY(rfac.alloc(static_cast<size_t>(Nmax)));
+    HIP_TRY(rfac2.alloc(static_cast<size_t>(Nmax)));
+    HIP_TRY(hipMemset(logc.p, 0, sizeof(double) * logc.n));
     // mode coefficients for the update kernels
     mc.nO = n_omega;
     mc.nT = nT;
@@ -693,6 +703,16 @@ int fts_engine::read_scal(int slot, int count, double2* out)
     return FTS_OK;
 }
 
+// host copy of LF(N-1) per species (FP32 renormalisation; zero in FP64)
+int fts_engine::read_log_gain()
+{
+    double2 v[FTS_MAX_SPECIES];
+    int rc = read_scal(SC_LF, static_cast<int>(species.size()), v);
+    if (rc) return rc;
+    for (size_t p = 0; p < species.size(); ++p) q_log_gain[p] = v[p].x;
+    return FTS_OK;
+}
+
 hipEvent_t fts_engine::get_event()
 {
     if (!event_pool.empty()) {
@@ -736,20 +756,20 @@ template <typename T> int fts_engine::solve_species(int p, bool accumulate)
     Cv* tf = static_cast<Cv*>(tmpF);
     Cv* tb = static_cast<Cv*>(tmpB);
     const int nc = static_cast<int>(sp.cls_type.size());
-    // FP32: shift each class's potential by Re<W_c> (log-renormalisation kept in FP64)
+    // FP32: shift each class's potential so that max_i exp(-Re W_c + shift_c) = exp(BOLTZ_CAP)
+    // (shift_c = min Re W_c + BOLTZ_CAP, kept in FP64)
     if (sizeof(T) == 4) {
         for (int c = 0; c < nc; ++c) {
-            hipLaunchKernelGGL(k_sum_class_w, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, f(W_type_s, sp.cls_type[static_cast<size_t>(c)]),
+            hipLaunchKernelGGL(k_min_class_w, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, f(W_type_s, sp.cls_type[static_cast<size_t>(c)]),
                                psi_s.p, electro() ? sp.cls_z[static_cast<size_t>(c)] : 0.0, M, partials.p);
+            hipLaunchKernelGGL(k_finalize_min, dim3(1), dim3(RED_THREADS), 0, stream, partials.p, scal.p + SC_SHIFT + c);
             HIP_TRY(hipGetLastError());
-            int rc = reduce_finalize(1, SC_SHIFT + c, 1.0 / static_cast<double>(M));
-            if (rc) return rc;
         }
         std::vector<double2> sh(static_cast<size_t>(nc));
         int rc = read_scal(SC_SHIFT, nc, sh.data());
         if (rc) return rc;
         double total = 0.0;
-        for (int c = 0; c < nc; ++c) sp.cls_shift[static_cast<size_t>(c)] = sh[static_cast<size_t>(c)].x;
+        for (int c = 0; c < nc; ++c) sp.cls_shift[static_cast<size_t>(c)] = sh[static_cast<size_t>(c)].x + BOLTZ_CAP;
         for (int s = 0; s < sp.N; ++s) total += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)])];
         q_shift_total[static_cast<size_t>(p)] = total;
         HIP_TRY(hipMemcpyAsync(sp.d_cls_shift.p, sp.cls_shift.data(), sizeof(double) * nc, hipMemcpyHostToDevice, stream));
@@ -763,7 +783,14 @@ template <typename T> int fts_engine::solve_species(int p, bool accumulate)
         HIP_TRY(hipMemcpyAsync(qf, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
         HIP_TRY(hipMemcpyAsync(qb, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
     } else {
-        PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0)}, {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1)}};
+        // per-bead power-of-two renormalisation through ping-pong per-plane norm buffers
+        const bool rn = true;
+        double* nF[2] = {norms.p, norms.p + m1};
+        double* nB[2] = {norms.p + 2 * m1, norms.p + 3 * m1};
+        double* lcF = logc.p;
+        double* lcB = logc.p + Nmax;
+        PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0), nullptr, rn ? nF[0] : nullptr, rn ? lcF : nullptr},
+                               {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), nullptr, rn ? nB[0] : nullptr, rn ? lcB + (N - 1) : nullptr}};
         HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, 2, stream));
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
         if (timing) {
@@ -778,7 +805,10 @@ template <typename T> int fts_engine::solve_species(int p, bool accumulate)
                                   {tb, kop_sep(sep_bond.p + static_cast<size_t>(bb) * (m1 + m2 + m3), invM)}};
             HIP_TRY(launch_pencil<T>(g, PENCIL_FWD_MUL_INV, pj, 2, stream));
             const int sf = s + 1, sb = N - 2 - s;
-            PlaneJob<T> st[2] = {{tf, tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf)}, {tb, tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb)}};
+            PlaneJob<T> st[2] = {{tf, tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf), rn ? nF[s % 2] : nullptr, rn ? nF[(s + 1) % 2] : nullptr,
+                                  rn ? lcF + sf : nullptr},
+                                 {tb, tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), rn ? nB[s % 2] : nullptr, rn ? nB[(s + 1) % 2] : nullptr,
+                                  rn ? lcB + sb : nullptr}};
             HIP_TRY(launch_plane<T>(g, s < N - 2 ? PLANE_STEP : PLANE_LAST, st, 2, stream));
         }
         if (timing) {
@@ -790,6 +820,10 @@ template <typename T> int fts_engine::solve_species(int p, bool accumulate)
             tstat.algorithmic_bytes += 2.0 * (N - 1) * 2.0 * static_cast<double>(sizeof(Cv)) * static_cast<double>(M);
         }
     }
+    if (N > 1) {
+        hipLaunchKernelGGL(k_scale_prefix, dim3(1), dim3(64), 0, stream, logc.p, logc.p + Nmax, N, rfac.p, rfac2.p, scal.p + SC_LF + p);
+        HIP_TRY(hipGetLastError());
+    }
     // Q = <qF_{N-1}> (polymer.cu:425)
     hipLaunchKernelGGL(k_sum<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, qf + static_cast<size_t>(N - 1) * Ms, M, partials.p);
     HIP_TRY(hipGetLastError());
@@ -799,7 +833,7 @@ template <typename T> int fts_engine::solve_species(int p, bool accumulate)
     // density contraction
     hipLaunchKernelGGL(k_contract_density<T>, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, qf, qb, sp.d_cls_start.p, sp.d_cls_beads.p,
                        sp.d_cls_type.p, sp.d_cls_z.p, sp.d_cls_shift.p, nc, W_type_s.p, psi_s.p, scal.p + SC_Q + p,
-                       static_cast<double>(sp.n) / V, electro() ? 1 : 0, M, rho_hat.p, rho_c_hat.p);
+                       static_cast<double>(sp.n) / V, electro() ? 1 : 0, N > 1 ? rfac.p : nullptr, M, rho_hat.p, rho_c_hat.p);
     HIP_TRY(hipGetLastError());
     return FTS_OK;
 }
@@ -942,12 +976,13 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
     double2 Q[FTS_MAX_SPECIES], Qi[FTS_MAX_IONS];
     if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
     if (!ions.empty() && (rc = read_scal(SC_QION, static_cast<int>(ions.size()), Qi))) return rc;
+    if ((rc = read_log_gain())) return rc;
     auto neglog = [](double2 z) { return double2{-std::log(std::sqrt(z.x * z.x + z.y * z.y)), -std::atan2(z.y, z.x)}; };
     double2 h{0.0, 0.0};
     for (size_t p = 0; p < species.size(); ++p) {
         double2 mu = neglog(Q[p]);
-        mu.x += q_shift_total[p]; // Q = Q~ e^{-sum of shifts}
-        const double scale_q = std::exp(-q_shift_total[p]);
+        mu.x += q_shift_total[p] + q_log_gain[p]; // Q = Q~ e^{-(sum of shifts + LF(N-1))}
+        const double scale_q = std::exp(-q_shift_total[p] - q_log_gain[p]);
         rec->Q_poly[p][0] = Q[p].x * scale_q;
         rec->Q_poly[p][1] = Q[p].y * scale_q;
         rec->mu_ex_poly[p][0] = mu.x;
@@ -1024,9 +1059,14 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
         std::vector<std::array<double2, 7>> acc(static_cast<size_t>(std::max(1, sp.N - 1)));
         rc = precision == FTS_PREC_FP32 ? stress_chain<float>(static_cast<int>(p), acc) : stress_chain<double>(static_cast<int>(p), acc);
         if (rc) return rc;
+        std::vector<double> r2(static_cast<size_t>(std::max(1, sp.N - 1)), 1.0);
+        if (sp.N > 1) {
+            HIP_TRY(hipMemcpyAsync(r2.data(), rfac2.p, sizeof(double) * (sp.N - 1), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
         double2 tot[6] = {};
         for (int s = 0; s + 1 < sp.N; ++s) {
-            const double w = sp.bond[static_cast<size_t>(s)] * sp.bond[static_cast<size_t>(s)] / 3.0;
+            const double w = sp.bond[static_cast<size_t>(s)] * sp.bond[static_cast<size_t>(s)] / 3.0 * r2[static_cast<size_t>(s)];
             for (int c = 0; c < 6; ++c) {
                 tot[c].x += w * acc[static_cast<size_t>(s)][c].x;
                 tot[c].y += w * acc[static_cast<size_t>(s)][c].y;
@@ -1430,20 +1470,23 @@ int fts_get_field(fts_engine* h, int id, double* out, size_t n_doubles)
         if (n_doubles != 2 * count) return fail(FTS_INVALID, "field size mismatch");
         const void* q = id == FTS_FIELD_QF ? h->qF : h->qB;
         // undo the FP32 potential shifts: qF_s = q~F_s e^{-sum_{t<=s} shift}, qB_s = q~B_s e^{-sum_{t>=s} shift}
+        std::vector<double> lc(2 * static_cast<size_t>(h->Nmax), 0.0);
+        HIP_TRY(hipMemcpy(lc.data(), h->logc.p, sizeof(double) * lc.size(), hipMemcpyDeviceToHost));
         for (int s = 0; s < sp.N; ++s) {
-            double shift = 0.0;
-            if (h->precision == FTS_PREC_FP32) {
-                for (int t = 0; t < sp.N; ++t)
-                    if ((id == FTS_FIELD_QF && t <= s) || (id == FTS_FIELD_QB && t >= s))
-                        shift += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(t)])];
-            }
+            double shift = 0.0, k2sum = 0.0;
+            for (int t = 0; t < sp.N; ++t)
+                if ((id == FTS_FIELD_QF && t <= s) || (id == FTS_FIELD_QB && t >= s)) {
+                    shift += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(t)])];
+                    k2sum += lc[static_cast<size_t>(id == FTS_FIELD_QF ? t : h->Nmax + t)];
+                }
+            const double scale = std::exp(-shift) * std::exp2(-k2sum);
             double2* dst = h->t1.p;
             if (h->precision == FTS_PREC_FP32)
                 hipLaunchKernelGGL((k_cast<float, double>), dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, h->stream,
-                                   static_cast<const float2*>(q) + static_cast<size_t>(s) * Ms, dst, std::exp(-shift), Ms);
+                                   static_cast<const float2*>(q) + static_cast<size_t>(s) * Ms, dst, scale, Ms);
             else
                 hipLaunchKernelGGL((k_cast<double, double>), dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, h->stream,
-                                   static_cast<const double2*>(q) + static_cast<size_t>(s) * Ms, dst, 1.0, Ms);
+                                   static_cast<const double2*>(q) + static_cast<size_t>(s) * Ms, dst, scale, Ms);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(out + 2 * static_cast<size_t>(s) * Ms, dst, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
         }
@@ -1465,9 +1508,10 @@ int fts_compute_densities(fts_engine* h, double* Q_out)
     if (rc) return rc;
     double2 Q[FTS_MAX_SPECIES];
     if ((rc = h->read_scal(SC_Q, static_cast<int>(h->species.size()), Q))) return rc;
+    if ((rc = h->read_log_gain())) return rc;
     bool ok = true;
     for (size_t p = 0; p < h->species.size(); ++p) {
-        const double sc = std::exp(-h->q_shift_total[p]);
+        const double sc = std::exp(-h->q_shift_total[p] - h->q_log_gain[p]);
         if (Q_out) {
             Q_out[2 * p] = Q[p].x * sc;
             Q_out[2 * p + 1] = Q[p].y * sc;
@@ -1484,7 +1528,7 @@ int fts_measure(fts_engine* h, fts_observables* out)
     if (rc) return rc;
     if ((rc = h->measure(h->omega.p, h->psi.p, out))) return rc;
     for (int p = 0; p < out->n_species; ++p)
-        if (!std::isfinite(out->Q_poly[p][0]) || !std::isfinite(out->Q_poly[p][1])) return fail(FTS_NONFINITE, "non-finite Q");
+        if (!std::isfinite(out->mu_ex_poly[p][0]) || !std::isfinite(out->mu_ex_poly[p][1])) return fail(FTS_NONFINITE, "non-finite Q");
     return FTS_OK;
 }
 

@@ -39,6 +39,15 @@ template <typename T> struct PlaneJob {
     V2<T>* dst;
     V2<T>* qout;
     const V2<T>* emw;
+    // Per-bead renormalisation (INIT / STEP / LAST).  When norm_out is set, the
+    // pass writes max |q_out|^2 of each x-plane to norm_out[x]; when norm_in is
+    // set, q_out is scaled by c = 2^k, k = round(-log2(max |q_prev|)), and block 0
+    // stores k in *log_scale.  Powers of two scale exactly, so the recursion's
+    // values change only by the exponent, and |q| stays below ~1.4 times the
+    // growth of one bead step.
+    const double* norm_in;
+    double* norm_out;
+    double* log_scale;
 };
 
 template <typename T> struct PencilJob {

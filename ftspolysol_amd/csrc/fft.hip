@@ -27,17 +27,48 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     constexpr int LS = N + 1;
     __shared__ V lds[N * LS];
     __shared__ V tw[N];
+    constexpr int NRED = NT % 64 == 0 ? NT / 64 : NT;
+    __shared__ double red[NRED];
+    __shared__ T scale_sh;
     const int tid = threadIdx.x;
     const PlaneJob<T> job = blockIdx.y == 0 ? j0 : j1;
     const size_t plane = static_cast<size_t>(blockIdx.x) * (N * N);
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
+    constexpr bool PRODUCES_Q = MODE == PLANE_INIT || MODE == PLANE_STEP || MODE == PLANE_LAST;
+
+    // renormalisation factor from the previous bead's per-plane norms (same fixed
+    // summation order in every block)
+    T c = T(1);
+    if constexpr (PRODUCES_Q) {
+        if (job.norm_in) {
+            if (tid == 0) {
+                double ms = 0.0;
+                for (int x = 0; x < gridDim.x; ++x) ms = fmax(ms, job.norm_in[x]);
+                // power-of-two scale: exact in binary floating point
+                const double kmax = sizeof(T) == 4 ? 120.0 : 1000.0;
+                double k = (ms > 0.0 && isfinite(ms)) ? rint(-0.5 * log2(ms)) : 0.0;
+                k = fmin(fmax(k, -kmax), kmax);
+                scale_sh = static_cast<T>(exp2(k));
+                if (blockIdx.x == 0) *job.log_scale = k;
+            }
+            __syncthreads();
+            c = scale_sh;
+        } else if (job.log_scale && blockIdx.x == 0 && tid == 0) {
+            *job.log_scale = 0.0;
+        }
+    }
+    double nrm = 0.0;
 
     const V* src = (MODE == PLANE_INIT) ? job.emw : job.src;
 #pragma unroll 4
     for (int i = tid; i < N * N; i += NT) {
-        const V v = src[plane + i];
+        V v = src[plane + i];
+        if constexpr (MODE == PLANE_INIT) {
+            v = cscale(v, c);
+            job.qout[plane + i] = v;
+            nrm = fmax(nrm, static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
+        }
         lds[(i / N) * LS + (i % N)] = v;
-        if (MODE == PLANE_INIT) job.qout[plane + i] = v;
     }
     __syncthreads();
 
@@ -49,11 +80,29 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) {
             V* p = &lds[(i / N) * LS + (i % N)];
-            const V v = cmul(*p, job.emw[plane + i]);
+            V v = cmul(*p, job.emw[plane + i]);
+            if (job.norm_in) v = cscale(v, c);
             *p = v;
             job.qout[plane + i] = v;
+            nrm = fmax(nrm, static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
         }
         __syncthreads();
+    }
+    if constexpr (PRODUCES_Q) {
+        if (job.norm_out) {
+            if constexpr (NT % 64 == 0) {
+                for (int off = 32; off > 0; off >>= 1) nrm = fmax(nrm, __shfl_down(nrm, off, 64));
+                if ((tid & 63) == 0) red[tid >> 6] = nrm;
+            } else {
+                red[tid] = nrm;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double s2 = 0.0;
+                for (int w = 0; w < NRED; ++w) s2 = fmax(s2, red[w]);
+                job.norm_out[blockIdx.x] = s2;
+            }
+        }
     }
     if constexpr (MODE == PLANE_FWD || MODE == PLANE_INIT || MODE == PLANE_STEP) {
         lds_fft<T, N, -1, N, LS, 1, false, NT>(lds, tw, tid);

@@ -106,17 +106,38 @@ __global__ void __launch_bounds__(RED_THREADS) k_ion_boltzmann(const double2* __
     if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
 }
 
-// Mean of Re W over a bead class: the FP32 log-renormalisation shift.
-__global__ void __launch_bounds__(RED_THREADS) k_sum_class_w(const double2* __restrict__ Wt, const double2* __restrict__ ps, double z, int M,
+// Minimum of Re W over a bead class (partials in .x): the FP32 shift that caps
+// the class's Boltzmann factor at exp(BOLTZ_CAP).
+constexpr double BOLTZ_CAP = 40.0;
+__global__ void __launch_bounds__(RED_THREADS) k_min_class_w(const double2* __restrict__ Wt, const double2* __restrict__ ps, double z, int M,
                                                              double2* __restrict__ partials)
 {
-    double2 v[1] = {d2(0.0, 0.0)};
+    __shared__ double sh[RED_THREADS];
+    double v = 1.0e300;
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < M; i += RED_BLOCKS * RED_THREADS) {
         const double2 w = cadd(Wt[i], cmul(ps[i], d2(0.0, z)));
-        v[0].x += w.x;
+        v = fmin(v, w.x);
     }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = d2(sh[0], 0.0);
+}
+
+// out[0].x = min over the RED_BLOCKS partials (.x)
+__global__ void k_finalize_min(const double2* __restrict__ partials, double2* __restrict__ out)
+{
+    __shared__ double sh[RED_THREADS];
+    sh[threadIdx.x] = partials[threadIdx.x].x;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = d2(sh[0], 0.0);
 }
 
 // k-space contraction: out_c = sum_k w_c(k) S(k) A(k) B(-k), c = xx yy zz xy xz yz 1
@@ -204,7 +225,7 @@ __global__ void k_contract_density(const V2<T>* __restrict__ qF, const V2<T>* __
                                    const int* __restrict__ cls_beads, const int* __restrict__ cls_type, const double* __restrict__ cls_z,
                                    const double* __restrict__ cls_shift, int ncls, const double2* __restrict__ Wt,
                                    const double2* __restrict__ ps, const double2* __restrict__ Qp, double chain_density, int electro,
-                                   int M, double2* __restrict__ rho_hat, double2* __restrict__ rho_c_hat)
+                                   const double* __restrict__ rfac, int M, double2* __restrict__ rho_hat, double2* __restrict__ rho_c_hat)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
@@ -213,9 +234,12 @@ __global__ void k_contract_density(const V2<T>* __restrict__ qF, const V2<T>* __
     for (int c = 0; c < ncls; ++c) {
         double2 acc = d2(0.0, 0.0);
         for (int t = cls_start[c]; t < cls_start[c + 1]; ++t) {
-            const size_t off = static_cast<size_t>(cls_beads[t]) * M + i;
+            const int bead = cls_beads[t];
+            const size_t off = static_cast<size_t>(bead) * M + i;
             const V2<T> a = qF[off], b = qB[off];
-            acc = cadd(acc, cmul(d2(a.x, a.y), d2(b.x, b.y)));
+            double2 p = cmul(d2(a.x, a.y), d2(b.x, b.y));
+            if (rfac) p = cscale(p, rfac[bead]);
+            acc = cadd(acc, p);
         }
         double2 w = Wt[static_cast<size_t>(cls_type[c]) * M + i];
         const double z = cls_z[c];
@@ -226,6 +250,33 @@ __global__ void k_contract_density(const V2<T>* __restrict__ qF, const V2<T>* __
         if (electro && z != 0.0) charge = cadd(charge, cscale(v, z));
     }
     if (electro) rho_c_hat[i] = cadd(rho_c_hat[i], charge);
+}
+
+// Renormalisation bookkeeping of one species' sweeps (one thread).  logc holds
+// the base-2 exponents of the per-bead scales: with LF(s) = sum_{t<=s} logcF[t]
+// and LB(s) = sum_{t>=s} logcB[t] the stored propagators are qF_s 2^{LF(s)} and
+// qB_s 2^{LB(s)}, so
+//   rfac[s]  = 2^{LF(N-1) - LF(s) - LB(s)}     (density term qF_s qB_s / Q)
+//   rfac2[s] = 2^{LF(N-1) - LF(s) - LB(s+1)}   (stress term qF_s qB_{s+1} / Q)
+// and out->x = LF(N-1) ln 2, the log-gain of Q.
+__global__ void k_scale_prefix(const double* __restrict__ logcF, const double* __restrict__ logcB, int N, double* __restrict__ rfac,
+                               double* __restrict__ rfac2, double2* __restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double LFtot = 0.0;
+    for (int s = 0; s < N; ++s) {
+        LFtot += logcF[s];
+        rfac[s] = LFtot; // LF(s), replaced below
+    }
+    double LB = 0.0; // sum_{t > s} logcB[t]
+    for (int s = N - 1; s >= 0; --s) {
+        const double LF = rfac[s];
+        if (s < N - 1) rfac2[s] = exp2(LFtot - LF - LB);
+        LB += logcB[s];
+        rfac[s] = exp2(LFtot - LF - LB);
+    }
+    out->x = LFtot * 0.69314718055994530942;
+    out->y = 0.0;
 }
 
 // rho = (n/V) b / Q   (normalizedDensityKernel, common.h:554-558)

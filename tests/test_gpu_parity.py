@@ -148,20 +148,32 @@ def test_copolymer_pc_trajectory():
 
 
 def test_fp32_engine_tracks_fp64_at_96():
-    """FP32 propagators (log-renormalised) vs the FP64 engine at the headline size."""
+    """FP32 propagators (per-bead renormalised) vs the FP64 engine at the headline size,
+    on equilibrated fields (100 plain EM steps), where |Q| spans many decades."""
     print("fp32 vs fp64 96^3 N=100", flush=True)
-    s64 = make_system(96, 100, stress_mask=0x7F, seed=5)
-    s32 = make_system(96, 100, stress_mask=0x7F, seed=5, precision=PREC_FP32)
-    g64, g32 = hip_engine(s64), hip_engine(s32)
-    set_random_fields([g64, g32], 1, s64.m, 0.05, 41)
+    s64 = make_system(96, 100, stress_mask=0x7F, seed=5, scheme=SCHEME_EM)
+    g64 = hip_engine(s64)
+    om, ps = initial_fields(s64, cfg=4)
+    g64.set_fields(om, ps)
+    g64.step(100)
+    om, ps = g64.get_field(FIELD_OMEGA), g64.get_field(FIELD_PSI)[0]
+    s32 = make_system(96, 100, stress_mask=0x7F, seed=5, scheme=SCHEME_EM, precision=PREC_FP32)
+    g32 = hip_engine(s32)
+    g32.set_fields(om, ps)
+    g64.set_fields(om, ps)
     a, b = g32.measure(), g64.measure()
-    r = obs_rel(a, b)
-    assert max(r.values()) < 1e-4, r
+    assert abs(a["H"] - b["H"]) / abs(b["H"]) < 1e-4, (a["H"], b["H"])
+    assert abs(a["p_iso"] - b["p_iso"]) / abs(b["p_iso"]) < 1e-4
+    assert rel(a["stress"], b["stress"]) < 1e-4
+    # mu = -ln Q compared absolutely: |Q| itself varies over decades between states
+    assert np.abs(a["mu_ex_poly"] - b["mu_ex_poly"]).max() < 1e-4 * max(1.0, np.abs(b["mu_ex_poly"]).max())
     # density means are exact: <rho_p> = n N / V and <rho_c> = 0
     rho_p = g64.get_field(FIELD_RHO_P)
     nN_V = s64.desc.species[0].num_chains * 100 / g64.info.volume
-    assert abs(rho_p.mean() - nN_V) < 1e-12 * nN_V
-    assert abs(g64.get_field(FIELD_RHO_C).mean()) < 1e-12
+    assert abs(rho_p.mean() - nN_V) < 1e-10 * nN_V
+    assert abs(g64.get_field(FIELD_RHO_C).mean()) < 1e-10
+    r32 = g32.get_field(FIELD_RHO_P)
+    assert rel(r32, rho_p) < 1e-3
 
 
 def test_headline_measure_matches_oracle_64():
