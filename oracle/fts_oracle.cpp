@@ -1396,3 +1396,22 @@ int fto_bead_step(fts_engine* h, const double* q, const double* emw, double bond
 }
 
 } // extern "C"
+
+// ---- test exports: the noise generator --------------------------------------
+extern "C" {
+
+void fto_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out)
+{
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    uint32_t k[2] = {key[0], key[1]};
+    uint32_t o[4];
+    oracle::philox4x32_10(c, k, o);
+    for (int i = 0; i < 4; ++i) out[i] = o[i];
+}
+
+double fto_cl_normal(unsigned long long seed, unsigned long long step, unsigned field, unsigned site)
+{
+    return oracle::cl_normal(seed, step, field, site);
+}
+
+} // extern "C"

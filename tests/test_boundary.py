@@ -1,0 +1,121 @@
+"""CPU checks of the C-ABI boundary and host logic (no GPU needed).
+
+- libfts_hip.so loads and exports every entry point include/fts.h declares, and the
+  oracle exports the same set under the fto_ prefix;
+- fts_create rejects invalid systems with FTS_INVALID and a message before it
+  touches a device (the reference returns false with "error: ..." on stderr,
+  errHandle.cu:5-9);
+- the ctypes structures have the C layout of the header;
+- bench.py's integrated autocorrelation time on AR(1) series with known tau.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ftspolysol_amd.api import FTS_INVALID, FtsError, Engine, Info, Observables, SystemDesc, load_hip_library
+from ftspolysol_amd.system import System, make_system
+from oracle_lib import oracle_library
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "fts.h")).read()
+    return sorted(set(re.findall(r"\b(fts_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for required in ("fts_create", "fts_destroy", "fts_last_error", "fts_set_fields", "fts_get_field",
+                     "fts_compute_densities", "fts_measure", "fts_step"):
+        assert required in names
+
+
+def test_hip_library_exports_every_declared_symbol():
+    lib = load_hip_library()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_oracle_exports_the_same_interface():
+    lib = oracle_library()
+    engine_only = {"fts_set_timing", "fts_get_timing"}
+    missing = [n for n in declared_functions() if n not in engine_only and not hasattr(lib, "fto_" + n[4:])]
+    assert not missing, missing
+
+
+def test_struct_layout(tmp_path):
+    """The ctypes mirrors have the C compiler's sizes and offsets for include/fts.h."""
+    from ftspolysol_amd.api import CounterionDesc, SaltDesc, SpeciesDesc, Timing
+    structs = {"fts_species_desc": SpeciesDesc, "fts_counterion_desc": CounterionDesc, "fts_salt_desc": SaltDesc,
+               "fts_system": SystemDesc, "fts_observables": Observables, "fts_info": Info, "fts_timing": Timing}
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "fts.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        src.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            src.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in out:
+        if not line:
+            continue
+        cname, what, val = line.split()
+        py = structs[cname]
+        got = ctypes.sizeof(py) if what == "size" else getattr(py, what).offset
+        assert got == int(val), (cname, what, got, val)
+
+
+def bad_create(lib_prefix, system):
+    lib = load_hip_library() if lib_prefix == "fts_" else oracle_library()
+    with pytest.raises(FtsError) as ei:
+        Engine(lib, lib_prefix, system)
+    return ei.value
+
+
+def test_hip_rejects_unsupported_mesh():
+    err = bad_create("fts_", make_system(10, 4))
+    assert err.code == FTS_INVALID and "mesh" in str(err)
+    err = bad_create("fts_", System(m=(32, 32, 16), dL=0.5, species=[dict(num_chains=1, types=[0], charges=[0.0])], xi=0.1))
+    assert err.code == FTS_INVALID
+
+
+@pytest.mark.parametrize("prefix", ["fts_", "fto_"])
+def test_rejects_invalid_physics(prefix):
+    # salt that is not neutral (ion.cu:26-30)
+    s = System(m=(16, 16, 16), dL=0.5, species=[dict(num_chains=2, types=[0] * 4, charges=[0.0] * 4)], xi=0.1, lB=1.0,
+               salt=dict(num_salt=3, cation_valence=2.0, anion_valence=-1.0))
+    assert bad_create(prefix, s).code == FTS_INVALID
+    # counterions cannot neutralise a fractional total charge (ion.cu:13-15)
+    s = System(m=(16, 16, 16), dL=0.5, species=[dict(num_chains=1, types=[0] * 3, charges=[-0.5] * 3)], xi=0.1, lB=1.0,
+               counterion=dict(auto=True, valence=1.0))
+    assert "counterion" in str(bad_create(prefix, s))
+    # chi must be symmetric with a zero diagonal (interactionMode.cu:36-48)
+    s = System(m=(16, 16, 16), dL=0.5, species=[dict(num_chains=1, types=[0, 1], charges=[0.0, 0.0])], monomer_types=2,
+               xi=0.1, chi=[[0.0, 0.2], [0.3, 0.0]])
+    assert "symmetric" in str(bad_create(prefix, s))
+    # bead type out of range
+    s = System(m=(16, 16, 16), dL=0.5, species=[dict(num_chains=1, types=[0, 3], charges=[0.0, 0.0])], xi=0.1)
+    assert bad_create(prefix, s).code == FTS_INVALID
+
+
+def test_tau_int_ar1():
+    import bench
+    rng = np.random.default_rng(0)
+    for rho in (0.0, 0.5, 0.8):
+        n = 200000
+        x = np.empty(n)
+        x[0] = 0.0
+        e = rng.standard_normal(n)
+        for i in range(1, n):
+            x[i] = rho * x[i - 1] + e[i]
+        tau, _ = bench.tau_int(x)
+        exact = 0.5 * (1 + rho) / (1 - rho)
+        assert abs(tau - exact) < 0.08 * exact + 0.02, (rho, tau, exact)
