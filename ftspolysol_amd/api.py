@@ -15,6 +15,7 @@ import numpy as np
 FTS_OK, FTS_NONFINITE, FTS_INVALID, FTS_DEVICE_ERROR = 0, 1, 2, 3
 SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
+SCHED_AUTO, SCHED_PLANE, SCHED_LINE = 0, 1, 2
 MAX_SPECIES = MAX_IONS = MAX_TYPES = 8
 STRESS_ALL = 0x3F
 STRESS_PISO = 0x40
@@ -48,7 +49,7 @@ class SystemDesc(C.Structure):
                 ("default_smearing_length", C.c_double), ("counterions", CounterionDesc), ("salt", SaltDesc),
                 ("dt", C.c_double), ("n_lambda_omega", C.c_int), ("lambda_omega", C.POINTER(C.c_double)),
                 ("lambda_psi", C.c_double), ("noise", C.c_int), ("scheme", C.c_int), ("precision", C.c_int),
-                ("seed", C.c_ulonglong), ("stress_mask", C.c_int)]
+                ("seed", C.c_ulonglong), ("stress_mask", C.c_int), ("fft_schedule", C.c_int)]
 
 
 class Observables(C.Structure):
@@ -79,12 +80,12 @@ class Observables(C.Structure):
 class Info(C.Structure):
     _fields_ = [("grid_size", C.c_longlong), ("step", C.c_longlong), ("n_omega", C.c_int),
                 ("monomer_types", C.c_int), ("n_species", C.c_int), ("n_ions", C.c_int),
-                ("n_segment_classes", C.c_int), ("precision", C.c_int), ("scheme", C.c_int), ("pad_", C.c_int),
+                ("n_segment_classes", C.c_int), ("precision", C.c_int), ("scheme", C.c_int), ("fft_schedule", C.c_int),
                 ("volume", C.c_double), ("rho0", C.c_double), ("short_range_self_pressure", C.c_double),
                 ("mode_eigenvalue", C.c_double * MAX_TYPES), ("omega_coefficient", C.c_double * MAX_TYPES),
                 ("gamma_is_imag", C.c_double * MAX_TYPES), ("mode_O", (C.c_double * MAX_TYPES) * MAX_TYPES),
                 ("ion_valence", C.c_double * MAX_IONS), ("ion_count", C.c_longlong * MAX_IONS),
-                ("ion_smearing", C.c_double * MAX_IONS)]
+                ("ion_smearing", C.c_double * MAX_IONS), ("tune_ms", C.c_double * 2)]
 
 
 class Timing(C.Structure):

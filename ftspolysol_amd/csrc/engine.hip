@@ -138,8 +138,8 @@ struct fts_engine {
     hipStream_t stream = nullptr;
     FftGeom g64{}, g32{};
     DevBuf<double> kvec, sep_gamma, sep_bond, sep_ion, si_table_omega, si_table_psi;
-    DevBuf<double2> tw64_x, tw64_yz;
-    DevBuf<float2> tw32_x, tw32_yz;
+    DevBuf<double2> tw64_x, tw64_yz, tw64_z;
+    DevBuf<float2> tw32_x, tw32_yz, tw32_z;
     int n_bond_cls = 0;
     std::vector<double> bond_cls_len;
     DevBuf<double2> omega, omega_bar, omega_s, drift, psi, psi_bar, psi_s, dpsi, dpsi_bar;
@@ -194,6 +194,13 @@ struct fts_engine {
 
     // hot path
     template <typename T> int solve_species(int p, bool accumulate = true);
+    template <typename T> int run_sweeps(int p, int schedule, int nsteps, bool timed);
+    int autotune_schedule();
+    template <typename T> int bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op);
+    int sched = FTS_SCHED_PLANE;
+    bool plane_ok = true;
+    size_t norm_stride = 0;
+    double tune_ms[2] = {0.0, 0.0};
     int compute_densities(const double2* om, const double2* ps);
     int psi_drift(const double2* ps, const double2* rho_src, double2* out);
     int check_finite(bool& ok);
@@ -223,6 +230,8 @@ void fts_engine::release()
     tw64_yz.release();
     tw32_x.release();
     tw32_yz.release();
+    tw64_z.release();
+    tw32_z.release();
     for (DevBuf<double2>* b : {&omega, &omega_bar, &omega_s, &drift, &psi, &psi_bar, &psi_s, &dpsi, &dpsi_bar, &W_type_s, &rho_hat,
                                &rho_type, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &ion_psi, &ion_b, &ion_rho, &t1, &t2,
                                &t3, &mode_coef, &partials, &scal})
@@ -475,10 +484,15 @@ int fts_engine::setup_device()
             t32.push_back(w);
         }
     };
-    std::vector<double2> tx64, tyz64;
-    std::vector<float2> tx32, tyz32;
+    std::vector<double2> tx64, tyz64, tz64;
+    std::vector<float2> tx32, tyz32, tz32;
     twiddles(m1, tx64, tx32);
     twiddles(m2, tyz64, tyz32);
+    twiddles(m3, tz64, tz32);
+    HIP_TRY(tw64_z.alloc(m3));
+    HIP_TRY(tw32_z.alloc(m3));
+    HIP_TRY(hipMemcpy(tw64_z.p, tz64.data(), sizeof(double2) * m3, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(tw32_z.p, tz32.data(), sizeof(float2) * m3, hipMemcpyHostToDevice));
     HIP_TRY(tw64_x.alloc(m1));
     HIP_TRY(tw64_yz.alloc(m2));
     HIP_TRY(tw32_x.alloc(m1));
@@ -565,7 +579,8 @@ int fts_engine::setup_device()
     }
     q_shift_total.assign(species.size(), 0.0);
     q_log_gain.assign(species.size(), 0.0);
-    HIP_TRY(norms.alloc(4 * static_cast<size_t>(m1)));
+    norm_stride = std::max(static_cast<size_t>(m1), static_cast<size_t>(m1) * m2 / LINE_TC + 1);
+    HIP_TRY(norms.alloc(4 * norm_stride));
     HIP_TRY(logc.alloc(2 * static_cast<size_t>(Nmax)));
     HIP_TRY(rfac.alloc(static_cast<size_t>(Nmax)));
     HIP_TRY(rfac2.alloc(static_cast<size_t>(Nmax)));
@@ -651,22 +666,42 @@ int fts_engine::build_si_tables()
 // dst = IFFT[K(k) FFT[src]]; callers fold 1/M into K (fft_apply_real_kernel, common.h:1079-1085)
 int fts_engine::fft_apply(const double2* src, double2* dst, const KOp& op)
 {
-    PlaneJob<double> pj{src, dst, nullptr, nullptr};
-    HIP_TRY(launch_plane<double>(g64, PLANE_FWD, &pj, 1, stream));
-    PencilJob<double> xj{dst, op};
-    HIP_TRY(launch_pencil<double>(g64, PENCIL_FWD_MUL_INV, &xj, 1, stream));
-    PlaneJob<double> ij{dst, dst, nullptr, nullptr};
-    HIP_TRY(launch_plane<double>(g64, PLANE_INV, &ij, 1, stream));
+    if (plane_ok) {
+        PlaneJob<double> pj{src, dst, nullptr, nullptr};
+        HIP_TRY(launch_plane<double>(g64, PLANE_FWD, &pj, 1, stream));
+        PencilJob<double> xj{dst, op};
+        HIP_TRY(launch_pencil<double>(g64, PENCIL_FWD_MUL_INV, &xj, 1, stream));
+        PlaneJob<double> ij{dst, dst, nullptr, nullptr};
+        HIP_TRY(launch_plane<double>(g64, PLANE_INV, &ij, 1, stream));
+        return FTS_OK;
+    }
+    if (dst != src) HIP_TRY(hipMemcpyAsync(dst, src, sizeof(double2) * M, hipMemcpyDeviceToDevice, stream));
+    LineJob<double> j{dst};
+    HIP_TRY(launch_line<double>(g64, 2, LINE_FWD, tw64_z.p, &j, 1, stream));
+    HIP_TRY(launch_line<double>(g64, 1, LINE_FWD, tw64_yz.p, &j, 1, stream));
+    LineJob<double> xj{dst, nullptr, nullptr, op};
+    HIP_TRY(launch_line<double>(g64, 0, LINE_FWD_MUL_INV, tw64_x.p, &xj, 1, stream));
+    HIP_TRY(launch_line<double>(g64, 1, LINE_INV, tw64_yz.p, &j, 1, stream));
+    HIP_TRY(launch_line<double>(g64, 2, LINE_INV, tw64_z.p, &j, 1, stream));
     return FTS_OK;
 }
 
 template <typename T> int fts_engine::fft_forward_t(const V2<T>* src, V2<T>* dst)
 {
     const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
-    PlaneJob<T> pj{src, dst, nullptr, nullptr};
-    HIP_TRY(launch_plane<T>(g, PLANE_FWD, &pj, 1, stream));
-    PencilJob<T> xj{dst, KOp{}};
-    HIP_TRY(launch_pencil<T>(g, PENCIL_FWD, &xj, 1, stream));
+    if (plane_ok) {
+        PlaneJob<T> pj{src, dst, nullptr, nullptr};
+        HIP_TRY(launch_plane<T>(g, PLANE_FWD, &pj, 1, stream));
+        PencilJob<T> xj{dst, KOp{}};
+        HIP_TRY(launch_pencil<T>(g, PENCIL_FWD, &xj, 1, stream));
+        return FTS_OK;
+    }
+    if (dst != src) HIP_TRY(hipMemcpyAsync(dst, src, sizeof(V2<T>) * M, hipMemcpyDeviceToDevice, stream));
+    const bool d = sizeof(T) == 8;
+    LineJob<T> j{dst};
+    HIP_TRY(launch_line<T>(g, 2, LINE_FWD, d ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p), &j, 1, stream));
+    HIP_TRY(launch_line<T>(g, 1, LINE_FWD, d ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p), &j, 1, stream));
+    HIP_TRY(launch_line<T>(g, 0, LINE_FWD, d ? static_cast<const void*>(tw64_x.p) : static_cast<const void*>(tw32_x.p), &j, 1, stream));
     return FTS_OK;
 }
 
@@ -741,6 +776,155 @@ int fts_engine::resolve_timing()
 }
 
 // ---------------------------------------------------------------------------
+// Both propagator sweeps of species p for nsteps bead steps (polymer.cu:402-416):
+//   forward  qF_{s+1} = e^{-W_{s+1}} S_{phi_s}[qF_s],       s = 0 .. nsteps-1
+//   backward qB_{t-1} = e^{-W_{t-1}} S_{phi_{t-1}}[qB_t],   t = N-1 .. N-nsteps
+// Every launch carries the two sweeps as two jobs.  Each pass that produces a
+// propagator rescales it by a power of two from the previous bead's maximum
+// (ping-pong norm buffers) and records the exponent in logc.
+// ---------------------------------------------------------------------------
+template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps, bool timed)
+{
+    using Cv = V2<T>;
+    HostSpecies& sp = species[static_cast<size_t>(p)];
+    const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
+    const size_t Ms = static_cast<size_t>(M);
+    Cv* qf = static_cast<Cv*>(qF);
+    Cv* qb = static_cast<Cv*>(qB);
+    Cv* e = static_cast<Cv*>(E);
+    Cv* tf = static_cast<Cv*>(tmpF);
+    Cv* tb = static_cast<Cv*>(tmpB);
+    const int N = sp.N;
+    auto E_of = [&](int s) { return e + static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)]) * Ms; };
+    double* nF[2] = {norms.p, norms.p + norm_stride};
+    double* nB[2] = {norms.p + 2 * norm_stride, norms.p + 3 * norm_stride};
+    double* lcF = logc.p;
+    double* lcB = logc.p + Nmax;
+    const double invM = 1.0 / static_cast<double>(M);
+    const size_t ks = static_cast<size_t>(m1 + m2 + m3);
+    const void* twx = sizeof(T) == 8 ? static_cast<const void*>(tw64_x.p) : static_cast<const void*>(tw32_x.p);
+    const void* twy = sizeof(T) == 8 ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p);
+    const void* twz = sizeof(T) == 8 ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p);
+    hipEvent_t ev0 = nullptr;
+    if (schedule == FTS_SCHED_PLANE) {
+        PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0), nullptr, nF[0], lcF},
+                               {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), nullptr, nB[0], lcB + (N - 1)}};
+        HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, 2, stream));
+        if (timed) {
+            ev0 = get_event();
+            HIP_TRY(hipEventRecord(ev0, stream));
+        }
+        for (int s = 0; s < nsteps; ++s) {
+            const int bf = sp.bond_cls[static_cast<size_t>(s)], bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
+            PencilJob<T> pj[2] = {{tf, kop_sep(sep_bond.p + static_cast<size_t>(bf) * ks, invM)},
+                                  {tb, kop_sep(sep_bond.p + static_cast<size_t>(bb) * ks, invM)}};
+            HIP_TRY(launch_pencil<T>(g, PENCIL_FWD_MUL_INV, pj, 2, stream));
+            const int sf = s + 1, sb = N - 2 - s;
+            PlaneJob<T> st[2] = {{tf, tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf), nF[s % 2], nF[(s + 1) % 2], lcF + sf},
+                                 {tb, tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), nB[s % 2], nB[(s + 1) % 2], lcB + sb}};
+            HIP_TRY(launch_plane<T>(g, s < nsteps - 1 ? PLANE_STEP : PLANE_LAST, st, 2, stream));
+        }
+    } else {
+        LineJob<T> init[2] = {{tf, qf, E_of(0), KOp{}, nullptr, nF[0], lcF},
+                              {tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), KOp{}, nullptr, nB[0], lcB + (N - 1)}};
+        HIP_TRY(launch_line<T>(g, 2, LINE_E_FWD, twz, init, 2, stream));
+        LineJob<T> yf[2] = {{tf}, {tb}};
+        HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, yf, 2, stream));
+        if (timed) {
+            ev0 = get_event();
+            HIP_TRY(hipEventRecord(ev0, stream));
+        }
+        for (int s = 0; s < nsteps; ++s) {
+            const int bf = sp.bond_cls[static_cast<size_t>(s)], bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
+            LineJob<T> xj[2] = {{tf, nullptr, nullptr, kop_sep(sep_bond.p + static_cast<size_t>(bf) * ks, invM)},
+                                {tb, nullptr, nullptr, kop_sep(sep_bond.p + static_cast<size_t>(bb) * ks, invM)}};
+            HIP_TRY(launch_line<T>(g, 0, LINE_FWD_MUL_INV, twx, xj, 2, stream));
+            HIP_TRY(launch_line<T>(g, 1, LINE_INV, twy, yf, 2, stream));
+            const int sf = s + 1, sb = N - 2 - s;
+            const bool last = s == nsteps - 1;
+            LineJob<T> zj[2] = {{tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf), KOp{}, nF[s % 2], nF[(s + 1) % 2], lcF + sf},
+                                {tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), KOp{}, nB[s % 2], nB[(s + 1) % 2], lcB + sb}};
+            HIP_TRY(launch_line<T>(g, 2, last ? LINE_INV_E : LINE_INV_E_FWD, twz, zj, 2, stream));
+            if (!last) HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, yf, 2, stream));
+        }
+    }
+    if (timed) {
+        hipEvent_t ev1 = get_event();
+        HIP_TRY(hipEventRecord(ev1, stream));
+        pending.emplace_back(ev0, ev1);
+        tstat.bead_steps += 2LL * nsteps;
+        tstat.sweeps += 1;
+        tstat.algorithmic_bytes += 2.0 * nsteps * 2.0 * static_cast<double>(sizeof(Cv)) * static_cast<double>(M);
+    }
+    return FTS_OK;
+}
+
+// One bead step on a single field with the engine's schedule (fts_bead_step): q is
+// overwritten with intermediates, qout = e^{-W} S_phi[q].
+template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op)
+{
+    const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
+    const bool d = sizeof(T) == 8;
+    if (sched == FTS_SCHED_PLANE) {
+        PlaneJob<T> j1{q, q, nullptr, nullptr};
+        HIP_TRY(launch_plane<T>(g, PLANE_FWD, &j1, 1, stream));
+        PencilJob<T> j2{q, op};
+        HIP_TRY(launch_pencil<T>(g, PENCIL_FWD_MUL_INV, &j2, 1, stream));
+        PlaneJob<T> j3{q, nullptr, qout, emw};
+        HIP_TRY(launch_plane<T>(g, PLANE_LAST, &j3, 1, stream));
+        return FTS_OK;
+    }
+    const void* twx = d ? static_cast<const void*>(tw64_x.p) : static_cast<const void*>(tw32_x.p);
+    const void* twy = d ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p);
+    const void* twz = d ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p);
+    LineJob<T> j{q};
+    HIP_TRY(launch_line<T>(g, 2, LINE_FWD, twz, &j, 1, stream));
+    HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, &j, 1, stream));
+    LineJob<T> xj{q, nullptr, nullptr, op};
+    HIP_TRY(launch_line<T>(g, 0, LINE_FWD_MUL_INV, twx, &xj, 1, stream));
+    HIP_TRY(launch_line<T>(g, 1, LINE_INV, twy, &j, 1, stream));
+    LineJob<T> zj{q, qout, emw};
+    HIP_TRY(launch_line<T>(g, 2, LINE_INV_E, twz, &zj, 1, stream));
+    return FTS_OK;
+}
+
+// Bead-step schedule chosen by timing both on this grid and precision (the FFT
+// autotuner): 2 x 8 bead steps of species 0 at zero fields, best of three.
+int fts_engine::autotune_schedule()
+{
+    if (!plane_ok) {
+        sched = FTS_SCHED_LINE;
+        return FTS_OK;
+    }
+    const HostSpecies& sp = species[0];
+    if (sp.N < 2) {
+        sched = FTS_SCHED_PLANE;
+        return FTS_OK;
+    }
+    const int steps = std::min(8, sp.N - 1);
+    double best[2] = {1e30, 1e30};
+    hipEvent_t a = get_event(), b = get_event();
+    for (int rep = 0; rep < 3; ++rep)
+        for (int k = 0; k < 2; ++k) {
+            const int schedule = k == 0 ? FTS_SCHED_PLANE : FTS_SCHED_LINE;
+            HIP_TRY(hipEventRecord(a, stream));
+            int rc = precision == FTS_PREC_FP32 ? run_sweeps<float>(0, schedule, steps, false) : run_sweeps<double>(0, schedule, steps, false);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(b, stream));
+            HIP_TRY(hipEventSynchronize(b));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, a, b));
+            best[k] = std::min(best[k], static_cast<double>(ms));
+        }
+    event_pool.push_back(a);
+    event_pool.push_back(b);
+    tune_ms[0] = best[0];
+    tune_ms[1] = best[1];
+    sched = best[0] <= best[1] ? FTS_SCHED_PLANE : FTS_SCHED_LINE;
+    return FTS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Propagator solve for species p (polymer.cu:418-426 + accumulate, 434-447)
 // ---------------------------------------------------------------------------
 template <typename T> int fts_engine::solve_species(int p, bool accumulate)
@@ -783,42 +967,8 @@ template <typename T> int fts_engine::solve_species(int p, bool accumulate)
         HIP_TRY(hipMemcpyAsync(qf, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
         HIP_TRY(hipMemcpyAsync(qb, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
     } else {
-        // per-bead power-of-two renormalisation through ping-pong per-plane norm buffers
-        const bool rn = true;
-        double* nF[2] = {norms.p, norms.p + m1};
-        double* nB[2] = {norms.p + 2 * m1, norms.p + 3 * m1};
-        double* lcF = logc.p;
-        double* lcB = logc.p + Nmax;
-        PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0), nullptr, rn ? nF[0] : nullptr, rn ? lcF : nullptr},
-                               {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), nullptr, rn ? nB[0] : nullptr, rn ? lcB + (N - 1) : nullptr}};
-        HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, 2, stream));
-        hipEvent_t ev0 = nullptr, ev1 = nullptr;
-        if (timing) {
-            ev0 = get_event();
-            HIP_TRY(hipEventRecord(ev0, stream));
-        }
-        const double invM = 1.0 / static_cast<double>(M);
-        for (int s = 0; s < N - 1; ++s) {
-            const int bf = sp.bond_cls[static_cast<size_t>(s)];
-            const int bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
-            PencilJob<T> pj[2] = {{tf, kop_sep(sep_bond.p + static_cast<size_t>(bf) * (m1 + m2 + m3), invM)},
-                                  {tb, kop_sep(sep_bond.p + static_cast<size_t>(bb) * (m1 + m2 + m3), invM)}};
-            HIP_TRY(launch_pencil<T>(g, PENCIL_FWD_MUL_INV, pj, 2, stream));
-            const int sf = s + 1, sb = N - 2 - s;
-            PlaneJob<T> st[2] = {{tf, tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf), rn ? nF[s % 2] : nullptr, rn ? nF[(s + 1) % 2] : nullptr,
-                                  rn ? lcF + sf : nullptr},
-                                 {tb, tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), rn ? nB[s % 2] : nullptr, rn ? nB[(s + 1) % 2] : nullptr,
-                                  rn ? lcB + sb : nullptr}};
-            HIP_TRY(launch_plane<T>(g, s < N - 2 ? PLANE_STEP : PLANE_LAST, st, 2, stream));
-        }
-        if (timing) {
-            ev1 = get_event();
-            HIP_TRY(hipEventRecord(ev1, stream));
-            pending.emplace_back(ev0, ev1);
-            tstat.bead_steps += 2LL * (N - 1);
-            tstat.sweeps += 1;
-            tstat.algorithmic_bytes += 2.0 * (N - 1) * 2.0 * static_cast<double>(sizeof(Cv)) * static_cast<double>(M);
-        }
+        int rc = run_sweeps<T>(p, sched, N - 1, timing);
+        if (rc) return rc;
     }
     if (N > 1) {
         hipLaunchKernelGGL(k_scale_prefix, dim3(1), dim3(64), 0, stream, logc.p, logc.p + Nmax, N, rfac.p, rfac2.p, scal.p + SC_LF + p);
@@ -1291,9 +1441,12 @@ int fts_create(const fts_system* s, fts_engine** out)
     const long long msize = static_cast<long long>(s->m1) * s->m2 * s->m3;
     if (msize > 2147483647LL) return fail(FTS_INVALID, "mesh size");
     if (s->precision != FTS_PREC_FP64 && s->precision != FTS_PREC_FP32) return fail(FTS_INVALID, "precision");
-    if (!fft_supported(s->m1, s->m2, s->m3, s->precision == FTS_PREC_FP32))
-        return fail(FTS_INVALID, "mesh not supported by the engine FFT: need m2 == m3, lengths in {8,12,16,24,32,48,64,96,128}, "
-                                 "and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
+    if (!line_supported(s->m1, s->m2, s->m3))
+        return fail(FTS_INVALID, "mesh not supported by the engine FFT: lengths must be in {8,12,16,24,32,48,64,96,128,192,256}");
+    const bool plane_fits = fft_supported(s->m1, s->m2, s->m3, s->precision == FTS_PREC_FP32);
+    if (s->fft_schedule == FTS_SCHED_PLANE && !plane_fits)
+        return fail(FTS_INVALID, "the plane schedule needs m2 == m3 and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
+    if (s->fft_schedule < 0 || s->fft_schedule > 2) return fail(FTS_INVALID, "fft_schedule");
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
     if (s->n_species <= 0 || s->n_species > FTS_MAX_SPECIES || !s->species) return fail(FTS_INVALID, "polymer species_count");
@@ -1321,6 +1474,8 @@ int fts_create(const fts_system* s, fts_engine** out)
     e->precision = s->precision;
     e->seed = s->seed;
     e->stress_mask = s->stress_mask;
+    e->plane_ok = plane_fits;
+    e->sched = s->fft_schedule == FTS_SCHED_AUTO ? (plane_fits ? FTS_SCHED_PLANE : FTS_SCHED_LINE) : s->fft_schedule;
     int rc = e->build_modes(s);
     if (rc) return bail(rc);
     e->total_type_segments.assign(static_cast<size_t>(e->nT), 0);
@@ -1395,6 +1550,7 @@ int fts_create(const fts_system* s, fts_engine** out)
     if ((rc = e->build_ions(s))) return bail(rc);
     if ((rc = e->setup_device())) return bail(rc);
     if (e->scheme == FTS_SCHEME_SEMI_IMPLICIT && (rc = e->build_si_tables())) return bail(rc);
+    if (s->fft_schedule == FTS_SCHED_AUTO && (rc = e->autotune_schedule())) return bail(rc);
     *out = e;
     return FTS_OK;
 }
@@ -1414,6 +1570,9 @@ int fts_get_info(fts_engine* h, fts_info* o)
     o->n_segment_classes = h->max_cls;
     o->precision = h->precision;
     o->scheme = h->scheme;
+    o->fft_schedule = h->sched;
+    o->tune_ms[0] = h->tune_ms[0];
+    o->tune_ms[1] = h->tune_ms[1];
     o->volume = h->V;
     o->rho0 = h->rho0;
     o->short_range_self_pressure = h->P_sr;
@@ -1566,10 +1725,19 @@ int fts_fft3d(fts_engine* h, const double* in, double* out, int sign)
     if (!h || !in || !out || (sign != 1 && sign != -1)) return fail(FTS_INVALID, "bad argument");
     const size_t Ms = static_cast<size_t>(h->M);
     HIP_TRY(hipMemcpyAsync(h->t1.p, in, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
-    PlaneJob<double> pj{h->t1.p, h->t2.p, nullptr, nullptr};
-    HIP_TRY(launch_plane<double>(h->g64, sign < 0 ? PLANE_FWD : PLANE_INV, &pj, 1, h->stream));
-    PencilJob<double> xj{h->t2.p, KOp{}};
-    HIP_TRY(launch_pencil<double>(h->g64, sign < 0 ? PENCIL_FWD : PENCIL_INV, &xj, 1, h->stream));
+    if (h->plane_ok) {
+        PlaneJob<double> pj{h->t1.p, h->t2.p, nullptr, nullptr};
+        HIP_TRY(launch_plane<double>(h->g64, sign < 0 ? PLANE_FWD : PLANE_INV, &pj, 1, h->stream));
+        PencilJob<double> xj{h->t2.p, KOp{}};
+        HIP_TRY(launch_pencil<double>(h->g64, sign < 0 ? PENCIL_FWD : PENCIL_INV, &xj, 1, h->stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(h->t2.p, h->t1.p, sizeof(double2) * Ms, hipMemcpyDeviceToDevice, h->stream));
+        LineJob<double> j{h->t2.p};
+        const int mode = sign < 0 ? LINE_FWD : LINE_INV;
+        HIP_TRY(launch_line<double>(h->g64, 2, mode, h->tw64_z.p, &j, 1, h->stream));
+        HIP_TRY(launch_line<double>(h->g64, 1, mode, h->tw64_yz.p, &j, 1, h->stream));
+        HIP_TRY(launch_line<double>(h->g64, 0, mode, h->tw64_x.p, &j, 1, h->stream));
+    }
     HIP_TRY(hipMemcpyAsync(out, h->t2.p, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return FTS_OK;
@@ -1590,28 +1758,19 @@ int fts_bead_step(fts_engine* h, const double* q, const double* emw, double bond
     HIP_TRY(hipMemcpyAsync(h->t2.p, emw, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
     const KOp op = h->kop_sep(dsep.p, 1.0 / static_cast<double>(h->M));
     const int nb = ew_blocks(Ms);
-    hipError_t err;
+    int rc = FTS_OK;
     if (h->precision == FTS_PREC_FP32) {
         float2* a = static_cast<float2*>(h->tmpF);
         float2* e = static_cast<float2*>(h->E);
         float2* o = static_cast<float2*>(h->qF);
         hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, h->stream, h->t1.p, a, 1.0, Ms);
         hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, h->stream, h->t2.p, e, 1.0, Ms);
-        PlaneJob<float> j1{a, a, nullptr, nullptr};
-        if ((err = launch_plane<float>(h->g32, PLANE_FWD, &j1, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
-        PencilJob<float> j2{a, op};
-        if ((err = launch_pencil<float>(h->g32, PENCIL_FWD_MUL_INV, &j2, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
-        PlaneJob<float> j3{a, nullptr, o, e};
-        if ((err = launch_plane<float>(h->g32, PLANE_LAST, &j3, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+        rc = h->bead_hook<float>(a, e, o, op);
         hipLaunchKernelGGL((k_cast<float, double>), dim3(nb), dim3(EW_THREADS), 0, h->stream, o, h->t3.p, 1.0, Ms);
     } else {
-        PlaneJob<double> j1{h->t1.p, h->t1.p, nullptr, nullptr};
-        if ((err = launch_plane<double>(h->g64, PLANE_FWD, &j1, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
-        PencilJob<double> j2{h->t1.p, op};
-        if ((err = launch_pencil<double>(h->g64, PENCIL_FWD_MUL_INV, &j2, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
-        PlaneJob<double> j3{h->t1.p, nullptr, h->t3.p, h->t2.p};
-        if ((err = launch_plane<double>(h->g64, PLANE_LAST, &j3, 1, h->stream)) != hipSuccess) return fail(FTS_DEVICE_ERROR, hipGetErrorString(err));
+        rc = h->bead_hook<double>(h->t1.p, h->t2.p, h->t3.p, op);
     }
+    if (rc) return rc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(q_out, h->t3.p, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));

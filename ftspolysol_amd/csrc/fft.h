@@ -55,6 +55,28 @@ template <typename T> struct PencilJob {
     KOp op;
 };
 
+// Line pass: a tile of LINE_TC lines of one axis, transformed along that axis in LDS.
+//   axis 0 (x): column j in [0, m2 m3),  point t at  t*m2*m3 + j
+//   axis 1 (y): column j in [0, m1 m3),  point t at  (j/m3)*m2*m3 + t*m3 + j%m3
+//   axis 2 (z): column j in [0, m1 m2),  point t at  j*m3 + t
+// Modes: FWD, INV, FWD_MUL_INV (k-space kernel, as the pencil pass), and on z only:
+// INV_E_FWD (inverse, x e^{-W} (x 2^k), store q, forward), INV_E (same without the
+// forward transform) and E_FWD (q = e^{-W} (x 2^k), store q, forward).  The bead step
+// with line passes is x(FWD_MUL_INV) -> y(INV) -> z(INV_E_FWD) -> y(FWD); it moves
+// more bytes than the plane/pencil pair but every pass has m^2/16 workgroups.
+constexpr int LINE_TC = 16;
+enum LineMode { LINE_FWD = 0, LINE_INV = 1, LINE_FWD_MUL_INV = 2, LINE_INV_E_FWD = 3, LINE_INV_E = 4, LINE_E_FWD = 5 };
+
+template <typename T> struct LineJob {
+    V2<T>* data;          // transformed in place (E_FWD writes its result here)
+    V2<T>* qout;          // z modes that produce a propagator
+    const V2<T>* emw;     // e^{-W} of the bead (z modes)
+    KOp op;               // FWD_MUL_INV
+    const double* norm_in;
+    double* norm_out;     // per-block max |q|^2 (gridDim.x entries)
+    double* log_scale;
+};
+
 struct FftGeom {
     int m1, m2, m3;
     const double* kvec;   // device: kx[m1], ky[m2], kz[m3]
@@ -69,8 +91,24 @@ bool fft_supported(int m1, int m2, int m3, bool fp32);
 template <typename T>
 hipError_t launch_plane(const FftGeom& g, int mode, const PlaneJob<T>* jobs, int njobs, hipStream_t s);
 
+// True when line kernels exist for this grid (any m1, m2, m3 in the size set).
+bool line_supported(int m1, int m2, int m3);
+
+// Launch one line pass along `axis` for njobs (1 or 2) jobs.  tw: twiddles of that axis.
+template <typename T>
+hipError_t launch_line(const FftGeom& g, int axis, int mode, const void* tw, const LineJob<T>* jobs, int njobs, hipStream_t s);
+
 // Launch one pencil pass for njobs (1 or 2) jobs.
 template <typename T>
 hipError_t launch_pencil(const FftGeom& g, int mode, const PencilJob<T>* jobs, int njobs, hipStream_t s);
+
+// The same passes over njobs jobs read from a device-resident job table (blockIdx.y
+// indexes the table): the propagator sweeps of many replicas share each launch.
+template <typename T>
+hipError_t launch_plane_table(const FftGeom& g, int mode, const PlaneJob<T>* dev_jobs, int njobs, hipStream_t s);
+template <typename T>
+hipError_t launch_pencil_table(const FftGeom& g, int mode, const PencilJob<T>* dev_jobs, int njobs, hipStream_t s);
+template <typename T>
+hipError_t launch_line_table(const FftGeom& g, int axis, int mode, const void* tw, const LineJob<T>* dev_jobs, int njobs, hipStream_t s);
 
 } // namespace fts

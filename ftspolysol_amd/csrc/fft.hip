@@ -20,7 +20,7 @@ template <int N> constexpr int plane_threads()
 template <int N1> constexpr int pencil_threads() { return PENCIL_ZC * (N1 / plan_e(N1)); }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j0, PlaneJob<T> j1, const V2<T>* __restrict__ twg)
+__global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j0, PlaneJob<T> j1, const PlaneJob<T>* __restrict__ table, const V2<T>* __restrict__ twg)
 {
     using V = V2<T>;
     constexpr int NT = plane_threads<N>();
@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     __shared__ double red[NRED];
     __shared__ T scale_sh;
     const int tid = threadIdx.x;
-    const PlaneJob<T> job = blockIdx.y == 0 ? j0 : j1;
+    const PlaneJob<T> job = table ? table[blockIdx.y] : (blockIdx.y == 0 ? j0 : j1);
     const size_t plane = static_cast<size_t>(blockIdx.x) * (N * N);
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
     constexpr bool PRODUCES_Q = MODE == PLANE_INIT || MODE == PLANE_STEP || MODE == PLANE_LAST;
@@ -132,7 +132,7 @@ __device__ __forceinline__ double kop_value(const KOp& op, const double* kvec, i
 }
 
 template <typename T, int N1, int MODE>
-__global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<T> j0, PencilJob<T> j1, FftGeom g)
+__global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<T> j0, PencilJob<T> j1, const PencilJob<T>* __restrict__ table, FftGeom g)
 {
     using V = V2<T>;
     constexpr int NT = pencil_threads<N1>();
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
     __shared__ V lds[N1 * LS];
     __shared__ V tw[N1];
     const int tid = threadIdx.x;
-    const PencilJob<T> job = blockIdx.y == 0 ? j0 : j1;
+    const PencilJob<T> job = table ? table[blockIdx.y] : (blockIdx.y == 0 ? j0 : j1);
     const size_t sx = static_cast<size_t>(g.m2) * g.m3;
     const size_t jbase = static_cast<size_t>(blockIdx.x) * ZC;
     const V* twg = static_cast<const V*>(g.tw_x);
@@ -150,18 +150,34 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
 #pragma unroll 4
     for (int i = tid; i < N1 * ZC; i += NT) {
         const int x = i / ZC, c = i % ZC;
-        lds[x * LS + c] = data[x * sx + jbase + c];
+        lds[x * LS + c] = data[static_cast<size_t>(x) * sx + jbase + c];
     }
     __syncthreads();
     if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_MUL_INV) lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
     if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL) {
+        // every element a thread touches shares its column c (NT is a multiple of ZC):
+        // the (y, z) part of the kernel is one per-thread constant
         const KOp op = job.op;
+        static_assert(NT % ZC == 0, "pencil threads must cover whole rows");
+        const int c = tid % ZC;
+        const int j = static_cast<int>(jbase) + c;
+        const int y = j / g.m3, z = j % g.m3;
+        const double syz = op.sep ? op.sep[g.m1 + y] * op.sep[g.m1 + g.m2 + z] : 1.0;
+        const double ky = g.kvec[g.m1 + y], kz = g.kvec[g.m1 + g.m2 + z];
 #pragma unroll 4
         for (int i = tid; i < N1 * ZC; i += NT) {
-            const int x = i / ZC, c = i % ZC;
-            const size_t j = jbase + c;
-            const int y = static_cast<int>(j / g.m3), z = static_cast<int>(j % g.m3);
-            const double kv = kop_value(op, g.kvec, g.m1, g.m2, x, y, z, x * sx + j);
+            const int x = i / ZC;
+            double kv = op.sep ? op.sep[x] * syz : 1.0;
+            if (op.table) kv *= op.table[static_cast<size_t>(x) * sx + j];
+            if (op.c1 != 0.0 || op.c2 != 0.0) {
+                const double k[3] = {g.kvec[x], ky, kz};
+                double poly = op.c0;
+                if (op.c1 != 0.0) poly += op.c1 * (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+                if (op.c2 != 0.0) poly += op.c2 * (k[op.ka] * k[op.kb]);
+                kv *= poly;
+            } else {
+                kv *= op.c0;
+            }
             lds[x * LS + c] = cscale(lds[x * LS + c], static_cast<T>(kv));
         }
         __syncthreads();
@@ -170,49 +186,52 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
 #pragma unroll 4
     for (int i = tid; i < N1 * ZC; i += NT) {
         const int x = i / ZC, c = i % ZC;
-        data[x * sx + jbase + c] = lds[x * LS + c];
+        data[static_cast<size_t>(x) * sx + jbase + c] = lds[x * LS + c];
     }
 }
 
 template <typename T, int N, int MODE>
-hipError_t plane_go(const FftGeom& g, const PlaneJob<T>* jobs, int njobs, hipStream_t s)
+hipError_t plane_go(const FftGeom& g, const PlaneJob<T>* jobs, const PlaneJob<T>* table, int njobs, hipStream_t s)
 {
     dim3 grid(g.m1, njobs);
-    hipLaunchKernelGGL((plane_kernel<T, N, MODE>), grid, dim3(plane_threads<N>()), 0, s, jobs[0], jobs[njobs > 1 ? 1 : 0],
-                       static_cast<const V2<T>*>(g.tw_yz));
+    const PlaneJob<T> none{};
+    hipLaunchKernelGGL((plane_kernel<T, N, MODE>), grid, dim3(plane_threads<N>()), 0, s, jobs ? jobs[0] : none,
+                       jobs ? jobs[njobs > 1 ? 1 : 0] : none, table, static_cast<const V2<T>*>(g.tw_yz));
     return hipGetLastError();
 }
 
 template <typename T, int N>
-hipError_t plane_mode(const FftGeom& g, int mode, const PlaneJob<T>* jobs, int njobs, hipStream_t s)
+hipError_t plane_mode(const FftGeom& g, int mode, const PlaneJob<T>* jobs, const PlaneJob<T>* table, int njobs, hipStream_t s)
 {
     switch (mode) {
-    case PLANE_FWD: return plane_go<T, N, PLANE_FWD>(g, jobs, njobs, s);
-    case PLANE_INV: return plane_go<T, N, PLANE_INV>(g, jobs, njobs, s);
-    case PLANE_INIT: return plane_go<T, N, PLANE_INIT>(g, jobs, njobs, s);
-    case PLANE_STEP: return plane_go<T, N, PLANE_STEP>(g, jobs, njobs, s);
-    case PLANE_LAST: return plane_go<T, N, PLANE_LAST>(g, jobs, njobs, s);
+    case PLANE_FWD: return plane_go<T, N, PLANE_FWD>(g, jobs, table, njobs, s);
+    case PLANE_INV: return plane_go<T, N, PLANE_INV>(g, jobs, table, njobs, s);
+    case PLANE_INIT: return plane_go<T, N, PLANE_INIT>(g, jobs, table, njobs, s);
+    case PLANE_STEP: return plane_go<T, N, PLANE_STEP>(g, jobs, table, njobs, s);
+    case PLANE_LAST: return plane_go<T, N, PLANE_LAST>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T, int N1, int MODE>
-hipError_t pencil_go(const FftGeom& g, const PencilJob<T>* jobs, int njobs, hipStream_t s)
+hipError_t pencil_go(const FftGeom& g, const PencilJob<T>* jobs, const PencilJob<T>* table, int njobs, hipStream_t s)
 {
     const size_t tiles = static_cast<size_t>(g.m2) * g.m3 / PENCIL_ZC;
     dim3 grid(static_cast<unsigned>(tiles), njobs);
-    hipLaunchKernelGGL((pencil_kernel<T, N1, MODE>), grid, dim3(pencil_threads<N1>()), 0, s, jobs[0], jobs[njobs > 1 ? 1 : 0], g);
+    const PencilJob<T> none{};
+    hipLaunchKernelGGL((pencil_kernel<T, N1, MODE>), grid, dim3(pencil_threads<N1>()), 0, s, jobs ? jobs[0] : none,
+                       jobs ? jobs[njobs > 1 ? 1 : 0] : none, table, g);
     return hipGetLastError();
 }
 
 template <typename T, int N1>
-hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, int njobs, hipStream_t s)
+hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, const PencilJob<T>* table, int njobs, hipStream_t s)
 {
     switch (mode) {
-    case PENCIL_FWD_MUL_INV: return pencil_go<T, N1, PENCIL_FWD_MUL_INV>(g, jobs, njobs, s);
-    case PENCIL_FWD: return pencil_go<T, N1, PENCIL_FWD>(g, jobs, njobs, s);
-    case PENCIL_INV: return pencil_go<T, N1, PENCIL_INV>(g, jobs, njobs, s);
-    case PENCIL_MUL: return pencil_go<T, N1, PENCIL_MUL>(g, jobs, njobs, s);
+    case PENCIL_FWD_MUL_INV: return pencil_go<T, N1, PENCIL_FWD_MUL_INV>(g, jobs, table, njobs, s);
+    case PENCIL_FWD: return pencil_go<T, N1, PENCIL_FWD>(g, jobs, table, njobs, s);
+    case PENCIL_INV: return pencil_go<T, N1, PENCIL_INV>(g, jobs, table, njobs, s);
+    case PENCIL_MUL: return pencil_go<T, N1, PENCIL_MUL>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -228,6 +247,13 @@ bool in_sizes(int n)
 
 } // namespace
 
+bool line_supported(int m1, int m2, int m3)
+{
+    auto ok = [](int n) { return n == 192 || n == 256 || in_sizes(n); };
+    if (!ok(m1) || !ok(m2) || !ok(m3)) return false;
+    return (m2 * m3) % LINE_TC == 0 && (m1 * m3) % LINE_TC == 0 && (m1 * m2) % LINE_TC == 0;
+}
+
 bool fft_supported(int m1, int m2, int m3, bool fp32)
 {
     if (!in_sizes(m1) || !in_sizes(m2) || m2 != m3) return false;
@@ -237,44 +263,69 @@ bool fft_supported(int m1, int m2, int m3, bool fp32)
 }
 
 template <typename T>
-hipError_t launch_plane(const FftGeom& g, int mode, const PlaneJob<T>* jobs, int njobs, hipStream_t s)
+hipError_t plane_any(const FftGeom& g, int mode, const PlaneJob<T>* jobs, const PlaneJob<T>* table, int njobs, hipStream_t s)
 {
     switch (g.m2) {
-    case 8: return plane_mode<T, 8>(g, mode, jobs, njobs, s);
-    case 12: return plane_mode<T, 12>(g, mode, jobs, njobs, s);
-    case 16: return plane_mode<T, 16>(g, mode, jobs, njobs, s);
-    case 24: return plane_mode<T, 24>(g, mode, jobs, njobs, s);
-    case 32: return plane_mode<T, 32>(g, mode, jobs, njobs, s);
-    case 48: return plane_mode<T, 48>(g, mode, jobs, njobs, s);
-    case 64: return plane_mode<T, 64>(g, mode, jobs, njobs, s);
-    case 96: return plane_mode<T, 96>(g, mode, jobs, njobs, s);
+    case 8: return plane_mode<T, 8>(g, mode, jobs, table, njobs, s);
+    case 12: return plane_mode<T, 12>(g, mode, jobs, table, njobs, s);
+    case 16: return plane_mode<T, 16>(g, mode, jobs, table, njobs, s);
+    case 24: return plane_mode<T, 24>(g, mode, jobs, table, njobs, s);
+    case 32: return plane_mode<T, 32>(g, mode, jobs, table, njobs, s);
+    case 48: return plane_mode<T, 48>(g, mode, jobs, table, njobs, s);
+    case 64: return plane_mode<T, 64>(g, mode, jobs, table, njobs, s);
+    case 96: return plane_mode<T, 96>(g, mode, jobs, table, njobs, s);
     case 128:
-        if constexpr (sizeof(T) == 4) return plane_mode<T, 128>(g, mode, jobs, njobs, s);
+        if constexpr (sizeof(T) == 4) return plane_mode<T, 128>(g, mode, jobs, table, njobs, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
-hipError_t launch_pencil(const FftGeom& g, int mode, const PencilJob<T>* jobs, int njobs, hipStream_t s)
+hipError_t pencil_any(const FftGeom& g, int mode, const PencilJob<T>* jobs, const PencilJob<T>* table, int njobs, hipStream_t s)
 {
     switch (g.m1) {
-    case 8: return pencil_mode<T, 8>(g, mode, jobs, njobs, s);
-    case 12: return pencil_mode<T, 12>(g, mode, jobs, njobs, s);
-    case 16: return pencil_mode<T, 16>(g, mode, jobs, njobs, s);
-    case 24: return pencil_mode<T, 24>(g, mode, jobs, njobs, s);
-    case 32: return pencil_mode<T, 32>(g, mode, jobs, njobs, s);
-    case 48: return pencil_mode<T, 48>(g, mode, jobs, njobs, s);
-    case 64: return pencil_mode<T, 64>(g, mode, jobs, njobs, s);
-    case 96: return pencil_mode<T, 96>(g, mode, jobs, njobs, s);
-    case 128: return pencil_mode<T, 128>(g, mode, jobs, njobs, s);
+    case 8: return pencil_mode<T, 8>(g, mode, jobs, table, njobs, s);
+    case 12: return pencil_mode<T, 12>(g, mode, jobs, table, njobs, s);
+    case 16: return pencil_mode<T, 16>(g, mode, jobs, table, njobs, s);
+    case 24: return pencil_mode<T, 24>(g, mode, jobs, table, njobs, s);
+    case 32: return pencil_mode<T, 32>(g, mode, jobs, table, njobs, s);
+    case 48: return pencil_mode<T, 48>(g, mode, jobs, table, njobs, s);
+    case 64: return pencil_mode<T, 64>(g, mode, jobs, table, njobs, s);
+    case 96: return pencil_mode<T, 96>(g, mode, jobs, table, njobs, s);
+    case 128: return pencil_mode<T, 128>(g, mode, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <typename T>
+hipError_t launch_plane(const FftGeom& g, int mode, const PlaneJob<T>* jobs, int njobs, hipStream_t s)
+{
+    return plane_any<T>(g, mode, jobs, nullptr, njobs, s);
+}
+template <typename T>
+hipError_t launch_plane_table(const FftGeom& g, int mode, const PlaneJob<T>* dev_jobs, int njobs, hipStream_t s)
+{
+    return plane_any<T>(g, mode, nullptr, dev_jobs, njobs, s);
+}
+template <typename T>
+hipError_t launch_pencil(const FftGeom& g, int mode, const PencilJob<T>* jobs, int njobs, hipStream_t s)
+{
+    return pencil_any<T>(g, mode, jobs, nullptr, njobs, s);
+}
+template <typename T>
+hipError_t launch_pencil_table(const FftGeom& g, int mode, const PencilJob<T>* dev_jobs, int njobs, hipStream_t s)
+{
+    return pencil_any<T>(g, mode, nullptr, dev_jobs, njobs, s);
 }
 
 template hipError_t launch_plane<double>(const FftGeom&, int, const PlaneJob<double>*, int, hipStream_t);
 template hipError_t launch_plane<float>(const FftGeom&, int, const PlaneJob<float>*, int, hipStream_t);
 template hipError_t launch_pencil<double>(const FftGeom&, int, const PencilJob<double>*, int, hipStream_t);
 template hipError_t launch_pencil<float>(const FftGeom&, int, const PencilJob<float>*, int, hipStream_t);
+template hipError_t launch_plane_table<double>(const FftGeom&, int, const PlaneJob<double>*, int, hipStream_t);
+template hipError_t launch_plane_table<float>(const FftGeom&, int, const PlaneJob<float>*, int, hipStream_t);
+template hipError_t launch_pencil_table<double>(const FftGeom&, int, const PencilJob<double>*, int, hipStream_t);
+template hipError_t launch_pencil_table<float>(const FftGeom&, int, const PencilJob<float>*, int, hipStream_t);
 
 } // namespace fts

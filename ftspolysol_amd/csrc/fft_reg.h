@@ -1,0 +1,127 @@
+// fft_reg.h -- register-resident line FFTs.
+//
+// A length-N line is split N = N1 * N2 over N2 threads: thread p holds the N1
+// points x[N2*j + p].  The forward transform is
+//   (A) DFT_N1 over j in registers -> Y[p][k1]
+//   (B) twiddle Y[p][k1] *= W_N^{p k1}
+//   (C) one LDS exchange, then DFT_N2 over p in registers -> X[k1 + N1 k2]
+// (Cooley-Tukey with N = N1 N2), and the inverse runs the same steps backwards,
+// starting from the k-ordered values a thread holds after (C).  A forward,
+// k-space multiply and inverse therefore cost two LDS exchanges, where the
+// staged Stockham line FFT needs a read and a write of LDS per radix stage.
+// The in-register DFTs are compile-time Cooley-Tukey trees over the radix-2/3/4/
+// 6/8 butterflies with constant twiddles.
+#pragma once
+
+#include "fft.h"
+
+namespace fts {
+namespace reg {
+
+// ---- compile-time sin / cos of 2 pi m / N --------------------------------------
+// Quadrant reduction to |r| <= pi/4, then Taylor series (|error| ~ 1e-17).
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double taylor_sin(double r)
+{
+    double term = r, sum = r;
+    for (int n = 1; n < 16; ++n) {
+        term *= -r * r / ((2.0 * n) * (2.0 * n + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_cos(double r)
+{
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 16; ++n) {
+        term *= -r * r / ((2.0 * n - 1.0) * (2.0 * n));
+        sum += term;
+    }
+    return sum;
+}
+// returns cos (which = 0) or sin (which = 1) of 2 pi m / N
+constexpr double turn(int m, int N, int which)
+{
+    m %= N;
+    if (m < 0) m += N;
+    // quadrant q = round(4 m / N), remainder angle r = 2 pi (m - q N / 4) / N
+    const int q = (8 * m + N) / (2 * N);
+    const double r = 2.0 * kPi * (static_cast<double>(m) - 0.25 * q * N) / N;
+    const double c = taylor_cos(r), s = taylor_sin(r);
+    const int qq = q % 4;
+    const double cs = qq == 0 ? c : qq == 1 ? -s : qq == 2 ? -c : s;
+    const double sn = qq == 0 ? s : qq == 1 ? c : qq == 2 ? -s : -c;
+    return which == 0 ? cs : sn;
+}
+
+// W_N^{m} with the transform's sign: e^{SIGN 2 pi i m / N}
+template <typename V, int N, int M, int SIGN> __device__ __forceinline__ V tw_const()
+{
+    using T = decltype(V::x);
+    constexpr double c = turn(M, N, 0);
+    constexpr double s = SIGN * turn(M, N, 1);
+    return cmk<V>(static_cast<T>(c), static_cast<T>(s));
+}
+
+template <int N> struct Split;            // N = A * B with a direct radix A
+template <> struct Split<12> { static constexpr int A = 4, B = 3; };
+template <> struct Split<16> { static constexpr int A = 4, B = 4; };
+template <> struct Split<24> { static constexpr int A = 8, B = 3; };
+template <> struct Split<32> { static constexpr int A = 8, B = 4; };
+
+template <int N> constexpr bool is_direct() { return N == 2 || N == 3 || N == 4 || N == 6 || N == 8; }
+
+// In-register DFT of length N, natural order in and out.
+template <typename V, int N, int SIGN> struct RegDft {
+    __device__ __forceinline__ static void run(V* x)
+    {
+        if constexpr (is_direct<N>()) {
+            Dft<V, N, SIGN>::run(x);
+        } else {
+            constexpr int A = Split<N>::A, B = Split<N>::B;
+            // n = B n1 + n2, k = k1 + A k2
+            V y[B][A];
+#pragma unroll
+            for (int n2 = 0; n2 < B; ++n2) {
+#pragma unroll
+                for (int n1 = 0; n1 < A; ++n1) y[n2][n1] = x[B * n1 + n2];
+                RegDft<V, A, SIGN>::run(y[n2]);
+            }
+            twiddle_all<A, B>(y, std::make_integer_sequence<int, A * B>{});
+#pragma unroll
+            for (int k1 = 0; k1 < A; ++k1) {
+                V z[B];
+#pragma unroll
+                for (int n2 = 0; n2 < B; ++n2) z[n2] = y[n2][k1];
+                RegDft<V, B, SIGN>::run(z);
+#pragma unroll
+                for (int k2 = 0; k2 < B; ++k2) x[k1 + A * k2] = z[k2];
+            }
+        }
+    }
+
+    template <int A, int B, int... I>
+    __device__ __forceinline__ static void twiddle_all(V (&y)[B][A], std::integer_sequence<int, I...>)
+    {
+        (twiddle_one<A, B, I>(y), ...);
+    }
+    template <int A, int B, int I> __device__ __forceinline__ static void twiddle_one(V (&y)[B][A])
+    {
+        constexpr int n2 = I / A, k1 = I % A;
+        if constexpr (n2 * k1 != 0) y[n2][k1] = cmul(y[n2][k1], tw_const<V, N, n2 * k1, SIGN>());
+    }
+};
+
+// register split of a line length over threads: N = N1 * N2
+template <int N> struct LineSplit { static constexpr int N1 = 0, N2 = 0; };
+template <> struct LineSplit<32> { static constexpr int N1 = 8, N2 = 4; };
+template <> struct LineSplit<48> { static constexpr int N1 = 12, N2 = 4; };
+template <> struct LineSplit<64> { static constexpr int N1 = 16, N2 = 4; };
+template <> struct LineSplit<96> { static constexpr int N1 = 24, N2 = 4; };
+template <> struct LineSplit<128> { static constexpr int N1 = 32, N2 = 4; };
+template <> struct LineSplit<192> { static constexpr int N1 = 24, N2 = 8; };
+template <> struct LineSplit<256> { static constexpr int N1 = 32, N2 = 8; };
+template <int N> constexpr bool reg_line_ok() { return LineSplit<N>::N2 != 0; }
+
+} // namespace reg
+} // namespace fts

@@ -43,6 +43,11 @@ extern "C" {
 #define FTS_MAX_IONS 8
 #define FTS_MAX_TYPES 8
 
+/* bead-step FFT schedules (fts_system.fft_schedule, fts_info.fft_schedule) */
+#define FTS_SCHED_AUTO 0  /* time both at creation, keep the faster */
+#define FTS_SCHED_PLANE 1 /* pencil(x: FFT, x phi/M, IFFT) + plane(yz: IFFT, x e^{-W}, store q, FFT); needs m2 == m3, plane in LDS */
+#define FTS_SCHED_LINE 2  /* four line passes x, y, z, y; any supported lengths */
+
 /* stress_mask bits: components in the reference's fixed order (stress.cu:56-61) */
 #define FTS_STRESS_XX 0x01
 #define FTS_STRESS_YY 0x02
@@ -120,6 +125,7 @@ typedef struct fts_system {
     int precision;               /* FTS_PREC_* */
     unsigned long long seed;     /* Philox4x32-10 key */
     int stress_mask;             /* FTS_STRESS_* bits; 0 = no stress observable */
+    int fft_schedule;            /* FTS_SCHED_*: bead-step pass schedule; 0 = timed at creation */
 } fts_system;
 
 /* observableRecord (observable.h:7-13), with the partition functions beside it. */
@@ -152,7 +158,7 @@ typedef struct fts_info {
     int n_segment_classes; /* distinct (type, charge) bead classes: e^{-W} fields kept per species */
     int precision;
     int scheme;
-    int pad_;
+    int fft_schedule; /* FTS_SCHED_PLANE or FTS_SCHED_LINE in use */
     double volume;
     double rho0;
     double short_range_self_pressure; /* param.cu:135-149 */
@@ -163,6 +169,7 @@ typedef struct fts_info {
     double ion_valence[FTS_MAX_IONS];
     long long ion_count[FTS_MAX_IONS];
     double ion_smearing[FTS_MAX_IONS];
+    double tune_ms[2]; /* autotuner: time of 8 bead steps, plane and line schedule (0 if not timed) */
 } fts_info;
 
 /* Accumulated device timing of the propagator sweeps (HIP events on the engine's

@@ -83,8 +83,13 @@ def bad_create(lib_prefix, system):
 def test_hip_rejects_unsupported_mesh():
     err = bad_create("fts_", make_system(10, 4))
     assert err.code == FTS_INVALID and "mesh" in str(err)
-    err = bad_create("fts_", System(m=(32, 32, 16), dL=0.5, species=[dict(num_chains=1, types=[0], charges=[0.0])], xi=0.1))
+    err = bad_create("fts_", System(m=(32, 32, 20), dL=0.5, species=[dict(num_chains=1, types=[0], charges=[0.0])], xi=0.1))
     assert err.code == FTS_INVALID
+    # the plane schedule needs m2 == m3 and the plane in LDS: FP64 128^2 does not fit
+    from ftspolysol_amd.api import SCHED_PLANE
+    err = bad_create("fts_", System(m=(128, 128, 128), dL=0.5, species=[dict(num_chains=1, types=[0], charges=[0.0])], xi=0.1,
+                                    fft_schedule=SCHED_PLANE))
+    assert err.code == FTS_INVALID and "plane" in str(err)
 
 
 @pytest.mark.parametrize("prefix", ["fts_", "fto_"])

@@ -8,7 +8,7 @@ engine 1e-4 relative to FP64 for H, mu and stress, 2e-5 for a bead step.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
+from ftspolysol_amd.api import (SCHED_LINE, SCHED_PLANE, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
                                 FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
                                 SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
 from ftspolysol_amd.system import System, initial_fields, make_system
@@ -36,19 +36,40 @@ def test_fft_matches_numpy(m):
     assert rel(e.fft3d(x, +1), np.fft.ifftn(x) * x.size) < 1e-13
 
 
+@pytest.mark.parametrize("m", [(32, 24, 48), (128, 128, 128), (16, 192, 8), (256, 16, 32)])
+def test_fft_line_path_matches_numpy(m):
+    """Grids the plane pass cannot hold run every FFT as line passes."""
+    print(f"fft line path {m}", flush=True)
+    e = hip_engine(bare_system(m))
+    assert e.info.fft_schedule == SCHED_LINE
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    assert rel(e.fft3d(x, -1), np.fft.fftn(x)) < 1e-13
+    assert rel(e.fft3d(x, +1), np.fft.ifftn(x) * x.size) < 1e-13
+
+
+@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE])
 @pytest.mark.parametrize("m,bond", [((32, 32, 32), 1.3), ((96, 96, 96), 1.0), ((16, 24, 24), 0.7)])
-def test_bead_step_matches_oracle(m, bond):
-    print(f"bead step {m} b={bond}", flush=True)
-    s = bare_system(m)
+def test_bead_step_matches_oracle(m, bond, sched):
+    print(f"bead step {m} b={bond} sched={sched}", flush=True)
+    s = bare_system(m, fft_schedule=sched)
     g, o = hip_engine(s), oracle_engine(s)
+    assert g.info.fft_schedule == sched
     rng = np.random.default_rng(5)
     q = 1 + 0.1 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
     w = 0.2 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
     emw = np.exp(-w)
     ref = o.bead_step(q, emw, bond)
     assert rel(g.bead_step(q, emw, bond), ref) < 1e-13
-    s32 = bare_system(m, precision=PREC_FP32)
+    s32 = bare_system(m, precision=PREC_FP32, fft_schedule=sched)
     assert rel(hip_engine(s32).bead_step(q, emw, bond), ref) < 2e-5
+
+
+def test_autotuner_times_both_schedules():
+    e = hip_engine(make_system(96, 100, precision=PREC_FP32))
+    print("tune ms (plane, line):", list(e.info.tune_ms), "chosen", e.info.fft_schedule, flush=True)
+    assert e.info.fft_schedule in (SCHED_PLANE, SCHED_LINE)
+    assert e.info.tune_ms[0] > 0 and e.info.tune_ms[1] > 0
 
 
 def c1_like(n=32, N=20, **kw):
@@ -118,10 +139,11 @@ def test_measure_matches_oracle(which):
     assert max(r.values()) < 1e-10, r
 
 
-@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT])
-def test_trajectory_matches_oracle(scheme):
-    print(f"trajectory scheme={scheme}", flush=True)
-    s = c1_like(stress_mask=0x7F, scheme=scheme)
+@pytest.mark.parametrize("scheme,sched", [(SCHEME_PC_REFERENCE, SCHED_PLANE), (SCHEME_PC_REFERENCE, SCHED_LINE),
+                                          (SCHEME_EM, SCHED_PLANE), (SCHEME_SEMI_IMPLICIT, SCHED_LINE)])
+def test_trajectory_matches_oracle(scheme, sched):
+    print(f"trajectory scheme={scheme} sched={sched}", flush=True)
+    s = c1_like(stress_mask=0x7F, scheme=scheme, fft_schedule=sched)
     g, o = hip_engine(s), oracle_engine(s)
     om, ps = initial_fields(s, sigma=1e-3, cfg=1)
     for e in (g, o):
@@ -174,6 +196,18 @@ def test_fp32_engine_tracks_fp64_at_96():
     assert abs(g64.get_field(FIELD_RHO_C).mean()) < 1e-10
     r32 = g32.get_field(FIELD_RHO_P)
     assert rel(r32, rho_p) < 1e-3
+
+
+def test_noncubic_line_schedule_measure():
+    """m2 != m3 only runs with line passes; densities and observables still match."""
+    print("non-cubic 32x24x16 measure", flush=True)
+    sp = [dict(num_chains=6, types=[0] * 10, charges=[-0.5] * 10)]
+    s = System(m=(32, 24, 16), dL=0.5, species=sp, xi=0.05, lB=1.0, counterion=dict(auto=True), salt=dict(num_salt=12),
+               stress_mask=0x7F, seed=3)
+    g, o = hip_engine(s), oracle_engine(s)
+    set_random_fields([g, o], 1, s.m, 0.1, 45)
+    r = obs_rel(g.measure(), o.measure())
+    assert max(r.values()) < 1e-10, r
 
 
 def test_headline_measure_matches_oracle_64():
