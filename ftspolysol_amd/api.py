@@ -16,6 +16,7 @@ FTS_OK, FTS_NONFINITE, FTS_INVALID, FTS_DEVICE_ERROR = 0, 1, 2, 3
 SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 SCHED_AUTO, SCHED_PLANE, SCHED_LINE = 0, 1, 2
+SYMMETRY_AUTO, SYMMETRY_OFF = 0, 1
 MAX_SPECIES = MAX_IONS = MAX_TYPES = 8
 STRESS_ALL = 0x3F
 STRESS_PISO = 0x40
@@ -49,7 +50,8 @@ class SystemDesc(C.Structure):
                 ("default_smearing_length", C.c_double), ("counterions", CounterionDesc), ("salt", SaltDesc),
                 ("dt", C.c_double), ("n_lambda_omega", C.c_int), ("lambda_omega", C.POINTER(C.c_double)),
                 ("lambda_psi", C.c_double), ("noise", C.c_int), ("scheme", C.c_int), ("precision", C.c_int),
-                ("seed", C.c_ulonglong), ("stress_mask", C.c_int), ("fft_schedule", C.c_int)]
+                ("seed", C.c_ulonglong), ("stress_mask", C.c_int), ("fft_schedule", C.c_int),
+                ("chain_symmetry", C.c_int)]
 
 
 class Observables(C.Structure):
@@ -85,12 +87,22 @@ class Info(C.Structure):
                 ("mode_eigenvalue", C.c_double * MAX_TYPES), ("omega_coefficient", C.c_double * MAX_TYPES),
                 ("gamma_is_imag", C.c_double * MAX_TYPES), ("mode_O", (C.c_double * MAX_TYPES) * MAX_TYPES),
                 ("ion_valence", C.c_double * MAX_IONS), ("ion_count", C.c_longlong * MAX_IONS),
-                ("ion_smearing", C.c_double * MAX_IONS), ("tune_ms", C.c_double * 2)]
+                ("ion_smearing", C.c_double * MAX_IONS), ("tune_ms", C.c_double * 2),
+                ("mirrored_species", C.c_int)]
 
 
 class Timing(C.Structure):
     _fields_ = [("sweep_ms", C.c_double), ("bead_steps", C.c_longlong), ("sweeps", C.c_longlong),
-                ("algorithmic_bytes", C.c_double)]
+                ("algorithmic_bytes", C.c_double), ("step_launches", C.c_longlong)]
+
+
+MAX_BATCH_GROUPS = 16
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("n_replicas", C.c_int), ("n_groups", C.c_int), ("group_size", C.c_int * MAX_BATCH_GROUPS),
+                ("group_mesh", (C.c_int * 3) * MAX_BATCH_GROUPS), ("group_precision", C.c_int * MAX_BATCH_GROUPS),
+                ("group_schedule", C.c_int * MAX_BATCH_GROUPS), ("group_tune_ms", (C.c_double * 2) * MAX_BATCH_GROUPS)]
 
 
 class FtsError(RuntimeError):
@@ -117,6 +129,13 @@ def _bind(lib, prefix):
         "bead_step": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)]),
         "set_timing": (C.c_int, [vp, C.c_int]),
         "get_timing": (C.c_int, [vp, P(Timing)]),
+        "batch_create": (C.c_int, [P(SystemDesc), C.c_int, P(vp)]),
+        "batch_destroy": (None, [vp]),
+        "batch_get_info": (C.c_int, [vp, P(BatchInfo)]),
+        "batch_replica": (C.c_int, [vp, C.c_int, P(vp), P(C.c_int)]),
+        "batch_step": (C.c_int, [vp, C.c_int, C.c_int, P(Observables), C.c_int, P(C.c_int)]),
+        "batch_set_timing": (C.c_int, [vp, C.c_int]),
+        "batch_get_timing": (C.c_int, [vp, C.c_int, P(Timing)]),
     }
     fns = {}
     for name, (res, args) in sig.items():
@@ -137,12 +156,16 @@ def _dptr(a):
 class Engine:
     """One engine instance over a library exporting the fts C ABI."""
 
-    def __init__(self, lib, prefix, system):
+    def __init__(self, lib, prefix, system, handle=None, owner=None):
         self._fns = _bind(lib, prefix)
         self._keep = system  # keeps the ctypes arrays of the description alive
-        h = C.c_void_p()
-        rc = self._fns["create"](C.byref(system.desc), C.byref(h))
-        self._check(rc)
+        self._owner = owner  # a batch that owns the handle (borrowed replica)
+        if handle is None:
+            h = C.c_void_p()
+            rc = self._fns["create"](C.byref(system.desc), C.byref(h))
+            self._check(rc)
+        else:
+            h = handle
         self._h = h
         self.info = self.get_info()
         self.M = int(self.info.grid_size)
@@ -155,7 +178,8 @@ class Engine:
 
     def close(self):
         if getattr(self, "_h", None):
-            self._fns["destroy"](self._h)
+            if self._owner is None:
+                self._fns["destroy"](self._h)
             self._h = None
 
     def __del__(self):
@@ -230,8 +254,72 @@ class Engine:
     def get_timing(self):
         t = Timing()
         self._check(self._fns["get_timing"](self._h, C.byref(t)))
-        return {"sweep_ms": t.sweep_ms, "bead_steps": t.bead_steps, "sweeps": t.sweeps,
-                "algorithmic_bytes": t.algorithmic_bytes}
+        return _timing_dict(t)
+
+
+def _timing_dict(t):
+    return {"sweep_ms": t.sweep_ms, "bead_steps": t.bead_steps, "sweeps": t.sweeps,
+            "algorithmic_bytes": t.algorithmic_bytes, "step_launches": t.step_launches}
+
+
+class Batch:
+    """A replica batch (fts_batch_*): replicas[r] are borrowed Engines on the batch's replicas."""
+
+    def __init__(self, lib, prefix, systems):
+        self._fns = _bind(lib, prefix)
+        self._keep = list(systems)
+        arr = (SystemDesc * len(systems))(*[s.desc for s in systems])
+        h = C.c_void_p()
+        rc = self._fns["batch_create"](arr, len(systems), C.byref(h))
+        self._check(rc)
+        self._h = h
+        self.replicas = []
+        self.group_of = []
+        for r, s in enumerate(systems):
+            eh, grp = C.c_void_p(), C.c_int()
+            self._check(self._fns["batch_replica"](self._h, r, C.byref(eh), C.byref(grp)))
+            self.replicas.append(Engine(lib, prefix, s, handle=eh, owner=self))
+            self.group_of.append(grp.value)
+        self.info = BatchInfo()
+        self._check(self._fns["batch_get_info"](self._h, C.byref(self.info)))
+
+    def _check(self, rc):
+        if rc != FTS_OK:
+            msg = self._fns["last_error"]()
+            raise FtsError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for e in self.replicas:
+                e._h = None
+            self._fns["batch_destroy"](self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n, record_every=0, capacity=None):
+        """n steps of every replica; returns one list of records per replica."""
+        if capacity is None:
+            capacity = n // record_every if record_every > 0 else 0
+        R = len(self.replicas)
+        recs = (Observables * max(1, R * capacity))()
+        nrec = (C.c_int * R)()
+        rc = self._fns["batch_step"](self._h, n, record_every, recs, capacity, nrec)
+        out = [[recs[r * capacity + i].as_dict() for i in range(nrec[r])] for r in range(R)]
+        self._check(rc)
+        return out
+
+    def set_timing(self, enabled):
+        self._check(self._fns["batch_set_timing"](self._h, int(enabled)))
+
+    def get_timing(self, group=0):
+        t = Timing()
+        self._check(self._fns["batch_get_timing"](self._h, group, C.byref(t)))
+        return _timing_dict(t)
 
 
 _HIP_LIB = None
@@ -255,3 +343,8 @@ def load_hip_library():
 def hip_engine(system):
     """An engine on the GPU (libfts_hip.so, symbols fts_*)."""
     return Engine(load_hip_library(), "fts_", system)
+
+
+def hip_batch(systems):
+    """A replica batch on the GPU."""
+    return Batch(load_hip_library(), "fts_", systems)

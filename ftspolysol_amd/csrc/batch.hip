@@ -1,0 +1,577 @@
+// batch.hip -- replica batches: many independent CL systems stepping together.
+//
+// A single 96^3 system cannot fill the chip during its propagator sweeps: one
+// plane pass is 96 workgroups on 256 CUs and each bead step waits for the one
+// before it.  A batch groups replicas by mesh, precision and grid spacing; the
+// replicas of a group share a stream, and every bead-step launch of a group
+// carries the jobs (sweep directions) of all its replicas at that bead index,
+// read from a device-resident job table built once.  Replicas with shorter
+// chains drop out of the later launches.  Groups run concurrently, each from
+// its own host thread on its own stream.  Everything outside the sweeps (the
+// smoothing FFTs, ions, drifts and the CL update) runs per replica on the group
+// stream; the host waits once per density evaluation for the Q checks of all
+// replicas of a group and once per step for their field checks.
+#include <memory>
+#include <thread>
+
+#include "engine.h"
+
+namespace {
+
+enum { L_PLANE = 0, L_PENCIL = 1, L_LINE = 2 };
+
+struct SweepLaunch {
+    int kind, mode, axis;
+    size_t off;
+    int count;
+};
+
+// The bead-step launches of one species index over a group: INIT, then per bead
+// index s the launches of that step; all jobs live in three device arrays.
+struct SweepTables {
+    int p = -1, sched = 0, steps = 0;
+    std::vector<SweepLaunch> launches;
+    size_t init_launches = 0;
+    long long jobs = 0;      // bead steps (jobs summed over the step launches)
+    void* dplane = nullptr;
+    void* dpencil = nullptr;
+    void* dline = nullptr;
+    ~SweepTables()
+    {
+        for (void* q : {dplane, dpencil, dline})
+            if (q) (void)hipFree(q);
+    }
+};
+
+struct Group {
+    int m1 = 0, m2 = 0, m3 = 0, precision = 0;
+    double dL = 0;
+    hipStream_t stream = nullptr;
+    std::vector<fts_engine*> reps;
+    std::vector<int> ids; // batch index of each replica
+    int sched = FTS_SCHED_PLANE;
+    bool plane_ok = true;
+    double tune_ms[2] = {0.0, 0.0};
+    std::vector<std::unique_ptr<SweepTables>> tables; // full sweeps, per species index and schedule
+    double2* hQ = nullptr;                             // pinned [reps][FTS_MAX_SPECIES]
+    int* hflag = nullptr;                              // pinned [reps]
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> pool;
+    fts_timing tstat{};
+    int rc = FTS_OK;
+    std::string err;
+};
+
+template <typename T> struct JobArrays {
+    std::vector<PlaneJob<T>> plane;
+    std::vector<PencilJob<T>> pencil;
+    std::vector<LineJob<T>> line;
+};
+
+// per-replica pointers of one species' sweep
+template <typename T> struct RepView {
+    fts_engine* e;
+    const HostSpecies* sp;
+    V2<T>*qf, *qb, *E, *tf, *tb;
+    double *nF[2], *nB[2], *lcF, *lcB;
+    size_t Ms;
+    V2<T>* E_of(int s) const { return E + static_cast<size_t>(sp->cls_of_bead[static_cast<size_t>(s)]) * Ms; }
+    KOp bond_op(int bond) const
+    {
+        const size_t ks = static_cast<size_t>(e->m1 + e->m2 + e->m3);
+        return e->kop_sep(e->sep_bond.p + static_cast<size_t>(sp->bond_cls[static_cast<size_t>(bond)]) * ks, 1.0 / static_cast<double>(e->M));
+    }
+};
+
+template <typename T> RepView<T> view(fts_engine* e, int p)
+{
+    using Cv = V2<T>;
+    RepView<T> v;
+    v.e = e;
+    v.sp = &e->species[static_cast<size_t>(p)];
+    v.qf = static_cast<Cv*>(e->qF);
+    v.qb = static_cast<Cv*>(e->qB);
+    v.E = static_cast<Cv*>(e->E);
+    v.tf = static_cast<Cv*>(e->tmpF);
+    v.tb = static_cast<Cv*>(e->tmpB);
+    v.Ms = static_cast<size_t>(e->M);
+    v.nF[0] = e->norms.p;
+    v.nF[1] = e->norms.p + e->norm_stride;
+    v.nB[0] = e->norms.p + 2 * e->norm_stride;
+    v.nB[1] = e->norms.p + 3 * e->norm_stride;
+    v.lcF = e->logc.p;
+    v.lcB = e->logc.p + e->Nmax;
+    return v;
+}
+
+template <typename X> int upload(const std::vector<X>& v, void** dst)
+{
+    if (v.empty()) return FTS_OK;
+    HIP_TRY(hipMalloc(dst, sizeof(X) * v.size()));
+    HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
+    return FTS_OK;
+}
+
+// Job tables of species index p over the group for max_steps bead steps (all if
+// max_steps < 0).  The bead-step chain of run_sweeps, with the jobs of every
+// replica whose chain still has bead index s + 1 in launch s.  A replica whose
+// chain ends before the group's longest takes the non-final pass form on its
+// last step (the extra forward FFT writes only its scratch field).
+template <typename T> int build_tables(Group& g, int p, int sched, int max_steps, SweepTables& tb)
+{
+    std::vector<RepView<T>> act;
+    for (fts_engine* e : g.reps)
+        if (static_cast<int>(e->species.size()) > p && e->species[static_cast<size_t>(p)].N > 1) act.push_back(view<T>(e, p));
+    std::stable_sort(act.begin(), act.end(), [](const RepView<T>& a, const RepView<T>& b) { return a.sp->N > b.sp->N; });
+    tb.p = p;
+    tb.sched = sched;
+    tb.launches.clear();
+    tb.jobs = 0;
+    if (act.empty()) {
+        tb.steps = 0;
+        return FTS_OK;
+    }
+    const int maxN = act[0].sp->N;
+    const int nsteps = max_steps < 0 ? maxN - 1 : std::min(maxN - 1, max_steps);
+    tb.steps = nsteps;
+    JobArrays<T> J;
+    auto add = [&](int kind, int mode, int axis, size_t off, size_t end) {
+        tb.launches.push_back(SweepLaunch{kind, mode, axis, off, static_cast<int>(end - off)});
+    };
+    if (sched == FTS_SCHED_PLANE) {
+        size_t o = J.plane.size();
+        for (const RepView<T>& r : act) {
+            const int N = r.sp->N;
+            J.plane.push_back(PlaneJob<T>{nullptr, r.tf, r.qf, r.E_of(0), nullptr, r.nF[0], r.lcF});
+            if (!r.sp->mirror)
+                J.plane.push_back(PlaneJob<T>{nullptr, r.tb, r.qb + static_cast<size_t>(N - 1) * r.Ms, r.E_of(N - 1), nullptr, r.nB[0], r.lcB + (N - 1)});
+        }
+        add(L_PLANE, PLANE_INIT, 0, o, J.plane.size());
+        tb.init_launches = tb.launches.size();
+        for (int s = 0; s < nsteps; ++s) {
+            const size_t op = J.pencil.size();
+            for (const RepView<T>& r : act) {
+                const int N = r.sp->N;
+                if (N - 1 <= s) break;
+                J.pencil.push_back(PencilJob<T>{r.tf, r.bond_op(s)});
+                if (!r.sp->mirror) J.pencil.push_back(PencilJob<T>{r.tb, r.bond_op(N - 2 - s)});
+            }
+            add(L_PENCIL, PENCIL_FWD_MUL_INV, 0, op, J.pencil.size());
+            const size_t ol = J.plane.size();
+            for (const RepView<T>& r : act) {
+                const int N = r.sp->N;
+                if (N - 1 <= s) break;
+                const int sf = s + 1, sb = N - 2 - s;
+                J.plane.push_back(PlaneJob<T>{r.tf, r.tf, r.qf + static_cast<size_t>(sf) * r.Ms, r.E_of(sf), r.nF[s % 2], r.nF[(s + 1) % 2], r.lcF + sf});
+                if (!r.sp->mirror)
+                    J.plane.push_back(PlaneJob<T>{r.tb, r.tb, r.qb + static_cast<size_t>(sb) * r.Ms, r.E_of(sb), r.nB[s % 2], r.nB[(s + 1) % 2], r.lcB + sb});
+            }
+            tb.jobs += static_cast<long long>(J.plane.size() - ol);
+            add(L_PLANE, s == maxN - 2 ? PLANE_LAST : PLANE_STEP, 0, ol, J.plane.size());
+        }
+    } else {
+        size_t o = J.line.size();
+        for (const RepView<T>& r : act) {
+            const int N = r.sp->N;
+            J.line.push_back(LineJob<T>{r.tf, r.qf, r.E_of(0), KOp{}, nullptr, r.nF[0], r.lcF});
+            if (!r.sp->mirror)
+                J.line.push_back(LineJob<T>{r.tb, r.qb + static_cast<size_t>(N - 1) * r.Ms, r.E_of(N - 1), KOp{}, nullptr, r.nB[0], r.lcB + (N - 1)});
+        }
+        add(L_LINE, LINE_E_FWD, 2, o, J.line.size());
+        // plain y-pass jobs of the replicas active at each step: the prefix of ybase
+        std::vector<size_t> ybase_end;
+        const size_t ybase = J.line.size();
+        for (const RepView<T>& r : act) {
+            J.line.push_back(LineJob<T>{r.tf});
+            if (!r.sp->mirror) J.line.push_back(LineJob<T>{r.tb});
+            ybase_end.push_back(J.line.size());
+        }
+        add(L_LINE, LINE_FWD, 1, ybase, J.line.size());
+        tb.init_launches = tb.launches.size();
+        for (int s = 0; s < nsteps; ++s) {
+            size_t nact = 0;
+            while (nact < act.size() && act[nact].sp->N - 1 > s) ++nact;
+            const size_t yend = ybase_end[nact - 1];
+            const size_t ox = J.line.size();
+            for (size_t k = 0; k < nact; ++k) {
+                const RepView<T>& r = act[k];
+                J.line.push_back(LineJob<T>{r.tf, nullptr, nullptr, r.bond_op(s)});
+                if (!r.sp->mirror) J.line.push_back(LineJob<T>{r.tb, nullptr, nullptr, r.bond_op(r.sp->N - 2 - s)});
+            }
+            add(L_LINE, LINE_FWD_MUL_INV, 0, ox, J.line.size());
+            add(L_LINE, LINE_INV, 1, ybase, yend);
+            const bool last = s == maxN - 2;
+            const size_t oz = J.line.size();
+            for (size_t k = 0; k < nact; ++k) {
+                const RepView<T>& r = act[k];
+                const int N = r.sp->N;
+                const int sf = s + 1, sb = N - 2 - s;
+                J.line.push_back(LineJob<T>{r.tf, r.qf + static_cast<size_t>(sf) * r.Ms, r.E_of(sf), KOp{}, r.nF[s % 2], r.nF[(s + 1) % 2], r.lcF + sf});
+                if (!r.sp->mirror)
+                    J.line.push_back(LineJob<T>{r.tb, r.qb + static_cast<size_t>(sb) * r.Ms, r.E_of(sb), KOp{}, r.nB[s % 2], r.nB[(s + 1) % 2], r.lcB + sb});
+            }
+            tb.jobs += static_cast<long long>(J.line.size() - oz);
+            add(L_LINE, last ? LINE_INV_E : LINE_INV_E_FWD, 2, oz, J.line.size());
+            if (!last) add(L_LINE, LINE_FWD, 1, ybase, yend);
+        }
+    }
+    int rc;
+    if ((rc = upload(J.plane, &tb.dplane)) || (rc = upload(J.pencil, &tb.dpencil)) || (rc = upload(J.line, &tb.dline))) return rc;
+    return FTS_OK;
+}
+
+template <typename T> int run_tables(Group& g, const SweepTables& tb, size_t first, size_t last)
+{
+    fts_engine* e0 = g.reps[0];
+    const FftGeom& geo = sizeof(T) == 8 ? e0->g64 : e0->g32;
+    const bool d = sizeof(T) == 8;
+    const void* tw[3] = {d ? static_cast<const void*>(e0->tw64_x.p) : static_cast<const void*>(e0->tw32_x.p),
+                         d ? static_cast<const void*>(e0->tw64_yz.p) : static_cast<const void*>(e0->tw32_yz.p),
+                         d ? static_cast<const void*>(e0->tw64_z.p) : static_cast<const void*>(e0->tw32_z.p)};
+    for (size_t i = first; i < last; ++i) {
+        const SweepLaunch& L = tb.launches[i];
+        if (L.count <= 0) continue;
+        switch (L.kind) {
+        case L_PLANE: HIP_TRY(launch_plane_table<T>(geo, L.mode, static_cast<const PlaneJob<T>*>(tb.dplane) + L.off, L.count, g.stream)); break;
+        case L_PENCIL: HIP_TRY(launch_pencil_table<T>(geo, L.mode, static_cast<const PencilJob<T>*>(tb.dpencil) + L.off, L.count, g.stream)); break;
+        default: HIP_TRY(launch_line_table<T>(geo, L.axis, L.mode, tw[L.axis], static_cast<const LineJob<T>*>(tb.dline) + L.off, L.count, g.stream));
+        }
+    }
+    return FTS_OK;
+}
+
+hipEvent_t group_event(Group& g)
+{
+    if (!g.pool.empty()) {
+        hipEvent_t e = g.pool.back();
+        g.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int resolve_group_timing(Group& g)
+{
+    if (g.pending.empty()) return FTS_OK;
+    HIP_TRY(hipEventSynchronize(g.pending.back().second));
+    for (auto& pr : g.pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
+        g.tstat.sweep_ms += ms;
+        g.pool.push_back(pr.first);
+        g.pool.push_back(pr.second);
+    }
+    g.pending.clear();
+    return FTS_OK;
+}
+
+template <typename T> int group_sweeps(Group& g, int p)
+{
+    SweepTables* tb = nullptr;
+    for (auto& t : g.tables)
+        if (t->p == p && t->sched == g.sched) tb = t.get();
+    if (!tb) {
+        g.tables.emplace_back(new SweepTables());
+        tb = g.tables.back().get();
+        int rc = build_tables<T>(g, p, g.sched, -1, *tb);
+        if (rc) return rc;
+    }
+    if (tb->steps == 0) return FTS_OK;
+    int rc = run_tables<T>(g, *tb, 0, tb->init_launches);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr;
+    if (g.timing) {
+        e0 = group_event(g);
+        if (!e0) return fail(FTS_DEVICE_ERROR, "hipEventCreate");
+        HIP_TRY(hipEventRecord(e0, g.stream));
+    }
+    if ((rc = run_tables<T>(g, *tb, tb->init_launches, tb->launches.size()))) return rc;
+    if (g.timing) {
+        hipEvent_t e1 = group_event(g);
+        if (!e1) return fail(FTS_DEVICE_ERROR, "hipEventCreate");
+        HIP_TRY(hipEventRecord(e1, g.stream));
+        g.pending.emplace_back(e0, e1);
+        g.tstat.sweeps += 1;
+        g.tstat.bead_steps += tb->jobs;
+        g.tstat.step_launches += tb->steps;
+        g.tstat.algorithmic_bytes += static_cast<double>(tb->jobs) * 2.0 * sizeof(V2<T>) * static_cast<double>(g.reps[0]->M);
+        if (g.pending.size() > 256 && (rc = resolve_group_timing(g))) return rc;
+    }
+    return FTS_OK;
+}
+
+// compute_densities of every replica of the group at (w, psi) or (w_bar, psi_bar)
+template <typename T> int group_densities(Group& g, bool bar)
+{
+    int rc;
+    size_t max_species = 0;
+    for (fts_engine* e : g.reps) {
+        if ((rc = e->densities_begin(bar ? e->omega_bar.p : e->omega.p, bar ? e->psi_bar.p : e->psi.p))) return rc;
+        max_species = std::max(max_species, e->species.size());
+    }
+    for (int p = 0; p < static_cast<int>(max_species); ++p) {
+        for (fts_engine* e : g.reps)
+            if (static_cast<int>(e->species.size()) > p && (rc = e->solve_begin<T>(p))) return rc;
+        if ((rc = group_sweeps<T>(g, p))) return rc;
+        for (fts_engine* e : g.reps)
+            if (static_cast<int>(e->species.size()) > p && (rc = e->solve_end<T>(p, true))) return rc;
+    }
+    for (fts_engine* e : g.reps)
+        if ((rc = e->densities_end())) return rc;
+    return FTS_OK;
+}
+
+bool finite_c(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
+
+int group_check_Q(Group& g, const char* what)
+{
+    int rc;
+    for (size_t r = 0; r < g.reps.size(); ++r)
+        if ((rc = g.reps[r]->q_queue(g.hQ + r * FTS_MAX_SPECIES))) return rc;
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    for (size_t r = 0; r < g.reps.size(); ++r)
+        for (size_t p = 0; p < g.reps[r]->species.size(); ++p)
+            if (!finite_c(g.hQ[r * FTS_MAX_SPECIES + p]))
+                return fail(FTS_NONFINITE, "replica " + std::to_string(g.ids[r]) + ": non-finite " + what + " at step " +
+                                               std::to_string(g.reps[r]->step_counter + 1));
+    return FTS_OK;
+}
+
+// One CL step of every replica of the group; rec[r] non-null where replica r records.
+template <typename T> int group_step(Group& g, int scheme, const std::vector<fts_observables*>& rec)
+{
+    int rc;
+    const size_t n = g.reps.size();
+    if ((rc = group_densities<T>(g, false)) || (rc = group_check_Q(g, "Q"))) return rc;
+    if (scheme == FTS_SCHEME_PC_REFERENCE) {
+        for (size_t r = 0; r < n; ++r)
+            if ((rc = g.reps[r]->pc_predict())) return rc;
+        if ((rc = group_densities<T>(g, true)) || (rc = group_check_Q(g, "Qbar"))) return rc;
+        for (size_t r = 0; r < n; ++r)
+            if ((rc = g.reps[r]->pc_correct(rec[r] != nullptr, rec[r]))) return rc;
+    } else {
+        for (size_t r = 0; r < n; ++r)
+            if ((rc = g.reps[r]->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT))) return rc;
+    }
+    for (size_t r = 0; r < n; ++r) {
+        ++g.reps[r]->step_counter;
+        if ((rc = g.reps[r]->finite_queue(g.hflag + r))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    for (size_t r = 0; r < n; ++r)
+        if (!g.hflag[r])
+            return fail(FTS_NONFINITE, "replica " + std::to_string(g.ids[r]) + ": updated fields have non-finite values at step " +
+                                           std::to_string(g.reps[r]->step_counter));
+    return FTS_OK;
+}
+
+// Schedule of a group's sweeps: both timed over 8 bead steps of species 0 with
+// all replicas in each launch, best of three.
+template <typename T> int group_autotune(Group& g)
+{
+    if (!g.plane_ok) {
+        g.sched = FTS_SCHED_LINE;
+        return FTS_OK;
+    }
+    SweepTables t[2];
+    int rc;
+    if ((rc = build_tables<T>(g, 0, FTS_SCHED_PLANE, 8, t[0])) || (rc = build_tables<T>(g, 0, FTS_SCHED_LINE, 8, t[1]))) return rc;
+    if (t[0].steps == 0) return FTS_OK;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    double best[2] = {1e30, 1e30};
+    for (int rep = 0; rep < 3 && rc == FTS_OK; ++rep)
+        for (int k = 0; k < 2 && rc == FTS_OK; ++k) {
+            if (hipEventRecord(a, g.stream) != hipSuccess) rc = fail(FTS_DEVICE_ERROR, "hipEventRecord");
+            if (!rc) rc = run_tables<T>(g, t[k], 0, t[k].launches.size());
+            if (!rc && (hipEventRecord(b, g.stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess)) rc = fail(FTS_DEVICE_ERROR, "hipEvent");
+            float ms = 0.0f;
+            if (!rc && hipEventElapsedTime(&ms, a, b) == hipSuccess) best[k] = std::min(best[k], static_cast<double>(ms));
+        }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    g.tune_ms[0] = best[0];
+    g.tune_ms[1] = best[1];
+    g.sched = best[0] <= best[1] ? FTS_SCHED_PLANE : FTS_SCHED_LINE;
+    return FTS_OK;
+}
+
+} // namespace
+
+struct fts_batch {
+    std::vector<fts_engine*> reps;
+    std::vector<int> group_of;
+    std::vector<std::unique_ptr<Group>> groups;
+    int scheme = 0;
+    ~fts_batch()
+    {
+        for (auto& g : groups) {
+            if (g->stream) (void)hipStreamSynchronize(g->stream);
+            for (auto& pr : g->pending) {
+                (void)hipEventDestroy(pr.first);
+                (void)hipEventDestroy(pr.second);
+            }
+            for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
+        }
+        for (fts_engine* e : reps) delete e; // engines do not own the group streams
+        for (auto& g : groups) {
+            g->tables.clear();
+            if (g->hQ) (void)hipHostFree(g->hQ);
+            if (g->hflag) (void)hipHostFree(g->hflag);
+            if (g->stream) (void)hipStreamDestroy(g->stream);
+        }
+    }
+};
+
+extern "C" {
+
+int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
+{
+    if (!systems || !out || n <= 0) return fail(FTS_INVALID, "batch: null argument or no replicas");
+    *out = nullptr;
+    for (int r = 1; r < n; ++r)
+        if (systems[r].scheme != systems[0].scheme) return fail(FTS_INVALID, "batch: every replica must use the same scheme");
+    std::unique_ptr<fts_batch> b(new fts_batch());
+    b->scheme = systems[0].scheme;
+    for (int r = 0; r < n; ++r) {
+        const fts_system& s = systems[r];
+        Group* g = nullptr;
+        for (auto& c : b->groups)
+            if (c->m1 == s.m1 && c->m2 == s.m2 && c->m3 == s.m3 && c->precision == s.precision && c->dL == s.dL) g = c.get();
+        if (!g) {
+            b->groups.emplace_back(new Group());
+            g = b->groups.back().get();
+            g->m1 = s.m1;
+            g->m2 = s.m2;
+            g->m3 = s.m3;
+            g->precision = s.precision;
+            g->dL = s.dL;
+            HIP_TRY(hipStreamCreate(&g->stream));
+        }
+        fts_engine* e = nullptr;
+        int rc = engine_create(&s, g->stream, false, &e);
+        if (rc) return fail(rc, "batch replica " + std::to_string(r) + ": " + g_error);
+        b->reps.push_back(e);
+        b->group_of.push_back(static_cast<int>(std::find_if(b->groups.begin(), b->groups.end(), [&](auto& c) { return c.get() == g; }) - b->groups.begin()));
+        g->reps.push_back(e);
+        g->ids.push_back(r);
+    }
+    for (auto& gp : b->groups) {
+        Group& g = *gp;
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hQ), sizeof(double2) * FTS_MAX_SPECIES * g.reps.size()));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hflag), sizeof(int) * g.reps.size()));
+        g.plane_ok = true;
+        for (fts_engine* e : g.reps) g.plane_ok = g.plane_ok && e->plane_ok;
+        const int want = systems[g.ids[0]].fft_schedule;
+        int rc = FTS_OK;
+        if (want == FTS_SCHED_AUTO) rc = g.precision == FTS_PREC_FP32 ? group_autotune<float>(g) : group_autotune<double>(g);
+        else if (want == FTS_SCHED_PLANE && !g.plane_ok) return fail(FTS_INVALID, "batch: the plane schedule does not fit this mesh");
+        else g.sched = want;
+        if (rc) return rc;
+        for (fts_engine* e : g.reps) {
+            e->sched = g.sched;
+            e->tune_ms[0] = g.tune_ms[0];
+            e->tune_ms[1] = g.tune_ms[1];
+        }
+    }
+    *out = b.release();
+    return FTS_OK;
+}
+
+void fts_batch_destroy(fts_batch* b) { delete b; }
+
+int fts_batch_get_info(fts_batch* b, fts_batch_info* o)
+{
+    if (!b || !o) return fail(FTS_INVALID, "null argument");
+    std::memset(o, 0, sizeof(*o));
+    o->n_replicas = static_cast<int>(b->reps.size());
+    o->n_groups = static_cast<int>(b->groups.size());
+    for (size_t k = 0; k < b->groups.size() && k < FTS_MAX_BATCH_GROUPS; ++k) {
+        const Group& g = *b->groups[k];
+        o->group_size[k] = static_cast<int>(g.reps.size());
+        o->group_mesh[k][0] = g.m1;
+        o->group_mesh[k][1] = g.m2;
+        o->group_mesh[k][2] = g.m3;
+        o->group_precision[k] = g.precision;
+        o->group_schedule[k] = g.sched;
+        o->group_tune_ms[k][0] = g.tune_ms[0];
+        o->group_tune_ms[k][1] = g.tune_ms[1];
+    }
+    return FTS_OK;
+}
+
+int fts_batch_replica(fts_batch* b, int r, fts_engine** engine, int* group)
+{
+    if (!b || r < 0 || r >= static_cast<int>(b->reps.size())) return fail(FTS_INVALID, "batch: replica index");
+    if (engine) *engine = b->reps[static_cast<size_t>(r)];
+    if (group) *group = b->group_of[static_cast<size_t>(r)];
+    return FTS_OK;
+}
+
+int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* records, int capacity, int* n_records)
+{
+    if (!b || n < 0) return fail(FTS_INVALID, "batch: bad argument");
+    const size_t R = b->reps.size();
+    if (n_records)
+        for (size_t r = 0; r < R; ++r) n_records[r] = 0;
+    auto run_group = [&](Group& g) {
+        g.rc = FTS_OK;
+        std::vector<fts_observables*> rec(g.reps.size(), nullptr);
+        for (int k = 0; k < n && g.rc == FTS_OK; ++k) {
+            for (size_t r = 0; r < g.reps.size(); ++r) {
+                const int id = g.ids[r];
+                const long long step = g.reps[r]->step_counter + 1;
+                const int have = n_records ? n_records[id] : 0;
+                const bool want = record_every > 0 && step % record_every == 0 && records && n_records && have < capacity;
+                rec[r] = want ? records + static_cast<size_t>(id) * capacity + have : nullptr;
+            }
+            g.rc = g.precision == FTS_PREC_FP32 ? group_step<float>(g, b->scheme, rec) : group_step<double>(g, b->scheme, rec);
+            for (size_t r = 0; r < g.reps.size(); ++r)
+                if (rec[r]) {
+                    rec[r]->step = g.reps[r]->step_counter;
+                    ++n_records[g.ids[r]];
+                }
+        }
+        if (g.rc) g.err = g_error;
+    };
+    if (b->groups.size() == 1) {
+        run_group(*b->groups[0]);
+    } else {
+        std::vector<std::thread> th;
+        for (auto& g : b->groups) th.emplace_back(run_group, std::ref(*g));
+        for (auto& t : th) t.join();
+    }
+    for (auto& g : b->groups)
+        if (g->rc) return fail(g->rc, g->err);
+    return FTS_OK;
+}
+
+int fts_batch_set_timing(fts_batch* b, int enabled)
+{
+    if (!b) return fail(FTS_INVALID, "null batch");
+    for (auto& g : b->groups) {
+        int rc = resolve_group_timing(*g);
+        if (rc) return rc;
+        g->timing = enabled != 0;
+        g->tstat = fts_timing{};
+    }
+    return FTS_OK;
+}
+
+int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out)
+{
+    if (!b || !out || group < 0 || group >= static_cast<int>(b->groups.size())) return fail(FTS_INVALID, "batch: bad argument");
+    Group& g = *b->groups[static_cast<size_t>(group)];
+    int rc = resolve_group_timing(g);
+    if (rc) return rc;
+    *out = g.tstat;
+    return FTS_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,239 @@
+// engine.h -- the host engine object shared by the single-system C ABI
+// (engine.hip) and the replica batch (batch.hip).
+//
+// One fts_engine owns every device buffer of one system (the role of
+// deviceState, deviceState.cu:8-105) and runs on one HIP stream of default
+// priority, its own or one a replica batch shares between its replicas.
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/fts.h"
+#include "fft.h"
+#include "kernel_params.h"
+
+using namespace fts;
+
+namespace fts_host {
+
+constexpr double Pi = 3.141592653589793238462643383279502884;
+inline thread_local std::string g_error;
+
+inline int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return fail(FTS_DEVICE_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline int ew_blocks(size_t n) { return static_cast<int>((n + EW_THREADS - 1) / EW_THREADS); }
+
+template <typename X> struct DevBuf {
+    X* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count)
+    {
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(X) * count);
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct HostSpecies {
+    long long n = 0;
+    int N = 0;
+    std::vector<int> type;
+    std::vector<double> charge, bond;
+    std::vector<int> bond_cls;   // sep-table class of each bond
+    // bead classes: distinct (type, charge)
+    std::vector<int> cls_of_bead, cls_type, cls_start, cls_beads;
+    std::vector<double> cls_z;
+    DevBuf<int> d_cls_type, d_cls_start, d_cls_beads;
+    DevBuf<double> d_cls_z, d_cls_shift;
+    std::vector<double> cls_shift; // host copy of d_cls_shift (min Re W_c + BOLTZ_CAP per class, FP32 only)
+    // mirrored sweep (palindromic chain): qB_s = qF_{N-1-s}; half_* list, per class,
+    // the beads s <= N-1-s the density contraction visits
+    bool mirror = false;
+    std::vector<int> half_start, half_beads;
+    DevBuf<int> d_half_start, d_half_beads;
+    double charge_square_sum() const
+    {
+        double t = 0;
+        for (double z : charge) t += z * z;
+        return t;
+    }
+    double net_charge() const
+    {
+        double t = 0;
+        for (double z : charge) t += z;
+        return t;
+    }
+};
+
+struct HostIon {
+    double z = 0, a = 0;
+    long long n = 0;
+};
+
+// scalar slots in the device scalar buffer
+enum {
+    SC_Q = 0,        // [8] polymer partition functions (of the stored, possibly shifted, propagators)
+    SC_QION = 8,     // [8] ion partition functions
+    SC_PSI_MEAN = 16,
+    SC_ETA_MEAN = 17,
+    SC_DRIFT_MEAN = 18,
+    SC_TMP = 19,
+    SC_CONTRACT = 24, // [7]
+    SC_H = 32,
+    SC_LF = 33,       // [8] per species: log-gain LF(N-1) of the renormalised forward sweep
+    SC_SHIFT = 40,    // [MAX_CLASSES]
+    SC_COUNT = 40 + MAX_CLASSES
+};
+
+} // namespace fts_host
+
+using namespace fts_host;
+
+struct fts_engine {
+    // ---- parameters ---------------------------------------------------------
+    int m1 = 0, m2 = 0, m3 = 0, M = 0;
+    double L1 = 0, L2 = 0, L3 = 0, V = 0, dV = 0, a = 0, lB = 0, dt = 0, lambda_psi = 0;
+    bool noise = true;
+    int scheme = 0, precision = 0, stress_mask = 0;
+    unsigned long long seed = 0;
+    long long step_counter = 0;
+    int nT = 0, n_omega = 0;
+    double xi = 0, rho0 = 0, P_sr = 0;
+    std::vector<double> K, eig, O;
+    std::vector<int> active;
+    std::vector<double> lambda_omega;
+    long long total_segments = 0;
+    std::vector<long long> total_type_segments;
+    std::vector<HostSpecies> species;
+    std::vector<HostIon> ions;
+    int Nmax = 0, max_cls = 0, last_species = 0;
+    double mode_term[6] = {0, 0, 0, 0, 0, 0};
+    ModeCoefs mc{};
+    std::vector<double2> mode_coef_host; // gamma O [nO][nT]
+
+    // ---- device ---------------------------------------------------------------
+    hipStream_t stream = nullptr;
+    bool own_stream = true; // false: the stream belongs to a replica batch
+    FftGeom g64{}, g32{};
+    DevBuf<double> kvec, sep_gamma, sep_bond, sep_ion, si_table_omega, si_table_psi;
+    DevBuf<double2> tw64_x, tw64_yz, tw64_z;
+    DevBuf<float2> tw32_x, tw32_yz, tw32_z;
+    int n_bond_cls = 0;
+    std::vector<double> bond_cls_len;
+    DevBuf<double2> omega, omega_bar, omega_s, drift, psi, psi_bar, psi_s, dpsi, dpsi_bar;
+    DevBuf<double2> W_type_s, rho_hat, rho_type, rho_c_hat, rho_c, rho_c_poly, rho_c_ion, rho_p;
+    DevBuf<double2> ion_psi, ion_b, ion_rho, t1, t2, t3, mode_coef;
+    DevBuf<double2> partials, scal;
+    DevBuf<int> flag;
+    // propagators (double or float complex)
+    void* qF = nullptr;
+    void* qB = nullptr;
+    void* E = nullptr;
+    void* tmpF = nullptr;
+    void* tmpB = nullptr;
+    void* kA = nullptr;
+    void* kB = nullptr;
+    std::vector<double> q_shift_total; // per species: sum of class shifts over the chain
+    DevBuf<double> norms, logc, rfac, rfac2; // FP32 per-bead renormalisation (see k_scale_prefix)
+    std::vector<double> q_log_gain;          // per species: LF(N-1) of the last solve (host copy)
+
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> event_pool;
+    fts_timing tstat{};
+
+    ~fts_engine() { release(); }
+    void release();
+
+    size_t csize() const { return precision == FTS_PREC_FP32 ? sizeof(float2) : sizeof(double2); }
+    double2* f(DevBuf<double2>& b, size_t k) { return b.p + k * static_cast<size_t>(M); }
+    bool electro() const { return lB > 0.0; }
+    double omega_coefficient(int l) const { return rho0 * (1.0 / std::fabs(eig[static_cast<size_t>(active[static_cast<size_t>(l)])])); }
+    bool gamma_imag(int l) const { return eig[static_cast<size_t>(active[static_cast<size_t>(l)])] > 0.0; }
+    double Ocoef(int l, int al) const { return O[static_cast<size_t>(active[static_cast<size_t>(l)]) * nT + al]; }
+
+    // setup
+    int build_modes(const fts_system* s);
+    int build_ions(const fts_system* s);
+    int setup_device();
+    int build_si_tables();
+
+    // primitives
+    KOp kop_sep(const double* sep, double c0) const { return KOp{sep, nullptr, c0, 0.0, 0.0, 0, 0}; }
+    int fft_apply(const double2* src, double2* dst, const KOp& op);
+    int fft_forward(const double2* src, double2* dst);
+    template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst);
+    int reduce_finalize(int K, int slot, double scale);
+    int sum_field(const double2* a, int slot, double scale);
+    int remove_mean(double2* a);
+    int read_scal(int slot, int count, double2* out);
+    int read_log_gain();
+
+    // hot path
+    template <typename T> int solve_species(int p, bool accumulate = true);
+    template <typename T> int run_sweeps(int p, int schedule, int nsteps, bool timed);
+    int autotune_schedule();
+    template <typename T> int bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op);
+    int sched = FTS_SCHED_PLANE;
+    bool plane_ok = true;
+    size_t norm_stride = 0;
+    double tune_ms[2] = {0.0, 0.0};
+    int compute_densities(const double2* om, const double2* ps);
+    // the phases of compute_densities, so that a replica batch can run the
+    // propagator sweeps of all its replicas in shared launches:
+    //   densities_begin; per species p: solve_begin(p), sweeps, solve_end(p); densities_end
+    int densities_begin(const double2* om, const double2* ps);
+    template <typename T> int solve_begin(int p);
+    template <typename T> int solve_end(int p, bool accumulate);
+    int densities_end();
+    const double2* dens_psi = nullptr; // the psi field densities_begin was given
+    int check_Q(const char* what);
+    int finite_queue(int* host_out);      // queue the field finiteness check; *host_out valid after a stream sync
+    int q_queue(double2* host_out);       // queue the copy of Q[n_species]; valid after a stream sync
+    int psi_drift(const double2* ps, const double2* rho_src, double2* out);
+    int check_finite(bool& ok);
+    int measure(const double2* om, const double2* ps, fts_observables* rec);
+    template <typename T> int stress_chain(int p, std::vector<std::array<double2, 7>>& acc);
+    int contract(const void* A, const void* B, bool fp32, const double* sep, double2 out[7]);
+    int step_pc(bool record, fts_observables* rec);
+    int step_em(bool record, fts_observables* rec, bool si);
+    // the parts of a CL step between its density computations
+    int pc_predict();                                          // after the densities at (w, psi)
+    int pc_correct(bool record, fts_observables* rec);         // after the densities at (w_bar, psi_bar)
+    int em_update(bool record, fts_observables* rec, bool si); // after the densities at (w, psi)
+    int finish_step();
+
+    // timing
+    hipEvent_t get_event();
+    int resolve_timing();
+};
+
+// Builds an engine for system s.  shared: a stream the engine uses instead of
+// creating its own (a replica batch's); tune: run the schedule autotuner when
+// s->fft_schedule is FTS_SCHED_AUTO.
+int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine** out);
+

@@ -13,207 +13,8 @@
 // followed by Gaussian smoothing, ions, drifts and the fused CL update kernels.
 // Reductions stay on the device; the host reads a scalar only where it needs
 // one (finiteness checks, records).
-#include <algorithm>
-#include <array>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <numeric>
-#include <string>
-#include <vector>
-
-#include "../../include/fts.h"
-#include "fft.h"
+#include "engine.h"
 #include "kernels.h"
-
-using namespace fts;
-
-namespace {
-
-constexpr double Pi = 3.141592653589793238462643383279502884;
-thread_local std::string g_error;
-
-int fail(int code, const std::string& msg)
-{
-    g_error = msg;
-    return code;
-}
-
-#define HIP_TRY(call)                                                                               \
-    do {                                                                                            \
-        hipError_t e_ = (call);                                                                     \
-        if (e_ != hipSuccess) return fail(FTS_DEVICE_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-inline int ew_blocks(size_t n) { return static_cast<int>((n + EW_THREADS - 1) / EW_THREADS); }
-
-template <typename X> struct DevBuf {
-    X* p = nullptr;
-    size_t n = 0;
-    hipError_t alloc(size_t count)
-    {
-        n = count;
-        if (count == 0) return hipSuccess;
-        return hipMalloc(reinterpret_cast<void**>(&p), sizeof(X) * count);
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-struct HostSpecies {
-    long long n = 0;
-    int N = 0;
-    std::vector<int> type;
-    std::vector<double> charge, bond;
-    std::vector<int> bond_cls;   // sep-table class of each bond
-    // bead classes: distinct (type, charge)
-    std::vector<int> cls_of_bead, cls_type, cls_start, cls_beads;
-    std::vector<double> cls_z;
-    DevBuf<int> d_cls_type, d_cls_start, d_cls_beads;
-    DevBuf<double> d_cls_z, d_cls_shift;
-    std::vector<double> cls_shift; // per class Re<W>, FP32 only
-    double charge_square_sum() const
-    {
-        double t = 0;
-        for (double z : charge) t += z * z;
-        return t;
-    }
-    double net_charge() const
-    {
-        double t = 0;
-        for (double z : charge) t += z;
-        return t;
-    }
-};
-
-struct HostIon {
-    double z = 0, a = 0;
-    long long n = 0;
-};
-
-// scalar slots in the device scalar buffer
-enum {
-    SC_Q = 0,        // [8] polymer partition functions (of the stored, possibly shifted, propagators)
-    SC_QION = 8,     // [8] ion partition functions
-    SC_PSI_MEAN = 16,
-    SC_ETA_MEAN = 17,
-    SC_DRIFT_MEAN = 18,
-    SC_TMP = 19,
-    SC_CONTRACT = 24, // [7]
-    SC_H = 32,
-    SC_LF = 33,       // [8] per species: log-gain LF(N-1) of the renormalised forward sweep
-    SC_SHIFT = 40,    // [MAX_CLASSES]
-    SC_COUNT = 40 + MAX_CLASSES
-};
-
-} // namespace
-
-struct fts_engine {
-    // ---- parameters ---------------------------------------------------------
-    int m1 = 0, m2 = 0, m3 = 0, M = 0;
-    double L1 = 0, L2 = 0, L3 = 0, V = 0, dV = 0, a = 0, lB = 0, dt = 0, lambda_psi = 0;
-    bool noise = true;
-    int scheme = 0, precision = 0, stress_mask = 0;
-    unsigned long long seed = 0;
-    long long step_counter = 0;
-    int nT = 0, n_omega = 0;
-    double xi = 0, rho0 = 0, P_sr = 0;
-    std::vector<double> K, eig, O;
-    std::vector<int> active;
-    std::vector<double> lambda_omega;
-    long long total_segments = 0;
-    std::vector<long long> total_type_segments;
-    std::vector<HostSpecies> species;
-    std::vector<HostIon> ions;
-    int Nmax = 0, max_cls = 0, last_species = 0;
-    double mode_term[6] = {0, 0, 0, 0, 0, 0};
-    ModeCoefs mc{};
-    std::vector<double2> mode_coef_host; // gamma O [nO][nT]
-
-    // ---- device ---------------------------------------------------------------
-    hipStream_t stream = nullptr;
-    FftGeom g64{}, g32{};
-    DevBuf<double> kvec, sep_gamma, sep_bond, sep_ion, si_table_omega, si_table_psi;
-    DevBuf<double2> tw64_x, tw64_yz, tw64_z;
-    DevBuf<float2> tw32_x, tw32_yz, tw32_z;
-    int n_bond_cls = 0;
-    std::vector<double> bond_cls_len;
-    DevBuf<double2> omega, omega_bar, omega_s, drift, psi, psi_bar, psi_s, dpsi, dpsi_bar;
-    DevBuf<double2> W_type_s, rho_hat, rho_type, rho_c_hat, rho_c, rho_c_poly, rho_c_ion, rho_p;
-    DevBuf<double2> ion_psi, ion_b, ion_rho, t1, t2, t3, mode_coef;
-    DevBuf<double2> partials, scal;
-    DevBuf<int> flag;
-    // propagators (double or float complex)
-    void* qF = nullptr;
-    void* qB = nullptr;
-    void* E = nullptr;
-    void* tmpF = nullptr;
-    void* tmpB = nullptr;
-    void* kA = nullptr;
-    void* kB = nullptr;
-    std::vector<double> q_shift_total; // per species: sum of class shifts over the chain
-    DevBuf<double> norms, logc, rfac, rfac2; // FP32 per-bead renormalisation (see k_scale_prefix)
-    std::vector<double> q_log_gain;          // per species: LF(N-1) of the last solve (host copy)
-
-    // timing
-    bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-    std::vector<hipEvent_t> event_pool;
-    fts_timing tstat{};
-
-    ~fts_engine() { release(); }
-    void release();
-
-    size_t csize() const { return precision == FTS_PREC_FP32 ? sizeof(float2) : sizeof(double2); }
-    double2* f(DevBuf<double2>& b, size_t k) { return b.p + k * static_cast<size_t>(M); }
-    bool electro() const { return lB > 0.0; }
-    double omega_coefficient(int l) const { return rho0 * (1.0 / std::fabs(eig[static_cast<size_t>(active[static_cast<size_t>(l)])])); }
-    bool gamma_imag(int l) const { return eig[static_cast<size_t>(active[static_cast<size_t>(l)])] > 0.0; }
-    double Ocoef(int l, int al) const { return O[static_cast<size_t>(active[static_cast<size_t>(l)]) * nT + al]; }
-
-    // setup
-    int build_modes(const fts_system* s);
-    int build_ions(const fts_system* s);
-    int setup_device();
-    int build_si_tables();
-
-    // primitives
-    KOp kop_sep(const double* sep, double c0) const { return KOp{sep, nullptr, c0, 0.0, 0.0, 0, 0}; }
-    int fft_apply(const double2* src, double2* dst, const KOp& op);
-    int fft_forward(const double2* src, double2* dst);
-    template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst);
-    int reduce_finalize(int K, int slot, double scale);
-    int sum_field(const double2* a, int slot, double scale);
-    int remove_mean(double2* a);
-    int read_scal(int slot, int count, double2* out);
-    int read_log_gain();
-
-    // hot path
-    template <typename T> int solve_species(int p, bool accumulate = true);
-    template <typename T> int run_sweeps(int p, int schedule, int nsteps, bool timed);
-    int autotune_schedule();
-    template <typename T> int bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op);
-    int sched = FTS_SCHED_PLANE;
-    bool plane_ok = true;
-    size_t norm_stride = 0;
-    double tune_ms[2] = {0.0, 0.0};
-    int compute_densities(const double2* om, const double2* ps);
-    int psi_drift(const double2* ps, const double2* rho_src, double2* out);
-    int check_finite(bool& ok);
-    int measure(const double2* om, const double2* ps, fts_observables* rec);
-    template <typename T> int stress_chain(int p, std::vector<std::array<double2, 7>>& acc);
-    int contract(const void* A, const void* B, bool fp32, const double* sep, double2 out[7]);
-    int step_pc(bool record, fts_observables* rec);
-    int step_em(bool record, fts_observables* rec, bool si);
-
-    // timing
-    hipEvent_t get_event();
-    int resolve_timing();
-};
 
 void fts_engine::release()
 {
@@ -243,12 +44,15 @@ void fts_engine::release()
         sp.d_cls_beads.release();
         sp.d_cls_z.release();
         sp.d_cls_shift.release();
+        sp.d_half_start.release();
+        sp.d_half_beads.release();
     }
     for (void** p : {&qF, &qB, &E, &tmpF, &tmpB, &kA, &kB}) {
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
     stream = nullptr;
 }
 
@@ -461,7 +265,10 @@ int fts_engine::build_ions(const fts_system* s)
 // Device buffers, FFT tables and per-species bead classes.
 int fts_engine::setup_device()
 {
-    HIP_TRY(hipStreamCreate(&stream));
+    if (!stream) {
+        HIP_TRY(hipStreamCreate(&stream));
+        own_stream = true;
+    }
     const size_t Ms = static_cast<size_t>(M);
     // wave numbers and twiddles
     std::vector<double> kv;
@@ -559,7 +366,12 @@ int fts_engine::setup_device()
     for (DevBuf<double2>* b : {&omega, &omega_bar, &psi, &psi_bar, &scal}) HIP_TRY(hipMemset(b->p, 0, sizeof(double2) * b->n));
     // propagators
     const size_t cs = csize();
-    for (void** p : {&qF, &qB}) HIP_TRY(hipMalloc(p, cs * static_cast<size_t>(Nmax) * Ms));
+    // qB is only stored for species without the mirror symmetry
+    int NmaxB = 0;
+    for (const HostSpecies& sp : species)
+        if (!sp.mirror) NmaxB = std::max(NmaxB, sp.N);
+    HIP_TRY(hipMalloc(&qF, cs * static_cast<size_t>(Nmax) * Ms));
+    if (NmaxB > 0) HIP_TRY(hipMalloc(&qB, cs * static_cast<size_t>(NmaxB) * Ms));
     HIP_TRY(hipMalloc(&E, cs * static_cast<size_t>(max_cls) * Ms));
     for (void** p : {&tmpF, &tmpB, &kA, &kB}) HIP_TRY(hipMalloc(p, cs * Ms));
     // bead classes
@@ -575,6 +387,12 @@ int fts_engine::setup_device()
         HIP_TRY(hipMemset(sp.d_cls_shift.p, 0, sizeof(double) * nc));
         HIP_TRY(hipMemcpy(sp.d_cls_start.p, sp.cls_start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(sp.d_cls_beads.p, sp.cls_beads.data(), sizeof(int) * sp.N, hipMemcpyHostToDevice));
+        if (sp.mirror) {
+            HIP_TRY(sp.d_half_start.alloc(nc + 1));
+            HIP_TRY(sp.d_half_beads.alloc(std::max<size_t>(1, sp.half_beads.size())));
+            HIP_TRY(hipMemcpy(sp.d_half_start.p, sp.half_start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(sp.d_half_beads.p, sp.half_beads.data(), sizeof(int) * sp.half_beads.size(), hipMemcpyHostToDevice));
+        }
         sp.cls_shift.assign(static_cast<size_t>(nc), 0.0);
     }
     q_shift_total.assign(species.size(), 0.0);
@@ -738,13 +556,22 @@ int fts_engine::read_scal(int slot, int count, double2* out)
     return FTS_OK;
 }
 
-// host copy of LF(N-1) per species (FP32 renormalisation; zero in FP64)
+// host copies of the scales of the last solve: LF(N-1) per species (per-bead
+// renormalisation) and the FP32 class shifts with their sum over the chain
 int fts_engine::read_log_gain()
 {
     double2 v[FTS_MAX_SPECIES];
     int rc = read_scal(SC_LF, static_cast<int>(species.size()), v);
     if (rc) return rc;
-    for (size_t p = 0; p < species.size(); ++p) q_log_gain[p] = v[p].x;
+    for (size_t p = 0; p < species.size(); ++p) {
+        HostSpecies& sp = species[p];
+        q_log_gain[p] = v[p].x;
+        HIP_TRY(hipMemcpyAsync(sp.cls_shift.data(), sp.d_cls_shift.p, sizeof(double) * sp.cls_shift.size(), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        double total = 0.0;
+        for (int s = 0; s < sp.N; ++s) total += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)])];
+        q_shift_total[p] = total;
+    }
     return FTS_OK;
 }
 
@@ -796,6 +623,8 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
     Cv* tb = static_cast<Cv*>(tmpB);
     const int N = sp.N;
     auto E_of = [&](int s) { return e + static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)]) * Ms; };
+    // a mirrored species runs the forward sweep only (one job per launch)
+    const int nj = sp.mirror ? 1 : 2;
     double* nF[2] = {norms.p, norms.p + norm_stride};
     double* nB[2] = {norms.p + 2 * norm_stride, norms.p + 3 * norm_stride};
     double* lcF = logc.p;
@@ -809,7 +638,7 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
     if (schedule == FTS_SCHED_PLANE) {
         PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0), nullptr, nF[0], lcF},
                                {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), nullptr, nB[0], lcB + (N - 1)}};
-        HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, 2, stream));
+        HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, nj, stream));
         if (timed) {
             ev0 = get_event();
             HIP_TRY(hipEventRecord(ev0, stream));
@@ -818,18 +647,18 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
             const int bf = sp.bond_cls[static_cast<size_t>(s)], bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
             PencilJob<T> pj[2] = {{tf, kop_sep(sep_bond.p + static_cast<size_t>(bf) * ks, invM)},
                                   {tb, kop_sep(sep_bond.p + static_cast<size_t>(bb) * ks, invM)}};
-            HIP_TRY(launch_pencil<T>(g, PENCIL_FWD_MUL_INV, pj, 2, stream));
+            HIP_TRY(launch_pencil<T>(g, PENCIL_FWD_MUL_INV, pj, nj, stream));
             const int sf = s + 1, sb = N - 2 - s;
             PlaneJob<T> st[2] = {{tf, tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf), nF[s % 2], nF[(s + 1) % 2], lcF + sf},
                                  {tb, tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), nB[s % 2], nB[(s + 1) % 2], lcB + sb}};
-            HIP_TRY(launch_plane<T>(g, s < nsteps - 1 ? PLANE_STEP : PLANE_LAST, st, 2, stream));
+            HIP_TRY(launch_plane<T>(g, s < nsteps - 1 ? PLANE_STEP : PLANE_LAST, st, nj, stream));
         }
     } else {
         LineJob<T> init[2] = {{tf, qf, E_of(0), KOp{}, nullptr, nF[0], lcF},
                               {tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), KOp{}, nullptr, nB[0], lcB + (N - 1)}};
-        HIP_TRY(launch_line<T>(g, 2, LINE_E_FWD, twz, init, 2, stream));
+        HIP_TRY(launch_line<T>(g, 2, LINE_E_FWD, twz, init, nj, stream));
         LineJob<T> yf[2] = {{tf}, {tb}};
-        HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, yf, 2, stream));
+        HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, yf, nj, stream));
         if (timed) {
             ev0 = get_event();
             HIP_TRY(hipEventRecord(ev0, stream));
@@ -838,23 +667,24 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
             const int bf = sp.bond_cls[static_cast<size_t>(s)], bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
             LineJob<T> xj[2] = {{tf, nullptr, nullptr, kop_sep(sep_bond.p + static_cast<size_t>(bf) * ks, invM)},
                                 {tb, nullptr, nullptr, kop_sep(sep_bond.p + static_cast<size_t>(bb) * ks, invM)}};
-            HIP_TRY(launch_line<T>(g, 0, LINE_FWD_MUL_INV, twx, xj, 2, stream));
-            HIP_TRY(launch_line<T>(g, 1, LINE_INV, twy, yf, 2, stream));
+            HIP_TRY(launch_line<T>(g, 0, LINE_FWD_MUL_INV, twx, xj, nj, stream));
+            HIP_TRY(launch_line<T>(g, 1, LINE_INV, twy, yf, nj, stream));
             const int sf = s + 1, sb = N - 2 - s;
             const bool last = s == nsteps - 1;
             LineJob<T> zj[2] = {{tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf), KOp{}, nF[s % 2], nF[(s + 1) % 2], lcF + sf},
                                 {tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), KOp{}, nB[s % 2], nB[(s + 1) % 2], lcB + sb}};
-            HIP_TRY(launch_line<T>(g, 2, last ? LINE_INV_E : LINE_INV_E_FWD, twz, zj, 2, stream));
-            if (!last) HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, yf, 2, stream));
+            HIP_TRY(launch_line<T>(g, 2, last ? LINE_INV_E : LINE_INV_E_FWD, twz, zj, nj, stream));
+            if (!last) HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, yf, nj, stream));
         }
     }
     if (timed) {
         hipEvent_t ev1 = get_event();
         HIP_TRY(hipEventRecord(ev1, stream));
         pending.emplace_back(ev0, ev1);
-        tstat.bead_steps += 2LL * nsteps;
+        tstat.bead_steps += static_cast<long long>(nj) * nsteps;
+        tstat.step_launches += nsteps;
         tstat.sweeps += 1;
-        tstat.algorithmic_bytes += 2.0 * nsteps * 2.0 * static_cast<double>(sizeof(Cv)) * static_cast<double>(M);
+        tstat.algorithmic_bytes += static_cast<double>(nj) * nsteps * 2.0 * static_cast<double>(sizeof(Cv)) * static_cast<double>(M);
     }
     return FTS_OK;
 }
@@ -927,73 +757,98 @@ int fts_engine::autotune_schedule()
 // ---------------------------------------------------------------------------
 // Propagator solve for species p (polymer.cu:418-426 + accumulate, 434-447)
 // ---------------------------------------------------------------------------
-template <typename T> int fts_engine::solve_species(int p, bool accumulate)
+// Before the sweeps: FP32 class shifts and the Boltzmann factors e^{-W_c}.
+template <typename T> int fts_engine::solve_begin(int p)
 {
     using Cv = V2<T>;
     HostSpecies& sp = species[static_cast<size_t>(p)];
     last_species = p;
-    const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
     const size_t Ms = static_cast<size_t>(M);
-    Cv* qf = static_cast<Cv*>(qF);
-    Cv* qb = static_cast<Cv*>(qB);
     Cv* e = static_cast<Cv*>(E);
-    Cv* tf = static_cast<Cv*>(tmpF);
-    Cv* tb = static_cast<Cv*>(tmpB);
     const int nc = static_cast<int>(sp.cls_type.size());
     // FP32: shift each class's potential so that max_i exp(-Re W_c + shift_c) = exp(BOLTZ_CAP)
-    // (shift_c = min Re W_c + BOLTZ_CAP, kept in FP64)
+    // (shift_c = min Re W_c + BOLTZ_CAP, kept in FP64 on the device)
     if (sizeof(T) == 4) {
         for (int c = 0; c < nc; ++c) {
             hipLaunchKernelGGL(k_min_class_w, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, f(W_type_s, sp.cls_type[static_cast<size_t>(c)]),
                                psi_s.p, electro() ? sp.cls_z[static_cast<size_t>(c)] : 0.0, M, partials.p);
-            hipLaunchKernelGGL(k_finalize_min, dim3(1), dim3(RED_THREADS), 0, stream, partials.p, scal.p + SC_SHIFT + c);
+            hipLaunchKernelGGL(k_finalize_min, dim3(1), dim3(RED_THREADS), 0, stream, partials.p, BOLTZ_CAP, sp.d_cls_shift.p + c);
             HIP_TRY(hipGetLastError());
         }
-        std::vector<double2> sh(static_cast<size_t>(nc));
-        int rc = read_scal(SC_SHIFT, nc, sh.data());
-        if (rc) return rc;
-        double total = 0.0;
-        for (int c = 0; c < nc; ++c) sp.cls_shift[static_cast<size_t>(c)] = sh[static_cast<size_t>(c)].x + BOLTZ_CAP;
-        for (int s = 0; s < sp.N; ++s) total += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)])];
-        q_shift_total[static_cast<size_t>(p)] = total;
-        HIP_TRY(hipMemcpyAsync(sp.d_cls_shift.p, sp.cls_shift.data(), sizeof(double) * nc, hipMemcpyHostToDevice, stream));
     }
     hipLaunchKernelGGL(k_boltzmann<T>, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, W_type_s.p, psi_s.p, sp.d_cls_type.p, sp.d_cls_z.p,
                        sp.d_cls_shift.p, nc, M, e);
     HIP_TRY(hipGetLastError());
-    auto E_of = [&](int s) { return e + static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)]) * Ms; };
-    const int N = sp.N;
-    if (N == 1) {
-        HIP_TRY(hipMemcpyAsync(qf, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
-        HIP_TRY(hipMemcpyAsync(qb, E_of(0), sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
-    } else {
-        int rc = run_sweeps<T>(p, sched, N - 1, timing);
-        if (rc) return rc;
+    if (sp.N == 1) {
+        const Cv* e0 = e + static_cast<size_t>(sp.cls_of_bead[0]) * Ms;
+        HIP_TRY(hipMemcpyAsync(qF, e0, sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
+        if (!sp.mirror) HIP_TRY(hipMemcpyAsync(qB, e0, sizeof(Cv) * Ms, hipMemcpyDeviceToDevice, stream));
     }
+    return FTS_OK;
+}
+
+// After the sweeps: renormalisation factors, Q = <qF_{N-1}> (polymer.cu:425) and,
+// with accumulate, the density contraction (polymer.cu:434-447).
+template <typename T> int fts_engine::solve_end(int p, bool accumulate)
+{
+    using Cv = V2<T>;
+    const HostSpecies& sp = species[static_cast<size_t>(p)];
+    const size_t Ms = static_cast<size_t>(M);
+    Cv* qf = static_cast<Cv*>(qF);
+    Cv* qb = static_cast<Cv*>(qB);
+    const int nc = static_cast<int>(sp.cls_type.size());
+    const int N = sp.N;
     if (N > 1) {
-        hipLaunchKernelGGL(k_scale_prefix, dim3(1), dim3(64), 0, stream, logc.p, logc.p + Nmax, N, rfac.p, rfac2.p, scal.p + SC_LF + p);
+        hipLaunchKernelGGL(k_scale_prefix, dim3(1), dim3(64), 0, stream, logc.p, logc.p + Nmax, N, sp.mirror ? 1 : 0, rfac.p, rfac2.p,
+                           scal.p + SC_LF + p);
         HIP_TRY(hipGetLastError());
     }
-    // Q = <qF_{N-1}> (polymer.cu:425)
     hipLaunchKernelGGL(k_sum<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, qf + static_cast<size_t>(N - 1) * Ms, M, partials.p);
     HIP_TRY(hipGetLastError());
     int rc = reduce_finalize(1, SC_Q + p, 1.0 / static_cast<double>(M));
     if (rc) return rc;
     if (!accumulate) return FTS_OK;
-    // density contraction
-    hipLaunchKernelGGL(k_contract_density<T>, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, qf, qb, sp.d_cls_start.p, sp.d_cls_beads.p,
-                       sp.d_cls_type.p, sp.d_cls_z.p, sp.d_cls_shift.p, nc, W_type_s.p, psi_s.p, scal.p + SC_Q + p,
-                       static_cast<double>(sp.n) / V, electro() ? 1 : 0, N > 1 ? rfac.p : nullptr, M, rho_hat.p, rho_c_hat.p);
+    hipLaunchKernelGGL(k_contract_density<T>, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, qf, qb,
+                       sp.mirror ? sp.d_half_start.p : sp.d_cls_start.p, sp.mirror ? sp.d_half_beads.p : sp.d_cls_beads.p, sp.d_cls_type.p,
+                       sp.d_cls_z.p, sp.d_cls_shift.p, nc, W_type_s.p, psi_s.p, scal.p + SC_Q + p, static_cast<double>(sp.n) / V,
+                       electro() ? 1 : 0, N > 1 ? rfac.p : nullptr, M, sp.mirror ? N : 0, rho_hat.p, rho_c_hat.p);
     HIP_TRY(hipGetLastError());
     return FTS_OK;
+}
+
+template int fts_engine::solve_begin<float>(int);
+template int fts_engine::solve_begin<double>(int);
+template int fts_engine::solve_end<float>(int, bool);
+template int fts_engine::solve_end<double>(int, bool);
+
+template <typename T> int fts_engine::solve_species(int p, bool accumulate)
+{
+    int rc = solve_begin<T>(p);
+    if (rc) return rc;
+    const int N = species[static_cast<size_t>(p)].N;
+    if (N > 1 && (rc = run_sweeps<T>(p, sched, N - 1, timing))) return rc;
+    return solve_end<T>(p, accumulate);
 }
 
 // complexLangevin::compute_densities (complexLangevin.cu:255-260)
 int fts_engine::compute_densities(const double2* om, const double2* ps)
 {
+    int rc = densities_begin(om, ps);
+    if (rc) return rc;
+    for (int p = 0; p < static_cast<int>(species.size()); ++p) {
+        rc = precision == FTS_PREC_FP32 ? solve_species<float>(p) : solve_species<double>(p);
+        if (rc) return rc;
+    }
+    return densities_end();
+}
+
+// smoothed sources of the state (om, ps) and cleared density accumulators
+int fts_engine::densities_begin(const double2* om, const double2* ps)
+{
     const size_t Ms = static_cast<size_t>(M);
     const double invM = 1.0 / static_cast<double>(M);
     int rc;
+    dens_psi = ps;
     // build_smoothed_sources (polymer.cu:341-363)
     for (int l = 0; l < n_omega; ++l)
         if ((rc = fft_apply(om + l * Ms, f(omega_s, l), kop_sep(sep_gamma.p, invM)))) return rc;
@@ -1006,11 +861,16 @@ int fts_engine::compute_densities(const double2* om, const double2* ps)
     }
     HIP_TRY(hipMemsetAsync(rho_hat.p, 0, sizeof(double2) * rho_hat.n, stream));
     HIP_TRY(hipMemsetAsync(rho_c_hat.p, 0, sizeof(double2) * Ms, stream));
-    for (int p = 0; p < static_cast<int>(species.size()); ++p) {
-        rc = precision == FTS_PREC_FP32 ? solve_species<float>(p) : solve_species<double>(p);
-        if (rc) return rc;
-    }
-    // smoothing of the densities (polymer.cu:467-476)
+    return FTS_OK;
+}
+
+// smoothing of the densities (polymer.cu:467-476) and the ion densities (ion.cu:54-72)
+int fts_engine::densities_end()
+{
+    const size_t Ms = static_cast<size_t>(M);
+    const double invM = 1.0 / static_cast<double>(M);
+    int rc;
+    const double2* ps = dens_psi;
     for (int al = 0; al < nT; ++al)
         if ((rc = fft_apply(f(rho_hat, al), f(rho_type, al), kop_sep(sep_gamma.p, invM)))) return rc;
     hipLaunchKernelGGL(k_sum_fields, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho_type.p, nT, M, rho_p.p);
@@ -1061,16 +921,28 @@ int fts_engine::psi_drift(const double2* ps, const double2* rho_src, double2* ou
     return remove_mean(out);
 }
 
-int fts_engine::check_finite(bool& ok)
+int fts_engine::finite_queue(int* host_out)
 {
-    const int one = 1;
-    HIP_TRY(hipMemcpyAsync(flag.p, &one, sizeof(int), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, stream, flag.p, 1);
     const size_t no = static_cast<size_t>(n_omega) * M;
     if (no) hipLaunchKernelGGL(k_finite, dim3(ew_blocks(no)), dim3(EW_THREADS), 0, stream, omega.p, no, flag.p);
     hipLaunchKernelGGL(k_finite, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi.p, static_cast<size_t>(M), flag.p);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host_out, flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    return FTS_OK;
+}
+
+int fts_engine::q_queue(double2* host_out)
+{
+    HIP_TRY(hipMemcpyAsync(host_out, scal.p + SC_Q, sizeof(double2) * species.size(), hipMemcpyDeviceToHost, stream));
+    return FTS_OK;
+}
+
+int fts_engine::check_finite(bool& ok)
+{
     int h = 0;
-    HIP_TRY(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    int rc = finite_queue(&h);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(stream));
     ok = h != 0;
     return FTS_OK;
@@ -1103,7 +975,7 @@ template <typename T> int fts_engine::stress_chain(int p, std::vector<std::array
     for (int s = 0; s + 1 < sp.N; ++s) {
         int rc = fft_forward_t<T>(qf + static_cast<size_t>(s) * Ms, static_cast<Cv*>(kA));
         if (rc) return rc;
-        rc = fft_forward_t<T>(qb + static_cast<size_t>(s + 1) * Ms, static_cast<Cv*>(kB));
+        rc = fft_forward_t<T>(sp.mirror ? qf + static_cast<size_t>(sp.N - 2 - s) * Ms : qb + static_cast<size_t>(s + 1) * Ms, static_cast<Cv*>(kB));
         if (rc) return rc;
         const double* sep = sep_bond.p + static_cast<size_t>(sp.bond_cls[static_cast<size_t>(s)]) * (m1 + m2 + m3);
         rc = contract(kB, kA, sizeof(T) == 4, sep, acc[static_cast<size_t>(s)].data());
@@ -1313,17 +1185,32 @@ namespace {
 bool finite2(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
 } // namespace
 
+int fts_engine::check_Q(const char* what)
+{
+    double2 Q[FTS_MAX_SPECIES];
+    int rc = read_scal(SC_Q, static_cast<int>(species.size()), Q);
+    if (rc) return rc;
+    for (size_t p = 0; p < species.size(); ++p)
+        if (!finite2(Q[p])) return fail(FTS_NONFINITE, std::string("non-finite ") + what + " at step " + std::to_string(step_counter + 1));
+    return FTS_OK;
+}
+
 // run_step (complexLangevin.cu:669-698): predictor-corrector with the same noise in both halves
 int fts_engine::step_pc(bool record, fts_observables* rec)
 {
+    int rc = compute_densities(omega.p, psi.p);
+    if (rc || (rc = check_Q("Q")) || (rc = pc_predict())) return rc;
+    if ((rc = compute_densities(omega_bar.p, psi_bar.p)) || (rc = check_Q("Qbar"))) return rc;
+    if ((rc = pc_correct(record, rec))) return rc;
+    return finish_step();
+}
+
+// predictor (complexLangevin.cu:328-345), after the densities at (w, psi)
+int fts_engine::pc_predict()
+{
     const unsigned long long t = static_cast<unsigned long long>(step_counter);
     const size_t Ms = static_cast<size_t>(M);
-    double2 Q[FTS_MAX_SPECIES];
-    int rc = compute_densities(omega.p, psi.p);
-    if (rc) return rc;
-    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
-    for (size_t p = 0; p < species.size(); ++p)
-        if (!finite2(Q[p])) return fail(FTS_NONFINITE, "non-finite Q at step " + std::to_string(step_counter + 1));
+    int rc;
     const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
     if (electro()) {
         hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, seed, t, 0x100u, sigma_psi, M, partials.p);
@@ -1344,10 +1231,17 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
     } else {
         HIP_TRY(hipMemsetAsync(psi_bar.p, 0, sizeof(double2) * Ms, stream));
     }
-    if ((rc = compute_densities(omega_bar.p, psi_bar.p))) return rc;
-    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
-    for (size_t p = 0; p < species.size(); ++p)
-        if (!finite2(Q[p])) return fail(FTS_NONFINITE, "non-finite Qbar at step " + std::to_string(step_counter + 1));
+    return FTS_OK;
+}
+
+// corrector (complexLangevin.cu:347-368), after the densities at (w_bar, psi_bar);
+// the record is taken at the predictor state (complexLangevin.cu:680-684)
+int fts_engine::pc_correct(bool record, fts_observables* rec)
+{
+    const unsigned long long t = static_cast<unsigned long long>(step_counter);
+    const size_t Ms = static_cast<size_t>(M);
+    int rc;
+    const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
     if ((rc = psi_drift(psi_bar.p, rho_c.p, dpsi_bar.p))) return rc;
     if (record && (rc = measure(omega_bar.p, psi_bar.p, rec))) return rc;
     if (n_omega > 0) {
@@ -1362,9 +1256,15 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
     } else {
         HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));
     }
+    return FTS_OK;
+}
+
+int fts_engine::finish_step()
+{
     ++step_counter;
     bool ok = true;
-    if ((rc = check_finite(ok))) return rc;
+    int rc = check_finite(ok);
+    if (rc) return rc;
     if (!ok) return fail(FTS_NONFINITE, "updated fields have non-finite values at step " + std::to_string(step_counter));
     return FTS_OK;
 }
@@ -1373,15 +1273,17 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
 // P(k) = 1/(1 + dt lam A(k)).  Records are taken at the state the step starts from.
 int fts_engine::step_em(bool record, fts_observables* rec, bool si)
 {
+    int rc = compute_densities(omega.p, psi.p);
+    if (rc || (rc = check_Q("Q")) || (rc = em_update(record, rec, si))) return rc;
+    return finish_step();
+}
+
+int fts_engine::em_update(bool record, fts_observables* rec, bool si)
+{
     const unsigned long long t = static_cast<unsigned long long>(step_counter);
     const size_t Ms = static_cast<size_t>(M);
     const double invM = 1.0 / static_cast<double>(M);
-    double2 Q[FTS_MAX_SPECIES];
-    int rc = compute_densities(omega.p, psi.p);
-    if (rc) return rc;
-    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
-    for (size_t p = 0; p < species.size(); ++p)
-        if (!finite2(Q[p])) return fail(FTS_NONFINITE, "non-finite Q at step " + std::to_string(step_counter + 1));
+    int rc;
     if (record && (rc = measure(omega.p, psi.p, rec))) return rc;
     const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
     if (electro()) {
@@ -1412,10 +1314,6 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool si)
     } else {
         HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));
     }
-    ++step_counter;
-    bool ok = true;
-    if ((rc = check_finite(ok))) return rc;
-    if (!ok) return fail(FTS_NONFINITE, "updated fields have non-finite values at step " + std::to_string(step_counter));
     return FTS_OK;
 }
 
@@ -1433,7 +1331,11 @@ extern "C" {
 
 const char* fts_last_error(void) { return g_error.c_str(); }
 
-int fts_create(const fts_system* s, fts_engine** out)
+int fts_create(const fts_system* s, fts_engine** out) { return engine_create(s, nullptr, true, out); }
+
+} // extern "C"
+
+int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine** out)
 {
     if (!s || !out) return fail(FTS_INVALID, "null argument");
     *out = nullptr;
@@ -1447,11 +1349,16 @@ int fts_create(const fts_system* s, fts_engine** out)
     if (s->fft_schedule == FTS_SCHED_PLANE && !plane_fits)
         return fail(FTS_INVALID, "the plane schedule needs m2 == m3 and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
     if (s->fft_schedule < 0 || s->fft_schedule > 2) return fail(FTS_INVALID, "fft_schedule");
+    if (s->chain_symmetry != FTS_SYMMETRY_AUTO && s->chain_symmetry != FTS_SYMMETRY_OFF) return fail(FTS_INVALID, "chain_symmetry");
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
     if (s->n_species <= 0 || s->n_species > FTS_MAX_SPECIES || !s->species) return fail(FTS_INVALID, "polymer species_count");
     if (s->scheme < 0 || s->scheme > 2) return fail(FTS_INVALID, "scheme");
     fts_engine* e = new fts_engine();
+    if (shared) {
+        e->stream = shared;
+        e->own_stream = false;
+    }
     auto bail = [&](int rc) {
         delete e;
         return rc;
@@ -1527,6 +1434,21 @@ int fts_create(const fts_system* s, fts_engine** out)
                 if (sp.cls_of_bead[static_cast<size_t>(s2)] == c) sp.cls_beads.push_back(s2);
             sp.cls_start[static_cast<size_t>(c + 1)] = static_cast<int>(sp.cls_beads.size());
         }
+        // palindrome: bead classes and bond classes read the same from both ends
+        bool pal = s->chain_symmetry == FTS_SYMMETRY_AUTO;
+        for (int s2 = 0; pal && s2 < d.N; ++s2)
+            pal = sp.cls_of_bead[static_cast<size_t>(s2)] == sp.cls_of_bead[static_cast<size_t>(d.N - 1 - s2)];
+        for (int s2 = 0; pal && s2 + 1 < d.N; ++s2)
+            pal = sp.bond_cls[static_cast<size_t>(s2)] == sp.bond_cls[static_cast<size_t>(d.N - 2 - s2)];
+        sp.mirror = pal;
+        if (pal) {
+            sp.half_start.assign(static_cast<size_t>(nc + 1), 0);
+            for (int c = 0; c < nc; ++c) {
+                for (int s2 = 0; 2 * s2 <= d.N - 1; ++s2)
+                    if (sp.cls_of_bead[static_cast<size_t>(s2)] == c) sp.half_beads.push_back(s2);
+                sp.half_start[static_cast<size_t>(c + 1)] = static_cast<int>(sp.half_beads.size());
+            }
+        }
         e->max_cls = std::max(e->max_cls, nc);
         e->Nmax = std::max(e->Nmax, d.N);
         e->total_segments += sp.n * d.N;
@@ -1550,10 +1472,12 @@ int fts_create(const fts_system* s, fts_engine** out)
     if ((rc = e->build_ions(s))) return bail(rc);
     if ((rc = e->setup_device())) return bail(rc);
     if (e->scheme == FTS_SCHEME_SEMI_IMPLICIT && (rc = e->build_si_tables())) return bail(rc);
-    if (s->fft_schedule == FTS_SCHED_AUTO && (rc = e->autotune_schedule())) return bail(rc);
+    if (tune && s->fft_schedule == FTS_SCHED_AUTO && (rc = e->autotune_schedule())) return bail(rc);
     *out = e;
     return FTS_OK;
 }
+
+extern "C" {
 
 void fts_destroy(fts_engine* h) { delete h; }
 
@@ -1573,6 +1497,9 @@ int fts_get_info(fts_engine* h, fts_info* o)
     o->fft_schedule = h->sched;
     o->tune_ms[0] = h->tune_ms[0];
     o->tune_ms[1] = h->tune_ms[1];
+    o->mirrored_species = 0;
+    for (size_t p = 0; p < h->species.size(); ++p)
+        if (h->species[p].mirror) o->mirrored_species |= 1 << p;
     o->volume = h->V;
     o->rho0 = h->rho0;
     o->short_range_self_pressure = h->P_sr;
@@ -1624,10 +1551,11 @@ int fts_get_field(fts_engine* h, int id, double* out, size_t n_doubles)
     case FTS_FIELD_ION_RHO: src = h->ion_rho.p; count = h->ions.size() * Ms; break;
     case FTS_FIELD_QF:
     case FTS_FIELD_QB: {
+        if (int rc = h->read_log_gain()) return rc;
         const HostSpecies& sp = h->species[static_cast<size_t>(h->last_species)];
         count = static_cast<size_t>(sp.N) * Ms;
         if (n_doubles != 2 * count) return fail(FTS_INVALID, "field size mismatch");
-        const void* q = id == FTS_FIELD_QF ? h->qF : h->qB;
+        const void* q = id == FTS_FIELD_QF || sp.mirror ? h->qF : h->qB;
         // undo the FP32 potential shifts: qF_s = q~F_s e^{-sum_{t<=s} shift}, qB_s = q~B_s e^{-sum_{t>=s} shift}
         std::vector<double> lc(2 * static_cast<size_t>(h->Nmax), 0.0);
         HIP_TRY(hipMemcpy(lc.data(), h->logc.p, sizeof(double) * lc.size(), hipMemcpyDeviceToHost));
@@ -1636,16 +1564,17 @@ int fts_get_field(fts_engine* h, int id, double* out, size_t n_doubles)
             for (int t = 0; t < sp.N; ++t)
                 if ((id == FTS_FIELD_QF && t <= s) || (id == FTS_FIELD_QB && t >= s)) {
                     shift += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(t)])];
-                    k2sum += lc[static_cast<size_t>(id == FTS_FIELD_QF ? t : h->Nmax + t)];
+                    k2sum += lc[static_cast<size_t>(id == FTS_FIELD_QF ? t : sp.mirror ? sp.N - 1 - t : h->Nmax + t)];
                 }
             const double scale = std::exp(-shift) * std::exp2(-k2sum);
+            const int src_s = id == FTS_FIELD_QB && sp.mirror ? sp.N - 1 - s : s;
             double2* dst = h->t1.p;
             if (h->precision == FTS_PREC_FP32)
                 hipLaunchKernelGGL((k_cast<float, double>), dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, h->stream,
-                                   static_cast<const float2*>(q) + static_cast<size_t>(s) * Ms, dst, scale, Ms);
+                                   static_cast<const float2*>(q) + static_cast<size_t>(src_s) * Ms, dst, scale, Ms);
             else
                 hipLaunchKernelGGL((k_cast<double, double>), dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, h->stream,
-                                   static_cast<const double2*>(q) + static_cast<size_t>(s) * Ms, dst, scale, Ms);
+                                   static_cast<const double2*>(q) + static_cast<size_t>(src_s) * Ms, dst, scale, Ms);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(out + 2 * static_cast<size_t>(s) * Ms, dst, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
         }

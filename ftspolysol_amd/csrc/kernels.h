@@ -7,13 +7,9 @@
 #pragma once
 
 #include "fts_device.h"
+#include "kernel_params.h"
 
 namespace fts {
-
-constexpr int RED_THREADS = 256;
-constexpr int RED_BLOCKS = 256;
-constexpr int EW_THREADS = 256;
-constexpr int MAX_CLASSES = 64;
 
 // Sum over the block of K complex values; the result is valid in thread 0.
 template <int K>
@@ -108,7 +104,6 @@ __global__ void __launch_bounds__(RED_THREADS) k_ion_boltzmann(const double2* __
 
 // Minimum of Re W over a bead class (partials in .x): the FP32 shift that caps
 // the class's Boltzmann factor at exp(BOLTZ_CAP).
-constexpr double BOLTZ_CAP = 40.0;
 __global__ void __launch_bounds__(RED_THREADS) k_min_class_w(const double2* __restrict__ Wt, const double2* __restrict__ ps, double z, int M,
                                                              double2* __restrict__ partials)
 {
@@ -127,8 +122,8 @@ __global__ void __launch_bounds__(RED_THREADS) k_min_class_w(const double2* __re
     if (threadIdx.x == 0) partials[blockIdx.x] = d2(sh[0], 0.0);
 }
 
-// out[0].x = min over the RED_BLOCKS partials (.x)
-__global__ void k_finalize_min(const double2* __restrict__ partials, double2* __restrict__ out)
+// out[0] = add + min over the RED_BLOCKS partials (.x)
+__global__ void k_finalize_min(const double2* __restrict__ partials, double add, double* __restrict__ out)
 {
     __shared__ double sh[RED_THREADS];
     sh[threadIdx.x] = partials[threadIdx.x].x;
@@ -137,7 +132,7 @@ __global__ void k_finalize_min(const double2* __restrict__ partials, double2* __
         if (threadIdx.x < off) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + off]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = d2(sh[0], 0.0);
+    if (threadIdx.x == 0) out[0] = sh[0] + add;
 }
 
 // k-space contraction: out_c = sum_k w_c(k) S(k) A(k) B(-k), c = xx yy zz xy xz yz 1
@@ -225,7 +220,8 @@ __global__ void k_contract_density(const V2<T>* __restrict__ qF, const V2<T>* __
                                    const int* __restrict__ cls_beads, const int* __restrict__ cls_type, const double* __restrict__ cls_z,
                                    const double* __restrict__ cls_shift, int ncls, const double2* __restrict__ Wt,
                                    const double2* __restrict__ ps, const double2* __restrict__ Qp, double chain_density, int electro,
-                                   const double* __restrict__ rfac, int M, double2* __restrict__ rho_hat, double2* __restrict__ rho_c_hat)
+                                   const double* __restrict__ rfac, int M, int mirrorN, double2* __restrict__ rho_hat,
+                                   double2* __restrict__ rho_c_hat)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
@@ -235,10 +231,13 @@ __global__ void k_contract_density(const V2<T>* __restrict__ qF, const V2<T>* __
         double2 acc = d2(0.0, 0.0);
         for (int t = cls_start[c]; t < cls_start[c + 1]; ++t) {
             const int bead = cls_beads[t];
-            const size_t off = static_cast<size_t>(bead) * M + i;
-            const V2<T> a = qF[off], b = qB[off];
+            const V2<T> a = qF[static_cast<size_t>(bead) * M + i];
+            // mirrored species: qB_s = qF_{N-1-s}; the lists hold beads s <= N-1-s and the
+            // pair (s, N-1-s) contributes twice the product (rfac is symmetric under s -> N-1-s)
+            const V2<T> b = mirrorN ? qF[static_cast<size_t>(mirrorN - 1 - bead) * M + i] : qB[static_cast<size_t>(bead) * M + i];
             double2 p = cmul(d2(a.x, a.y), d2(b.x, b.y));
             if (rfac) p = cscale(p, rfac[bead]);
+            if (mirrorN && 2 * bead != mirrorN - 1) p = cscale(p, 2.0);
             acc = cadd(acc, p);
         }
         double2 w = Wt[static_cast<size_t>(cls_type[c]) * M + i];
@@ -259,9 +258,10 @@ __global__ void k_contract_density(const V2<T>* __restrict__ qF, const V2<T>* __
 //   rfac[s]  = 2^{LF(N-1) - LF(s) - LB(s)}     (density term qF_s qB_s / Q)
 //   rfac2[s] = 2^{LF(N-1) - LF(s) - LB(s+1)}   (stress term qF_s qB_{s+1} / Q)
 // and out->x = LF(N-1) ln 2, the log-gain of Q.
-__global__ void k_scale_prefix(const double* __restrict__ logcF, const double* __restrict__ logcB, int N, double* __restrict__ rfac,
+__global__ void k_scale_prefix(const double* __restrict__ logcF, const double* __restrict__ logcB, int N, int mirror, double* __restrict__ rfac,
                                double* __restrict__ rfac2, double2* __restrict__ out)
 {
+    // mirrored species: the backward exponents are the forward ones reversed
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double LFtot = 0.0;
     for (int s = 0; s < N; ++s) {
@@ -272,7 +272,7 @@ __global__ void k_scale_prefix(const double* __restrict__ logcF, const double* _
     for (int s = N - 1; s >= 0; --s) {
         const double LF = rfac[s];
         if (s < N - 1) rfac2[s] = exp2(LFtot - LF - LB);
-        LB += logcB[s];
+        LB += mirror ? logcF[N - 1 - s] : logcB[s];
         rfac[s] = exp2(LFtot - LF - LB);
     }
     out->x = LFtot * 0.69314718055994530942;
@@ -317,14 +317,6 @@ __global__ void k_psi_drift(const double2* __restrict__ lap, const double2* __re
     v.y += rho[i].x;
     out[i] = v;
 }
-
-struct ModeCoefs {
-    int nO, nT;
-    double c[8];        // rho0 / |d_l|
-    double2 g[8 * 8];   // gamma_l O[l][alpha]
-    double lam_dt[8];   // dt * lambda_l
-    double sigma[8];    // noise standard deviation of mode l
-};
 
 __device__ __forceinline__ double2 mode_drift(const ModeCoefs& mc, int l, const double2* __restrict__ om, const double2* __restrict__ rho, int M, int i)
 {
@@ -416,6 +408,11 @@ __global__ void k_axpy1(double2* __restrict__ a, const double2* __restrict__ b, 
 }
 
 // flag[0] = 0 if any value is not finite
+__global__ void k_set_flag(int* __restrict__ flag, int v)
+{
+    if (threadIdx.x == 0) flag[0] = v;
+}
+
 __global__ void k_finite(const double2* __restrict__ a, size_t n, int* __restrict__ flag)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * EW_THREADS + threadIdx.x;

@@ -36,7 +36,7 @@ class System:
                  counterion=None, salt=None, ions_enabled=None, default_smearing_length=0.0,
                  dt=0.01, lambda_omega=(1.0,), lambda_psi=1.0, noise=True,
                  scheme=SCHEME_PC_REFERENCE, precision=PREC_FP64, seed=0, stress_mask=0,
-                 eigen_tol_rel=-1.0, fft_schedule=0):
+                 eigen_tol_rel=-1.0, fft_schedule=0, chain_symmetry=0):
         self._arrays = []
         d = SystemDesc()
         d.m1, d.m2, d.m3 = m
@@ -102,6 +102,7 @@ class System:
         d.seed = seed
         d.stress_mask = stress_mask
         d.fft_schedule = fft_schedule
+        d.chain_symmetry = chain_symmetry
         self.desc = d
         self.m = tuple(m)
         self.dL = dL

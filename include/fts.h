@@ -43,6 +43,13 @@ extern "C" {
 #define FTS_MAX_IONS 8
 #define FTS_MAX_TYPES 8
 
+/* chain symmetry (fts_system.chain_symmetry).  A chain whose bead sequence and bonds
+ * read the same from both ends has qB_s = qF_{N-1-s} exactly (the backward recursion
+ * repeats the forward one operation for operation), so the engine runs the forward
+ * sweep only and reads qB from the stored qF. */
+#define FTS_SYMMETRY_AUTO 0 /* mirror every palindromic species */
+#define FTS_SYMMETRY_OFF 1  /* always run both sweeps */
+
 /* bead-step FFT schedules (fts_system.fft_schedule, fts_info.fft_schedule) */
 #define FTS_SCHED_AUTO 0  /* time both at creation, keep the faster */
 #define FTS_SCHED_PLANE 1 /* pencil(x: FFT, x phi/M, IFFT) + plane(yz: IFFT, x e^{-W}, store q, FFT); needs m2 == m3, plane in LDS */
@@ -126,6 +133,7 @@ typedef struct fts_system {
     unsigned long long seed;     /* Philox4x32-10 key */
     int stress_mask;             /* FTS_STRESS_* bits; 0 = no stress observable */
     int fft_schedule;            /* FTS_SCHED_*: bead-step pass schedule; 0 = timed at creation */
+    int chain_symmetry;          /* FTS_SYMMETRY_*: mirrored sweep for palindromic chains */
 } fts_system;
 
 /* observableRecord (observable.h:7-13), with the partition functions beside it. */
@@ -170,6 +178,7 @@ typedef struct fts_info {
     long long ion_count[FTS_MAX_IONS];
     double ion_smearing[FTS_MAX_IONS];
     double tune_ms[2]; /* autotuner: time of 8 bead steps, plane and line schedule (0 if not timed) */
+    int mirrored_species; /* bit p set: species p runs the mirrored (forward-only) sweep */
 } fts_info;
 
 /* Accumulated device timing of the propagator sweeps (HIP events on the engine's
@@ -179,6 +188,9 @@ typedef struct fts_timing {
     long long bead_steps;     /* bead steps (q_s -> q_{s+1}) executed inside them, both directions */
     long long sweeps;         /* number of timed sweep launches */
     double algorithmic_bytes; /* 2*c*M per bead step, summed */
+    long long step_launches;  /* bead-step launch groups: one per bead index, carrying every job (sweep
+                               * direction, replica) of that index; sweep_ms / step_launches is the
+                               * time of one group (pencil + plane pass, or the four line passes) */
 } fts_timing;
 
 typedef struct fts_engine fts_engine;
@@ -227,6 +239,47 @@ int fts_bead_step(fts_engine* h, const double* q, const double* exp_minus_w, dou
 /* Device timing of the propagator sweeps (engine only; the oracle ignores it). */
 int fts_set_timing(fts_engine* h, int enabled);
 int fts_get_timing(fts_engine* h, fts_timing* out);
+
+/* ---- Replica batches ------------------------------------------------------
+ * No counterpart in the reference, which runs one system per process
+ * (main.cu:9-93); this is the batched form: many independent CL replicas of
+ * uneven mesh, chain length and salt stepping together.  Replicas that share
+ * mesh, precision and grid spacing form a group: every bead-step launch of a
+ * group carries the propagator jobs of all its replicas, and groups run
+ * concurrently on their own streams.  Each replica is an fts_engine owned by
+ * the batch: fts_set_fields, fts_get_field, fts_measure, fts_get_info and the
+ * other single-engine calls work on it (never fts_destroy it, and never call
+ * fts_step on it while a batch step runs).  All replicas must use one scheme. */
+#define FTS_MAX_BATCH_GROUPS 16
+
+typedef struct fts_batch fts_batch;
+
+typedef struct fts_batch_info {
+    int n_replicas;
+    int n_groups;
+    int group_size[FTS_MAX_BATCH_GROUPS];
+    int group_mesh[FTS_MAX_BATCH_GROUPS][3];
+    int group_precision[FTS_MAX_BATCH_GROUPS];
+    int group_schedule[FTS_MAX_BATCH_GROUPS];        /* FTS_SCHED_PLANE or FTS_SCHED_LINE */
+    double group_tune_ms[FTS_MAX_BATCH_GROUPS][2];   /* autotuner: 8 batched bead steps, plane and line */
+} fts_batch_info;
+
+/* Build n replicas from systems[0..n-1] (each with its own seed); fields start at 0.
+ * A group whose first system asks for FTS_SCHED_AUTO has its schedule timed with all
+ * of its replicas in each launch. */
+int fts_batch_create(const fts_system* systems, int n, fts_batch** out);
+void fts_batch_destroy(fts_batch* b);
+int fts_batch_get_info(fts_batch* b, fts_batch_info* out);
+/* Borrowed engine handle of replica r and the index of its group. */
+int fts_batch_replica(fts_batch* b, int r, fts_engine** engine, int* group);
+/* n CL steps of every replica, each exactly as fts_step would take it (same
+ * noise stream, same records).  records: [n_replicas][capacity], replica r's
+ * records from records + r * capacity; n_records: [n_replicas].  On a failure
+ * the call returns its status, and fts_last_error names the replica. */
+int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* records, int capacity, int* n_records);
+/* Device timing of the batched sweeps of one group (fts_timing as for an engine). */
+int fts_batch_set_timing(fts_batch* b, int enabled);
+int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,97 @@
+"""GPU: replica batches (fts_batch_*) against the same systems stepped one by one.
+
+A batch runs the bead steps of all replicas of a group in shared launches and
+everything else per replica, so each replica must reproduce the single-engine
+trajectory of its own system (same Philox stream): identical arithmetic, checked
+at 1e-12 relative.  The single engines themselves are pinned to the oracle by
+test_gpu_parity.py.
+"""
+import numpy as np
+import pytest
+
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, PREC_FP32, PREC_FP64, SCHED_LINE, SCHED_PLANE, SCHEME_EM,
+                                SCHEME_PC_REFERENCE, SYMMETRY_OFF, hip_batch, hip_engine)
+from ftspolysol_amd.system import initial_fields, make_system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def replica_systems(scheme, precision, sched):
+    """Four replicas in two groups: three 32^3 systems (N = 20, 12 and 16, different
+    salt and lB, one with both sweeps forced) and one 16^3 system."""
+    kw = dict(scheme=scheme, precision=precision, fft_schedule=sched, stress_mask=0x7F, dt=0.01)
+    return [
+        make_system(32, 20, salt=0.05, lB=1.0, seed=101, **kw),
+        make_system(32, 12, salt=0.2, lB=0.7, seed=102, **kw),
+        make_system(32, 16, salt=0.01, lB=1.5, seed=103, chain_symmetry=SYMMETRY_OFF, **kw),
+        make_system(16, 10, salt=0.1, lB=1.0, seed=104, **kw),
+    ]
+
+
+def start_fields(engines, systems):
+    for r, (e, s) in enumerate(zip(engines, systems)):
+        om, ps = initial_fields(s, sigma=0.05, seed=500 + r)
+        e.set_fields(om, ps)
+
+
+@pytest.mark.parametrize("scheme,precision,sched", [(SCHEME_PC_REFERENCE, PREC_FP64, SCHED_PLANE),
+                                                    (SCHEME_EM, PREC_FP64, SCHED_LINE),
+                                                    (SCHEME_EM, PREC_FP32, SCHED_PLANE),
+                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_LINE)])
+def test_batch_matches_single_engines(scheme, precision, sched):
+    print(f"batch scheme={scheme} precision={precision} sched={sched}", flush=True)
+    systems = replica_systems(scheme, precision, sched)
+    b = hip_batch(systems)
+    assert b.info.n_replicas == 4 and b.info.n_groups == 2
+    assert b.group_of == [0, 0, 0, 1]
+    assert list(b.info.group_size[:2]) == [3, 1]
+    assert [b.replicas[r].info.mirrored_species for r in range(4)] == [1, 1, 0, 1]
+    singles = [hip_engine(s) for s in replica_systems(scheme, precision, sched)]
+    start_fields(b.replicas, systems)
+    start_fields(singles, systems)
+    rb = b.step(4, record_every=2)
+    for r, e in enumerate(singles):
+        rs = e.step(4, record_every=2)
+        assert len(rb[r]) == len(rs) == 2
+        for x, y in zip(rb[r], rs):
+            assert x["step"] == y["step"]
+            assert abs(x["H"] - y["H"]) <= 1e-12 * abs(y["H"]), (r, x["H"], y["H"])
+            assert rel(x["stress"], y["stress"]) <= 1e-12
+            assert rel(x["mu_ex_poly"], y["mu_ex_poly"]) <= 1e-12
+        assert rel(b.replicas[r].get_field(FIELD_OMEGA), e.get_field(FIELD_OMEGA)) <= 1e-12
+        assert rel(b.replicas[r].get_field(FIELD_PSI), e.get_field(FIELD_PSI)) <= 1e-12
+    # a second call continues the noise stream of every replica
+    rb2 = b.step(2, record_every=1)
+    for r, e in enumerate(singles):
+        rs = e.step(2, record_every=1)
+        assert [x["step"] for x in rb2[r]] == [5, 6]
+        for x, y in zip(rb2[r], rs):
+            assert abs(x["H"] - y["H"]) <= 1e-12 * abs(y["H"])
+
+
+def test_batch_headline_autotuned():
+    """Four FP32 replicas at 96^3, N=100: the autotuner times both schedules with
+    every replica in each launch, and the batch reproduces a single engine."""
+    print("batch 96^3 x4 autotuned", flush=True)
+    systems = [make_system(96, 100, scheme=SCHEME_EM, precision=PREC_FP32, seed=7 + r) for r in range(4)]
+    b = hip_batch(systems)
+    sched = b.info.group_schedule[0]
+    print("batch tune ms (plane, line):", list(b.info.group_tune_ms[0]), "chosen", sched, flush=True)
+    assert sched in (SCHED_PLANE, SCHED_LINE)
+    assert b.info.group_tune_ms[0][0] > 0 and b.info.group_tune_ms[0][1] > 0
+    start_fields(b.replicas, systems)
+    b.set_timing(True)
+    rb = b.step(3, record_every=1)
+    t = b.get_timing(0)
+    assert t["step_launches"] == 3 * 99 and t["bead_steps"] == 4 * 3 * 99 and t["sweep_ms"] > 0
+    single = hip_engine(make_system(96, 100, scheme=SCHEME_EM, precision=PREC_FP32, seed=9, fft_schedule=sched))
+    om, ps = initial_fields(systems[2], sigma=0.05, seed=502)
+    single.set_fields(om, ps)
+    rs = single.step(3, record_every=1)
+    for x, y in zip(rb[2], rs):
+        assert abs(x["H"] - y["H"]) <= 1e-12 * abs(y["H"])

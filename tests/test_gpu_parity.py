@@ -8,7 +8,7 @@ engine 1e-4 relative to FP64 for H, mu and stress, 2e-5 for a bead step.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (SCHED_LINE, SCHED_PLANE, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
+from ftspolysol_amd.api import (SCHED_LINE, SCHED_PLANE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
                                 FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
                                 SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
 from ftspolysol_amd.system import System, initial_fields, make_system
@@ -167,6 +167,49 @@ def test_copolymer_pc_trajectory():
     for a, b in zip(rg, ro):
         r = obs_rel(a, b)
         assert max(r.values()) < 1e-10, r
+
+
+def triblock_system(**kw):
+    """A palindromic chain with three bead classes, an odd N (a middle bead) and
+    symmetric bonds: A4 B5 A4 with charges -1 on A, +0.5 on B."""
+    types = [0] * 4 + [1] * 5 + [0] * 4
+    charges = [-1.0] * 4 + [0.5] * 5 + [-1.0] * 4
+    bonds = [0.9, 1.1, 1.0, 1.2, 0.8, 1.0, 1.0, 0.8, 1.2, 1.0, 1.1, 0.9]
+    sp = dict(num_chains=8, types=types, charges=charges, bonds=bonds)
+    rho0 = 8 * 13 / (8.0 ** 3)
+    return System(m=(16, 16, 16), dL=0.5, a=1.0, species=[sp], monomer_types=2, xi=0.5 * rho0,
+                  chi=[[0.0, 0.4], [0.4, 0.0]], lB=0.9, counterion=dict(auto=True, valence=1.0),
+                  salt=dict(num_salt=12), dt=0.005, seed=41, **kw)
+
+
+@pytest.mark.parametrize("which", ["c1", "triblock"])
+def test_mirrored_sweep_matches_both_sweeps_and_oracle(which):
+    """A palindromic species runs the forward sweep only (qB_s = qF_{N-1-s}); the
+    result must match the engine with both sweeps and the oracle."""
+    print(f"mirrored sweep {which}", flush=True)
+    mk = (lambda **kw: c1_like(stress_mask=0x7F, **kw)) if which == "c1" else (lambda **kw: triblock_system(stress_mask=0x7F, **kw))
+    gm, gb, o = hip_engine(mk()), hip_engine(mk(chain_symmetry=SYMMETRY_OFF)), oracle_engine(mk())
+    assert gm.info.mirrored_species == 1 and gb.info.mirrored_species == 0
+    set_random_fields([gm, gb, o], gm.info.n_omega, mk().m, 0.1, 29)
+    Qm, Qb, Qo = gm.compute_densities(), gb.compute_densities(), o.compute_densities()
+    assert rel(Qm, Qo) < 1e-11 and rel(Qm, Qb) < 1e-12
+    for fid in FIELDS:
+        assert rel(gm.get_field(fid), o.get_field(fid)) < 1e-10, fid
+    N = mk().desc.species[0].N
+    assert rel(gm.get_field(FIELD_QB, N), o.get_field(FIELD_QB, N)) < 1e-11
+    assert rel(gm.get_field(FIELD_QB, N), gb.get_field(FIELD_QB, N)) < 1e-12
+    r = obs_rel(gm.measure(), o.measure())
+    assert max(r.values()) < 1e-10, r
+    rg, ro = gm.step(3, record_every=1), o.step(3, record_every=1)
+    for a, b in zip(rg, ro):
+        r = obs_rel(a, b)
+        assert max(r.values()) < 1e-10, r
+    # FP32 with the mirror against FP32 with both sweeps: the same propagators
+    f32m, f32b = hip_engine(mk(precision=PREC_FP32)), hip_engine(mk(precision=PREC_FP32, chain_symmetry=SYMMETRY_OFF))
+    set_random_fields([f32m, f32b], f32m.info.n_omega, mk().m, 0.1, 29)
+    a, b = f32m.measure(), f32b.measure()
+    assert abs(a["H"] - b["H"]) / abs(b["H"]) < 1e-6
+    assert rel(a["stress"], b["stress"]) < 1e-5
 
 
 def test_fp32_engine_tracks_fp64_at_96():
