@@ -1,23 +1,39 @@
 """Headline benchmark: decorrelated complex-Langevin samples/s at 96^3, N = 100.
 
-Workload (one system, one GPU): 96^3 grid, L = 48 b, discrete Gaussian chains
-N = 100, b = 1, charge -0.5 on every bead, counterions, 1:1 salt 0.05 b^-3,
-bead density 0.2 b^-3 (221 chains), excluded volume B = 0.5 b^3, lB = 1 b,
-a = 1 b, lambda = 1.  Synthetic initial fields (w, psi = 1e-3 complex Gaussian).
+Workload (one GPU): 96^3 grid, L = 48 b, discrete Gaussian chains N = 100,
+b = 1, charge -0.5 on every bead, counterions, 1:1 salt 0.05 b^-3, bead density
+0.2 b^-3 (221 chains), excluded volume B = 0.5 b^3, lB = 1 b, a = 1 b,
+lambda = 1.  Synthetic initial fields (w, psi = 1e-3 complex Gaussian).
 
-Two engines run the same workload:
-  fast   complex-FP32 propagators and FFTs with per-bead renormalisation, FP64
-         reductions and field accumulation, scheme and dt_fast selectable
-         (default: Euler-Maruyama, dt = 0.01)
-  plain  complex-FP64 engine, plain explicit Euler-Maruyama at dt_plain = 0.01
-Each takes `warmup` untimed steps, then `steps` timed steps recording Re H every
-step.  Decorrelated samples/s = (CL steps/s) / (2 tau_int), tau_int of the Re H
-series by Sokal's automatic window (c = 6).  `value` is the fast engine's.
+Two engines run the workload, both with the reference's predictor-corrector
+step (complexLangevin.cu:669-698) at dt = 0.01 by default:
+  fast   a batch of R independent replicas (own seeds and initial fields) with
+         complex-FP32 propagators and FFTs, per-bead renormalisation, FP64
+         reductions and field accumulation; every bead-step launch carries all
+         replicas.
+  plain  one complex-FP64 system per GPU, as the reference runs it.
+Replica 0 of the fast batch has the plain engine's seed and initial fields, so
+the two see the same Philox noise stream.  Each takes `warmup` untimed steps,
+then `steps` timed steps recording Re H every step.  Every replica keeps a
+device checkpoint every 50 steps; a replica whose step turns non-finite is
+restored from it with a new noise key (the reference's non-finite restart) and
+the repeated steps count in the elapsed time; the restarts are reported.
 
-roofline: the dominant kernel pair is the propagator bead step (one pencil pass
-plus one plane pass advancing the forward and backward sweeps together).
-Algorithmic bytes per bead step = read q_s + write q_{s+1} = 2 c M (c = 8 B for
-complex FP32), timed with HIP events on the engine's stream around every sweep.
+<H> error, two ways: `H_err_same_state` (the precision error) measures H, mu
+and the stress tensor with the FP32 engine and an FP64 engine on the same
+fields (the fast replica 0's final state); `H_err_stream` compares the time
+averages of fast replica 0 and the plain engine on the same noise stream (a
+statistical difference once the trajectories decorrelate; `H_err_sigma` is its
+standard error).  Decorrelated samples/s =
+(replica CL steps/s) / (2 tau_int), tau_int by Sokal's automatic window (c = 6)
+on the replica-averaged normalised autocorrelation of Re H.  `value` is the
+fast batch's whole-GPU rate.
+
+roofline: the dominant kernels are the propagator bead-step passes (one pencil
+pass and one plane pass per bead index, or the four line passes, every replica's
+job in each launch).  Algorithmic bytes per bead step = read q_s + write q_{s+1}
+= 2 c M (c = 8 B for complex FP32), summed over the jobs of a launch group and
+divided by the group's average duration from HIP events on the batch stream.
 
 cpu_baseline: the FP64 CPU restatement (oracle/) running the plain scheme on the
 host cores for a bounded sample of steps of the same workload.
@@ -34,25 +50,34 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from ftspolysol_amd.api import (Engine, PREC_FP32, PREC_FP64, SCHEME_EM, SCHEME_PC_REFERENCE,  # noqa: E402
-                                SCHEME_SEMI_IMPLICIT, hip_engine)
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, Engine, PREC_FP32, PREC_FP64, SCHEME_EM,  # noqa: E402
+                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_batch, hip_engine)
 from ftspolysol_amd.system import initial_fields, make_system  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
 
-def tau_int(x, c=6.0):
-    """Integrated autocorrelation time (in samples) with Sokal's automatic window."""
+def autocorr(x):
     x = np.asarray(x, dtype=np.float64)
     n = x.size
-    if n < 8:
-        return float("nan"), 0
     y = x - x.mean()
     f = np.fft.rfft(y, 2 * n)
     acf = np.fft.irfft(f * np.conj(f))[:n]
-    if acf[0] <= 0:
+    return acf / acf[0] if acf[0] > 0 else None
+
+
+def tau_int(x, c=6.0):
+    """Integrated autocorrelation time (in samples) with Sokal's automatic window.
+    x: one series, or a list of equally long series whose normalised
+    autocorrelations are averaged first."""
+    series = [np.asarray(x, dtype=np.float64)] if np.ndim(x) == 1 else [np.asarray(v, dtype=np.float64) for v in x]
+    n = series[0].size
+    if n < 8:
+        return float("nan"), 0
+    rhos = [r for r in (autocorr(v) for v in series) if r is not None]
+    if not rhos:
         return 0.5, 0
-    rho = acf / acf[0]
+    rho = np.mean(rhos, axis=0)
     tau = 0.5
     for w in range(1, n):
         tau += rho[w]
@@ -61,44 +86,71 @@ def tau_int(x, c=6.0):
     return float(tau), n - 1
 
 
-def run_engine(system, warmup, steps, label):
-    eng = hip_engine(system)
-    om, ps = initial_fields(system, sigma=1e-3, cfg=4)
-    eng.set_fields(om, ps)
+def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50):
+    b = hip_batch(systems)
+    for r, (e, s) in enumerate(zip(b.replicas, systems)):
+        om, ps = initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r)
+        e.set_fields(om, ps)
+        e.set_restart(ckpt_every, max_restarts)
+    R = len(systems)
     t0 = time.perf_counter()
     done = 0
     while done < warmup:
         k = min(100, warmup - done)
-        eng.step(k)
+        b.step(k)
         done += k
     print(f"[bench] {label}: warmup {warmup} steps in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    eng.set_timing(True)
-    H = []
+    b.set_timing(True)
+    H = [[] for _ in range(R)]
     t0 = time.perf_counter()
     done = 0
     while done < steps:
         k = min(200, steps - done)
-        recs = eng.step(k, record_every=1)
-        H.extend(r["H"] for r in recs)
+        recs = b.step(k, record_every=1)
+        for r in range(R):
+            H[r].extend(x["H"] for x in recs[r])
         done += k
         print(f"[bench] {label}: {done}/{steps} timed steps", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
-    timing = eng.get_timing()
-    eng.close()
+    restarts = [e.get_info().restarts for e in b.replicas]
+    final = (b.replicas[0].get_field(FIELD_OMEGA), b.replicas[0].get_field(FIELD_PSI)[0])
+    timing = b.get_timing(0)
+    sched = b.info.group_schedule[0]
+    tune = list(b.info.group_tune_ms[0])
+    b.close()
     H = np.array(H)
     tau, window = tau_int(H.real)
-    sps = steps / elapsed
-    stderr_H = float(H.real.std(ddof=1) * np.sqrt(2 * tau / len(H))) if len(H) > 1 else float("nan")
-    return dict(steps_per_s=sps, ms_per_step=1e3 * elapsed / steps, tau_int=tau, window=window,
+    sps = R * steps / elapsed
+    stderr_H = float(H.real.std(ddof=1) * np.sqrt(2 * tau / H.size))
+    return dict(replicas=R, replica_steps_per_s=sps, ms_per_step=1e3 * elapsed / steps, tau_int=tau, window=window,
                 samples_per_s=sps / (2 * tau), H_mean=float(H.real.mean()), H_imag_mean=float(H.imag.mean()),
-                H_stderr=stderr_H, timing=timing, elapsed=elapsed)
+                H_stderr=stderr_H, H0=H[0].real, timing=timing, elapsed=elapsed, schedule=sched, tune_ms=tune,
+                restarts=int(sum(restarts)), final=final)
 
 
-def cpu_baseline(system, n_steps, tau_plain):
+def same_state_errors(wl, fields):
+    """FP32 engine vs FP64 engine observables on the same fields."""
+    out = {}
+    e32 = hip_engine(make_system(**wl, precision=PREC_FP32, stress_mask=0x7F))
+    e64 = hip_engine(make_system(**wl, precision=PREC_FP64, stress_mask=0x7F))
+    for e in (e32, e64):
+        e.set_fields(*fields)
+    a, b = e32.measure(), e64.measure()
+    out["H"] = abs(a["H"] - b["H"]) / abs(b["H"])
+    out["mu_ex_poly"] = float(np.abs(a["mu_ex_poly"] - b["mu_ex_poly"]).max() / max(1.0, np.abs(b["mu_ex_poly"]).max()))
+    out["stress"] = float(np.abs(a["stress"] - b["stress"]).max() / np.abs(b["stress"]).max())
+    out["p_iso"] = abs(a["p_iso"] - b["p_iso"]) / abs(b["p_iso"])
+    out["H_fp64"] = b["H"].real
+    e32.close()
+    e64.close()
+    return out
+
+
+def cpu_baseline(system, n_steps, tau_plain, scheme_name):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "libfts_oracle.so"))
     eng = Engine(lib, "fto_", system)
-    om, ps = initial_fields(system, sigma=1e-3, cfg=4)
+    om, ps = initial_fields(system, sigma=1e-3, seed=0x5EED0004)
     eng.set_fields(om, ps)
     t0 = time.perf_counter()
     eng.step(n_steps)
@@ -107,7 +159,7 @@ def cpu_baseline(system, n_steps, tau_plain):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": sps / (2 * tau_plain), "unit": "decorrelated CL samples/s", "cores": cores, "kind": "port",
             "steps_per_s": sps,
-            "sample": f"{n_steps} plain FP64 Euler-Maruyama steps of the 96^3 N=100 workload on the host "
+            "sample": f"{n_steps} plain FP64 {scheme_name} steps of the 96^3 N=100 workload on the host "
                       f"({el:.1f} s, own mixed-radix FFT, OpenMP); samples/s uses the GPU plain engine's tau_int"}
 
 
@@ -116,36 +168,48 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--replicas", type=int, default=8, help="replicas in the fast batch")
+    ap.add_argument("--plain-replicas", type=int, default=1)
     ap.add_argument("--dt-fast", type=float, default=0.01)
-    ap.add_argument("--fast-scheme", choices=["em", "si", "pc"], default="em")
+    ap.add_argument("--fast-scheme", choices=["em", "si", "pc"], default="pc")
+    ap.add_argument("--plain-scheme", choices=["em", "si", "pc"], default="pc")
     ap.add_argument("--dt-plain", type=float, default=0.01)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs one system on one GPU; multi-GPU runs are not part of this build "
+        print("bench.py measures one GPU; runs on several GPUs are not part of this build "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
-    fast_scheme = {"em": SCHEME_EM, "si": SCHEME_SEMI_IMPLICIT, "pc": SCHEME_PC_REFERENCE}[args.fast_scheme]
-    fast_sys = make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=fast_scheme, seed=0xC4)
-    plain_sys = make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_EM, seed=0xC4)
+    schemes = {"em": SCHEME_EM, "si": SCHEME_SEMI_IMPLICIT, "pc": SCHEME_PC_REFERENCE}
+    fast_scheme, plain_scheme = schemes[args.fast_scheme], schemes[args.plain_scheme]
+    fast_sys = [make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=fast_scheme, seed=0xC4 + r)
+                for r in range(args.replicas)]
+    plain_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=plain_scheme, seed=0xC4 + r)
+                 for r in range(args.plain_replicas)]
 
-    fast = run_engine(fast_sys, args.warmup, args.steps, f"fast fp32 {args.fast_scheme}")
-    plain = run_engine(plain_sys, args.warmup, args.steps, "plain fp64 EM")
+    fast = run_batch(fast_sys, args.warmup, args.steps, f"fast fp32 {args.fast_scheme} x{args.replicas}")
+    plain = run_batch(plain_sys, args.warmup, args.steps, f"plain fp64 {args.plain_scheme} x{args.plain_replicas}")
+    same = same_state_errors(wl, fast["final"])
 
-    t = fast["timing"]
-    per_pair_s = (t["sweep_ms"] * 1e-3) / (t["bead_steps"] / 2) if t["bead_steps"] else float("nan")
-    bytes_per_pair = t["algorithmic_bytes"] / (t["bead_steps"] / 2) if t["bead_steps"] else float("nan")
-    achieved = bytes_per_pair / per_pair_s / 1e9
-    tp = plain["timing"]
-    plain_pair_s = (tp["sweep_ms"] * 1e-3) / (tp["bead_steps"] / 2) if tp["bead_steps"] else float("nan")
-    plain_achieved = (tp["algorithmic_bytes"] / (tp["bead_steps"] / 2)) / plain_pair_s / 1e9
+    def roof(t):
+        if not t["step_launches"]:
+            return float("nan"), float("nan"), float("nan")
+        us = 1e3 * t["sweep_ms"] / t["step_launches"]
+        byt = t["algorithmic_bytes"] / t["step_launches"]
+        return us, byt, byt / (us * 1e-6) / 1e9
 
-    H_err = abs(fast["H_mean"] - plain["H_mean"]) / abs(plain["H_mean"])
+    us, byt, achieved = roof(fast["timing"])
+    _, _, plain_achieved = roof(plain["timing"])
+    # same Philox stream: fast replica 0 and plain replica 0
+    n = min(fast["H0"].size, plain["H0"].size)
+    H_err_stream = abs(fast["H0"][:n].mean() - plain["H0"][:n].mean()) / abs(plain["H0"][:n].mean())
     H_err_sigma = np.hypot(fast["H_stderr"], plain["H_stderr"]) / abs(plain["H_mean"])
-
+    keys = ("replicas", "replica_steps_per_s", "ms_per_step", "tau_int", "samples_per_s", "H_mean", "H_stderr",
+            "schedule", "tune_ms", "restarts")
+    sched_name = {1: "plane", 2: "line"}
     out = {
         "metric": "decorrelated CL samples/s at 96^3 N=100, FP32+learned vs plain FP64, <H> err",
         "value": fast["samples_per_s"],
@@ -160,21 +224,26 @@ def main():
         "dtype": "c64 propagators/FFT, f64 reductions",
         "data": "synthetic",
         "config": {"workload": "96^3 L=48 N=100 z=-0.5 salt=0.05 B=0.5 lB=1 (221 chains)",
-                   "fast": f"FP32 {args.fast_scheme} dt={args.dt_fast} (learned preconditioner not yet in this engine)",
-                   "plain": f"FP64 Euler-Maruyama dt={args.dt_plain}", "parallelism": "single GPU"},
-        "fast": {k: fast[k] for k in ("steps_per_s", "tau_int", "samples_per_s", "H_mean", "H_stderr")},
-        "plain_fp64": {k: plain[k] for k in ("steps_per_s", "tau_int", "samples_per_s", "H_mean", "H_stderr")},
+                   "fast": f"FP32 {args.fast_scheme} dt={args.dt_fast}, {args.replicas} replicas per launch "
+                           f"(learned preconditioner not yet in this engine)",
+                   "plain": f"FP64 {args.plain_scheme} dt={args.dt_plain}, {args.plain_replicas} replica(s)",
+                   "parallelism": "single GPU"},
+        "fast": {k: fast[k] for k in keys},
+        "plain_fp64": {k: plain[k] for k in keys},
         "speedup_vs_plain_fp64": fast["samples_per_s"] / plain["samples_per_s"],
-        "H_err": H_err,
-        "H_err_sigma": H_err_sigma,
+        "H_err_same_state": float(same["H"]),
+        "same_state_rel_err": {k: float(v) for k, v in same.items() if k != "H_fp64"},
+        "H_err_stream": float(H_err_stream),
+        "H_err_sigma": float(H_err_sigma),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "bead step (pencil + plane pass, fwd and bwd sweeps), FP32",
-                     "us_per_launch_pair": per_pair_s * 1e6, "alg_bytes_per_pair": bytes_per_pair,
+                     "kernel": f"bead-step launch group ({sched_name.get(fast['schedule'], '?')} schedule), FP32, "
+                               f"{args.replicas} replica jobs per launch",
+                     "us_per_launch_group": us, "alg_bytes_per_launch_group": byt,
                      "plain_fp64_achieved": plain_achieved},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(plain_sys, args.cpu_steps, plain["tau_int"])
+        out["cpu_baseline"] = cpu_baseline(plain_sys[0], args.cpu_steps, plain["tau_int"], args.plain_scheme.upper())
     print(json.dumps(out))
 
 

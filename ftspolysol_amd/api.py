@@ -88,7 +88,7 @@ class Info(C.Structure):
                 ("gamma_is_imag", C.c_double * MAX_TYPES), ("mode_O", (C.c_double * MAX_TYPES) * MAX_TYPES),
                 ("ion_valence", C.c_double * MAX_IONS), ("ion_count", C.c_longlong * MAX_IONS),
                 ("ion_smearing", C.c_double * MAX_IONS), ("tune_ms", C.c_double * 2),
-                ("mirrored_species", C.c_int)]
+                ("mirrored_species", C.c_int), ("restarts", C.c_int), ("noise_epoch", C.c_int)]
 
 
 class Timing(C.Structure):
@@ -125,6 +125,8 @@ def _bind(lib, prefix):
         "measure": (C.c_int, [vp, P(Observables)]),
         "step": (C.c_int, [vp, C.c_int, C.c_int, P(Observables), C.c_int, P(C.c_int)]),
         "set_step": (C.c_int, [vp, C.c_longlong]),
+        "set_restart": (C.c_int, [vp, C.c_int, C.c_int]),
+        "inject_nonfinite": (C.c_int, [vp, C.c_longlong]),
         "fft3d": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_int]),
         "bead_step": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)]),
         "set_timing": (C.c_int, [vp, C.c_int]),
@@ -233,6 +235,12 @@ class Engine:
 
     def set_step(self, step):
         self._check(self._fns["set_step"](self._h, step))
+
+    def set_restart(self, checkpoint_every, max_restarts):
+        self._check(self._fns["set_restart"](self._h, checkpoint_every, max_restarts))
+
+    def inject_nonfinite(self, step):
+        self._check(self._fns["inject_nonfinite"](self._h, step))
 
     def fft3d(self, x, sign):
         xin = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).reshape(-1)

@@ -30,6 +30,7 @@ struct SweepLaunch {
 // index s the launches of that step; all jobs live in three device arrays.
 struct SweepTables {
     int p = -1, sched = 0, steps = 0;
+    std::vector<char> mask; // participating replicas of the group
     std::vector<SweepLaunch> launches;
     size_t init_launches = 0;
     long long jobs = 0;      // bead steps (jobs summed over the step launches)
@@ -119,11 +120,14 @@ template <typename X> int upload(const std::vector<X>& v, void** dst)
 // replica whose chain still has bead index s + 1 in launch s.  A replica whose
 // chain ends before the group's longest takes the non-final pass form on its
 // last step (the extra forward FFT writes only its scratch field).
-template <typename T> int build_tables(Group& g, int p, int sched, int max_steps, SweepTables& tb)
+template <typename T> int build_tables(Group& g, int p, int sched, int max_steps, const std::vector<char>& mask, SweepTables& tb)
 {
     std::vector<RepView<T>> act;
-    for (fts_engine* e : g.reps)
-        if (static_cast<int>(e->species.size()) > p && e->species[static_cast<size_t>(p)].N > 1) act.push_back(view<T>(e, p));
+    for (size_t r = 0; r < g.reps.size(); ++r) {
+        fts_engine* e = g.reps[r];
+        if (mask[r] && static_cast<int>(e->species.size()) > p && e->species[static_cast<size_t>(p)].N > 1) act.push_back(view<T>(e, p));
+    }
+    tb.mask = mask;
     std::stable_sort(act.begin(), act.end(), [](const RepView<T>& a, const RepView<T>& b) { return a.sp->N > b.sp->N; });
     tb.p = p;
     tb.sched = sched;
@@ -269,15 +273,20 @@ int resolve_group_timing(Group& g)
     return FTS_OK;
 }
 
-template <typename T> int group_sweeps(Group& g, int p)
+template <typename T> int group_sweeps(Group& g, int p, const std::vector<char>& mask)
 {
     SweepTables* tb = nullptr;
     for (auto& t : g.tables)
-        if (t->p == p && t->sched == g.sched) tb = t.get();
+        if (t->p == p && t->sched == g.sched && t->mask == mask) tb = t.get();
     if (!tb) {
+        // tables are cached per participant set; sets change only around restarts
+        if (g.tables.size() >= 32) {
+            HIP_TRY(hipStreamSynchronize(g.stream));
+            g.tables.clear();
+        }
         g.tables.emplace_back(new SweepTables());
         tb = g.tables.back().get();
-        int rc = build_tables<T>(g, p, g.sched, -1, *tb);
+        int rc = build_tables<T>(g, p, g.sched, -1, mask, *tb);
         if (rc) return rc;
     }
     if (tb->steps == 0) return FTS_OK;
@@ -304,68 +313,77 @@ template <typename T> int group_sweeps(Group& g, int p)
     return FTS_OK;
 }
 
-// compute_densities of every replica of the group at (w, psi) or (w_bar, psi_bar)
-template <typename T> int group_densities(Group& g, bool bar)
+// compute_densities of the replicas in mask at (w, psi) or (w_bar, psi_bar)
+template <typename T> int group_densities(Group& g, bool bar, const std::vector<char>& mask)
 {
     int rc;
     size_t max_species = 0;
-    for (fts_engine* e : g.reps) {
+    for (size_t r = 0; r < g.reps.size(); ++r) {
+        if (!mask[r]) continue;
+        fts_engine* e = g.reps[r];
         if ((rc = e->densities_begin(bar ? e->omega_bar.p : e->omega.p, bar ? e->psi_bar.p : e->psi.p))) return rc;
         max_species = std::max(max_species, e->species.size());
     }
     for (int p = 0; p < static_cast<int>(max_species); ++p) {
-        for (fts_engine* e : g.reps)
-            if (static_cast<int>(e->species.size()) > p && (rc = e->solve_begin<T>(p))) return rc;
-        if ((rc = group_sweeps<T>(g, p))) return rc;
-        for (fts_engine* e : g.reps)
-            if (static_cast<int>(e->species.size()) > p && (rc = e->solve_end<T>(p, true))) return rc;
+        for (size_t r = 0; r < g.reps.size(); ++r)
+            if (mask[r] && static_cast<int>(g.reps[r]->species.size()) > p && (rc = g.reps[r]->solve_begin<T>(p))) return rc;
+        if ((rc = group_sweeps<T>(g, p, mask))) return rc;
+        for (size_t r = 0; r < g.reps.size(); ++r)
+            if (mask[r] && static_cast<int>(g.reps[r]->species.size()) > p && (rc = g.reps[r]->solve_end<T>(p, true))) return rc;
     }
-    for (fts_engine* e : g.reps)
-        if ((rc = e->densities_end())) return rc;
+    for (size_t r = 0; r < g.reps.size(); ++r)
+        if (mask[r] && (rc = g.reps[r]->densities_end())) return rc;
     return FTS_OK;
 }
 
 bool finite_c(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
 
-int group_check_Q(Group& g, const char* what)
+// Q of the replicas in mask; a non-finite one leaves mask and is reported in why
+int group_check_Q(Group& g, const char* what, std::vector<char>& mask, std::vector<std::string>& why)
 {
     int rc;
     for (size_t r = 0; r < g.reps.size(); ++r)
-        if ((rc = g.reps[r]->q_queue(g.hQ + r * FTS_MAX_SPECIES))) return rc;
+        if (mask[r] && (rc = g.reps[r]->q_queue(g.hQ + r * FTS_MAX_SPECIES))) return rc;
     HIP_TRY(hipStreamSynchronize(g.stream));
-    for (size_t r = 0; r < g.reps.size(); ++r)
+    for (size_t r = 0; r < g.reps.size(); ++r) {
+        if (!mask[r]) continue;
         for (size_t p = 0; p < g.reps[r]->species.size(); ++p)
-            if (!finite_c(g.hQ[r * FTS_MAX_SPECIES + p]))
-                return fail(FTS_NONFINITE, "replica " + std::to_string(g.ids[r]) + ": non-finite " + what + " at step " +
-                                               std::to_string(g.reps[r]->step_counter + 1));
+            if (!finite_c(g.hQ[r * FTS_MAX_SPECIES + p])) {
+                why[r] = std::string("non-finite ") + what + " at step " + std::to_string(g.reps[r]->step_counter + 1);
+                mask[r] = 0;
+                break;
+            }
+    }
     return FTS_OK;
 }
 
-// One CL step of every replica of the group; rec[r] non-null where replica r records.
-template <typename T> int group_step(Group& g, int scheme, const std::vector<fts_observables*>& rec)
+// One CL step of the replicas in mask; rec[r] non-null where replica r records.
+// A replica whose step fails leaves the step with its reason in why[r] (the
+// caller restarts it or gives up); device errors return a status.
+template <typename T> int group_step(Group& g, int scheme, const std::vector<fts_observables*>& rec, std::vector<char> mask,
+                                     std::vector<std::string>& why)
 {
     int rc;
     const size_t n = g.reps.size();
-    if ((rc = group_densities<T>(g, false)) || (rc = group_check_Q(g, "Q"))) return rc;
+    if ((rc = group_densities<T>(g, false, mask)) || (rc = group_check_Q(g, "Q", mask, why))) return rc;
     if (scheme == FTS_SCHEME_PC_REFERENCE) {
         for (size_t r = 0; r < n; ++r)
-            if ((rc = g.reps[r]->pc_predict())) return rc;
-        if ((rc = group_densities<T>(g, true)) || (rc = group_check_Q(g, "Qbar"))) return rc;
+            if (mask[r] && (rc = g.reps[r]->pc_predict())) return rc;
+        if ((rc = group_densities<T>(g, true, mask)) || (rc = group_check_Q(g, "Qbar", mask, why))) return rc;
         for (size_t r = 0; r < n; ++r)
-            if ((rc = g.reps[r]->pc_correct(rec[r] != nullptr, rec[r]))) return rc;
+            if (mask[r] && (rc = g.reps[r]->pc_correct(rec[r] != nullptr, rec[r]))) return rc;
     } else {
         for (size_t r = 0; r < n; ++r)
-            if ((rc = g.reps[r]->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT))) return rc;
+            if (mask[r] && (rc = g.reps[r]->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT))) return rc;
     }
     for (size_t r = 0; r < n; ++r) {
+        if (!mask[r]) continue;
         ++g.reps[r]->step_counter;
         if ((rc = g.reps[r]->finite_queue(g.hflag + r))) return rc;
     }
     HIP_TRY(hipStreamSynchronize(g.stream));
     for (size_t r = 0; r < n; ++r)
-        if (!g.hflag[r])
-            return fail(FTS_NONFINITE, "replica " + std::to_string(g.ids[r]) + ": updated fields have non-finite values at step " +
-                                           std::to_string(g.reps[r]->step_counter));
+        if (mask[r] && !g.hflag[r]) why[r] = "updated fields have non-finite values at step " + std::to_string(g.reps[r]->step_counter);
     return FTS_OK;
 }
 
@@ -378,8 +396,9 @@ template <typename T> int group_autotune(Group& g)
         return FTS_OK;
     }
     SweepTables t[2];
+    const std::vector<char> all(g.reps.size(), 1);
     int rc;
-    if ((rc = build_tables<T>(g, 0, FTS_SCHED_PLANE, 8, t[0])) || (rc = build_tables<T>(g, 0, FTS_SCHED_LINE, 8, t[1]))) return rc;
+    if ((rc = build_tables<T>(g, 0, FTS_SCHED_PLANE, 8, all, t[0])) || (rc = build_tables<T>(g, 0, FTS_SCHED_LINE, 8, all, t[1]))) return rc;
     if (t[0].steps == 0) return FTS_OK;
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
@@ -522,21 +541,53 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
         for (size_t r = 0; r < R; ++r) n_records[r] = 0;
     auto run_group = [&](Group& g) {
         g.rc = FTS_OK;
-        std::vector<fts_observables*> rec(g.reps.size(), nullptr);
-        for (int k = 0; k < n && g.rc == FTS_OK; ++k) {
-            for (size_t r = 0; r < g.reps.size(); ++r) {
+        const size_t n_rep = g.reps.size();
+        std::vector<fts_observables*> rec(n_rep, nullptr);
+        std::vector<long long> target(n_rep);
+        for (size_t r = 0; r < n_rep; ++r) target[r] = g.reps[r]->step_counter + n;
+        std::vector<char> mask(n_rep);
+        std::vector<std::string> why(n_rep);
+        std::vector<long long> label(n_rep);
+        while (g.rc == FTS_OK) {
+            bool any = false;
+            for (size_t r = 0; r < n_rep; ++r) {
+                mask[r] = g.reps[r]->step_counter < target[r];
+                any = any || mask[r];
+            }
+            if (!any) break;
+            for (size_t r = 0; r < n_rep && g.rc == FTS_OK; ++r) {
+                rec[r] = nullptr;
+                why[r].clear();
+                if (!mask[r]) continue;
+                g.rc = g.reps[r]->maybe_checkpoint();
                 const int id = g.ids[r];
-                const long long step = g.reps[r]->step_counter + 1;
+                const long long step = g.reps[r]->step_counter + 1; // 1-based label, as fts_step
                 const int have = n_records ? n_records[id] : 0;
                 const bool want = record_every > 0 && step % record_every == 0 && records && n_records && have < capacity;
                 rec[r] = want ? records + static_cast<size_t>(id) * capacity + have : nullptr;
+                label[r] = step;
             }
-            g.rc = g.precision == FTS_PREC_FP32 ? group_step<float>(g, b->scheme, rec) : group_step<double>(g, b->scheme, rec);
-            for (size_t r = 0; r < g.reps.size(); ++r)
+            if (g.rc) break;
+            g.rc = g.precision == FTS_PREC_FP32 ? group_step<float>(g, b->scheme, rec, mask, why) : group_step<double>(g, b->scheme, rec, mask, why);
+            if (g.rc) break;
+            for (size_t r = 0; r < n_rep; ++r) {
+                const int id = g.ids[r];
                 if (rec[r]) {
-                    rec[r]->step = g.reps[r]->step_counter;
-                    ++n_records[g.ids[r]];
+                    rec[r]->step = label[r];
+                    ++n_records[id];
                 }
+                if (why[r].empty()) continue;
+                // non-finite: restart this replica from its checkpoint with new noise, or give up
+                fts_engine* e = g.reps[r];
+                g_error = why[r];
+                int rr = e->restart_from_checkpoint(e->step_counter);
+                if (rr) {
+                    g.rc = fail(rr, "replica " + std::to_string(id) + ": " + g_error);
+                    break;
+                }
+                if (n_records)
+                    while (n_records[id] > 0 && records[static_cast<size_t>(id) * capacity + n_records[id] - 1].step > e->ckpt_step) --n_records[id];
+            }
         }
         if (g.rc) g.err = g_error;
     };

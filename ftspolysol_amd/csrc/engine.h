@@ -102,10 +102,10 @@ enum {
     SC_DRIFT_MEAN = 18,
     SC_TMP = 19,
     SC_CONTRACT = 24, // [7]
-    SC_H = 32,
-    SC_LF = 33,       // [8] per species: log-gain LF(N-1) of the renormalised forward sweep
-    SC_SHIFT = 40,    // [MAX_CLASSES]
-    SC_COUNT = 40 + MAX_CLASSES
+    SC_LF = 32,       // [8] per species: log-gain LF(N-1) of the renormalised forward sweep
+    SC_H = 40,        // [8] per interaction mode: (c_l / 2) <omega_l^2>
+    SC_PSI_S = 48,    // [7] k-space contraction of psi with itself
+    SC_COUNT = 56
 };
 
 } // namespace fts_host
@@ -217,8 +217,14 @@ struct fts_engine {
     int psi_drift(const double2* ps, const double2* rho_src, double2* out);
     int check_finite(bool& ok);
     int measure(const double2* om, const double2* ps, fts_observables* rec);
-    template <typename T> int stress_chain(int p, std::vector<std::array<double2, 7>>& acc);
-    int contract(const void* A, const void* B, bool fp32, const double* sep, double2 out[7]);
+    template <typename T> int stress_chain(int p);
+    int contract_to(const void* A, const void* B, bool fp32, const double* sep, double2* dst);
+    // measure(): device results gathered in stress_acc and copied, with the scalar
+    // slots, the FP32 shifts and rfac2, into pinned host memory behind one sync
+    DevBuf<double2> stress_acc;
+    size_t stress_chain_base(int p) const; // first stress_acc row of species p's chain
+    void* pin = nullptr;                   // pinned staging
+    size_t pin_bytes = 0;
     int step_pc(bool record, fts_observables* rec);
     int step_em(bool record, fts_observables* rec, bool si);
     // the parts of a CL step between its density computations
@@ -230,6 +236,18 @@ struct fts_engine {
     // timing
     hipEvent_t get_event();
     int resolve_timing();
+
+    // Non-finite restart (complexLangevin.cu:518-667, 640-650): a device copy of the
+    // fields every ckpt_every steps; a non-finite step restores it and moves the
+    // noise to a new Philox key (seed0 + epoch * 0x9E3779B97F4A7C15), so that the
+    // retried steps do not replay the noise that led to the failure.
+    int ckpt_every = 0, max_restarts = 0, restarts = 0, noise_epoch = 0;
+    long long ckpt_step = -1;
+    long long inject_step = -1; // test hook: the step ending at this count leaves a NaN in psi (once)
+    unsigned long long seed0 = 0;
+    DevBuf<double2> ckpt_omega, ckpt_psi;
+    int maybe_checkpoint();
+    int restart_from_checkpoint(long long failed_step);
 };
 
 // Builds an engine for system s.  shared: a stream the engine uses instead of

@@ -38,6 +38,11 @@ void fts_engine::release()
                                &t3, &mode_coef, &partials, &scal})
         b->release();
     flag.release();
+    ckpt_omega.release();
+    ckpt_psi.release();
+    stress_acc.release();
+    if (pin) (void)hipHostFree(pin);
+    pin = nullptr;
     for (HostSpecies& sp : species) {
         sp.d_cls_type.release();
         sp.d_cls_start.release();
@@ -353,6 +358,9 @@ int fts_engine::setup_device()
     for (DevBuf<double2>* b : {&ion_psi, &ion_b, &ion_rho}) HIP_TRY(b->alloc(nI * Ms));
     HIP_TRY(partials.alloc(8 * RED_BLOCKS));
     HIP_TRY(scal.alloc(SC_COUNT));
+    HIP_TRY(stress_acc.alloc((stress_chain_base(static_cast<int>(species.size())) + nTs + 1 + nI) * 7));
+    pin_bytes = sizeof(double2) * (SC_COUNT + stress_acc.n) + sizeof(double) * species.size() * (static_cast<size_t>(Nmax) + MAX_CLASSES);
+    HIP_TRY(hipHostMalloc(&pin, pin_bytes));
     HIP_TRY(flag.alloc(1));
     HIP_TRY(mode_coef.alloc(std::max<size_t>(1, nO * nTs)));
     mode_coef_host.assign(std::max<size_t>(1, nO * nTs), double2{0.0, 0.0});
@@ -923,6 +931,10 @@ int fts_engine::psi_drift(const double2* ps, const double2* rho_src, double2* ou
 
 int fts_engine::finite_queue(int* host_out)
 {
+    if (inject_step >= 0 && step_counter == inject_step) {
+        hipLaunchKernelGGL(k_fill, dim3(1), dim3(EW_THREADS), 0, stream, psi.p, double2{std::nan(""), 0.0}, 1);
+        inject_step = -1;
+    }
     hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, stream, flag.p, 1);
     const size_t no = static_cast<size_t>(n_omega) * M;
     if (no) hipLaunchKernelGGL(k_finite, dim3(ew_blocks(no)), dim3(EW_THREADS), 0, stream, omega.p, no, flag.p);
@@ -948,8 +960,8 @@ int fts_engine::check_finite(bool& ok)
     return FTS_OK;
 }
 
-// out_c = sum_k w_c(k) S(k) A(k) B(-k)
-int fts_engine::contract(const void* A, const void* B, bool fp32, const double* sep, double2 out[7])
+// dst[c] = sum_k w_c(k) S(k) A(k) B(-k), c = xx yy zz xy xz yz 1 (device slots)
+int fts_engine::contract_to(const void* A, const void* B, bool fp32, const double* sep, double2* dst)
 {
     if (fp32)
         hipLaunchKernelGGL(k_contract_kspace<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, static_cast<const float2*>(A),
@@ -957,34 +969,57 @@ int fts_engine::contract(const void* A, const void* B, bool fp32, const double* 
     else
         hipLaunchKernelGGL(k_contract_kspace<double>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, static_cast<const double2*>(A),
                            static_cast<const double2*>(B), kvec.p, sep, m1, m2, m3, partials.p);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, partials.p, 7, dst, 1.0);
     HIP_TRY(hipGetLastError());
-    int rc = reduce_finalize(7, SC_CONTRACT, 1.0);
-    if (rc) return rc;
-    return read_scal(SC_CONTRACT, 7, out);
+    return FTS_OK;
 }
 
-// Chain-connectivity sums of species p: acc[s][c] = sum_k w_c phi_s(k) FFT[qB_{s+1}](k) FFT[qF_s](-k),
-// the k-space form of <qF_s S_{-k_a k_b}[e^{W_s} qB_s]> with FFT[e^{W_s} qB_s] = phi_s FFT[qB_{s+1}].
-template <typename T> int fts_engine::stress_chain(int p, std::vector<std::array<double2, 7>>& acc)
+size_t fts_engine::stress_chain_base(int p) const
+{
+    size_t base = 0;
+    for (int q = 0; q < p; ++q) base += static_cast<size_t>(std::max(1, species[static_cast<size_t>(q)].N - 1));
+    return base;
+}
+
+// Chain-connectivity sums of species p into stress_acc rows base + s:
+// sum_k w_c phi_s(k) FFT[qB_{s+1}](k) FFT[qF_s](-k), the k-space form of
+// <qF_s S_{-k_a k_b}[e^{W_s} qB_s]> with FFT[e^{W_s} qB_s] = phi_s FFT[qB_{s+1}].
+template <typename T> int fts_engine::stress_chain(int p)
 {
     using Cv = V2<T>;
     const HostSpecies& sp = species[static_cast<size_t>(p)];
     const size_t Ms = static_cast<size_t>(M);
     Cv* qf = static_cast<Cv*>(qF);
     Cv* qb = static_cast<Cv*>(qB);
+    double2* rows = stress_acc.p + stress_chain_base(p) * 7;
     for (int s = 0; s + 1 < sp.N; ++s) {
         int rc = fft_forward_t<T>(qf + static_cast<size_t>(s) * Ms, static_cast<Cv*>(kA));
         if (rc) return rc;
         rc = fft_forward_t<T>(sp.mirror ? qf + static_cast<size_t>(sp.N - 2 - s) * Ms : qb + static_cast<size_t>(s + 1) * Ms, static_cast<Cv*>(kB));
         if (rc) return rc;
         const double* sep = sep_bond.p + static_cast<size_t>(sp.bond_cls[static_cast<size_t>(s)]) * (m1 + m2 + m3);
-        rc = contract(kB, kA, sizeof(T) == 4, sep, acc[static_cast<size_t>(s)].data());
-        if (rc) return rc;
+        if ((rc = contract_to(kB, kA, sizeof(T) == 4, sep, rows + static_cast<size_t>(s) * 7))) return rc;
     }
     return FTS_OK;
 }
 
-// observable::measure (observable.cu:53-67) at (om, ps), after compute_densities at the same state.
+namespace {
+double2 cinv_host(double2 q)
+{
+    // Smith's scaled division (complex_inverse, common.h:127-139)
+    const double ax = std::fabs(q.x), ay = std::fabs(q.y);
+    if (ax >= ay) {
+        const double r = q.y / q.x, d = q.x * (1.0 + r * r);
+        return double2{1.0 / d, -r / d};
+    }
+    const double r = q.x / q.y, d = q.y * (1.0 + r * r);
+    return double2{r / d, -1.0 / d};
+}
+} // namespace
+
+// observable::measure (observable.cu:53-67) at (om, ps), after compute_densities at
+// the same state.  All device reductions are queued first; their results, the
+// scalar slots and the renormalisation data come back in one copy to pinned memory.
 int fts_engine::measure(const double2* om, const double2* ps, fts_observables* rec)
 {
     std::memset(rec, 0, sizeof(*rec));
@@ -994,14 +1029,75 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
     rec->stress_mask = stress_mask;
     const double invM = 1.0 / static_cast<double>(M);
     const double invM2 = invM * invM;
+    const size_t nsp = species.size();
+    const bool want_stress = (stress_mask & 0x7F) != 0;
     int rc;
-    double2 Q[FTS_MAX_SPECIES], Qi[FTS_MAX_IONS];
-    if ((rc = read_scal(SC_Q, static_cast<int>(species.size()), Q))) return rc;
-    if (!ions.empty() && (rc = read_scal(SC_QION, static_cast<int>(ions.size()), Qi))) return rc;
-    if ((rc = read_log_gain())) return rc;
+    // ---- queue ----------------------------------------------------------------
+    // H/V field terms: (c_l/2) <omega_l^2> and (1/(8 pi lB)) <psi S_{k^2} psi>  (observable.cu:160-182)
+    for (int l = 0; l < n_omega; ++l) {
+        const double2* w = om + static_cast<size_t>(l) * M;
+        hipLaunchKernelGGL(k_sum_product, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, w, w, M, partials.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = reduce_finalize(1, SC_H + l, 0.5 * omega_coefficient(l) * invM))) return rc;
+    }
+    if (electro()) {
+        if ((rc = fft_forward(ps, t3.p))) return rc;
+        if ((rc = contract_to(t3.p, t3.p, false, nullptr, scal.p + SC_PSI_S))) return rc;
+    }
+    // pinned staging: [scal][stress_acc][rfac2: nsp x Nmax doubles][shifts: nsp x MAX_CLASSES doubles]
+    double2* h_scal = static_cast<double2*>(pin);
+    double2* h_acc = h_scal + SC_COUNT;
+    double* h_rfac2 = reinterpret_cast<double*>(h_acc + stress_acc.n);
+    double* h_shift = h_rfac2 + nsp * static_cast<size_t>(Nmax);
+    const size_t fixed_rows = stress_chain_base(static_cast<int>(nsp)); // rows after the chains
+    if (want_stress) {
+        for (size_t p = 0; p < nsp; ++p) {
+            if (nsp > 1) {
+                // the propagator store holds one species: re-solve species p at the same state
+                rc = precision == FTS_PREC_FP32 ? solve_species<float>(static_cast<int>(p), false) : solve_species<double>(static_cast<int>(p), false);
+                if (rc) return rc;
+            }
+            rc = precision == FTS_PREC_FP32 ? stress_chain<float>(static_cast<int>(p)) : stress_chain<double>(static_cast<int>(p));
+            if (rc) return rc;
+            if (species[p].N > 1)
+                HIP_TRY(hipMemcpyAsync(h_rfac2 + p * Nmax, rfac2.p, sizeof(double) * (species[p].N - 1), hipMemcpyDeviceToHost, stream));
+        }
+        // polymer field coupling (one row per type) and, with electrostatics, the psi row and one row per ion
+        double2* fixed = stress_acc.p + fixed_rows * 7;
+        for (int al = 0; al < nT; ++al) {
+            if ((rc = fft_forward(f(W_type_s, al), t1.p)) || (rc = fft_forward(f(rho_hat, al), t2.p))) return rc;
+            if ((rc = contract_to(t1.p, t2.p, false, nullptr, fixed + static_cast<size_t>(al) * 7))) return rc;
+        }
+        if (electro()) {
+            if ((rc = fft_forward(psi_s.p, t1.p)) || (rc = fft_forward(rho_c_hat.p, t2.p))) return rc;
+            if ((rc = contract_to(t1.p, t2.p, false, nullptr, fixed + static_cast<size_t>(nT) * 7))) return rc;
+            for (size_t j = 0; j < ions.size(); ++j) {
+                if ((rc = fft_forward(f(ion_psi, j), t1.p)) || (rc = fft_forward(f(ion_b, j), t2.p))) return rc;
+                if ((rc = contract_to(t1.p, t2.p, false, nullptr, fixed + (static_cast<size_t>(nT) + 1 + j) * 7))) return rc;
+            }
+        }
+        HIP_TRY(hipMemcpyAsync(h_acc, stress_acc.p, sizeof(double2) * stress_acc.n, hipMemcpyDeviceToHost, stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h_scal, scal.p, sizeof(double2) * SC_COUNT, hipMemcpyDeviceToHost, stream));
+    for (size_t p = 0; p < nsp; ++p)
+        HIP_TRY(hipMemcpyAsync(h_shift + p * MAX_CLASSES, species[p].d_cls_shift.p, sizeof(double) * species[p].cls_shift.size(),
+                               hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+
+    // ---- host arithmetic ----------------------------------------------------------
+    const double2* Q = h_scal + SC_Q;
+    const double2* Qi = h_scal + SC_QION;
+    for (size_t p = 0; p < nsp; ++p) {
+        HostSpecies& sp = species[p];
+        q_log_gain[p] = h_scal[SC_LF + p].x;
+        double total = 0.0;
+        for (size_t c = 0; c < sp.cls_shift.size(); ++c) sp.cls_shift[c] = h_shift[p * MAX_CLASSES + c];
+        for (int s = 0; s < sp.N; ++s) total += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)])];
+        q_shift_total[p] = total;
+    }
     auto neglog = [](double2 z) { return double2{-std::log(std::sqrt(z.x * z.x + z.y * z.y)), -std::atan2(z.y, z.x)}; };
     double2 h{0.0, 0.0};
-    for (size_t p = 0; p < species.size(); ++p) {
+    for (size_t p = 0; p < nsp; ++p) {
         double2 mu = neglog(Q[p]);
         mu.x += q_shift_total[p] + q_log_gain[p]; // Q = Q~ e^{-(sum of shifts + LF(N-1))}
         const double scale_q = std::exp(-q_shift_total[p] - q_log_gain[p]);
@@ -1031,28 +1127,19 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
         h.x += w * mu.x;
         h.y += w * mu.y;
     }
-    // H/V: sum_l (c_l/2) <omega_l^2> + (1/(8 pi lB)) <psi S_{k^2} psi> + sum n mu / V  (observable.cu:160-182)
     for (int l = 0; l < n_omega; ++l) {
-        const double2* w = om + static_cast<size_t>(l) * M;
-        hipLaunchKernelGGL(k_sum_product, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, w, w, M, partials.p);
-        HIP_TRY(hipGetLastError());
-        if ((rc = reduce_finalize(1, SC_H, 0.5 * omega_coefficient(l) * invM))) return rc;
-        double2 v;
-        if ((rc = read_scal(SC_H, 1, &v))) return rc;
-        h.x += v.x;
-        h.y += v.y;
+        h.x += h_scal[SC_H + l].x;
+        h.y += h_scal[SC_H + l].y;
     }
-    double2 S_psi[7] = {};
+    const double2* S_psi = h_scal + SC_PSI_S;
     if (electro()) {
-        if ((rc = fft_forward(ps, t3.p))) return rc;
-        if ((rc = contract(t3.p, t3.p, false, nullptr, S_psi))) return rc;
-        const double s = 1.0 / (8.0 * Pi * lB) * invM2;
-        h.x += s * (S_psi[0].x + S_psi[1].x + S_psi[2].x);
-        h.y += s * (S_psi[0].y + S_psi[1].y + S_psi[2].y);
+        const double sc = 1.0 / (8.0 * Pi * lB) * invM2;
+        h.x += sc * (S_psi[0].x + S_psi[1].x + S_psi[2].x);
+        h.y += sc * (S_psi[0].y + S_psi[1].y + S_psi[2].y);
     }
     rec->H[0] = h.x;
     rec->H[1] = h.y;
-    if (!(stress_mask & 0x7F)) return FTS_OK;
+    if (!want_stress) return FTS_OK;
 
     // stress tensor (stress.cu:161-321) in k-space
     double2 sig[6];
@@ -1062,53 +1149,30 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
     const double ideal = (static_cast<double>(chains) + static_cast<double>(ion_count)) / V;
     for (int c = 0; c < 6; ++c) sig[c] = double2{(ideal + P_sr) * (c < 3 ? 1.0 : 0.0) + mode_term[c], 0.0};
     if (electro()) {
-        const double s = 1.0 / (4.0 * Pi * lB) * invM2;
+        const double sc = 1.0 / (4.0 * Pi * lB) * invM2;
         const double2 k2s{S_psi[0].x + S_psi[1].x + S_psi[2].x, S_psi[0].y + S_psi[1].y + S_psi[2].y};
         for (int c = 0; c < 6; ++c) {
             const double d = c < 3 ? 1.0 / 3.0 : 0.0;
-            sig[c].x += s * (S_psi[c].x - d * k2s.x);
-            sig[c].y += s * (S_psi[c].y - d * k2s.y);
+            sig[c].x += sc * (S_psi[c].x - d * k2s.x);
+            sig[c].y += sc * (S_psi[c].y - d * k2s.y);
         }
     }
-    // chain connectivity: + (n/(V Q M^2)) sum_s (b_s^2/3) sum_k k_a k_b phi_s qB^_{s+1}(k) qF^_s(-k)
-    for (size_t p = 0; p < species.size(); ++p) {
-        if (species.size() > 1) {
-            // the propagator store holds one species: re-solve species p at the same state
-            rc = precision == FTS_PREC_FP32 ? solve_species<float>(static_cast<int>(p), false) : solve_species<double>(static_cast<int>(p), false);
-            if (rc) return rc;
-        }
+    // chain connectivity: + (n/(V Q M^2)) sum_s (b_s^2/3) rfac2_s sum_k k_a k_b phi_s qB^_{s+1}(k) qF^_s(-k)
+    for (size_t p = 0; p < nsp; ++p) {
         const HostSpecies& sp = species[p];
-        std::vector<std::array<double2, 7>> acc(static_cast<size_t>(std::max(1, sp.N - 1)));
-        rc = precision == FTS_PREC_FP32 ? stress_chain<float>(static_cast<int>(p), acc) : stress_chain<double>(static_cast<int>(p), acc);
-        if (rc) return rc;
-        std::vector<double> r2(static_cast<size_t>(std::max(1, sp.N - 1)), 1.0);
-        if (sp.N > 1) {
-            HIP_TRY(hipMemcpyAsync(r2.data(), rfac2.p, sizeof(double) * (sp.N - 1), hipMemcpyDeviceToHost, stream));
-            HIP_TRY(hipStreamSynchronize(stream));
-        }
+        const double2* rows = h_acc + stress_chain_base(static_cast<int>(p)) * 7;
         double2 tot[6] = {};
         for (int s = 0; s + 1 < sp.N; ++s) {
-            const double w = sp.bond[static_cast<size_t>(s)] * sp.bond[static_cast<size_t>(s)] / 3.0 * r2[static_cast<size_t>(s)];
+            const double w = sp.bond[static_cast<size_t>(s)] * sp.bond[static_cast<size_t>(s)] / 3.0 * h_rfac2[p * Nmax + s];
             for (int c = 0; c < 6; ++c) {
-                tot[c].x += w * acc[static_cast<size_t>(s)][c].x;
-                tot[c].y += w * acc[static_cast<size_t>(s)][c].y;
+                tot[c].x += w * rows[static_cast<size_t>(s) * 7 + c].x;
+                tot[c].y += w * rows[static_cast<size_t>(s) * 7 + c].y;
             }
         }
         // prefactor (n / V) / Q~ (the shifts cancel between qF_s qB_{s+1} and Q~)
-        const double2 pref = [&]() {
-            const double2 q = Q[p];
-            const double ax = std::fabs(q.x), ay = std::fabs(q.y);
-            double2 inv;
-            if (ax >= ay) {
-                const double r = q.y / q.x, d = q.x * (1.0 + r * r);
-                inv = double2{1.0 / d, -r / d};
-            } else {
-                const double r = q.x / q.y, d = q.y * (1.0 + r * r);
-                inv = double2{r / d, -1.0 / d};
-            }
-            const double sc = static_cast<double>(sp.n) / V * invM2;
-            return double2{inv.x * sc, inv.y * sc};
-        }();
+        const double2 inv = cinv_host(Q[p]);
+        const double sc = static_cast<double>(sp.n) / V * invM2;
+        const double2 pref{inv.x * sc, inv.y * sc};
         for (int c = 0; c < 6; ++c) {
             sig[c].x += pref.x * tot[c].x - pref.y * tot[c].y;
             sig[c].y += pref.x * tot[c].y + pref.y * tot[c].x;
@@ -1116,11 +1180,9 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
     }
     // polymer field coupling: sum_alpha [ -a^2/M^2 S_ab(W^_a, P^_a) + (delta/2) S_1/M^2 ] (+ psi terms)
     const double a2 = a * a;
+    const double2* fixed = h_acc + fixed_rows * 7;
     for (int al = 0; al < nT; ++al) {
-        if ((rc = fft_forward(f(W_type_s, al), t1.p))) return rc;
-        if ((rc = fft_forward(f(rho_hat, al), t2.p))) return rc;
-        double2 S[7];
-        if ((rc = contract(t1.p, t2.p, false, nullptr, S))) return rc;
+        const double2* S = fixed + static_cast<size_t>(al) * 7;
         for (int c = 0; c < 6; ++c) {
             const double d = c < 3 ? 0.5 : 0.0;
             sig[c].x += invM2 * (-a2 * S[c].x + d * S[6].x);
@@ -1128,10 +1190,7 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
         }
     }
     if (electro()) {
-        if ((rc = fft_forward(psi_s.p, t1.p))) return rc;
-        if ((rc = fft_forward(rho_c_hat.p, t2.p))) return rc;
-        double2 S[7];
-        if ((rc = contract(t1.p, t2.p, false, nullptr, S))) return rc;
+        const double2* S = fixed + static_cast<size_t>(nT) * 7;
         for (int c = 0; c < 6; ++c) {
             const double d = c < 3 ? 1.0 / 6.0 : 0.0;
             const double2 v{invM2 * (-a2 * S[c].x + d * S[6].x), invM2 * (-a2 * S[c].y + d * S[6].y)};
@@ -1140,26 +1199,14 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
         }
         // ions: i (n_j z_j / (V Q_j)) [ -a_j^2/M^2 S_ab(psi~^_j, b^_j) + (delta/6) S_1/M^2 ]
         for (size_t j = 0; j < ions.size(); ++j) {
-            if ((rc = fft_forward(f(ion_psi, j), t1.p))) return rc;
-            if ((rc = fft_forward(f(ion_b, j), t2.p))) return rc;
-            double2 S[7];
-            if ((rc = contract(t1.p, t2.p, false, nullptr, S))) return rc;
-            const double2 q = Qi[j];
-            const double ax = std::fabs(q.x), ay = std::fabs(q.y);
-            double2 inv;
-            if (ax >= ay) {
-                const double r = q.y / q.x, d = q.x * (1.0 + r * r);
-                inv = double2{1.0 / d, -r / d};
-            } else {
-                const double r = q.x / q.y, d = q.y * (1.0 + r * r);
-                inv = double2{r / d, -1.0 / d};
-            }
+            const double2* Sj = fixed + (static_cast<size_t>(nT) + 1 + j) * 7;
+            const double2 inv = cinv_host(Qi[j]);
             const double sc = static_cast<double>(ions[j].n) * ions[j].z / V;
             const double2 pref{-inv.y * sc, inv.x * sc}; // i * inv * sc
             const double aj2 = ions[j].a * ions[j].a;
             for (int c = 0; c < 6; ++c) {
                 const double d = c < 3 ? 1.0 / 6.0 : 0.0;
-                const double2 v{invM2 * (-aj2 * S[c].x + d * S[6].x), invM2 * (-aj2 * S[c].y + d * S[6].y)};
+                const double2 v{invM2 * (-aj2 * Sj[c].x + d * Sj[6].x), invM2 * (-aj2 * Sj[c].y + d * Sj[6].y)};
                 sig[c].x += pref.x * v.x - pref.y * v.y;
                 sig[c].y += pref.x * v.y + pref.y * v.x;
             }
@@ -1317,6 +1364,43 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
     return FTS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Checkpoint and non-finite restart
+// ---------------------------------------------------------------------------
+int fts_engine::maybe_checkpoint()
+{
+    if (ckpt_every <= 0) return FTS_OK;
+    if (ckpt_step >= 0 && (step_counter % ckpt_every != 0 || step_counter == ckpt_step)) return FTS_OK;
+    const size_t Ms = static_cast<size_t>(M);
+    const size_t no = static_cast<size_t>(n_omega) * Ms;
+    if (!ckpt_psi.p) {
+        HIP_TRY(ckpt_omega.alloc(std::max<size_t>(1, no)));
+        HIP_TRY(ckpt_psi.alloc(Ms));
+    }
+    if (no) HIP_TRY(hipMemcpyAsync(ckpt_omega.p, omega.p, sizeof(double2) * no, hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(ckpt_psi.p, psi.p, sizeof(double2) * Ms, hipMemcpyDeviceToDevice, stream));
+    ckpt_step = step_counter;
+    return FTS_OK;
+}
+
+int fts_engine::restart_from_checkpoint(long long failed_step)
+{
+    const std::string why = g_error;
+    if (ckpt_every <= 0 || ckpt_step < 0) return fail(FTS_NONFINITE, why);
+    if (restarts >= max_restarts)
+        return fail(FTS_NONFINITE, why + "; maximum non-finite restarts (" + std::to_string(max_restarts) + ") exceeded");
+    const size_t Ms = static_cast<size_t>(M);
+    const size_t no = static_cast<size_t>(n_omega) * Ms;
+    if (no) HIP_TRY(hipMemcpyAsync(omega.p, ckpt_omega.p, sizeof(double2) * no, hipMemcpyDeviceToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(psi.p, ckpt_psi.p, sizeof(double2) * Ms, hipMemcpyDeviceToDevice, stream));
+    ++restarts;
+    ++noise_epoch;
+    seed = seed0 + static_cast<unsigned long long>(noise_epoch) * 0x9E3779B97F4A7C15ull;
+    (void)failed_step;
+    step_counter = ckpt_step;
+    return FTS_OK;
+}
+
 template <typename S, typename D>
 __global__ void k_cast(const V2<S>* __restrict__ src, V2<D>* __restrict__ dst, double scale, size_t n)
 {
@@ -1380,6 +1464,7 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     e->scheme = s->scheme;
     e->precision = s->precision;
     e->seed = s->seed;
+    e->seed0 = s->seed;
     e->stress_mask = s->stress_mask;
     e->plane_ok = plane_fits;
     e->sched = s->fft_schedule == FTS_SCHED_AUTO ? (plane_fits ? FTS_SCHED_PLANE : FTS_SCHED_LINE) : s->fft_schedule;
@@ -1497,6 +1582,8 @@ int fts_get_info(fts_engine* h, fts_info* o)
     o->fft_schedule = h->sched;
     o->tune_ms[0] = h->tune_ms[0];
     o->tune_ms[1] = h->tune_ms[1];
+    o->restarts = h->restarts;
+    o->noise_epoch = h->noise_epoch;
     o->mirrored_species = 0;
     for (size_t p = 0; p < h->species.size(); ++p)
         if (h->species[p].mirror) o->mirrored_species |= 1 << p;
@@ -1531,6 +1618,7 @@ int fts_set_fields(fts_engine* h, const double* om, const double* ps)
     if (rc) return rc;
     if (!h->electro()) HIP_TRY(hipMemsetAsync(h->psi.p, 0, sizeof(double2) * Ms, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    h->ckpt_step = -1; // new fields: the next step takes a new checkpoint
     return FTS_OK;
 }
 
@@ -1622,11 +1710,13 @@ int fts_measure(fts_engine* h, fts_observables* out)
 
 int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, int capacity, int* n_records)
 {
-    if (!h) return fail(FTS_INVALID, "null engine");
+    if (!h || n < 0) return fail(FTS_INVALID, "null engine or negative step count");
     int nrec = 0;
     if (n_records) *n_records = 0;
     int rc = FTS_OK;
-    for (int k = 0; k < n; ++k) {
+    const long long target = h->step_counter + n;
+    while (h->step_counter < target) {
+        if ((rc = h->maybe_checkpoint())) break;
         const long long step = h->step_counter + 1; // 1-based step label (complexLangevin.cu:206)
         const bool record = record_every > 0 && step % record_every == 0 && records && nrec < capacity;
         fts_observables* rec = record ? &records[nrec] : nullptr;
@@ -1635,6 +1725,11 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
             rec->step = step;
             ++nrec;
         }
+        if (rc == FTS_NONFINITE && h->ckpt_every > 0) {
+            // restart from the checkpoint with fresh noise; drop the records taken after it
+            if ((rc = h->restart_from_checkpoint(step))) break;
+            while (nrec > 0 && records[nrec - 1].step > h->ckpt_step) --nrec;
+        }
         if (n_records) *n_records = nrec;
         if (rc != FTS_OK) break;
         if (h->timing && h->pending.size() > 256 && (rc = h->resolve_timing())) break;
@@ -1642,10 +1737,28 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
     return rc;
 }
 
+int fts_inject_nonfinite(fts_engine* h, long long step)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    h->inject_step = step;
+    return FTS_OK;
+}
+
+int fts_set_restart(fts_engine* h, int checkpoint_every, int max_restarts)
+{
+    if (!h || checkpoint_every < 0 || max_restarts < 0) return fail(FTS_INVALID, "bad restart settings");
+    h->ckpt_every = checkpoint_every;
+    h->max_restarts = max_restarts;
+    h->restarts = 0;
+    h->ckpt_step = -1;
+    return FTS_OK;
+}
+
 int fts_set_step(fts_engine* h, long long step)
 {
     if (!h || step < 0) return fail(FTS_INVALID, "bad step");
     h->step_counter = step;
+    h->ckpt_step = -1;
     return FTS_OK;
 }
 

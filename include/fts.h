@@ -179,6 +179,8 @@ typedef struct fts_info {
     double ion_smearing[FTS_MAX_IONS];
     double tune_ms[2]; /* autotuner: time of 8 bead steps, plane and line schedule (0 if not timed) */
     int mirrored_species; /* bit p set: species p runs the mirrored (forward-only) sweep */
+    int restarts;         /* non-finite restarts taken (fts_set_restart) */
+    int noise_epoch;      /* Philox key in use: seed + noise_epoch * 0x9E3779B97F4A7C15 */
 } fts_info;
 
 /* Accumulated device timing of the propagator sweeps (HIP events on the engine's
@@ -228,6 +230,21 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
 
 /* Noise counter: the next step draws Philox counter `step`. */
 int fts_set_step(fts_engine* h, long long step);
+
+/* Checkpoint and non-finite restart (the reference's Ntmp checkpoint and
+ * max_nan_restarts, complexLangevin.cu:518-667).  With checkpoint_every > 0 the
+ * engine keeps a device copy of (w, psi) taken at the first step of every
+ * fts_step call and whenever the step counter is a multiple of checkpoint_every.
+ * A step that fails with FTS_NONFINITE restores that copy, rewinds the step
+ * counter to it, moves the noise to a new Philox key (seed + epoch *
+ * 0x9E3779B97F4A7C15; the reference draws new wall-clock seeds) and drops the
+ * records taken after the checkpoint; fts_step still advances the counter by n.
+ * After max_restarts restarts a failure is returned.  0, 0 turns it off. */
+int fts_set_restart(fts_engine* h, int checkpoint_every, int max_restarts);
+/* Test hook (the proposal's fts_inject_nonfinite): the step that brings the step
+ * counter to `step` ends with a NaN in psi, once, so that the failure path and the
+ * restart can be tested without a physical blow-up. */
+int fts_inject_nonfinite(fts_engine* h, long long step);
 
 /* Test hooks on the hot path. */
 /* 3D DFT of one field on the engine's FFT, unnormalised; sign -1 forward (e^{-ikx}), +1 inverse

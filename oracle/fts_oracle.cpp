@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -122,6 +123,13 @@ struct fts_engine {
     unsigned long long seed = 0;
     long long step_counter = 0;
     int stress_mask = 0;
+
+    // checkpoint and non-finite restart (complexLangevin.cu:518-667); the restart
+    // moves the noise to the key seed0 + epoch * 0x9E3779B97F4A7C15
+    int ckpt_every = 0, max_restarts = 0, restarts = 0, noise_epoch = 0;
+    long long ckpt_step = -1, inject_step = -1;
+    unsigned long long seed0 = 0;
+    std::vector<Cx> ckpt_omega, ckpt_psi;
 
     // modes
     int nT = 0;                      // monomer types
@@ -272,6 +280,14 @@ struct fts_engine {
     Cx isotropic_pressure(const std::vector<Cx>& Qp, const Cx* om, const Cx* ps);
     bool fields_finite() const;
     int step_pc(bool record, fts_observables* rec);
+    // test hook: the step that ends at inject_step leaves a NaN in psi (once)
+    void inject()
+    {
+        if (inject_step >= 0 && step_counter == inject_step) {
+            psi[0].x = std::numeric_limits<double>::quiet_NaN();
+            inject_step = -1;
+        }
+    }
     int step_em(bool record, fts_observables* rec, bool semi_implicit);
 };
 
@@ -1066,6 +1082,7 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
     }
     remove_psi_zero_mode(psi.data());
     ++step_counter;
+    inject();
     if (!fields_finite()) return fail(FTS_NONFINITE, "updated fields have non-finite values");
     return FTS_OK;
 }
@@ -1114,6 +1131,7 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     }
     remove_psi_zero_mode(psi.data());
     ++step_counter;
+    inject();
     if (!fields_finite()) return fail(FTS_NONFINITE, "updated fields have non-finite values");
     return FTS_OK;
 }
@@ -1167,6 +1185,7 @@ int fto_create(const fts_system* s, fts_engine** out)
     e->noise = s->noise != 0;
     e->scheme = s->scheme;
     e->seed = s->seed;
+    e->seed0 = s->seed;
     e->stress_mask = s->stress_mask;
     int rc = e->build_modes(s);
     if (rc != FTS_OK) {
@@ -1258,6 +1277,8 @@ int fto_get_info(fts_engine* h, fts_info* o)
     std::memset(o, 0, sizeof(*o));
     o->grid_size = h->M;
     o->step = h->step_counter;
+    o->restarts = h->restarts;
+    o->noise_epoch = h->noise_epoch;
     o->n_omega = h->n_omega;
     o->monomer_types = h->nT;
     o->n_species = static_cast<int>(h->species.size());
@@ -1295,6 +1316,7 @@ int fto_set_fields(fts_engine* h, const double* om, const double* ps)
     if (ps) std::memcpy(h->psi.data(), ps, sizeof(Cx) * M);
     else std::fill(h->psi.begin(), h->psi.end(), Cx{0, 0});
     h->remove_psi_zero_mode(h->psi.data());
+    h->ckpt_step = -1;
     return FTS_OK;
 }
 
@@ -1349,10 +1371,17 @@ int fto_measure(fts_engine* h, fts_observables* out)
 
 int fto_step(fts_engine* h, int n, int record_every, fts_observables* records, int capacity, int* n_records)
 {
-    if (!h) return fail(FTS_INVALID, "null engine");
+    if (!h || n < 0) return fail(FTS_INVALID, "null engine or negative step count");
     int nrec = 0;
     if (n_records) *n_records = 0;
-    for (int k = 0; k < n; ++k) {
+    const long long target = h->step_counter + n;
+    while (h->step_counter < target) {
+        // checkpoint at the first step of the call and at multiples of ckpt_every
+        if (h->ckpt_every > 0 && (h->ckpt_step < 0 || (h->step_counter % h->ckpt_every == 0 && h->step_counter != h->ckpt_step))) {
+            h->ckpt_omega = h->omega;
+            h->ckpt_psi = h->psi;
+            h->ckpt_step = h->step_counter;
+        }
         const long long step = h->step_counter + 1; // 1-based step label (complexLangevin.cu:206)
         const bool record = record_every > 0 && step % record_every == 0 && records && nrec < capacity;
         fts_observables* rec = record ? &records[nrec] : nullptr;
@@ -1363,9 +1392,38 @@ int fto_step(fts_engine* h, int n, int record_every, fts_observables* records, i
             rec->step = step;
             ++nrec;
         }
+        if (rc == FTS_NONFINITE && h->ckpt_every > 0) {
+            if (h->ckpt_step < 0) return rc;
+            if (h->restarts >= h->max_restarts) return fail(FTS_NONFINITE, "maximum non-finite restarts exceeded");
+            h->omega = h->ckpt_omega;
+            h->psi = h->ckpt_psi;
+            ++h->restarts;
+            ++h->noise_epoch;
+            h->seed = h->seed0 + static_cast<unsigned long long>(h->noise_epoch) * 0x9E3779B97F4A7C15ull;
+            h->step_counter = h->ckpt_step;
+            while (nrec > 0 && records[nrec - 1].step > h->ckpt_step) --nrec;
+            rc = FTS_OK;
+        }
         if (n_records) *n_records = nrec;
         if (rc != FTS_OK) return rc;
     }
+    return FTS_OK;
+}
+
+int fto_set_restart(fts_engine* h, int checkpoint_every, int max_restarts)
+{
+    if (!h || checkpoint_every < 0 || max_restarts < 0) return fail(FTS_INVALID, "bad restart settings");
+    h->ckpt_every = checkpoint_every;
+    h->max_restarts = max_restarts;
+    h->restarts = 0;
+    h->ckpt_step = -1;
+    return FTS_OK;
+}
+
+int fto_inject_nonfinite(fts_engine* h, long long step)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    h->inject_step = step;
     return FTS_OK;
 }
 
@@ -1373,6 +1431,7 @@ int fto_set_step(fts_engine* h, long long step)
 {
     if (!h || step < 0) return fail(FTS_INVALID, "bad step");
     h->step_counter = step;
+    h->ckpt_step = -1;
     return FTS_OK;
 }
 
