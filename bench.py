@@ -202,6 +202,14 @@ def main():
         return us, byt, byt / (us * 1e-6) / 1e9
 
     us, byt, achieved = roof(fast["timing"])
+    # measured HBM traffic per launch group (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated
+    # on a copy kernel) when the committed measurement is for this configuration
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pmc_traffic_96cubed_N100.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))
+        if t.get("replicas") == args.replicas and t.get("schedule") == fast["schedule"] and t.get("precision") == "fp32":
+            traffic = t["traffic_bytes"]
     _, _, plain_achieved = roof(plain["timing"])
     # same Philox stream: fast replica 0 and plain replica 0
     n = min(fast["H0"].size, plain["H0"].size)
@@ -236,7 +244,7 @@ def main():
         "H_err_stream": float(H_err_stream),
         "H_err_sigma": float(H_err_sigma),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"bead-step launch group ({sched_name.get(fast['schedule'], '?')} schedule), FP32, "
                                f"{args.replicas} replica jobs per launch",
                      "us_per_launch_group": us, "alg_bytes_per_launch_group": byt,

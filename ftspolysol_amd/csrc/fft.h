@@ -77,6 +77,75 @@ template <typename T> struct LineJob {
     double* log_scale;
 };
 
+// The job of a workgroup: row blockIdx.y of a device job table, or (no table) j0
+// for blockIdx.y == 0 and j1 otherwise.  Built member by member: a select between
+// the two by-value kernel arguments as whole structs is lowered through private
+// per-lane copies of both (120 bytes of scratch written by every lane).
+__device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
+{
+    KOp k;
+    k.sep = f ? a.sep : b.sep;
+    k.table = f ? a.table : b.table;
+    k.c0 = f ? a.c0 : b.c0;
+    k.c1 = f ? a.c1 : b.c1;
+    k.c2 = f ? a.c2 : b.c2;
+    k.ka = f ? a.ka : b.ka;
+    k.kb = f ? a.kb : b.kb;
+    return k;
+}
+template <typename T> __device__ __forceinline__ PlaneJob<T> pick_job(PlaneJob<T> a, PlaneJob<T> b, const PlaneJob<T>* table)
+{
+    PlaneJob<T> j;
+    const PlaneJob<T>* t = table + blockIdx.y;
+    const bool tb = table != nullptr, f = blockIdx.y == 0;
+    j.src = tb ? t->src : f ? a.src : b.src;
+    j.dst = tb ? t->dst : f ? a.dst : b.dst;
+    j.qout = tb ? t->qout : f ? a.qout : b.qout;
+    j.emw = tb ? t->emw : f ? a.emw : b.emw;
+    j.norm_in = tb ? t->norm_in : f ? a.norm_in : b.norm_in;
+    j.norm_out = tb ? t->norm_out : f ? a.norm_out : b.norm_out;
+    j.log_scale = tb ? t->log_scale : f ? a.log_scale : b.log_scale;
+    return j;
+}
+template <typename T> __device__ __forceinline__ PencilJob<T> pick_job(PencilJob<T> a, PencilJob<T> b, const PencilJob<T>* table)
+{
+    PencilJob<T> j;
+    const bool f = blockIdx.y == 0;
+    if (table) {
+        const PencilJob<T>* t = table + blockIdx.y;
+        j.data = t->data;
+        j.op = pick_kop(t->op, t->op, true);
+    } else {
+        j.data = f ? a.data : b.data;
+        j.op = pick_kop(a.op, b.op, f);
+    }
+    return j;
+}
+template <typename T> __device__ __forceinline__ LineJob<T> pick_job(LineJob<T> a, LineJob<T> b, const LineJob<T>* table)
+{
+    LineJob<T> j;
+    const bool f = blockIdx.y == 0;
+    if (table) {
+        const LineJob<T>* t = table + blockIdx.y;
+        j.data = t->data;
+        j.qout = t->qout;
+        j.emw = t->emw;
+        j.op = pick_kop(t->op, t->op, true);
+        j.norm_in = t->norm_in;
+        j.norm_out = t->norm_out;
+        j.log_scale = t->log_scale;
+    } else {
+        j.data = f ? a.data : b.data;
+        j.qout = f ? a.qout : b.qout;
+        j.emw = f ? a.emw : b.emw;
+        j.op = pick_kop(a.op, b.op, f);
+        j.norm_in = f ? a.norm_in : b.norm_in;
+        j.norm_out = f ? a.norm_out : b.norm_out;
+        j.log_scale = f ? a.log_scale : b.log_scale;
+    }
+    return j;
+}
+
 struct FftGeom {
     int m1, m2, m3;
     const double* kvec;   // device: kx[m1], ky[m2], kz[m3]

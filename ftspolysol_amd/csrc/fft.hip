@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     __shared__ double red[NRED];
     __shared__ T scale_sh;
     const int tid = threadIdx.x;
-    const PlaneJob<T> job = table ? table[blockIdx.y] : (blockIdx.y == 0 ? j0 : j1);
+    const PlaneJob<T> job = pick_job<T>(j0, j1, table);
     const size_t plane = static_cast<size_t>(blockIdx.x) * (N * N);
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
     constexpr bool PRODUCES_Q = MODE == PLANE_INIT || MODE == PLANE_STEP || MODE == PLANE_LAST;
@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
     __shared__ V lds[N1 * LS];
     __shared__ V tw[N1];
     const int tid = threadIdx.x;
-    const PencilJob<T> job = table ? table[blockIdx.y] : (blockIdx.y == 0 ? j0 : j1);
+    const PencilJob<T> job = pick_job<T>(j0, j1, table);
     const size_t sx = static_cast<size_t>(g.m2) * g.m3;
     const size_t jbase = static_cast<size_t>(blockIdx.x) * ZC;
     const V* twg = static_cast<const V*>(g.tw_x);

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
     __shared__ V tw[N];
     __shared__ double red[NRED];
     const int tid = threadIdx.x;
-    const LineJob<T> job = table ? table[blockIdx.y] : (blockIdx.y == 0 ? j0 : j1);
+    const LineJob<T> job = pick_job<T>(j0, j1, table);
     const int col0 = blockIdx.x * TC;
     const size_t S = AXIS == 0 ? static_cast<size_t>(g.m2) * g.m3 : AXIS == 1 ? static_cast<size_t>(g.m3) : 1;
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(rline_threads<N>()) rline_kernel(LineJob<T> j0
     __shared__ V tw[N];
     const int tid = threadIdx.x;
     const int c = tid % TC, p = tid / TC;
-    const LineJob<T> job = table ? table[blockIdx.y] : (blockIdx.y == 0 ? j0 : j1);
+    const LineJob<T> job = pick_job<T>(j0, j1, table);
     const size_t S = AXIS == 0 ? static_cast<size_t>(g.m2) * g.m3 : static_cast<size_t>(g.m3);
     const size_t base = line_base(AXIS, blockIdx.x * TC + c, g.m2, g.m3);
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
