@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 FTS_OK, FTS_NONFINITE, FTS_INVALID, FTS_DEVICE_ERROR = 0, 1, 2, 3
-SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT = 0, 1, 2
+SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 SCHED_AUTO, SCHED_PLANE, SCHED_LINE = 0, 1, 2
 SYMMETRY_AUTO, SYMMETRY_OFF = 0, 1
@@ -23,6 +23,14 @@ STRESS_PISO = 0x40
 
 FIELD_OMEGA, FIELD_PSI, FIELD_RHO_TYPE, FIELD_RHO_P, FIELD_RHO_C = 0, 1, 2, 3, 4
 FIELD_RHO_C_POLY, FIELD_RHO_C_ION, FIELD_ION_RHO, FIELD_QF, FIELD_QB = 5, 6, 7, 8, 9
+
+# learned linear-response kernel (FTS_LRK_* in include/fts.h)
+LRK_FEATURES, LRK_IN, LRK_HIDDEN, LRK_OUT, LRK_BINS, LRK_MOMENTS = 7, 8, 64, 2, 128, 6
+LRK_OFF = {"shift": 0, "scale": 8, "W1": 16, "b1": 528, "W2": 592, "b2": 4688, "W3": 4752, "b3": 8848,
+           "W4": 8912, "b4": 9040}
+LRK_NPARAMS = 9042
+LRK_REC_MOMENTS, LRK_REC_E, LRK_REC_S1, LRK_REC_S2 = 0, 6, 8, 8 + 2 * LRK_BINS
+LRK_RECORD = 8 + 4 * LRK_BINS
 
 
 class SpeciesDesc(C.Structure):
@@ -96,6 +104,18 @@ class Timing(C.Structure):
                 ("algorithmic_bytes", C.c_double), ("step_launches", C.c_longlong)]
 
 
+class LrkStats(C.Structure):
+    _fields_ = [("judged", C.c_longlong), ("learned_steps", C.c_longlong), ("fallback_steps", C.c_longlong),
+                ("records", C.c_longlong), ("last_excess", C.c_double),
+                ("sum_rel_residual_learned", C.c_double * 2), ("sum_rel_residual_debye", C.c_double * 2)]
+
+    def as_dict(self):
+        return {"judged": self.judged, "learned_steps": self.learned_steps, "fallback_steps": self.fallback_steps,
+                "records": self.records, "last_excess": self.last_excess,
+                "sum_rel_residual_learned": np.array(self.sum_rel_residual_learned[:]),
+                "sum_rel_residual_debye": np.array(self.sum_rel_residual_debye[:])}
+
+
 MAX_BATCH_GROUPS = 16
 
 
@@ -138,6 +158,18 @@ def _bind(lib, prefix):
         "batch_step": (C.c_int, [vp, C.c_int, C.c_int, P(Observables), C.c_int, P(C.c_int)]),
         "batch_set_timing": (C.c_int, [vp, C.c_int]),
         "batch_get_timing": (C.c_int, [vp, C.c_int, P(Timing)]),
+        "lrk_set_network": (C.c_int, [vp, P(C.c_float), C.c_double]),
+        "lrk_set_recording": (C.c_int, [vp, C.c_int]),
+        "lrk_get_records": (C.c_int, [vp, P(C.c_double), C.c_int, P(C.c_int)]),
+        "lrk_get_stats": (C.c_int, [vp, P(LrkStats)]),
+        "lrk_forward": (C.c_int, [P(C.c_float), P(C.c_float), C.c_int, P(C.c_float)]),
+        "lrk_forward64": (C.c_int, [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double)]),
+        "lrk_loss_grad": (C.c_int, [P(C.c_float), P(C.c_double), C.c_int, P(C.c_double), P(C.c_float)]),
+        "lrk_loss_grad64": (C.c_int, [P(C.c_double), P(C.c_double), C.c_int, P(C.c_double), P(C.c_double)]),
+        "lrk_trainer_create": (C.c_int, [P(C.c_double), C.c_int, P(vp)]),
+        "lrk_trainer_destroy": (None, [vp]),
+        "lrk_trainer_run": (C.c_int, [vp, P(C.c_float), C.c_int, C.c_int, C.c_double, C.c_double, C.c_ulonglong,
+                                      P(C.c_double)]),
     }
     fns = {}
     for name, (res, args) in sig.items():
@@ -153,6 +185,10 @@ def _bind(lib, prefix):
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 class Engine:
@@ -264,6 +300,85 @@ class Engine:
         self._check(self._fns["get_timing"](self._h, C.byref(t)))
         return _timing_dict(t)
 
+    # ---- learned linear-response kernel ----
+    def lrk_set_network(self, params, threshold=1e-3):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+        if p.size != LRK_NPARAMS:
+            raise ValueError(f"expected {LRK_NPARAMS} parameters, got {p.size}")
+        self._check(self._fns["lrk_set_network"](self._h, _fptr(p), float(threshold)))
+
+    def lrk_set_recording(self, capacity):
+        self._check(self._fns["lrk_set_recording"](self._h, int(capacity)))
+
+    def lrk_get_records(self, max_records):
+        out = np.zeros((max(1, max_records), LRK_RECORD))
+        n = C.c_int(0)
+        self._check(self._fns["lrk_get_records"](self._h, _dptr(out), int(max_records), C.byref(n)))
+        return out[:n.value].copy()
+
+    def lrk_get_stats(self):
+        st = LrkStats()
+        self._check(self._fns["lrk_get_stats"](self._h, C.byref(st)))
+        return st.as_dict()
+
+
+class LrkNet:
+    """The learned kernel's network functions of a library exporting the fts ABI
+    (no engine needed): forward pass, loss and gradient, and the Adam trainer."""
+
+    def __init__(self, lib, prefix):
+        self._fns = _bind(lib, prefix)
+
+    def _check(self, rc):
+        if rc != FTS_OK:
+            msg = self._fns["last_error"]()
+            raise FtsError(rc, msg.decode() if msg else "")
+
+    def forward(self, params, features):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+        f = np.ascontiguousarray(np.asarray(features, dtype=np.float32).reshape(-1, LRK_FEATURES))
+        out = np.zeros((f.shape[0], LRK_OUT), dtype=np.float32)
+        self._check(self._fns["lrk_forward"](_fptr(p), _fptr(f), f.shape[0], _fptr(out)))
+        return out
+
+    def forward64(self, params, features):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        f = np.ascontiguousarray(np.asarray(features, dtype=np.float64).reshape(-1, LRK_FEATURES))
+        out = np.zeros((f.shape[0], LRK_OUT))
+        self._check(self._fns["lrk_forward64"](_dptr(p), _dptr(f), f.shape[0], _dptr(out)))
+        return out
+
+    def loss_grad(self, params, records, grad=True):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float32))
+        r = np.ascontiguousarray(np.asarray(records, dtype=np.float64).reshape(-1, LRK_RECORD))
+        loss = C.c_double(0.0)
+        g = np.zeros(LRK_NPARAMS, dtype=np.float32) if grad else None
+        self._check(self._fns["lrk_loss_grad"](_fptr(p), _dptr(r), r.shape[0], C.byref(loss),
+                                               _fptr(g) if grad else None))
+        return (loss.value, g) if grad else loss.value
+
+    def loss_grad64(self, params, records):
+        p = np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+        r = np.ascontiguousarray(np.asarray(records, dtype=np.float64).reshape(-1, LRK_RECORD))
+        loss = C.c_double(0.0)
+        g = np.zeros(LRK_NPARAMS)
+        self._check(self._fns["lrk_loss_grad64"](_dptr(p), _dptr(r), r.shape[0], C.byref(loss), _dptr(g)))
+        return loss.value, g
+
+    def train(self, params, records, epochs, batch_records=64, lr0=3e-3, lr1=3e-4, seed=1):
+        """Adam on the device; returns (trained params, per-epoch mean loss)."""
+        r = np.ascontiguousarray(np.asarray(records, dtype=np.float64).reshape(-1, LRK_RECORD))
+        h = C.c_void_p()
+        self._check(self._fns["lrk_trainer_create"](_dptr(r), r.shape[0], C.byref(h)))
+        try:
+            p = np.ascontiguousarray(np.array(params, dtype=np.float32))
+            hist = np.zeros(max(1, epochs))
+            self._check(self._fns["lrk_trainer_run"](h, _fptr(p), int(epochs), int(batch_records), float(lr0),
+                                                     float(lr1), int(seed), _dptr(hist)))
+        finally:
+            self._fns["lrk_trainer_destroy"](h)
+        return p, hist[:epochs]
+
 
 def _timing_dict(t):
     return {"sweep_ms": t.sweep_ms, "bead_steps": t.bead_steps, "sweeps": t.sweeps,
@@ -356,3 +471,8 @@ def hip_engine(system):
 def hip_batch(systems):
     """A replica batch on the GPU."""
     return Batch(load_hip_library(), "fts_", systems)
+
+
+def hip_lrk():
+    """The learned kernel's network kernels on the GPU."""
+    return LrkNet(load_hip_library(), "fts_")

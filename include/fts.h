@@ -35,6 +35,7 @@ extern "C" {
 #define FTS_SCHEME_PC_REFERENCE 0 /* predictor-corrector, complexLangevin.cu:328-368 */
 #define FTS_SCHEME_EM 1           /* plain explicit Euler-Maruyama (no reference counterpart) */
 #define FTS_SCHEME_SEMI_IMPLICIT 2 /* k-space Debye-preconditioned semi-implicit EM (no reference counterpart) */
+#define FTS_SCHEME_SI_LEARNED 3    /* semi-implicit EM with the learned linear-response kernel (no reference counterpart) */
 
 #define FTS_PREC_FP64 0 /* complex FP64 everywhere, as the reference (common.h:23) */
 #define FTS_PREC_FP32 1 /* complex FP32 propagators/FFTs, per-bead log-renormalisation, FP64 reductions */
@@ -297,6 +298,107 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
 /* Device timing of the batched sweeps of one group (fts_timing as for an engine). */
 int fts_batch_set_timing(fts_batch* b, int enabled);
 int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out);
+
+/* ---- Learned linear-response kernel (LRK) ---------------------------------
+ * No counterpart in the reference, which has no semi-implicit scheme at all.
+ *
+ * The semi-implicit update (FTS_SCHEME_SEMI_IMPLICIT) is
+ *     w <- w + S_P[-dt lam u(w) + eta],   P(k) = 1 / (1 + dt lam A(k)),
+ * with A = A_D, the Debye (discrete-chain RPA) linear response of the
+ * homogeneous state.  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
+ *     A(k) = A_D(k) exp(r_c(bin(|k|))),   r_c = 4 tanh(z_c / 4),
+ * where z = MLP(|k|, moments of the current fields): field c = 0 is interaction
+ * mode 0, c = 1 is psi.  The MLP has FTS_LRK_FEATURES inputs (padded to
+ * FTS_LRK_IN), three tanh layers of FTS_LRK_HIDDEN units and FTS_LRK_OUT linear
+ * outputs, FP32 weights.  Features: x0 = (bin + 0.5) / FTS_LRK_BINS, the centre of
+ * the |k| bin on the scale k_ref = sqrt(3) pi / dL (bin = floor(BINS |k| / k_ref)
+ * clamped to BINS - 1), then the moments std Re w0, std Im w0, mean Re w0,
+ * mean Im w0, std Re psi, std Im psi over the grid; feature j enters the first
+ * layer as (f_j - shift_j) * scale_j.
+ *
+ * The exact force judges the kernel on every step: with dW = FFT[w_t - w_{t-1}]
+ * and dU = FFT[u(w_t) - u(w_{t-1})], the residual of a kernel A on that step is
+ *     R_c(A) = sum_{k != 0} |dU - A dW|^2
+ *            = E_c - 2 sum_b e^{r_b} S1_{c,b} + sum_b e^{2 r_b} S2_{c,b}
+ * with E_c = sum |dU|^2, S1_{c,b} = sum_{k in b} A_D Re(dU conj dW) and
+ * S2_{c,b} = sum_{k in b} A_D^2 |dW|^2.  The learned kernel is used for the next
+ * step while its excess residual max_c (R_c(learned) - R_c(A_D)) / E_c stays at or
+ * below the threshold; otherwise the step falls back to A_D (the kernel is still
+ * judged, and comes back when it does well again).  The first step after creation,
+ * fts_set_fields or a restart has nothing judged yet and uses A_D.
+ *
+ * The network minimises the same residual: a training record is one judged step
+ * (moments at its start, E, S1, S2), and the loss of a record is
+ *     sum_c sum_b (S2_{c,b} / E_c) (e^{r_{c,b}} - S1_{c,b} / S2_{c,b})^2
+ *     = sum_c (R_c(learned) - min_r R_c) / E_c.
+ * The batch loss is the mean over its records. */
+#define FTS_LRK_FEATURES 7
+#define FTS_LRK_IN 8
+#define FTS_LRK_HIDDEN 64
+#define FTS_LRK_OUT 2
+#define FTS_LRK_BINS 128
+#define FTS_LRK_MOMENTS 6
+/* parameter block (floats): shift[8] scale[8] W1[64][8] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[2][64] b4[2]; W[out][in] row-major */
+#define FTS_LRK_OFF_SHIFT 0
+#define FTS_LRK_OFF_SCALE 8
+#define FTS_LRK_OFF_W1 16
+#define FTS_LRK_OFF_B1 528
+#define FTS_LRK_OFF_W2 592
+#define FTS_LRK_OFF_B2 4688
+#define FTS_LRK_OFF_W3 4752
+#define FTS_LRK_OFF_B3 8848
+#define FTS_LRK_OFF_W4 8912
+#define FTS_LRK_OFF_B4 9040
+#define FTS_LRK_NPARAMS 9042
+/* training record (doubles): moments[6] E[2] S1[2][BINS] S2[2][BINS] */
+#define FTS_LRK_REC_MOMENTS 0
+#define FTS_LRK_REC_E 6
+#define FTS_LRK_REC_S1 8
+#define FTS_LRK_REC_S2 (8 + 2 * FTS_LRK_BINS)
+#define FTS_LRK_RECORD (8 + 4 * FTS_LRK_BINS)
+
+typedef struct fts_lrk_stats {
+    long long judged;         /* steps judged by the exact force */
+    long long learned_steps;  /* steps taken with the learned kernel */
+    long long fallback_steps; /* steps taken with A_D (first steps and fallbacks) */
+    long long records;        /* training records written since recording started */
+    double last_excess;       /* excess residual of the last judged step */
+    double sum_rel_residual_learned[2]; /* sum over judged steps of R_c(learned) / E_c */
+    double sum_rel_residual_debye[2];   /* sum over judged steps of R_c(A_D) / E_c */
+} fts_lrk_stats;
+
+/* Load the network (FTS_LRK_NPARAMS floats, copied) and the fallback threshold
+ * into an engine.  Engines start with all-zero parameters (r = 0, i.e. A_D) and
+ * threshold 1e-3. */
+int fts_lrk_set_network(fts_engine* h, const float* params, double threshold);
+/* Write a training record for every judged step into a device ring of `capacity`
+ * records (0 stops recording and frees it).  Needs FTS_SCHEME_SEMI_IMPLICIT or
+ * FTS_SCHEME_SI_LEARNED, and 1 <= n_omega + (lB > 0) with n_omega <= 1. */
+int fts_lrk_set_recording(fts_engine* h, int capacity);
+/* Copy the records written since the last call (at most max_records, the oldest
+ * first; older ones are lost if the ring wrapped) and empty the ring. */
+int fts_lrk_get_records(fts_engine* h, double* out, int max_records, int* n_out);
+int fts_lrk_get_stats(fts_engine* h, fts_lrk_stats* out);
+
+/* The network on the device, outside an engine (training and tests).
+ * forward: features [n][FTS_LRK_FEATURES] -> r [n][FTS_LRK_OUT]. */
+int fts_lrk_forward(const float* params, const float* features, int n, float* r_out);
+/* Loss of n_records training records (mean over records) and, if grad is not
+ * NULL, its gradient with respect to every parameter except shift and scale
+ * (those entries are set to 0). */
+int fts_lrk_loss_grad(const float* params, const double* records, int n_records, double* loss, float* grad);
+
+/* Trainer: Adam on minibatches of records, on the device. */
+typedef struct fts_lrk_trainer fts_lrk_trainer;
+int fts_lrk_trainer_create(const double* records, int n_records, fts_lrk_trainer** out);
+void fts_lrk_trainer_destroy(fts_lrk_trainer* t);
+/* `epochs` passes over the records in an order shuffled per epoch from `seed`,
+ * batch_records records per Adam step, learning rate decaying geometrically
+ * from lr0 to lr1 over the run.  params is updated in place (shift and scale are
+ * kept); loss_history[e] (if not NULL) receives the mean minibatch loss of epoch e. */
+int fts_lrk_trainer_run(fts_lrk_trainer* t, float* params, int epochs, int batch_records, double lr0, double lr1,
+                        unsigned long long seed, double* loss_history);
 
 #ifdef __cplusplus
 }

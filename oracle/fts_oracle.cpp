@@ -24,10 +24,12 @@
 // M = 1 / M = 2 mode closed forms; the Heun step's second-order convergence;
 // the discrete-chain linear response for the semi-implicit kernel.
 #include "../include/fts.h"
+#include "lrk_mlp.h"
 #include "oracle_fft.h"
 #include "philox_host.h"
 
 #include <algorithm>
+#include <deque>
 #include <cmath>
 #include <limits>
 #include <cstdio>
@@ -153,6 +155,22 @@ struct fts_engine {
     std::vector<double> k2, Gamma, PHI, ion_kernels, kab, minus_kab;
     double mode_term[6] = {0, 0, 0, 0, 0, 0};
     std::vector<double> A_omega, A_psi; // semi-implicit linear-response kernels
+
+    // learned linear-response kernel (FTS_SCHEME_SI_LEARNED, include/fts.h FTS_LRK_*)
+    std::vector<double> lrk_P = std::vector<double>(FTS_LRK_NPARAMS, 0.0);
+    double lrk_threshold = 1.0e-3;
+    std::vector<int> lrk_binof;   // [M] |k| bin of every wave vector (k = 0 in bin 0)
+    bool lrk_have_prev = false;
+    Field lrk_W[2], lrk_U[2];     // k-space w and u(w) of the previous step, field c
+    double lrk_m_prev[FTS_LRK_MOMENTS] = {};
+    double lrk_r_prev[2][FTS_LRK_BINS] = {};
+    fts_lrk_stats lrk_st{};
+    int lrk_capacity = 0;
+    std::deque<std::vector<double>> lrk_ring;
+    bool lrk_active() const { return scheme == FTS_SCHEME_SI_LEARNED || lrk_capacity > 0; }
+    void lrk_build_bins();
+    void lrk_moments(double* m);
+    void lrk_step(std::vector<double> (&mult)[2]);
 
     oracle::Fft3d* fft = nullptr;
 
@@ -561,6 +579,148 @@ void fts_engine::build_kernels()
         const double delta = c < 3 ? 1.0 : 0.0;
         mode_term[c] = electrostatics() ? ((1.0 / 3.0) * static_cast<double>(modes) * delta - sum_over_k2[c]) / V : 0.0;
     }
+}
+
+// ---- learned linear-response kernel (no reference counterpart) --------------
+// The formulas are the ones include/fts.h states for FTS_SCHEME_SI_LEARNED.
+void fts_engine::lrk_build_bins()
+{
+    lrk_binof.assign(static_cast<size_t>(M), 0);
+    for (int x = 0; x < m1; ++x)
+        for (int y = 0; y < m2; ++y)
+            for (int z = 0; z < m3; ++z)
+                lrk_binof[(static_cast<size_t>(x) * m2 + y) * m3 + z] =
+                    oracle::lrk_bin(spectral_mode(x, m1), spectral_mode(y, m2), spectral_mode(z, m3), m1, m2, m3);
+}
+
+// std Re w0, std Im w0, mean Re w0, mean Im w0, std Re psi, std Im psi (population moments)
+void fts_engine::lrk_moments(double* m)
+{
+    auto mom = [&](const Cx* f, double* out4) {
+        long double s[4] = {0.0L, 0.0L, 0.0L, 0.0L};
+        for (int i = 0; i < M; ++i) {
+            s[0] += f[i].x;
+            s[1] += static_cast<long double>(f[i].x) * f[i].x;
+            s[2] += f[i].y;
+            s[3] += static_cast<long double>(f[i].y) * f[i].y;
+        }
+        const double inv = 1.0 / static_cast<double>(M);
+        const double mx = static_cast<double>(s[0]) * inv, my = static_cast<double>(s[2]) * inv;
+        out4[0] = std::sqrt(std::max(0.0, static_cast<double>(s[1]) * inv - mx * mx));
+        out4[1] = std::sqrt(std::max(0.0, static_cast<double>(s[3]) * inv - my * my));
+        out4[2] = mx;
+        out4[3] = my;
+    };
+    double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    if (n_omega > 0) mom(omega.data(), a);
+    if (electrostatics()) mom(psi.data(), b);
+    m[0] = a[0];
+    m[1] = a[1];
+    m[2] = a[2];
+    m[3] = a[3];
+    m[4] = b[0];
+    m[5] = b[1];
+}
+
+// Called by the semi-implicit step after the drifts u(w_t) are known (drift, dpsi):
+// judges the previous step with the exact force, writes its training record, and
+// returns in mult[c] the per-k factor exp(r) that multiplies A_D for this step
+// (empty: A_D unchanged).
+void fts_engine::lrk_step(std::vector<double> (&mult)[2])
+{
+    constexpr int NB = FTS_LRK_BINS;
+    Field Wc[2], Uc[2];
+    for (int c = 0; c < 2; ++c) {
+        Wc[c].assign(static_cast<size_t>(M), Cx{0.0, 0.0});
+        Uc[c].assign(static_cast<size_t>(M), Cx{0.0, 0.0});
+    }
+    if (n_omega > 0) {
+        copy(Wc[0].data(), omega.data());
+        copy(Uc[0].data(), drift.data());
+        fft->run(Wc[0].data(), -1);
+        fft->run(Uc[0].data(), -1);
+    }
+    if (electrostatics()) {
+        copy(Wc[1].data(), psi.data());
+        copy(Uc[1].data(), dpsi.data());
+        fft->run(Wc[1].data(), -1);
+        fft->run(Uc[1].data(), -1);
+    }
+    double m[FTS_LRK_MOMENTS];
+    lrk_moments(m);
+    double r[2][NB];
+    for (int b = 0; b < NB; ++b) {
+        double f[FTS_LRK_FEATURES];
+        oracle::lrk_features(b, m, f);
+        oracle::LrkActs act;
+        oracle::lrk_forward(lrk_P.data(), f, act);
+        r[0][b] = act.r[0];
+        r[1][b] = act.r[1];
+    }
+    bool use = false;
+    if (lrk_have_prev) {
+        double E[2] = {0.0, 0.0}, S1[2][NB] = {}, S2[2][NB] = {};
+        for (int i = 1; i < M; ++i) {
+            const int b = lrk_binof[static_cast<size_t>(i)];
+            for (int c = 0; c < 2; ++c) {
+                const Cx dW = csub(Wc[c][static_cast<size_t>(i)], lrk_W[c][static_cast<size_t>(i)]);
+                const Cx dU = csub(Uc[c][static_cast<size_t>(i)], lrk_U[c][static_cast<size_t>(i)]);
+                const double AD = c == 0 ? (n_omega > 0 ? A_omega[static_cast<size_t>(i)] : 0.0) : A_psi[static_cast<size_t>(i)];
+                E[c] += dU.x * dU.x + dU.y * dU.y;
+                S1[c][b] += AD * (dU.x * dW.x + dU.y * dW.y);
+                S2[c][b] += AD * AD * (dW.x * dW.x + dW.y * dW.y);
+            }
+        }
+        double ex = -std::numeric_limits<double>::infinity();
+        bool any = false;
+        for (int c = 0; c < 2; ++c) {
+            if (!(E[c] > 0.0)) continue;
+            double RL = E[c], RD = E[c];
+            for (int b = 0; b < NB; ++b) {
+                const double e = std::exp(lrk_r_prev[c][b]);
+                RL += -2.0 * e * S1[c][b] + e * e * S2[c][b];
+                RD += -2.0 * S1[c][b] + S2[c][b];
+            }
+            lrk_st.sum_rel_residual_learned[c] += RL / E[c];
+            lrk_st.sum_rel_residual_debye[c] += RD / E[c];
+            ex = std::max(ex, (RL - RD) / E[c]);
+            any = true;
+        }
+        if (!any) ex = 0.0;
+        ++lrk_st.judged;
+        lrk_st.last_excess = ex;
+        use = ex <= lrk_threshold;
+        if (lrk_capacity > 0) {
+            std::vector<double> rec(FTS_LRK_RECORD, 0.0);
+            for (int j = 0; j < FTS_LRK_MOMENTS; ++j) rec[static_cast<size_t>(FTS_LRK_REC_MOMENTS + j)] = lrk_m_prev[j];
+            for (int c = 0; c < 2; ++c) {
+                rec[static_cast<size_t>(FTS_LRK_REC_E + c)] = E[c];
+                for (int b = 0; b < NB; ++b) {
+                    rec[static_cast<size_t>(FTS_LRK_REC_S1 + c * NB + b)] = S1[c][b];
+                    rec[static_cast<size_t>(FTS_LRK_REC_S2 + c * NB + b)] = S2[c][b];
+                }
+            }
+            lrk_ring.push_back(std::move(rec));
+            if (static_cast<int>(lrk_ring.size()) > lrk_capacity) lrk_ring.pop_front();
+            ++lrk_st.records;
+        }
+    }
+    if (scheme == FTS_SCHEME_SI_LEARNED) {
+        if (use) ++lrk_st.learned_steps;
+        else ++lrk_st.fallback_steps;
+        if (use)
+            for (int c = 0; c < 2; ++c) {
+                mult[c].assign(static_cast<size_t>(M), 1.0);
+                for (int i = 0; i < M; ++i) mult[c][static_cast<size_t>(i)] = std::exp(r[c][lrk_binof[static_cast<size_t>(i)]]);
+            }
+    }
+    for (int c = 0; c < 2; ++c) {
+        lrk_W[c] = std::move(Wc[c]);
+        lrk_U[c] = std::move(Uc[c]);
+        for (int b = 0; b < NB; ++b) lrk_r_prev[c][b] = r[c][b];
+    }
+    for (int j = 0; j < FTS_LRK_MOMENTS; ++j) lrk_m_prev[j] = m[j];
+    lrk_have_prev = true;
 }
 
 // Linear-response kernels of the semi-implicit scheme (no reference counterpart).
@@ -1101,6 +1261,8 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     generate_noises(step_counter);
     compute_drifts(omega.data(), drift.data());
     compute_psi_drift(psi.data(), rho_c.data(), dpsi.data());
+    std::vector<double> mult[2];
+    if (semi_implicit && lrk_active()) lrk_step(mult);
     std::vector<double> P(static_cast<size_t>(M));
     for (int l = 0; l < n_omega; ++l) {
         const double lam = lambda_omega[static_cast<size_t>(l)];
@@ -1112,7 +1274,10 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
             r[i].x += eta[i];
         }
         if (semi_implicit) {
-            for (int i = 0; i < M; ++i) P[static_cast<size_t>(i)] = 1.0 / (1.0 + dt * lam * A_omega[static_cast<size_t>(l) * M + i]);
+            for (int i = 0; i < M; ++i) {
+                const double A = A_omega[static_cast<size_t>(l) * M + i] * (l == 0 && !mult[0].empty() ? mult[0][static_cast<size_t>(i)] : 1.0);
+                P[static_cast<size_t>(i)] = 1.0 / (1.0 + dt * lam * A);
+            }
             apply_kernel(r, r, P.data());
         }
         axpy(w, r, 1.0);
@@ -1124,7 +1289,10 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
             r[i].x += etaPsi[static_cast<size_t>(i)];
         }
         if (semi_implicit) {
-            for (int i = 0; i < M; ++i) P[static_cast<size_t>(i)] = 1.0 / (1.0 + dt * lambda_psi * A_psi[static_cast<size_t>(i)]);
+            for (int i = 0; i < M; ++i) {
+                const double A = A_psi[static_cast<size_t>(i)] * (!mult[1].empty() ? mult[1][static_cast<size_t>(i)] : 1.0);
+                P[static_cast<size_t>(i)] = 1.0 / (1.0 + dt * lambda_psi * A);
+            }
             apply_kernel(r, r, P.data());
         }
         axpy(psi.data(), r, 1.0);
@@ -1167,7 +1335,7 @@ int fto_create(const fts_system* s, fts_engine** out)
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
     if (s->n_species <= 0 || s->n_species > FTS_MAX_SPECIES || !s->species) return fail(FTS_INVALID, "polymer species_count");
-    if (s->scheme < 0 || s->scheme > 2) return fail(FTS_INVALID, "scheme");
+    if (s->scheme < 0 || s->scheme > 3) return fail(FTS_INVALID, "scheme");
     fts_engine* e = new fts_engine();
     e->m1 = s->m1;
     e->m2 = s->m2;
@@ -1246,7 +1414,14 @@ int fto_create(const fts_system* s, fts_engine** out)
     }
     e->fft = new oracle::Fft3d(e->m1, e->m2, e->m3);
     e->build_kernels();
-    if (e->scheme == FTS_SCHEME_SEMI_IMPLICIT) e->build_si_kernels();
+    if (e->scheme == FTS_SCHEME_SI_LEARNED && e->n_omega > 1) {
+        delete e;
+        return fail(FTS_INVALID, "the learned kernel covers one interaction mode and psi (n_omega <= 1)");
+    }
+    if (e->scheme == FTS_SCHEME_SEMI_IMPLICIT || e->scheme == FTS_SCHEME_SI_LEARNED) {
+        e->build_si_kernels();
+        e->lrk_build_bins();
+    }
     const size_t M = static_cast<size_t>(e->M);
     const size_t nO = static_cast<size_t>(e->n_omega), nT = static_cast<size_t>(e->nT), nI = e->ions.size(), Nm = static_cast<size_t>(e->Nmax);
     const Cx zero{0.0, 0.0};
@@ -1317,6 +1492,7 @@ int fto_set_fields(fts_engine* h, const double* om, const double* ps)
     else std::fill(h->psi.begin(), h->psi.end(), Cx{0, 0});
     h->remove_psi_zero_mode(h->psi.data());
     h->ckpt_step = -1;
+    h->lrk_have_prev = false;
     return FTS_OK;
 }
 
@@ -1387,7 +1563,7 @@ int fto_step(fts_engine* h, int n, int record_every, fts_observables* records, i
         fts_observables* rec = record ? &records[nrec] : nullptr;
         int rc;
         if (h->scheme == FTS_SCHEME_PC_REFERENCE) rc = h->step_pc(record, rec);
-        else rc = h->step_em(record, rec, h->scheme == FTS_SCHEME_SEMI_IMPLICIT);
+        else rc = h->step_em(record, rec, h->scheme == FTS_SCHEME_SEMI_IMPLICIT || h->scheme == FTS_SCHEME_SI_LEARNED);
         if (record) {
             rec->step = step;
             ++nrec;
@@ -1401,6 +1577,7 @@ int fto_step(fts_engine* h, int n, int record_every, fts_observables* records, i
             ++h->noise_epoch;
             h->seed = h->seed0 + static_cast<unsigned long long>(h->noise_epoch) * 0x9E3779B97F4A7C15ull;
             h->step_counter = h->ckpt_step;
+            h->lrk_have_prev = false;
             while (nrec > 0 && records[nrec - 1].step > h->ckpt_step) --nrec;
             rc = FTS_OK;
         }
@@ -1451,6 +1628,83 @@ int fto_bead_step(fts_engine* h, const double* q, const double* emw, double bond
     Cx* o = reinterpret_cast<Cx*>(q_out);
     h->apply_kernel(reinterpret_cast<const Cx*>(q), o, phi.data());
     h->mul(o, reinterpret_cast<const Cx*>(emw));
+    return FTS_OK;
+}
+
+// ---- learned linear-response kernel ------------------------------------------
+int fto_lrk_set_network(fts_engine* h, const float* params, double threshold)
+{
+    if (!h || !params) return fail(FTS_INVALID, "null argument");
+    for (int j = 0; j < FTS_LRK_NPARAMS; ++j) h->lrk_P[static_cast<size_t>(j)] = params[j];
+    h->lrk_threshold = threshold;
+    return FTS_OK;
+}
+
+int fto_lrk_set_recording(fts_engine* h, int capacity)
+{
+    if (!h || capacity < 0) return fail(FTS_INVALID, "bad argument");
+    if (capacity > 0 && h->scheme != FTS_SCHEME_SEMI_IMPLICIT && h->scheme != FTS_SCHEME_SI_LEARNED)
+        return fail(FTS_INVALID, "training records need a semi-implicit scheme");
+    if (capacity > 0 && (h->n_omega > 1 || (h->n_omega == 0 && !h->electrostatics())))
+        return fail(FTS_INVALID, "training records need n_omega <= 1 and at least one field");
+    h->lrk_capacity = capacity;
+    h->lrk_ring.clear();
+    h->lrk_st.records = 0;
+    return FTS_OK;
+}
+
+int fto_lrk_get_records(fts_engine* h, double* out, int max_records, int* n_out)
+{
+    if (!h || !n_out || max_records < 0 || (max_records > 0 && !out)) return fail(FTS_INVALID, "bad argument");
+    const int n = std::min(max_records, static_cast<int>(h->lrk_ring.size()));
+    for (int k = 0; k < n; ++k) std::memcpy(out + static_cast<size_t>(k) * FTS_LRK_RECORD, h->lrk_ring[static_cast<size_t>(k)].data(), sizeof(double) * FTS_LRK_RECORD);
+    h->lrk_ring.clear();
+    *n_out = n;
+    return FTS_OK;
+}
+
+int fto_lrk_get_stats(fts_engine* h, fts_lrk_stats* out)
+{
+    if (!h || !out) return fail(FTS_INVALID, "null argument");
+    *out = h->lrk_st;
+    return FTS_OK;
+}
+
+int fto_lrk_forward64(const double* params, const double* features, int n, double* r_out)
+{
+    if (!params || n < 0 || (n > 0 && (!features || !r_out))) return fail(FTS_INVALID, "bad argument");
+    for (int k = 0; k < n; ++k) {
+        oracle::LrkActs a;
+        oracle::lrk_forward(params, features + static_cast<size_t>(k) * FTS_LRK_FEATURES, a);
+        for (int c = 0; c < FTS_LRK_OUT; ++c) r_out[static_cast<size_t>(k) * FTS_LRK_OUT + c] = a.r[c];
+    }
+    return FTS_OK;
+}
+
+int fto_lrk_forward(const float* params, const float* features, int n, float* r_out)
+{
+    if (!params || n < 0 || (n > 0 && (!features || !r_out))) return fail(FTS_INVALID, "bad argument");
+    std::vector<double> P(params, params + FTS_LRK_NPARAMS), f(features, features + static_cast<size_t>(n) * FTS_LRK_FEATURES),
+        r(static_cast<size_t>(n) * FTS_LRK_OUT);
+    fto_lrk_forward64(P.data(), f.data(), n, r.data());
+    for (size_t j = 0; j < r.size(); ++j) r_out[j] = static_cast<float>(r[j]);
+    return FTS_OK;
+}
+
+int fto_lrk_loss_grad64(const double* params, const double* records, int n_records, double* loss, double* grad)
+{
+    if (!params || !loss || n_records < 0 || (n_records > 0 && !records)) return fail(FTS_INVALID, "bad argument");
+    *loss = oracle::lrk_loss_grad(params, records, n_records, grad);
+    return FTS_OK;
+}
+
+int fto_lrk_loss_grad(const float* params, const double* records, int n_records, double* loss, float* grad)
+{
+    if (!params || !loss || n_records < 0 || (n_records > 0 && !records)) return fail(FTS_INVALID, "bad argument");
+    std::vector<double> P(params, params + FTS_LRK_NPARAMS), G(FTS_LRK_NPARAMS, 0.0);
+    *loss = oracle::lrk_loss_grad(P.data(), records, n_records, grad ? G.data() : nullptr);
+    if (grad)
+        for (int j = 0; j < FTS_LRK_NPARAMS; ++j) grad[j] = static_cast<float>(G[static_cast<size_t>(j)]);
     return FTS_OK;
 }
 

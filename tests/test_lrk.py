@@ -1,0 +1,339 @@
+"""CPU pins of the oracle's learned linear-response kernel (FTS_SCHEME_SI_LEARNED).
+
+What fixes each part independently of the oracle's own code:
+  - network forward: a numpy forward pass written from the documented parameter
+    layout, and the closed form of a network whose only non-zero parameters are
+    the output biases (r = 4 tanh(b4 / 4));
+  - loss gradient: central finite differences of the FP64 loss;
+  - loss: the residual identity  loss = sum_c (R_c(A_D e^r) - R_c,min) / E_c  with
+    R_c(A) = sum_{k != 0} |dU - A dW|^2 summed directly over a grid;
+  - training records: recomputed in numpy from the fields the engine exports
+    (omega, psi, rho_type, rho_c), numpy FFTs, the Debye kernel written out for a
+    homopolymer, and the exact integer |k| binning;
+  - learned step: with r = 0 it is the Debye semi-implicit step bit for bit; with a
+    constant r0 one deterministic step multiplies a weak mode by
+    (1 + dt lam (A_L - A_D)) / (1 + dt lam A_L), A_L = A_D e^{r0};
+  - judge: excess 0 for r = 0, and the thresholds that force or forbid fallback.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, FIELD_RHO_C, FIELD_RHO_TYPE, LRK_BINS, LRK_FEATURES,
+                                LRK_NPARAMS, LRK_OFF, LRK_RECORD, LrkNet, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED)
+from ftspolysol_amd.lrk import features, init_params, relative_residuals
+from ftspolysol_amd.system import System
+from oracle_lib import oracle_engine, oracle_library
+
+H = 64
+
+
+def oracle_net():
+    return LrkNet(oracle_library(), "fto_")
+
+
+def random_params(seed, scale=0.4):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-scale, scale, LRK_NPARAMS)
+    p[LRK_OFF["shift"]:LRK_OFF["shift"] + 8] = rng.normal(0, 0.3, 8)
+    p[LRK_OFF["scale"]:LRK_OFF["scale"] + 8] = rng.uniform(0.5, 2.0, 8)
+    return p
+
+
+def np_forward(P, F):
+    """The documented architecture, written out in numpy."""
+    o = LRK_OFF
+    x = np.zeros((F.shape[0], 8))
+    x[:, :LRK_FEATURES] = (F - P[o["shift"]:o["shift"] + LRK_FEATURES]) * P[o["scale"]:o["scale"] + LRK_FEATURES]
+    W1 = P[o["W1"]:o["W1"] + H * 8].reshape(H, 8)
+    W2 = P[o["W2"]:o["W2"] + H * H].reshape(H, H)
+    W3 = P[o["W3"]:o["W3"] + H * H].reshape(H, H)
+    W4 = P[o["W4"]:o["W4"] + 2 * H].reshape(2, H)
+    h1 = np.tanh(x @ W1.T + P[o["b1"]:o["b1"] + H])
+    h2 = np.tanh(h1 @ W2.T + P[o["b2"]:o["b2"] + H])
+    h3 = np.tanh(h2 @ W3.T + P[o["b3"]:o["b3"] + H])
+    z = h3 @ W4.T + P[o["b4"]:o["b4"] + 2]
+    return 4 * np.tanh(z / 4)
+
+
+def synthetic_records(n, seed):
+    """Records with the statistical shape of real ones: S2 > 0, |S1| <= sqrt(S2 E_b)."""
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((n, LRK_RECORD))
+    rec[:, 0:6] = rng.normal(0, 1, (n, 6))
+    for k in range(n):
+        for c in range(2):
+            Eb = rng.uniform(0.1, 1.0, LRK_BINS)
+            S2 = rng.uniform(0.1, 1.0, LRK_BINS) * (rng.uniform(size=LRK_BINS) > 0.1)
+            S1 = np.sqrt(S2 * Eb) * rng.uniform(0.2, 1.0, LRK_BINS)
+            rec[k, 6 + c] = Eb.sum()
+            rec[k, 8 + c * LRK_BINS:8 + (c + 1) * LRK_BINS] = S1
+            rec[k, 8 + 2 * LRK_BINS + c * LRK_BINS:8 + 2 * LRK_BINS + (c + 1) * LRK_BINS] = S2
+    return rec
+
+
+# ---------------------------------------------------------------------------- network
+def test_forward_matches_numpy_layout():
+    P = random_params(1)
+    F = np.random.default_rng(2).normal(0, 1, (50, LRK_FEATURES))
+    r = oracle_net().forward64(P, F)
+    np.testing.assert_allclose(r, np_forward(P, F), rtol=1e-12, atol=1e-13)
+
+
+def test_forward_output_bias_closed_form():
+    P = np.zeros(LRK_NPARAMS)
+    P[LRK_OFF["b4"]:LRK_OFF["b4"] + 2] = (0.7, -9.0)
+    F = np.random.default_rng(3).normal(0, 1, (7, LRK_FEATURES))
+    r = oracle_net().forward64(P, F)
+    np.testing.assert_allclose(r, np.tile([4 * math.tanh(0.7 / 4), 4 * math.tanh(-9.0 / 4)], (7, 1)), rtol=1e-15)
+
+
+def test_gradient_matches_finite_differences():
+    P = random_params(4, scale=0.3)
+    rec = synthetic_records(3, 5)
+    net = oracle_net()
+    L, G = net.loss_grad64(P, rec)
+    assert L > 0
+    rng = np.random.default_rng(6)
+    idx = []
+    for key, size in (("W1", H * 8), ("b1", H), ("W2", H * H), ("b2", H), ("W3", H * H), ("b3", H), ("W4", 2 * H),
+                      ("b4", 2)):
+        idx += list(LRK_OFF[key] + rng.choice(size, min(size, 6), replace=False))
+    for j in idx:
+        if LRK_OFF["W1"] <= j < LRK_OFF["b1"] and (j - LRK_OFF["W1"]) % 8 == 7:
+            continue  # padded input column: x_7 = 0, zero gradient by construction
+        h = 1e-5 * max(1.0, abs(P[j]))
+        Pp, Pm = P.copy(), P.copy()
+        Pp[j] += h
+        Pm[j] -= h
+        fd = (net.loss_grad64(Pp, rec)[0] - net.loss_grad64(Pm, rec)[0]) / (2 * h)
+        assert abs(fd - G[j]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(G).max()), (j, fd, G[j])
+    assert np.all(G[:16] == 0)  # shift and scale are not trained
+
+
+def test_loss_is_residual_excess():
+    """Build records from explicit k-space dW, dU on a grid and check the loss
+    against the residual sum_k |dU - A_D e^{r_b} dW|^2 evaluated directly."""
+    rng = np.random.default_rng(7)
+    m = 12
+    kx = np.fft.fftfreq(m) * m
+    nx, ny, nz = np.meshgrid(kx, kx, kx, indexing="ij")
+    s = (nx ** 2 + ny ** 2 + nz ** 2).astype(np.int64).reshape(-1)
+    b = np.minimum(LRK_BINS - 1, np.array([math.isqrt(int(v)) for v in (4 * LRK_BINS ** 2 * s) // (3 * m * m)]))
+    M = m ** 3
+    P = random_params(8, scale=0.2)
+    moms = rng.normal(0, 1, 6)
+    rec = np.zeros((1, LRK_RECORD))
+    rec[0, :6] = moms
+    F = np.concatenate([((np.arange(LRK_BINS) + 0.5) / LRK_BINS)[:, None], np.tile(moms, (LRK_BINS, 1))], axis=1)
+    r = np_forward(P, F)  # [BINS, 2]
+    expected = 0.0
+    for c in range(2):
+        dW = rng.normal(size=M) + 1j * rng.normal(size=M)
+        AD = rng.uniform(0.5, 3.0, M)
+        dU = AD * dW * rng.uniform(0.5, 2.0) + 0.3 * (rng.normal(size=M) + 1j * rng.normal(size=M))
+        keep = np.arange(M) != 0
+        E = np.sum(np.abs(dU[keep]) ** 2)
+        S1 = np.bincount(b[keep], weights=(AD * np.real(dU * np.conj(dW)))[keep], minlength=LRK_BINS)
+        S2 = np.bincount(b[keep], weights=(AD ** 2 * np.abs(dW) ** 2)[keep], minlength=LRK_BINS)
+        rec[0, 6 + c] = E
+        rec[0, 8 + c * LRK_BINS:8 + (c + 1) * LRK_BINS] = S1
+        rec[0, 8 + 2 * LRK_BINS + c * LRK_BINS:8 + 2 * LRK_BINS + (c + 1) * LRK_BINS] = S2
+        A = AD * np.exp(r[b, c])
+        R = np.sum(np.abs(dU[keep] - A[keep] * dW[keep]) ** 2)
+        occupied = S2 > 0
+        Rmin = E - np.sum(S1[occupied] ** 2 / S2[occupied])
+        expected += (R - Rmin) / E
+    L, _ = oracle_net().loss_grad64(P, rec)
+    assert abs(L - expected) <= 1e-10 * expected, (L, expected)
+    # the relative residual helper reproduces the direct sums
+    rl = relative_residuals(rec, r[None])
+    assert np.all(np.isfinite(rl))
+
+
+# ---------------------------------------------------------------------------- engine
+def charged_system(scheme, m=12, N=6, n_chains=40, dt=0.01, noise=True, seed=11, lam_psi=1.0):
+    sp = [dict(num_chains=n_chains, types=[0] * N, charges=[-0.5] * N)]
+    return System(m=(m, m, m), dL=0.5, a=1.0, species=sp, xi=0.1, lB=1.0, counterion=dict(auto=True, valence=1.0),
+                  salt=dict(num_salt=20), dt=dt, lambda_omega=(1.0,), lambda_psi=lam_psi, noise=noise,
+                  scheme=scheme, seed=seed)
+
+
+def start_fields(m, seed=3, sigma=0.05):
+    rng = np.random.default_rng(seed)
+    om = sigma * (rng.normal(size=(1, m, m, m)) + 1j * rng.normal(size=(1, m, m, m)))
+    ps = sigma * (rng.normal(size=(m, m, m)) + 1j * rng.normal(size=(m, m, m)))
+    return om, ps
+
+
+def debye_kernels(e, m, dL, N, n_chains, z, a=1.0):
+    """A_D of mode 0 and psi for a uniform homopolymer (bond 1), in numpy."""
+    k = 2 * np.pi * np.where(np.arange(m) > m // 2, np.arange(m) - m, np.arange(m)) / (dL * m)
+    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    phi = np.exp(-k2 / 6)
+    pair = N + sum(2 * (N - j) * phi ** j for j in range(1, N))
+    info = e.info
+    V = info.volume
+    G2 = np.exp(-a * a * k2)
+    O = info.mode_O[0][0]
+    A_om = info.omega_coefficient[0] + G2 * (n_chains / V) * O * O * pair
+    A_ps = k2 / (4 * np.pi * 1.0) + G2 * (n_chains / V) * z * z * pair
+    for j in range(info.n_ions):
+        A_ps = A_ps + info.ion_valence[j] ** 2 * info.ion_count[j] / V * np.exp(-info.ion_smearing[j] ** 2 * k2)
+    return A_om, A_ps, k2
+
+
+def integer_bins(m):
+    kx = np.where(np.arange(m) > m // 2, np.arange(m) - m, np.arange(m))
+    nx, ny, nz = np.meshgrid(kx, kx, kx, indexing="ij")
+    s = (nx ** 2 + ny ** 2 + nz ** 2).reshape(-1).astype(np.int64)
+    return np.minimum(LRK_BINS - 1, np.array([math.isqrt(int(v)) for v in (4 * LRK_BINS ** 2 * s) // (3 * m * m)]))
+
+
+def test_records_match_numpy_recomputation():
+    m, N, n = 12, 6, 40
+    e = oracle_engine(charged_system(SCHEME_SEMI_IMPLICIT, m=m, N=N, n_chains=n))
+    e.set_fields(*start_fields(m))
+    e.lrk_set_recording(16)
+    A_om, A_ps, k2 = debye_kernels(e, m, 0.5, N, n, -0.5)
+    c = e.info.omega_coefficient[0]
+    O = e.info.mode_O[0][0]
+    bins = integer_bins(m)
+    W, U, moms = [], [], []
+    for t in range(4):
+        om = e.get_field(FIELD_OMEGA)[0]
+        ps = e.get_field(FIELD_PSI)[0]
+        e.step(1)
+        rho = e.get_field(FIELD_RHO_TYPE)[0]
+        rc = e.get_field(FIELD_RHO_C)[0]
+        u_om = c * om + 1j * O * rho
+        u_ps = np.fft.ifftn(k2 * np.fft.fftn(ps)) / (4 * np.pi) + 1j * rc
+        u_ps -= u_ps.mean()
+        W.append((np.fft.fftn(om).reshape(-1), np.fft.fftn(ps).reshape(-1)))
+        U.append((np.fft.fftn(u_om).reshape(-1), np.fft.fftn(u_ps).reshape(-1)))
+        moms.append([om.real.std(), om.imag.std(), om.real.mean(), om.imag.mean(), ps.real.std(), ps.imag.std()])
+    rec = e.lrk_get_records(16)
+    assert rec.shape == (3, LRK_RECORD)
+    keep = np.arange(m ** 3) != 0
+    for t in range(1, 4):
+        r = rec[t - 1]
+        np.testing.assert_allclose(r[:6], moms[t - 1], rtol=1e-12, atol=1e-14)
+        for ci, AD in enumerate((A_om.reshape(-1), A_ps.reshape(-1))):
+            dW = (W[t][ci] - W[t - 1][ci])[keep]
+            dU = (U[t][ci] - U[t - 1][ci])[keep]
+            E = np.sum(np.abs(dU) ** 2)
+            S1 = np.bincount(bins[keep], weights=AD[keep] * np.real(dU * np.conj(dW)), minlength=LRK_BINS)
+            S2 = np.bincount(bins[keep], weights=AD[keep] ** 2 * np.abs(dW) ** 2, minlength=LRK_BINS)
+            assert abs(r[6 + ci] - E) <= 1e-10 * E
+            got1 = r[8 + ci * LRK_BINS:8 + (ci + 1) * LRK_BINS]
+            got2 = r[8 + 2 * LRK_BINS + ci * LRK_BINS:8 + 2 * LRK_BINS + (ci + 1) * LRK_BINS]
+            np.testing.assert_allclose(got1, S1, rtol=1e-9, atol=1e-10 * np.abs(S1).max())
+            np.testing.assert_allclose(got2, S2, rtol=1e-9, atol=1e-10 * S2.max())
+    st = e.lrk_get_stats()
+    assert st["judged"] == 3 and st["records"] == 3
+
+
+def test_zero_network_is_the_debye_step():
+    m = 12
+    fields = start_fields(m)
+    a = oracle_engine(charged_system(SCHEME_SEMI_IMPLICIT, m=m))
+    b = oracle_engine(charged_system(SCHEME_SI_LEARNED, m=m))
+    for e in (a, b):
+        e.set_fields(*fields)
+    b.lrk_set_network(np.zeros(LRK_NPARAMS, dtype=np.float32), 1e-3)
+    ra, rb = a.step(4, record_every=1), b.step(4, record_every=1)
+    for x, y in zip(ra, rb):
+        assert x["H"] == y["H"]
+    for fid in (FIELD_OMEGA, FIELD_PSI):
+        assert np.array_equal(a.get_field(fid), b.get_field(fid))
+    st = b.lrk_get_stats()
+    assert st["judged"] == 3 and st["last_excess"] == 0.0
+    assert st["learned_steps"] == 3 and st["fallback_steps"] == 1
+
+
+def test_constant_output_damps_weak_mode():
+    """Deterministic steps around the homogeneous state: the first step (nothing
+    judged) damps a weak omega mode by 1/(1 + dt A_D); the second, with the network
+    trusted, preconditions with A_L = A_D e^{r0} while the true response stays A_D,
+    so it multiplies the mode by (1 + dt (A_L - A_D)) / (1 + dt A_L)."""
+    m, N, n, xi, dt = 16, 10, 6, 0.05, 0.01
+    sp = [dict(num_chains=n, types=[0] * N, charges=[0.0] * N)]
+    s = System(m=(m, m, m), dL=0.5, a=1.0, species=sp, xi=xi, lB=0.0, dt=dt, lambda_omega=(1.0,), noise=False,
+               scheme=SCHEME_SI_LEARNED)
+    e = oracle_engine(s)
+    P = np.zeros(LRK_NPARAMS, dtype=np.float32)
+    z0 = 1.3
+    P[LRK_OFF["b4"]] = z0
+    e.lrk_set_network(P, 1e300)
+    k0 = 2 * np.pi / 8.0
+    x = np.arange(m) * 0.5
+    eps = 1e-6
+    om = np.full((1, m, m, m), -1j * xi, dtype=complex)
+    om[0] += -1j * eps * np.cos(k0 * x)[:, None, None]
+    e.set_fields(om)
+    amps = []
+    for _ in range(2):
+        e.step(1)
+        w = e.get_field(FIELD_OMEGA)[0] + 1j * xi
+        amps.append(2 * np.mean(w * np.cos(k0 * x)[:, None, None]) / (-1j * eps))
+    V = e.info.volume
+    phi = np.exp(-k0 ** 2 / 6)
+    A = n * N / V / xi + np.exp(-k0 ** 2) * (n / V) * (N + sum(2 * (N - j) * phi ** j for j in range(1, N)))
+    r0 = 4 * math.tanh(np.float32(z0) / 4)
+    assert abs(amps[0] - 1 / (1 + dt * A)) < 1e-4
+    AL = A * math.exp(r0)
+    assert abs(amps[1] / amps[0] - (1 + dt * (AL - A)) / (1 + dt * AL)) < 1e-4, (amps, A, r0)
+    st = e.lrk_get_stats()
+    assert st["learned_steps"] == 1 and st["fallback_steps"] == 1
+
+
+def test_thresholds_force_and_forbid_fallback():
+    m = 12
+    fields = start_fields(m)
+    P = np.zeros(LRK_NPARAMS, dtype=np.float32)
+    P[LRK_OFF["b4"]:LRK_OFF["b4"] + 2] = (1.5, 1.5)
+    ref = oracle_engine(charged_system(SCHEME_SEMI_IMPLICIT, m=m))
+    never = oracle_engine(charged_system(SCHEME_SI_LEARNED, m=m))
+    always = oracle_engine(charged_system(SCHEME_SI_LEARNED, m=m))
+    for e in (ref, never, always):
+        e.set_fields(*fields)
+    never.lrk_set_network(P, -1e300)
+    always.lrk_set_network(P, 1e300)
+    ref.lrk_set_recording(8)  # the Debye scheme judges its steps while it records
+    for e in (ref, never, always):
+        e.step(4)
+    assert np.array_equal(ref.get_field(FIELD_OMEGA), never.get_field(FIELD_OMEGA))
+    assert not np.array_equal(ref.get_field(FIELD_OMEGA), always.get_field(FIELD_OMEGA))
+    sn, sa = never.lrk_get_stats(), always.lrk_get_stats()
+    assert sn["learned_steps"] == 0 and sn["fallback_steps"] == 4
+    assert sa["learned_steps"] == 3 and sa["fallback_steps"] == 1
+    # the judge sees the same residuals of the Debye kernel on the same (Debye) steps
+    np.testing.assert_allclose(sn["sum_rel_residual_debye"], ref.lrk_get_stats()["sum_rel_residual_debye"])
+
+
+def test_records_of_a_set_fields_restart_and_ring():
+    m = 12
+    e = oracle_engine(charged_system(SCHEME_SEMI_IMPLICIT, m=m))
+    e.set_fields(*start_fields(m))
+    e.lrk_set_recording(2)
+    e.step(5)
+    rec = e.lrk_get_records(10)
+    assert rec.shape[0] == 2  # 4 judged steps, ring keeps the newest 2
+    e.set_fields(*start_fields(m, seed=9))  # nothing judged across new fields
+    e.step(1)
+    assert e.lrk_get_records(10).shape[0] == 0
+    e.step(2)
+    assert e.lrk_get_records(10).shape[0] == 2
+
+
+def test_init_params_start_at_debye():
+    rec = synthetic_records(4, 12)
+    P = init_params(rec, seed=1)
+    r = oracle_net().forward(P, features(rec))
+    assert np.all(r == 0.0)
+    L = oracle_net().loss_grad(P, rec, grad=False)
+    rd = relative_residuals(rec, None)
+    # with r = 0 the loss is the Debye kernel's excess over the per-bin optimum
+    assert L > 0 and np.all(np.isfinite(rd))
