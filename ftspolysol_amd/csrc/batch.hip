@@ -374,7 +374,7 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
             if (mask[r] && (rc = g.reps[r]->pc_correct(rec[r] != nullptr, rec[r]))) return rc;
     } else {
         for (size_t r = 0; r < n; ++r)
-            if (mask[r] && (rc = g.reps[r]->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT))) return rc;
+            if (mask[r] && (rc = g.reps[r]->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT || scheme == FTS_SCHEME_SI_LEARNED))) return rc;
     }
     for (size_t r = 0; r < n; ++r) {
         if (!mask[r]) continue;

@@ -18,6 +18,7 @@
 #include "../../include/fts.h"
 #include "fft.h"
 #include "kernel_params.h"
+#include "lrk.h"
 
 using namespace fts;
 
@@ -105,7 +106,8 @@ enum {
     SC_LF = 32,       // [8] per species: log-gain LF(N-1) of the renormalised forward sweep
     SC_H = 40,        // [8] per interaction mode: (c_l / 2) <omega_l^2>
     SC_PSI_S = 48,    // [7] k-space contraction of psi with itself
-    SC_COUNT = 56
+    SC_LRK = 56,      // [4] (mean, mean square) of Re w0, Im w0, Re psi, Im psi (learned kernel's moments)
+    SC_COUNT = 60
 };
 
 } // namespace fts_host
@@ -248,6 +250,25 @@ struct fts_engine {
     DevBuf<double2> ckpt_omega, ckpt_psi;
     int maybe_checkpoint();
     int restart_from_checkpoint(long long failed_step);
+
+    // Learned linear-response kernel (FTS_SCHEME_SI_LEARNED; the Debye scheme runs the
+    // same judge while it records training data).  A_D tables, |k| bins, the wave
+    // vectors sorted by bin (with A_D in that order) and their partial-sum chunks;
+    // k-space w0, u0, psi, u_psi of the current and previous step (ping-pong).
+    DevBuf<double> si_A_omega, si_A_psi;
+    DevBuf<unsigned char> lrk_bin;
+    DevBuf<int> lrk_perm, lrk_chunk_start, lrk_bin_chunk;
+    DevBuf<double> lrk_ad_perm, lrk_chunk_sums, lrk_ring;
+    DevBuf<double2> lrk_u, lrk_hat[2][2][2]; // [field][w, u][parity]
+    DevBuf<float> lrk_params;
+    DevBuf<LrkDevState> lrk_state;
+    int lrk_nperm = 0, lrk_nchunks = 0, lrk_par = 0, lrk_capacity = 0;
+    long long lrk_read_idx = 0;
+    bool lrk_ready = false, lrk_have_prev = false;
+    double lrk_threshold = 1.0e-3;
+    bool lrk_active() const { return scheme == FTS_SCHEME_SI_LEARNED || lrk_capacity > 0; }
+    int lrk_setup();
+    int lrk_step();
 };
 
 // Builds an engine for system s.  shared: a stream the engine uses instead of

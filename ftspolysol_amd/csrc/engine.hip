@@ -56,6 +56,15 @@ void fts_engine::release()
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
+    for (DevBuf<double>* b : {&si_A_omega, &si_A_psi, &lrk_ad_perm, &lrk_chunk_sums, &lrk_ring}) b->release();
+    for (DevBuf<int>* b : {&lrk_perm, &lrk_chunk_start, &lrk_bin_chunk}) b->release();
+    lrk_bin.release();
+    lrk_u.release();
+    for (auto& f : lrk_hat)
+        for (auto& k : f)
+            for (auto& b : k) b.release();
+    lrk_params.release();
+    lrk_state.release();
     if (stream && own_stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
     stream = nullptr;
@@ -434,6 +443,7 @@ int fts_engine::build_si_tables()
     std::vector<double> kv(static_cast<size_t>(m1 + m2 + m3));
     HIP_TRY(hipMemcpy(kv.data(), kvec.p, sizeof(double) * kv.size(), hipMemcpyDeviceToHost));
     std::vector<double> Pw(static_cast<size_t>(n_omega) * M), Pp(static_cast<size_t>(M), 1.0);
+    std::vector<double> Aw(static_cast<size_t>(n_omega) * M), Ap(static_cast<size_t>(M), 0.0);
     auto pair_sum = [&](const HostSpecies& sp, const std::vector<double>& w, double ksq) {
         double total = 0.0, h = 0.0;
         for (int s = 0; s < sp.N; ++s) {
@@ -463,6 +473,7 @@ int fts_engine::build_si_tables()
             for (size_t p = 0; p < species.size(); ++p)
                 resp += static_cast<double>(species[p].n) / V * pair_sum(species[p], wmode[static_cast<size_t>(l)][p], ksq);
             const double A = omega_coefficient(l) + (gamma_imag(l) ? 1.0 : -1.0) * G2 * resp;
+            Aw[static_cast<size_t>(l) * M + i] = A;
             Pw[static_cast<size_t>(l) * M + i] = 1.0 / (1.0 + dt * lambda_omega[static_cast<size_t>(l)] * A);
         }
         if (electro()) {
@@ -474,6 +485,7 @@ int fts_engine::build_si_tables()
                 ion_resp += ion.z * ion.z * static_cast<double>(ion.n) / V * g * g;
             }
             const double A = ksq / (4.0 * Pi * lB) + G2 * resp + ion_resp;
+            Ap[static_cast<size_t>(i)] = A;
             Pp[static_cast<size_t>(i)] = 1.0 / (1.0 + dt * lambda_psi * A);
         }
     }
@@ -483,6 +495,132 @@ int fts_engine::build_si_tables()
     HIP_TRY(si_table_psi.alloc(Pp.size()));
     HIP_TRY(hipMemcpy(si_table_omega.p, Pw.data(), sizeof(double) * Pw.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(si_table_psi.p, Pp.data(), sizeof(double) * Pp.size(), hipMemcpyHostToDevice));
+    HIP_TRY(si_A_omega.alloc(Aw.size()));
+    HIP_TRY(si_A_psi.alloc(Ap.size()));
+    if (!Aw.empty()) HIP_TRY(hipMemcpy(si_A_omega.p, Aw.data(), sizeof(double) * Aw.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(si_A_psi.p, Ap.data(), sizeof(double) * Ap.size(), hipMemcpyHostToDevice));
+    return FTS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned linear-response kernel (no reference counterpart)
+// ---------------------------------------------------------------------------
+// Exact |k| bin of the wave vector with spectral indices (nx, ny, nz): the largest b
+// with 3 b^2 m1^2 m2^2 m3^2 <= 4 B^2 (nx^2 m2^2 m3^2 + ny^2 m1^2 m3^2 + nz^2 m1^2 m2^2),
+// i.e. b = floor(B |k| / k_ref) with k_ref = sqrt(3) pi / dL, clamped to B - 1.
+// Integer arithmetic, so no rounding moves a wave vector across a bin edge.
+static int lrk_bin_of(long long nx, long long ny, long long nz, long long m1, long long m2, long long m3)
+{
+    using u128 = unsigned __int128;
+    const u128 a1 = static_cast<u128>(m1 * m1), a2 = static_cast<u128>(m2 * m2), a3 = static_cast<u128>(m3 * m3);
+    const u128 rhs = static_cast<u128>(4) * LRK_NB * LRK_NB *
+                     (static_cast<u128>(nx * nx) * a2 * a3 + static_cast<u128>(ny * ny) * a1 * a3 + static_cast<u128>(nz * nz) * a1 * a2);
+    const u128 den = static_cast<u128>(3) * a1 * a2 * a3;
+    int lo = 0, hi = LRK_NB - 1; // largest b in [0, B-1] with den b^2 <= rhs
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (den * static_cast<u128>(mid) * static_cast<u128>(mid) <= rhs) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Device buffers of the learned kernel and its judge (needs the A_D tables).
+int fts_engine::lrk_setup()
+{
+    if (lrk_ready) return FTS_OK;
+    if (n_omega > 1 || (n_omega == 0 && !electro()))
+        return fail(FTS_INVALID, "the learned kernel covers interaction mode 0 and psi: needs n_omega <= 1 and at least one field");
+    const size_t Ms = static_cast<size_t>(M);
+    auto mode = [](int i, int m) { return i > m / 2 ? i - m : i; };
+    std::vector<unsigned char> bin(Ms);
+    for (int x = 0; x < m1; ++x)
+        for (int y = 0; y < m2; ++y)
+            for (int z = 0; z < m3; ++z)
+                bin[(static_cast<size_t>(x) * m2 + y) * m3 + z] = static_cast<unsigned char>(lrk_bin_of(mode(x, m1), mode(y, m2), mode(z, m3), m1, m2, m3));
+    // wave vectors k != 0 sorted by bin (stable in the grid index), split into chunks
+    std::vector<int> count(LRK_NB + 1, 0);
+    for (size_t i = 1; i < Ms; ++i) ++count[bin[i] + 1];
+    for (int b = 0; b < LRK_NB; ++b) count[b + 1] += count[b];
+    lrk_nperm = M - 1;
+    std::vector<int> perm(static_cast<size_t>(std::max(1, lrk_nperm))), fill(count.begin(), count.end() - 1);
+    for (size_t i = 1; i < Ms; ++i) perm[static_cast<size_t>(fill[bin[i]]++)] = static_cast<int>(i);
+    std::vector<int> chunk_start, bin_chunk(LRK_NB + 1, 0);
+    for (int b = 0; b < LRK_NB; ++b) {
+        bin_chunk[b] = static_cast<int>(chunk_start.size());
+        for (int j = count[b]; j < count[b + 1]; j += LRK_CHUNK) chunk_start.push_back(j);
+    }
+    bin_chunk[LRK_NB] = static_cast<int>(chunk_start.size());
+    lrk_nchunks = static_cast<int>(chunk_start.size());
+    // chunk c covers [cs[c], cs[c + 1]): after a bin's last chunk comes the first chunk of the
+    // next non-empty bin, which starts where the bin ends
+    std::vector<int> cs(chunk_start);
+    cs.push_back(lrk_nperm);
+    std::vector<double> Aw(Ms, 0.0), Ap(Ms, 0.0), ad(2 * static_cast<size_t>(std::max(1, lrk_nperm)), 0.0);
+    if (n_omega > 0) HIP_TRY(hipMemcpy(Aw.data(), si_A_omega.p, sizeof(double) * Ms, hipMemcpyDeviceToHost));
+    if (electro()) HIP_TRY(hipMemcpy(Ap.data(), si_A_psi.p, sizeof(double) * Ms, hipMemcpyDeviceToHost));
+    for (int j = 0; j < lrk_nperm; ++j) {
+        ad[static_cast<size_t>(j)] = Aw[static_cast<size_t>(perm[static_cast<size_t>(j)])];
+        ad[static_cast<size_t>(lrk_nperm + j)] = Ap[static_cast<size_t>(perm[static_cast<size_t>(j)])];
+    }
+    HIP_TRY(lrk_bin.alloc(Ms));
+    HIP_TRY(lrk_perm.alloc(perm.size()));
+    HIP_TRY(lrk_chunk_start.alloc(cs.size()));
+    HIP_TRY(lrk_bin_chunk.alloc(bin_chunk.size()));
+    HIP_TRY(lrk_ad_perm.alloc(ad.size()));
+    HIP_TRY(lrk_chunk_sums.alloc(6 * static_cast<size_t>(std::max(1, lrk_nchunks))));
+    HIP_TRY(hipMemcpy(lrk_bin.p, bin.data(), Ms, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(lrk_perm.p, perm.data(), sizeof(int) * perm.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(lrk_chunk_start.p, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(lrk_bin_chunk.p, bin_chunk.data(), sizeof(int) * bin_chunk.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(lrk_ad_perm.p, ad.data(), sizeof(double) * ad.size(), hipMemcpyHostToDevice));
+    for (int c = 0; c < 2; ++c) {
+        if ((c == 0 && n_omega == 0) || (c == 1 && !electro())) continue;
+        for (int k = 0; k < 2; ++k)
+            for (int par = 0; par < 2; ++par) HIP_TRY(lrk_hat[c][k][par].alloc(Ms));
+    }
+    if (n_omega > 0) HIP_TRY(lrk_u.alloc(Ms));
+    if (!lrk_params.p) {
+        HIP_TRY(lrk_params.alloc(FTS_LRK_NPARAMS));
+        HIP_TRY(hipMemset(lrk_params.p, 0, sizeof(float) * FTS_LRK_NPARAMS));
+    }
+    HIP_TRY(lrk_state.alloc(1));
+    HIP_TRY(hipMemset(lrk_state.p, 0, sizeof(LrkDevState)));
+    lrk_ready = true;
+    lrk_have_prev = false;
+    return FTS_OK;
+}
+
+// One judged step: called by the semi-implicit update once u(w_t) is known (lrk_u after
+// k_mode0_drift, dpsi after psi_drift).  Judges the previous step, writes its record,
+// decides this step's kernel and (SI_LEARNED) rebuilds the P tables the update applies.
+int fts_engine::lrk_step()
+{
+    int rc;
+    const int cur = lrk_par, prv = lrk_par ^ 1;
+    const bool f0 = n_omega > 0, f1 = electro();
+    if (f0) {
+        hipLaunchKernelGGL(k_mode0_drift, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, omega.p, rho_type.p, M, lrk_u.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = fft_forward(omega.p, lrk_hat[0][0][cur].p)) || (rc = fft_forward(lrk_u.p, lrk_hat[0][1][cur].p))) return rc;
+    }
+    if (f1 && ((rc = fft_forward(psi.p, lrk_hat[1][0][cur].p)) || (rc = fft_forward(dpsi.p, lrk_hat[1][1][cur].p)))) return rc;
+    HIP_TRY(lrk_moment_sums(f0 ? omega.p : nullptr, f1 ? psi.p : nullptr, M, partials.p, stream));
+    if ((rc = reduce_finalize(4, SC_LRK, 1.0 / static_cast<double>(M)))) return rc;
+    HIP_TRY(lrk_infer(lrk_params.p, scal.p + SC_LRK, lrk_state.p, stream));
+    if (lrk_have_prev)
+        HIP_TRY(lrk_bin_chunks(lrk_nchunks, lrk_chunk_start.p, lrk_perm.p, lrk_ad_perm.p, lrk_nperm, f0 ? lrk_hat[0][0][cur].p : nullptr,
+                               f0 ? lrk_hat[0][0][prv].p : nullptr, f0 ? lrk_hat[0][1][cur].p : nullptr, f0 ? lrk_hat[0][1][prv].p : nullptr,
+                               f1 ? lrk_hat[1][0][cur].p : nullptr, f1 ? lrk_hat[1][0][prv].p : nullptr, f1 ? lrk_hat[1][1][cur].p : nullptr,
+                               f1 ? lrk_hat[1][1][prv].p : nullptr, lrk_chunk_sums.p, stream));
+    HIP_TRY(lrk_judge(lrk_state.p, lrk_have_prev ? 1 : 0, lrk_bin_chunk.p, lrk_chunk_sums.p, lrk_threshold,
+                      scheme == FTS_SCHEME_SI_LEARNED ? 1 : 0, lrk_ring.p, lrk_capacity, stream));
+    if (scheme == FTS_SCHEME_SI_LEARNED)
+        HIP_TRY(lrk_ptable(lrk_state.p, lrk_bin.p, f0 ? si_A_omega.p : nullptr, f1 ? si_A_psi.p : nullptr,
+                           f0 ? dt * lambda_omega[0] : 0.0, dt * lambda_psi, M, f0 ? si_table_omega.p : nullptr, f1 ? si_table_psi.p : nullptr,
+                           stream));
+    lrk_par = prv;
+    lrk_have_prev = true;
     return FTS_OK;
 }
 
@@ -1339,6 +1477,7 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
         if ((rc = reduce_finalize(1, SC_ETA_MEAN, invM))) return rc;
     }
     if ((rc = psi_drift(psi.p, rho_c.p, dpsi.p))) return rc;
+    if (si && lrk_active() && (rc = lrk_step())) return rc;
     if (n_omega > 0) {
         hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, seed, t, M, si ? 0 : 1, omega.p, drift.p);
         HIP_TRY(hipGetLastError());
@@ -1398,6 +1537,7 @@ int fts_engine::restart_from_checkpoint(long long failed_step)
     seed = seed0 + static_cast<unsigned long long>(noise_epoch) * 0x9E3779B97F4A7C15ull;
     (void)failed_step;
     step_counter = ckpt_step;
+    lrk_have_prev = false;
     return FTS_OK;
 }
 
@@ -1437,7 +1577,7 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
     if (s->n_species <= 0 || s->n_species > FTS_MAX_SPECIES || !s->species) return fail(FTS_INVALID, "polymer species_count");
-    if (s->scheme < 0 || s->scheme > 2) return fail(FTS_INVALID, "scheme");
+    if (s->scheme < 0 || s->scheme > 3) return fail(FTS_INVALID, "scheme");
     fts_engine* e = new fts_engine();
     if (shared) {
         e->stream = shared;
@@ -1556,7 +1696,9 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
         if (!std::isfinite(x) || x < 0.0) return bail(fail(FTS_INVALID, "lambda_omega"));
     if ((rc = e->build_ions(s))) return bail(rc);
     if ((rc = e->setup_device())) return bail(rc);
-    if (e->scheme == FTS_SCHEME_SEMI_IMPLICIT && (rc = e->build_si_tables())) return bail(rc);
+    const bool semi = e->scheme == FTS_SCHEME_SEMI_IMPLICIT || e->scheme == FTS_SCHEME_SI_LEARNED;
+    if (semi && (rc = e->build_si_tables())) return bail(rc);
+    if (e->scheme == FTS_SCHEME_SI_LEARNED && (rc = e->lrk_setup())) return bail(rc);
     if (tune && s->fft_schedule == FTS_SCHED_AUTO && (rc = e->autotune_schedule())) return bail(rc);
     *out = e;
     return FTS_OK;
@@ -1619,6 +1761,7 @@ int fts_set_fields(fts_engine* h, const double* om, const double* ps)
     if (!h->electro()) HIP_TRY(hipMemsetAsync(h->psi.p, 0, sizeof(double2) * Ms, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->ckpt_step = -1; // new fields: the next step takes a new checkpoint
+    h->lrk_have_prev = false;
     return FTS_OK;
 }
 
@@ -1720,7 +1863,9 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
         const long long step = h->step_counter + 1; // 1-based step label (complexLangevin.cu:206)
         const bool record = record_every > 0 && step % record_every == 0 && records && nrec < capacity;
         fts_observables* rec = record ? &records[nrec] : nullptr;
-        rc = h->scheme == FTS_SCHEME_PC_REFERENCE ? h->step_pc(record, rec) : h->step_em(record, rec, h->scheme == FTS_SCHEME_SEMI_IMPLICIT);
+        rc = h->scheme == FTS_SCHEME_PC_REFERENCE
+                 ? h->step_pc(record, rec)
+                 : h->step_em(record, rec, h->scheme == FTS_SCHEME_SEMI_IMPLICIT || h->scheme == FTS_SCHEME_SI_LEARNED);
         if (record) {
             rec->step = step;
             ++nrec;
@@ -1836,6 +1981,87 @@ int fts_get_timing(fts_engine* h, fts_timing* out)
     int rc = h->resolve_timing();
     if (rc) return rc;
     *out = h->tstat;
+    return FTS_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// C ABI: the learned linear-response kernel of an engine
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int fts_lrk_set_network(fts_engine* h, const float* params, double threshold)
+{
+    if (!h || !params) return fail(FTS_INVALID, "lrk_set_network: null argument");
+    if (std::isnan(threshold)) return fail(FTS_INVALID, "lrk_set_network: threshold is NaN");
+    for (int j = 0; j < FTS_LRK_NPARAMS; ++j)
+        if (!std::isfinite(params[j])) return fail(FTS_INVALID, "lrk_set_network: non-finite parameter");
+    if (!h->lrk_params.p) HIP_TRY(h->lrk_params.alloc(FTS_LRK_NPARAMS));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(h->lrk_params.p, params, sizeof(float) * FTS_LRK_NPARAMS, hipMemcpyHostToDevice));
+    h->lrk_threshold = threshold;
+    return FTS_OK;
+}
+
+int fts_lrk_set_recording(fts_engine* h, int capacity)
+{
+    if (!h || capacity < 0) return fail(FTS_INVALID, "lrk_set_recording: bad argument");
+    if (capacity > 0 && h->scheme != FTS_SCHEME_SEMI_IMPLICIT && h->scheme != FTS_SCHEME_SI_LEARNED)
+        return fail(FTS_INVALID, "training records need a semi-implicit scheme");
+    int rc;
+    if (capacity > 0 && (rc = h->lrk_setup())) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->lrk_ring.release();
+    if (capacity > 0) HIP_TRY(h->lrk_ring.alloc(static_cast<size_t>(capacity) * FTS_LRK_RECORD));
+    h->lrk_capacity = capacity;
+    if (h->lrk_state.p) {
+        LrkDevState st;
+        HIP_TRY(hipMemcpy(&st, h->lrk_state.p, sizeof(st), hipMemcpyDeviceToHost));
+        st.records = 0;
+        h->lrk_read_idx = st.write_idx;
+        HIP_TRY(hipMemcpy(h->lrk_state.p, &st, sizeof(st), hipMemcpyHostToDevice));
+    }
+    return FTS_OK;
+}
+
+int fts_lrk_get_records(fts_engine* h, double* out, int max_records, int* n_out)
+{
+    if (!h || !n_out || max_records < 0 || (max_records > 0 && !out)) return fail(FTS_INVALID, "lrk_get_records: bad argument");
+    *n_out = 0;
+    if (!h->lrk_state.p || h->lrk_capacity <= 0) return FTS_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    LrkDevState st;
+    HIP_TRY(hipMemcpy(&st, h->lrk_state.p, sizeof(st), hipMemcpyDeviceToHost));
+    const long long first = std::max(h->lrk_read_idx, st.write_idx - h->lrk_capacity);
+    const long long n = std::min<long long>(max_records, st.write_idx - first);
+    for (long long k = 0; k < n; ++k) {
+        const size_t slot = static_cast<size_t>((first + k) % h->lrk_capacity);
+        HIP_TRY(hipMemcpy(out + static_cast<size_t>(k) * FTS_LRK_RECORD, h->lrk_ring.p + slot * FTS_LRK_RECORD, sizeof(double) * FTS_LRK_RECORD,
+                          hipMemcpyDeviceToHost));
+    }
+    h->lrk_read_idx = st.write_idx;
+    *n_out = static_cast<int>(n);
+    return FTS_OK;
+}
+
+int fts_lrk_get_stats(fts_engine* h, fts_lrk_stats* out)
+{
+    if (!h || !out) return fail(FTS_INVALID, "lrk_get_stats: null argument");
+    std::memset(out, 0, sizeof(*out));
+    if (!h->lrk_state.p) return FTS_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    LrkDevState st;
+    HIP_TRY(hipMemcpy(&st, h->lrk_state.p, sizeof(st), hipMemcpyDeviceToHost));
+    out->judged = st.judged;
+    out->learned_steps = st.learned;
+    out->fallback_steps = st.fallback;
+    out->records = st.records;
+    out->last_excess = st.last_excess;
+    for (int c = 0; c < 2; ++c) {
+        out->sum_rel_residual_learned[c] = st.sum_rl[c];
+        out->sum_rel_residual_debye[c] = st.sum_rd[c];
+    }
     return FTS_OK;
 }
 

@@ -375,6 +375,14 @@ __global__ void k_omega_increment(ModeCoefs mc, const double2* __restrict__ rho,
     }
 }
 
+// u of interaction mode 0 (the force the learned kernel is judged against)
+__global__ void k_mode0_drift(ModeCoefs mc, const double2* __restrict__ om, const double2* __restrict__ rho, int M, double2* __restrict__ u)
+{
+    const int i = blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= M) return;
+    u[i] = mode_drift(mc, 0, om, rho, M, i);
+}
+
 // psi update: out = base + c1 d1 + c2 d2 + (sigma n_i - eta_mean) on the real part
 __global__ void k_psi_update(const double2* __restrict__ base, const double2* __restrict__ d1, double c1, const double2* __restrict__ dd2,
                              double c2, double sigma, const double2* __restrict__ eta_mean, unsigned long long seed, unsigned long long step,

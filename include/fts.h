@@ -322,10 +322,11 @@ int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out);
  *            = E_c - 2 sum_b e^{r_b} S1_{c,b} + sum_b e^{2 r_b} S2_{c,b}
  * with E_c = sum |dU|^2, S1_{c,b} = sum_{k in b} A_D Re(dU conj dW) and
  * S2_{c,b} = sum_{k in b} A_D^2 |dW|^2.  The learned kernel is used for the next
- * step while its excess residual max_c (R_c(learned) - R_c(A_D)) / E_c stays at or
- * below the threshold; otherwise the step falls back to A_D (the kernel is still
- * judged, and comes back when it does well again).  The first step after creation,
- * fts_set_fields or a restart has nothing judged yet and uses A_D.
+ * step while its excess force residual over the Debye kernel on the same step,
+ * max_c (R_c(learned) - R_c(A_D)) / E_c, stays at or below the threshold; otherwise
+ * the step falls back to A_D (the kernel is still judged, and comes back when it
+ * does well again).  The first step after creation, fts_set_fields or a restart has
+ * nothing judged yet and uses A_D.
  *
  * The network minimises the same residual: a training record is one judged step
  * (moments at its start, E, S1, S2), and the loss of a record is
@@ -363,7 +364,7 @@ typedef struct fts_lrk_stats {
     long long learned_steps;  /* steps taken with the learned kernel */
     long long fallback_steps; /* steps taken with A_D (first steps and fallbacks) */
     long long records;        /* training records written since recording started */
-    double last_excess;       /* excess residual of the last judged step */
+    double last_excess;       /* max_c (R_c(learned) - R_c(A_D)) / E_c of the last judged step */
     double sum_rel_residual_learned[2]; /* sum over judged steps of R_c(learned) / E_c */
     double sum_rel_residual_debye[2];   /* sum over judged steps of R_c(A_D) / E_c */
 } fts_lrk_stats;

@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -1705,6 +1706,65 @@ int fto_lrk_loss_grad(const float* params, const double* records, int n_records,
     *loss = oracle::lrk_loss_grad(P.data(), records, n_records, grad ? G.data() : nullptr);
     if (grad)
         for (int j = 0; j < FTS_LRK_NPARAMS; ++j) grad[j] = static_cast<float>(G[static_cast<size_t>(j)]);
+    return FTS_OK;
+}
+
+// Adam on minibatches, as fts_lrk_trainer_run, in FP64: the same per-epoch shuffle
+// (std::mt19937_64 + std::shuffle of the record order), minibatch loss = mean over its
+// records, learning rate lr0 (lr1/lr0)^(step/(total-1)), beta 0.9/0.999, eps 1e-8.
+struct fts_lrk_trainer {
+    std::vector<double> records;
+    int n = 0;
+};
+
+int fto_lrk_trainer_create(const double* records, int n_records, fts_lrk_trainer** out)
+{
+    if (!records || n_records <= 0 || !out) return fail(FTS_INVALID, "lrk_trainer_create: bad argument");
+    fts_lrk_trainer* t = new fts_lrk_trainer();
+    t->records.assign(records, records + static_cast<size_t>(n_records) * FTS_LRK_RECORD);
+    t->n = n_records;
+    *out = t;
+    return FTS_OK;
+}
+
+void fto_lrk_trainer_destroy(fts_lrk_trainer* t) { delete t; }
+
+int fto_lrk_trainer_run(fts_lrk_trainer* t, float* params, int epochs, int batch_records, double lr0, double lr1,
+                        unsigned long long seed, double* loss_history)
+{
+    if (!t || !params || epochs < 0 || batch_records <= 0 || !(lr0 > 0.0) || !(lr1 > 0.0)) return fail(FTS_INVALID, "lrk_trainer_run: bad argument");
+    const int B = std::min(batch_records, t->n);
+    const int per_epoch = (t->n + B - 1) / B;
+    const long long total = static_cast<long long>(per_epoch) * epochs;
+    std::vector<double> P(params, params + FTS_LRK_NPARAMS), m1(FTS_LRK_NPARAMS, 0.0), m2(FTS_LRK_NPARAMS, 0.0), G(FTS_LRK_NPARAMS);
+    std::vector<double> batch;
+    std::mt19937_64 rng(seed);
+    std::vector<int> order(static_cast<size_t>(t->n));
+    std::iota(order.begin(), order.end(), 0);
+    long long step = 0;
+    for (int ep = 0; ep < epochs; ++ep) {
+        std::shuffle(order.begin(), order.end(), rng);
+        double el = 0.0;
+        for (int b0 = 0; b0 < t->n; b0 += B) {
+            const int nb = std::min(B, t->n - b0);
+            batch.resize(static_cast<size_t>(nb) * FTS_LRK_RECORD);
+            for (int k = 0; k < nb; ++k)
+                std::memcpy(batch.data() + static_cast<size_t>(k) * FTS_LRK_RECORD,
+                            t->records.data() + static_cast<size_t>(order[static_cast<size_t>(b0 + k)]) * FTS_LRK_RECORD, sizeof(double) * FTS_LRK_RECORD);
+            el += oracle::lrk_loss_grad(P.data(), batch.data(), nb, G.data()) / per_epoch;
+            ++step;
+            const double frac = total > 1 ? static_cast<double>(step - 1) / static_cast<double>(total - 1) : 0.0;
+            const double lr = lr0 * std::pow(lr1 / lr0, frac);
+            const double bc1 = 1.0 - std::pow(0.9, static_cast<double>(step)), bc2 = 1.0 - std::pow(0.999, static_cast<double>(step));
+            for (int j = 16; j < FTS_LRK_NPARAMS; ++j) {
+                m1[static_cast<size_t>(j)] = 0.9 * m1[static_cast<size_t>(j)] + 0.1 * G[static_cast<size_t>(j)];
+                m2[static_cast<size_t>(j)] = 0.999 * m2[static_cast<size_t>(j)] + 0.001 * G[static_cast<size_t>(j)] * G[static_cast<size_t>(j)];
+                P[static_cast<size_t>(j)] -= lr * (m1[static_cast<size_t>(j)] / bc1) / (std::sqrt(m2[static_cast<size_t>(j)] / bc2) + 1e-8);
+            }
+        }
+        if (loss_history) loss_history[ep] = el;
+    }
+    for (int j = 0; j < FTS_LRK_NPARAMS; ++j) params[j] = static_cast<float>(P[static_cast<size_t>(j)]);
     return FTS_OK;
 }
 

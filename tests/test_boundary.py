@@ -52,10 +52,10 @@ def test_oracle_exports_the_same_interface():
 
 def test_struct_layout(tmp_path):
     """The ctypes mirrors have the C compiler's sizes and offsets for include/fts.h."""
-    from ftspolysol_amd.api import BatchInfo, CounterionDesc, SaltDesc, SpeciesDesc, Timing
+    from ftspolysol_amd.api import BatchInfo, CounterionDesc, LrkStats, SaltDesc, SpeciesDesc, Timing
     structs = {"fts_species_desc": SpeciesDesc, "fts_counterion_desc": CounterionDesc, "fts_salt_desc": SaltDesc,
                "fts_system": SystemDesc, "fts_observables": Observables, "fts_info": Info, "fts_timing": Timing,
-               "fts_batch_info": BatchInfo}
+               "fts_batch_info": BatchInfo, "fts_lrk_stats": LrkStats}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "fts.h"', "int main(void) {"]
     for cname, py in structs.items():
         src.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

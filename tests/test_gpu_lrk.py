@@ -1,0 +1,177 @@
+"""GPU parity of the learned linear-response kernel (FTS_SCHEME_SI_LEARNED) against
+the FP64 oracle, through the C ABI.
+
+Tolerances: the network runs in FP32 on the device and in FP64 in the oracle, so
+r agrees to ~1e-6 absolute (checked at 3e-5), gradients to 1e-4 of their norm,
+and trajectories that use the learned kernel to 1e-8 relative (the FP32 r enters
+the update only through P = 1 / (1 + dt lam A_D e^r)).  Records and the Debye-side
+judge are FP64 on both sides and agree to 1e-9.
+"""
+import numpy as np
+import pytest
+
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, LRK_FEATURES, LRK_NPARAMS, LRK_OFF, LrkNet, SCHEME_SEMI_IMPLICIT,
+                                SCHEME_SI_LEARNED, hip_batch, hip_engine, hip_lrk)
+from ftspolysol_amd.lrk import init_params, network_r, relative_residuals
+from oracle_lib import oracle_engine, oracle_library
+from test_lrk import charged_system, random_params, start_fields, synthetic_records
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_net():
+    return LrkNet(oracle_library(), "fto_")
+
+
+def test_forward_matches_oracle():
+    P = random_params(21, scale=0.3).astype(np.float32)
+    F = np.random.default_rng(22).normal(0, 1, (300, LRK_FEATURES)).astype(np.float32)
+    r_gpu = hip_lrk().forward(P, F)
+    r_ref = oracle_net().forward64(P.astype(np.float64), F.astype(np.float64))
+    assert np.abs(r_gpu - r_ref).max() < 3e-5
+
+
+def test_gradient_matches_oracle():
+    P = random_params(23, scale=0.3).astype(np.float32)
+    rec = synthetic_records(6, 24)
+    Lg, Gg = hip_lrk().loss_grad(P, rec)
+    Lo, Go = oracle_net().loss_grad64(P.astype(np.float64), rec)
+    assert abs(Lg - Lo) <= 1e-5 * Lo
+    assert np.linalg.norm(Gg - Go) <= 1e-4 * np.linalg.norm(Go)
+    for key, size in (("W1", 512), ("W2", 4096), ("W3", 4096), ("W4", 128), ("b1", 64), ("b2", 64), ("b3", 64), ("b4", 2)):
+        o = LRK_OFF[key]
+        blk_o, blk_g = Go[o:o + size], Gg[o:o + size]
+        assert np.linalg.norm(blk_g - blk_o) <= 2e-4 * max(np.linalg.norm(blk_o), 1e-6 * np.linalg.norm(Go)), key
+    assert np.all(Gg[:16] == 0)
+
+
+def test_trainer_matches_oracle():
+    rec = synthetic_records(8, 25)
+    P0 = init_params(rec, seed=3)
+    Pg, hg = hip_lrk().train(P0, rec, epochs=3, batch_records=3, lr0=1e-3, lr1=5e-4, seed=7)
+    Po, ho = oracle_net().train(P0, rec, epochs=3, batch_records=3, lr0=1e-3, lr1=5e-4, seed=7)
+    np.testing.assert_allclose(hg, ho, rtol=1e-4)
+    assert np.linalg.norm(Pg - Po) <= 1e-4 * np.linalg.norm(Po - P0) + 1e-6
+    assert hg[-1] < hg[0]
+
+
+def learned_pair(P, threshold, steps, scheme=SCHEME_SI_LEARNED, m=12):
+    fields = start_fields(m)
+    out = []
+    for make in (hip_engine, oracle_engine):
+        e = make(charged_system(scheme, m=m))
+        e.set_fields(*fields)
+        e.lrk_set_network(P, threshold)
+        e.lrk_set_recording(32)
+        recs = e.step(steps, record_every=1)
+        out.append((e, recs))
+    return out
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def small_network(seed):
+    P = random_params(seed, scale=0.15)
+    P[LRK_OFF["shift"]:LRK_OFF["shift"] + 8] = 0.0
+    P[LRK_OFF["scale"]:LRK_OFF["scale"] + 8] = 1.0
+    return P.astype(np.float32)
+
+
+def test_learned_trajectory_matches_oracle():
+    (g, rg), (o, ro) = learned_pair(small_network(31), 1e300, 5)
+    for a, b in zip(rg, ro):
+        assert abs(a["H"] - b["H"]) <= 1e-8 * abs(b["H"])
+    for fid in (FIELD_OMEGA, FIELD_PSI):
+        assert rel(g.get_field(fid), o.get_field(fid)) < 1e-8
+    sg, so = g.lrk_get_stats(), o.lrk_get_stats()
+    assert (sg["learned_steps"], sg["fallback_steps"], sg["judged"]) == (so["learned_steps"], so["fallback_steps"], so["judged"])
+    assert sg["learned_steps"] == 4
+    np.testing.assert_allclose(sg["sum_rel_residual_learned"], so["sum_rel_residual_learned"], rtol=1e-6)
+    np.testing.assert_allclose(sg["sum_rel_residual_debye"], so["sum_rel_residual_debye"], rtol=1e-8)
+    recg, reco = g.lrk_get_records(32), o.lrk_get_records(32)
+    assert recg.shape == reco.shape == (4, reco.shape[1])
+    np.testing.assert_allclose(recg, reco, rtol=1e-7, atol=1e-9 * np.abs(reco).max())
+
+
+def test_debye_records_match_oracle():
+    (g, _), (o, _) = learned_pair(np.zeros(LRK_NPARAMS, dtype=np.float32), 1e-3, 4, scheme=SCHEME_SEMI_IMPLICIT)
+    recg, reco = g.lrk_get_records(32), o.lrk_get_records(32)
+    assert recg.shape == reco.shape == (3, reco.shape[1])
+    np.testing.assert_allclose(recg, reco, rtol=1e-9, atol=1e-11 * np.abs(reco).max())
+    for fid in (FIELD_OMEGA, FIELD_PSI):
+        assert rel(g.get_field(fid), o.get_field(fid)) < 1e-10
+
+
+def test_fallback_decisions_match_oracle():
+    """A network that overestimates A everywhere (r = 4 tanh(1.5/4)): the judge
+    finds it worse than A_D and both sides fall back on the same steps."""
+    P = np.zeros(LRK_NPARAMS, dtype=np.float32)
+    P[LRK_OFF["b4"]:LRK_OFF["b4"] + 2] = 1.5
+    (g, rg), (o, ro) = learned_pair(P, 1e-3, 5)
+    sg, so = g.lrk_get_stats(), o.lrk_get_stats()
+    assert (sg["learned_steps"], sg["fallback_steps"]) == (so["learned_steps"], so["fallback_steps"])
+    assert sg["fallback_steps"] >= 2
+    assert abs(sg["last_excess"] - so["last_excess"]) <= 1e-6 * abs(so["last_excess"])
+    for fid in (FIELD_OMEGA, FIELD_PSI):
+        assert rel(g.get_field(fid), o.get_field(fid)) < 1e-8
+
+
+def test_batch_learned_matches_single_engines():
+    P = small_network(41)
+    systems = [charged_system(SCHEME_SI_LEARNED, m=12, seed=50 + r) for r in range(3)]
+    b = hip_batch(systems)
+    singles = [hip_engine(charged_system(SCHEME_SI_LEARNED, m=12, seed=50 + r)) for r in range(3)]
+    for r in range(3):
+        f = start_fields(12, seed=60 + r)
+        for e in (b.replicas[r], singles[r]):
+            e.set_fields(*f)
+            e.lrk_set_network(P, 1e300)
+    b.step(4, record_every=1)
+    for r in range(3):
+        singles[r].step(4, record_every=1)
+        assert rel(b.replicas[r].get_field(FIELD_OMEGA), singles[r].get_field(FIELD_OMEGA)) < 1e-12
+        assert b.replicas[r].lrk_get_stats()["learned_steps"] == singles[r].lrk_get_stats()["learned_steps"] == 3
+    b.close()
+
+
+def test_trained_network_beats_debye_on_held_out_steps():
+    """A/B on a strongly fluctuating small system (24^3, dV = 0.125 b^3, N = 20,
+    rho = 0.2, salt 0.05, lB = 1): records of plain (Debye) semi-implicit runs; a
+    network trained on 3/4 of them predicts the force change of the held-out 1/4
+    better than A_D, and the learned scheme run with it is judged better than A_D
+    on its own steps."""
+    from ftspolysol_amd.system import initial_fields, make_system
+    mk = lambda scheme, seed: make_system(24, 20, rho=0.2, salt=0.05, B=0.5, lB=1.0, z=-0.5, dt=0.01, scheme=scheme,
+                                          seed=seed)
+    systems = [mk(SCHEME_SEMI_IMPLICIT, 90 + r) for r in range(4)]
+    b = hip_batch(systems)
+    for r, (e, s) in enumerate(zip(b.replicas, systems)):
+        e.set_fields(*initial_fields(s, sigma=1e-3, seed=100 + r))
+    b.step(60)
+    for e in b.replicas:
+        e.lrk_set_recording(200)
+    b.step(150)
+    rec = np.concatenate([e.lrk_get_records(200) for e in b.replicas])
+    b.close()
+    assert rec.shape[0] == 600
+    order = np.random.default_rng(3).permutation(600)
+    train, val = rec[order[:450]], rec[order[450:]]
+    net = hip_lrk()
+    P0 = init_params(train, seed=5)
+    P, hist = net.train(P0, train, epochs=30, batch_records=32, lr0=3e-3, lr1=3e-4, seed=9)
+    assert hist[-1] < hist[0]
+    rl = np.nanmean(relative_residuals(val, network_r(net, P, val)), axis=0)
+    rd = np.nanmean(relative_residuals(val, None), axis=0)
+    print("held-out relative force residual: learned", rl, "Debye", rd)
+    assert np.all(rl < rd), (rl, rd)
+    sys_l = mk(SCHEME_SI_LEARNED, 120)
+    s = hip_engine(sys_l)
+    s.set_fields(*initial_fields(sys_l, sigma=1e-3, seed=121))
+    s.lrk_set_network(P, 1e-3)
+    s.step(100)
+    st = s.lrk_get_stats()
+    print("learned scheme:", st)
+    assert st["learned_steps"] > 50
+    assert np.all(st["sum_rel_residual_learned"] < st["sum_rel_residual_debye"])
