@@ -116,6 +116,11 @@ class LrkStats(C.Structure):
                 "sum_rel_residual_debye": np.array(self.sum_rel_residual_debye[:])}
 
 
+class RunSettings(C.Structure):
+    _fields_ = [("Nt", C.c_int), ("Nss", C.c_int), ("Ns", C.c_int), ("Ntmp", C.c_int), ("Nso", C.c_int),
+                ("N_stat", C.c_int), ("max_nan_restarts", C.c_int), ("legacy_ab", C.c_int), ("n_ion_names", C.c_int)]
+
+
 MAX_BATCH_GROUPS = 16
 
 
@@ -158,6 +163,14 @@ def _bind(lib, prefix):
         "batch_step": (C.c_int, [vp, C.c_int, C.c_int, P(Observables), C.c_int, P(C.c_int)]),
         "batch_set_timing": (C.c_int, [vp, C.c_int]),
         "batch_get_timing": (C.c_int, [vp, C.c_int, P(Timing)]),
+        "deck_read": (C.c_int, [C.c_char_p, C.c_char_p, P(vp)]),
+        "deck_destroy": (None, [vp]),
+        "deck_system": (C.c_int, [vp, P(SystemDesc)]),
+        "deck_settings": (C.c_int, [vp, P(RunSettings)]),
+        "deck_string": (C.c_char_p, [vp, C.c_char_p]),
+        "rf_read": (C.c_int, [C.c_char_p, P(C.c_int), P(C.c_int), P(C.c_double), C.c_size_t]),
+        "rf_write": (C.c_int, [C.c_char_p, P(C.c_int), P(C.c_double), C.c_int, P(C.c_double)]),
+        "field_resize": (C.c_int, [P(C.c_double), P(C.c_int), C.c_int, P(C.c_double), P(C.c_int)]),
         "lrk_set_network": (C.c_int, [vp, P(C.c_float), C.c_double]),
         "lrk_set_recording": (C.c_int, [vp, C.c_int]),
         "lrk_get_records": (C.c_int, [vp, P(C.c_double), C.c_int, P(C.c_int)]),
@@ -378,6 +391,92 @@ class LrkNet:
         finally:
             self._fns["lrk_trainer_destroy"](h)
         return p, hist[:epochs]
+
+
+class Deck:
+    """A parameter file plus its sequence file (fts_deck_*): `system` is an object
+    with the fts_system description (`desc`), usable wherever a System is."""
+
+    def __init__(self, lib, prefix, param_file, sequence_file=None):
+        self._fns = _bind(lib, prefix)
+        h = C.c_void_p()
+        rc = self._fns["deck_read"](os.fsencode(param_file), None if sequence_file is None else os.fsencode(sequence_file),
+                                    C.byref(h))
+        if rc != FTS_OK:
+            raise FtsError(rc, self._fns["last_error"]().decode())
+        self._h = h
+        desc = SystemDesc()
+        self._fns["deck_system"](self._h, C.byref(desc))
+        self.settings = RunSettings()
+        self._fns["deck_settings"](self._h, C.byref(self.settings))
+        deck = self
+
+        class _System:
+            pass
+
+        self.system = _System()
+        self.system.desc = desc
+        self.system.m = (desc.m1, desc.m2, desc.m3)
+        self.system.dL = desc.dL
+        self.system._deck = deck
+
+    def string(self, key):
+        v = self._fns["deck_string"](self._h, key.encode())
+        return None if v is None else v.decode()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._fns["deck_destroy"](self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rf_read(lib, path, mesh=(0, 0, 0), prefix="fts_"):
+    """Fields of an .rf table: complex array [n_fields, m1, m2, m3]."""
+    fns = _bind(lib, prefix)
+    m = (C.c_int * 3)(*mesh)
+    nf = C.c_int(0)
+    rc = fns["rf_read"](os.fsencode(path), m, C.byref(nf), None, 0)
+    if rc != FTS_OK:
+        raise FtsError(rc, fns["last_error"]().decode())
+    shape = (nf.value, m[0], m[1], m[2])
+    out = np.zeros(shape, dtype=np.complex128)
+    v = out.reshape(-1).view(np.float64)
+    rc = fns["rf_read"](os.fsencode(path), m, C.byref(nf), _dptr(v), v.size)
+    if rc != FTS_OK:
+        raise FtsError(rc, fns["last_error"]().decode())
+    return out
+
+
+def rf_write(lib, path, fields, L, prefix="fts_"):
+    fns = _bind(lib, prefix)
+    f = np.ascontiguousarray(np.asarray(fields, dtype=np.complex128))
+    if f.ndim == 3:
+        f = f[None]
+    m = (C.c_int * 3)(*f.shape[1:])
+    Ls = (C.c_double * 3)(*L)
+    rc = fns["rf_write"](os.fsencode(path), m, Ls, f.shape[0], _dptr(f.reshape(-1).view(np.float64)))
+    if rc != FTS_OK:
+        raise FtsError(rc, fns["last_error"]().decode())
+
+
+def field_resize(lib, fields, mesh, prefix="fts_"):
+    fns = _bind(lib, prefix)
+    f = np.ascontiguousarray(np.asarray(fields, dtype=np.complex128))
+    if f.ndim == 3:
+        f = f[None]
+    out = np.zeros((f.shape[0],) + tuple(mesh), dtype=np.complex128)
+    n = (C.c_int * 3)(*f.shape[1:])
+    m = (C.c_int * 3)(*mesh)
+    rc = fns["field_resize"](_dptr(f.reshape(-1).view(np.float64)), n, f.shape[0], _dptr(out.reshape(-1).view(np.float64)), m)
+    if rc != FTS_OK:
+        raise FtsError(rc, fns["last_error"]().decode())
+    return out
 
 
 def _timing_dict(t):

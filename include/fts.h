@@ -299,6 +299,45 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
 int fts_batch_set_timing(fts_batch* b, int enabled);
 int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out);
 
+/* ---- Front end: input decks and field files -------------------------------
+ * The reference reads an INI-like parameter file (param.cu:165-436) and a block
+ * sequence file (polymer.cu:219-273) into its param object, and reads and writes
+ * fields as .rf text tables (common.h:229-471, fieldIO.cu:213-726).  A deck holds
+ * both files, parsed with the reference's keys, aliases, defaults and checks;
+ * fts_deck_system fills the engine's system description from it.  Keys this
+ * engine adds, all in [dynamics]: scheme = pc | em | si | si_learned (default pc,
+ * the reference's), precision = fp64 | fp32 (default fp64) and seed (Philox key,
+ * default 0x5EED; the reference draws wall-clock seeds, common.h:922-927). */
+typedef struct fts_deck fts_deck;
+
+typedef struct fts_run_settings {
+    int Nt, Nss, Ns, Ntmp, Nso, N_stat, max_nan_restarts; /* [dynamics] / [output] (param.h:62-70) */
+    int legacy_ab;   /* no [monomers] and no chi: omega files hold the legacy A/B columns (param.cu:145-148) */
+    int n_ion_names; /* ion species after merging; names via fts_deck_string(d, "ion.<j>") */
+} fts_run_settings;
+
+/* param::read + polymerSpecies::setup + param::validate (main.cu:15-24);
+ * sequence_file overrides [polymer] sequence_file when not NULL (--seq). */
+int fts_deck_read(const char* param_file, const char* sequence_file, fts_deck** out);
+void fts_deck_destroy(fts_deck* d);
+/* The system description of the deck; its pointers stay valid while the deck lives. */
+int fts_deck_system(fts_deck* d, fts_system* out);
+int fts_deck_settings(fts_deck* d, fts_run_settings* out);
+/* [output] file names ("omega_file", "psi_file", "observable_file", ...), "sequence_file",
+ * and the labels "species.<p>", "ion.<j>", "monomer.<a>"; NULL if unknown. */
+const char* fts_deck_string(fts_deck* d, const char* key);
+
+/* Read an .rf table: m holds the mesh to assume when the file has no header and
+ * receives the file's mesh; *n_fields receives the complex columns per row.  With
+ * data == NULL only m and n_fields are filled; otherwise data receives the fields
+ * field-major [n_fields][M] interleaved complex (capacity in doubles). */
+int fts_rf_read(const char* path, int* m, int* n_fields, double* data, size_t capacity_doubles);
+/* Periodic trilinear resize of `count` complex fields from mesh n to mesh m
+ * (trilinear::resize, trilinear.cu:11-64); in [count][n1 n2 n3], out [count][m1 m2 m3]. */
+int fts_field_resize(const double* in, const int* n, int count, double* out, const int* m);
+/* Write an .rf table (header "m1 m2 m3" and "L1 L2 L3", rows of "%.10f  %.10f" per field). */
+int fts_rf_write(const char* path, const int* m, const double* L, int n_fields, const double* fields);
+
 /* ---- Learned linear-response kernel (LRK) ---------------------------------
  * No counterpart in the reference, which has no semi-implicit scheme at all.
  *

@@ -1,0 +1,270 @@
+"""CPU tests of the front end (include/fts.h fts_deck_*, fts_rf_*, fts_field_resize).
+
+The reference ships no input decks, so the decks here are written for the tests
+in the reference's grammar (param.cu:165-334, polymer.cu:219-273): sections,
+aliases, multi-line bracket lists, quoted '#', bare-sign charges, compact monomer
+words, auto-neutralised counterions and merged salt.  Each deck is checked
+against the system the same physics builds through make_system, by creating the
+oracle engine from both and comparing what the engine derives (mesh, modes, ion
+species and counts), and by one oracle step on both at identical fields and noise.
+"""
+import numpy as np
+import pytest
+
+from ftspolysol_amd.api import (Deck, FtsError, FTS_INVALID, SCHEME_PC_REFERENCE, SCHEME_SI_LEARNED, field_resize,
+                                load_hip_library, rf_read, rf_write)
+from ftspolysol_amd.system import initial_fields, make_system
+from oracle_lib import oracle_engine
+
+C1_DECK = """# a config-1 style deck on an 8^3 grid
+[grid]
+m1 = 8
+m2 = 8   # comment after a value
+m3 = 8
+a = 1.0
+dL = 0.5
+
+[monomers]
+M = 1
+names = [ "A" ]
+
+[polymer]
+species_count = 1
+sequence_file = "{seq}"
+
+[polymer.0]
+name = "pe"
+num_chains = {n_chains}
+N = 6
+
+[interactions]
+chi = [
+   0.0
+]
+v = {xi}
+lB = 1.0
+
+[dynamics]
+dt = 0.01
+lambda_w = [1.0]
+lambdaPsi = 1.0
+Nt = 20
+Nss = 2
+Ntmp = 10
+noise = yes
+seed = 0x1234
+
+[ions]
+enabled = true
+
+[ions.counterions]
+enabled = true
+number = auto_neutralize
+valence = 1.0
+
+[ions.salt]
+enabled = true
+num_salt = {num_salt}
+cation_valence = 1
+anion_valence = -1
+
+[observables]
+stress = true
+component_xx = true
+component_xy = true
+isotropic_pressure = on
+
+[structure_factor]
+enabled = false
+kmax = 3.0
+
+[output]
+observable_file = "obs # not a comment.dat"
+"""
+
+SEQ = """[polymer.0]
+monomers = AAAAAA
+charges = -0.5 -0.5, [-0.5 -0.5]
+charges = -0.5 -0.5
+"""
+
+
+def write_deck(tmp_path, text, seq=SEQ):
+    sp = tmp_path / "seq.txt"
+    sp.write_text(seq)
+    pp = tmp_path / "deck.param"
+    pp.write_text(text.replace("{seq}", str(sp)))
+    return str(pp)
+
+
+def c1_like(tmp_path):
+    ref = make_system(8, 6, rho=0.2, salt=0.05, B=0.5, lB=1.0, z=-0.5, dt=0.01, seed=0x1234, stress_mask=0x49)
+    n_chains = ref.desc.species[0].num_chains
+    text = C1_DECK.format(seq="{seq}", n_chains=n_chains, xi=ref.desc.xi, num_salt=ref.desc.salt.num_salt)
+    return ref, write_deck(tmp_path, text)
+
+
+def test_deck_builds_the_same_system(tmp_path):
+    ref, path = c1_like(tmp_path)
+    deck = Deck(load_hip_library(), "fts_", path)
+    d = deck.system.desc
+    assert (d.m1, d.m2, d.m3, d.dL, d.a, d.monomer_types, d.lB) == (8, 8, 8, 0.5, 1.0, 1, 1.0)
+    assert d.xi == ref.desc.xi and d.seed == 0x1234 and d.scheme == SCHEME_PC_REFERENCE
+    assert d.stress_mask == 0x01 | 0x08 | 0x40
+    sp = d.species[0]
+    assert sp.N == 6 and sp.num_chains == ref.desc.species[0].num_chains
+    assert [sp.charges[i] for i in range(6)] == [-0.5] * 6
+    s = deck.settings
+    assert (s.Nt, s.Nss, s.Ns, s.Ntmp, s.Nso, s.N_stat, s.max_nan_restarts, s.legacy_ab) == (20, 2, 20, 10, -1, -1, 20, 0)
+    assert deck.string("observable_file") == "obs # not a comment.dat"
+    assert deck.string("omega_file") == "omega.rf" and deck.string("species.0") == "pe"
+    # counterions and salt cations have z = +1 and the same smearing: merged (ion.cu:117-127)
+    assert s.n_ion_names == 2 and deck.string("ion.0") == "counterion" and deck.string("ion.1") == "salt_anion"
+    a, b = oracle_engine(deck.system), oracle_engine(ref)
+    ia, ib = a.info, b.info
+    assert (ia.n_omega, ia.n_ions) == (ib.n_omega, ib.n_ions)
+    assert [ia.ion_count[j] for j in range(ia.n_ions)] == [ib.ion_count[j] for j in range(ib.n_ions)]
+    fields = initial_fields(ref, sigma=0.05, seed=3)
+    for e in (a, b):
+        e.set_fields(*fields)
+    ra, rb = a.step(2, record_every=1), b.step(2, record_every=1)
+    for x, y in zip(ra, rb):
+        assert x["H"] == y["H"]
+        assert np.array_equal(x["stress"][[0, 3]], y["stress"][[0, 3]]) and x["p_iso"] == y["p_iso"]
+
+
+def test_sequence_grammar(tmp_path):
+    text = """[grid]
+m1 = 8
+m2 = 8
+m3 = 8
+a = 1
+dL = 0.5
+[monomers]
+names = [A, B]
+[polymer]
+species_count = 2
+sequence_file = {seq}
+[polymer.0]
+name = "di block"
+num_chains = 3
+N = 5
+[polymer.1]
+name = homo
+num_chains = 2
+N = 3
+[interactions]
+chi = [[0, 1.5], [1.5, 0]]
+xi = 2
+[dynamics]
+dt = 0.01
+scheme = si_learned
+precision = fp32
+"""
+    seq = """# two species
+[polymer.0]
+monomers = A A B
+monomers = B B
+charges = + - 0 -1.5 +2
+bonds = 1.0 1.2 1.0 0.8
+[polymer.1]
+monomers = "AAB"
+charges = 0 0 0
+"""
+    deck = Deck(load_hip_library(), "fts_", write_deck(tmp_path, text, seq))
+    d = deck.system.desc
+    assert d.monomer_types == 2 and d.n_species == 2 and d.scheme == SCHEME_SI_LEARNED and d.precision == 1
+    assert [d.chi[i] for i in range(4)] == [0, 1.5, 1.5, 0]
+    s0, s1 = d.species[0], d.species[1]
+    assert [s0.types[i] for i in range(5)] == [0, 0, 1, 1, 1]
+    assert [s0.charges[i] for i in range(5)] == [1.0, -1.0, 0.0, -1.5, 2.0]
+    assert [s0.bonds[i] for i in range(4)] == [1.0, 1.2, 1.0, 0.8]
+    assert [s1.types[i] for i in range(3)] == [0, 0, 1] and [s1.bonds[i] for i in range(2)] == [1.0, 1.0]
+    assert deck.string("species.0") == "di block" and deck.string("monomer.1") == "B"
+    assert deck.settings.Ns == 0
+
+
+def test_legacy_ab_deck_defaults(tmp_path):
+    text = """[grid]
+m1 = 8
+m2 = 8
+m3 = 8
+a = 1
+dL = 0.5
+[polymer]
+species_count = 1
+sequence_file = {seq}
+[polymer.0]
+name = p
+num_chains = 4
+N = 2
+[interactions]
+xi = 1.0
+"""
+    deck = Deck(load_hip_library(), "fts_", write_deck(tmp_path, text, "[polymer.0]\nmonomers = A B\ncharges = 0 0\n"))
+    assert deck.settings.legacy_ab == 1 and deck.system.desc.monomer_types == 2
+    # K = xi [[1, 1], [1, 1]]: one active mode (interactionMode.cu:9-298), gamma = i
+    info = oracle_engine(deck.system).info
+    assert info.n_omega == 1 and info.gamma_is_imag[0] == 1.0
+
+
+@pytest.mark.parametrize("bad, msg", [
+    ("[grid]\nm1 = 8\nbogus = 1\n", "unknown parameter [grid] bogus"),
+    ("[grid]\nm1 = eight\n", "cannot parse [grid] m1"),
+    ("[interactions]\nchi = [1, 2\n", "unbalanced brackets"),
+    ("[grid]\nm1 = 8\nm2 = 8\nm3 = 8\na = 1\ndL = 0.5\n[polymer]\nsequence_file = x\n", "species_count is required"),
+])
+def test_deck_errors(tmp_path, bad, msg):
+    p = tmp_path / "bad.param"
+    p.write_text(bad)
+    with pytest.raises(FtsError) as ei:
+        Deck(load_hip_library(), "fts_", str(p))
+    assert ei.value.code == FTS_INVALID and msg in str(ei.value)
+
+
+def test_sequence_errors(tmp_path):
+    ref, path = c1_like(tmp_path)
+    (tmp_path / "seq.txt").write_text("[polymer.0]\nmonomers = AAAAA\ncharges = -0.5\n")
+    with pytest.raises(FtsError) as ei:
+        Deck(load_hip_library(), "fts_", path)
+    assert "sequence length mismatch" in str(ei.value)
+    (tmp_path / "seq2.txt").write_text("[polymer.0]\nmonomers = AAAAAC\ncharges = 0 0 0 0 0 0\n")
+    with pytest.raises(FtsError) as ei:
+        Deck(load_hip_library(), "fts_", path, str(tmp_path / "seq2.txt"))
+    assert "unknown monomer label" in str(ei.value)
+
+
+def test_rf_round_trip_and_headerless(tmp_path):
+    lib = load_hip_library()
+    rng = np.random.default_rng(1)
+    f = rng.normal(size=(2, 4, 6, 5)) + 1j * rng.normal(size=(2, 4, 6, 5))
+    path = str(tmp_path / "w.rf")
+    rf_write(lib, path, f, (2.0, 3.0, 2.5))
+    lines = open(path).read().splitlines()
+    assert lines[0] == "4 6 5" and lines[1] == "2 3 2.5" and len(lines) == 2 + 120
+    assert lines[2] == "%.10f  %.10f  %.10f  %.10f" % (f[0, 0, 0, 0].real, f[0, 0, 0, 0].imag, f[1, 0, 0, 0].real, f[1, 0, 0, 0].imag)
+    g = rf_read(lib, path)
+    assert g.shape == f.shape and np.abs((g - f).view(np.float64)).max() <= 5e-11  # %.10f per component
+    # no header: the caller's mesh is assumed; comments and blank lines are skipped
+    p2 = tmp_path / "p.rf"
+    p2.write_text("# psi\n" + "\n".join("%.10f %.10f  # row" % (x.real, x.imag) for x in f[0].reshape(-1)) + "\n\n")
+    h = rf_read(lib, str(p2), mesh=(4, 6, 5))
+    assert np.abs((h[0] - f[0]).view(np.float64)).max() <= 5e-11
+    with pytest.raises(FtsError):
+        rf_read(lib, str(p2), mesh=(4, 6, 6))
+
+
+def test_trilinear_resize():
+    lib = load_hip_library()
+    rng = np.random.default_rng(2)
+    f = rng.normal(size=(1, 4, 6, 8)) + 1j * rng.normal(size=(1, 4, 6, 8))
+    up = field_resize(lib, f, (8, 12, 16))
+    # target points at even indices are the source points (x n / m = x / 2)
+    assert np.array_equal(up[0, ::2, ::2, ::2], f[0])
+    back = field_resize(lib, up, (4, 6, 8))
+    assert np.array_equal(back, f)
+    # midpoints along one axis: averages of the neighbours, periodic
+    assert np.allclose(up[0, 1::2, ::2, ::2], 0.5 * (f[0] + np.roll(f[0], -1, axis=0)))
+    # a linear field along z stays linear between nodes; constants stay constant
+    c = field_resize(lib, np.full((1, 3, 3, 3), 2 - 1j), (6, 9, 12))
+    assert np.allclose(c, 2 - 1j)
