@@ -43,10 +43,13 @@ def test_hip_library_exports_every_declared_symbol():
 def test_oracle_exports_the_same_interface():
     lib = oracle_library()
     # timing and replica batches are device-scheduling concepts: the oracle checks a
-    # batch replica by replica through the single-system interface
-    engine_only = {"fts_set_timing", "fts_get_timing"}
+    # batch replica by replica through the single-system interface.  The front end
+    # (decks, .rf files, resize) is host I/O that every caller shares; its own tests
+    # pin it, and there is no device arithmetic for the oracle to check.
+    engine_only = {"fts_set_timing", "fts_get_timing", "fts_field_resize"}
     missing = [n for n in declared_functions()
-               if n not in engine_only and not n.startswith("fts_batch_") and not hasattr(lib, "fto_" + n[4:])]
+               if n not in engine_only and not n.startswith(("fts_batch_", "fts_deck_", "fts_rf_"))
+               and not hasattr(lib, "fto_" + n[4:])]
     assert not missing, missing
 
 
