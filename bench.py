@@ -5,29 +5,32 @@ b = 1, charge -0.5 on every bead, counterions, 1:1 salt 0.05 b^-3, bead density
 0.2 b^-3 (221 chains), excluded volume B = 0.5 b^3, lB = 1 b, a = 1 b,
 lambda = 1.  Synthetic initial fields (w, psi = 1e-3 complex Gaussian).
 
-Two engines run the workload, both with the reference's predictor-corrector
-step (complexLangevin.cu:669-698) at dt = 0.01 by default:
-  fast   a batch of R independent replicas (own seeds and initial fields) with
-         complex-FP32 propagators and FFTs, per-bead renormalisation, FP64
-         reductions and field accumulation; every bead-step launch carries all
-         replicas.
-  plain  one complex-FP64 system per GPU, as the reference runs it.
-Replica 0 of the fast batch has the plain engine's seed and initial fields, so
-the two see the same Philox noise stream.  Each takes `warmup` untimed steps,
-then `steps` timed steps recording Re H every step.  Every replica keeps a
-device checkpoint every 50 steps; a replica whose step turns non-finite is
-restored from it with a new noise key (the reference's non-finite restart) and
-the repeated steps count in the elapsed time; the restarts are reported.
+Three engines run the workload:
+  fast    FP32 + learned: a batch of R independent replicas (own seeds and
+          initial fields) with complex-FP32 propagators and FFTs, per-bead
+          renormalisation, FP64 reductions and field accumulation, stepping the
+          semi-implicit scheme with the learned linear-response kernel
+          (FTS_SCHEME_SI_LEARNED; network trained by ftspolysol_amd.train_lrk on
+          48^3 data, ftspolysol_amd/lrk_c4_48cubed.npy), judged every step by the
+          exact force, falling back to the Debye kernel when its excess force
+          residual passes 1e-3.  Every bead-step launch carries all replicas.
+  fp32_pc the same batch without the learned part: FP32, the reference's
+          predictor-corrector step (complexLangevin.cu:669-698).
+  plain   one complex-FP64 system, predictor-corrector, as the reference runs it.
+Replica 0 of each batch has the plain engine's seed and initial fields.  Each
+takes `warmup` untimed steps, then `steps` timed steps recording Re H every step.
+Every replica keeps a device checkpoint every 50 steps; a replica whose step
+turns non-finite is restored from it with a new noise key and the repeated steps
+count in the elapsed time; the restarts are reported.
 
-<H> error, two ways: `H_err_same_state` (the precision error) measures H, mu
-and the stress tensor with the FP32 engine and an FP64 engine on the same
-fields (the fast replica 0's final state); `H_err_stream` compares the time
-averages of fast replica 0 and the plain engine on the same noise stream (a
-statistical difference once the trajectories decorrelate; `H_err_sigma` is its
-standard error).  Decorrelated samples/s =
-(replica CL steps/s) / (2 tau_int), tau_int by Sokal's automatic window (c = 6)
-on the replica-averaged normalised autocorrelation of Re H.  `value` is the
-fast batch's whole-GPU rate.
+<H> error: `H_err_same_state` (the precision error) measures H, mu and the
+stress tensor with an FP32 engine and an FP64 engine on the same fields (fast
+replica 0's final state); `H_err_vs_plain` compares the time averages of the fast
+batch and the plain engine (statistical once trajectories decorrelate, plus the
+time-step bias of the fast scheme against the predictor-corrector; `H_err_sigma`
+is the standard error).  Decorrelated samples/s = (replica CL steps/s) /
+(2 tau_int), tau_int by Sokal's automatic window (c = 6) on the replica-averaged
+normalised autocorrelation of Re H.  `value` is the fast batch's whole-GPU rate.
 
 roofline: the dominant kernels are the propagator bead-step passes (one pencil
 pass and one plane pass per bead index, or the four line passes, every replica's
@@ -51,10 +54,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, Engine, PREC_FP32, PREC_FP64, SCHEME_EM,  # noqa: E402
-                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_batch, hip_engine)
+                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, hip_batch, hip_engine)
+from ftspolysol_amd.lrk import load_params  # noqa: E402
 from ftspolysol_amd.system import initial_fields, make_system  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+LRK_PARAMS = os.path.join(ROOT, "ftspolysol_amd", "lrk_c4_48cubed.npy")
 
 
 def autocorr(x):
@@ -86,12 +91,14 @@ def tau_int(x, c=6.0):
     return float(tau), n - 1
 
 
-def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50):
+def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50, lrk=None, lrk_threshold=1e-3):
     b = hip_batch(systems)
     for r, (e, s) in enumerate(zip(b.replicas, systems)):
         om, ps = initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r)
         e.set_fields(om, ps)
         e.set_restart(ckpt_every, max_restarts)
+        if lrk is not None:
+            e.lrk_set_network(lrk, lrk_threshold)
     R = len(systems)
     t0 = time.perf_counter()
     done = 0
@@ -113,6 +120,14 @@ def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50):
         print(f"[bench] {label}: {done}/{steps} timed steps", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     restarts = [e.get_info().restarts for e in b.replicas]
+    lrk_stats = None
+    if lrk is not None:
+        st = [e.lrk_get_stats() for e in b.replicas]
+        judged = max(1, sum(x["judged"] for x in st))
+        lrk_stats = {"learned_steps": int(sum(x["learned_steps"] for x in st)),
+                     "fallback_steps": int(sum(x["fallback_steps"] for x in st)),
+                     "rel_force_residual_learned": [float(sum(x["sum_rel_residual_learned"][c] for x in st) / judged) for c in range(2)],
+                     "rel_force_residual_debye": [float(sum(x["sum_rel_residual_debye"][c] for x in st) / judged) for c in range(2)]}
     final = (b.replicas[0].get_field(FIELD_OMEGA), b.replicas[0].get_field(FIELD_PSI)[0])
     timing = b.get_timing(0)
     sched = b.info.group_schedule[0]
@@ -125,7 +140,7 @@ def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50):
     return dict(replicas=R, replica_steps_per_s=sps, ms_per_step=1e3 * elapsed / steps, tau_int=tau, window=window,
                 samples_per_s=sps / (2 * tau), H_mean=float(H.real.mean()), H_imag_mean=float(H.imag.mean()),
                 H_stderr=stderr_H, H0=H[0].real, timing=timing, elapsed=elapsed, schedule=sched, tune_ms=tune,
-                restarts=int(sum(restarts)), final=final)
+                restarts=int(sum(restarts)), final=final, lrk=lrk_stats)
 
 
 def same_state_errors(wl, fields):
@@ -169,11 +184,13 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--replicas", type=int, default=8, help="replicas in the fast batch")
-    ap.add_argument("--plain-replicas", type=int, default=1)
     ap.add_argument("--dt-fast", type=float, default=0.01)
-    ap.add_argument("--fast-scheme", choices=["em", "si", "pc"], default="pc")
-    ap.add_argument("--plain-scheme", choices=["em", "si", "pc"], default="pc")
+    ap.add_argument("--fast-scheme", choices=["si_learned", "si", "em", "pc"], default="si_learned")
+    ap.add_argument("--lrk", default=LRK_PARAMS, help="parameters of the learned kernel (.npy)")
+    ap.add_argument("--lrk-threshold", type=float, default=1e-3)
     ap.add_argument("--dt-plain", type=float, default=0.01)
+    ap.add_argument("--plain-replicas", type=int, default=1)
+    ap.add_argument("--no-fp32-pc", action="store_true", help="skip the FP32 predictor-corrector comparison batch")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -183,15 +200,19 @@ def main():
         sys.exit(2)
 
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
-    schemes = {"em": SCHEME_EM, "si": SCHEME_SEMI_IMPLICIT, "pc": SCHEME_PC_REFERENCE}
-    fast_scheme, plain_scheme = schemes[args.fast_scheme], schemes[args.plain_scheme]
-    fast_sys = [make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=fast_scheme, seed=0xC4 + r)
+    schemes = {"em": SCHEME_EM, "si": SCHEME_SEMI_IMPLICIT, "pc": SCHEME_PC_REFERENCE, "si_learned": SCHEME_SI_LEARNED}
+    lrk = load_params(args.lrk) if args.fast_scheme == "si_learned" else None
+    fast_sys = [make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=schemes[args.fast_scheme], seed=0xC4 + r)
                 for r in range(args.replicas)]
-    plain_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=plain_scheme, seed=0xC4 + r)
+    pc_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP32, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r)
+              for r in range(args.replicas)]
+    plain_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r)
                  for r in range(args.plain_replicas)]
 
-    fast = run_batch(fast_sys, args.warmup, args.steps, f"fast fp32 {args.fast_scheme} x{args.replicas}")
-    plain = run_batch(plain_sys, args.warmup, args.steps, f"plain fp64 {args.plain_scheme} x{args.plain_replicas}")
+    fast = run_batch(fast_sys, args.warmup, args.steps, f"fast fp32 {args.fast_scheme} x{args.replicas}", lrk=lrk,
+                     lrk_threshold=args.lrk_threshold)
+    fp32_pc = None if args.no_fp32_pc else run_batch(pc_sys, args.warmup, args.steps, f"fp32 pc x{args.replicas}")
+    plain = run_batch(plain_sys, args.warmup, args.steps, f"plain fp64 pc x{args.plain_replicas}")
     same = same_state_errors(wl, fast["final"])
 
     def roof(t):
@@ -211,9 +232,7 @@ def main():
         if t.get("replicas") == args.replicas and t.get("schedule") == fast["schedule"] and t.get("precision") == "fp32":
             traffic = t["traffic_bytes"]
     _, _, plain_achieved = roof(plain["timing"])
-    # same Philox stream: fast replica 0 and plain replica 0
-    n = min(fast["H0"].size, plain["H0"].size)
-    H_err_stream = abs(fast["H0"][:n].mean() - plain["H0"][:n].mean()) / abs(plain["H0"][:n].mean())
+    H_err_vs_plain = abs(fast["H_mean"] - plain["H_mean"]) / abs(plain["H_mean"])
     H_err_sigma = np.hypot(fast["H_stderr"], plain["H_stderr"]) / abs(plain["H_mean"])
     keys = ("replicas", "replica_steps_per_s", "ms_per_step", "tau_int", "samples_per_s", "H_mean", "H_stderr",
             "schedule", "tune_ms", "restarts")
@@ -232,16 +251,21 @@ def main():
         "dtype": "c64 propagators/FFT, f64 reductions",
         "data": "synthetic",
         "config": {"workload": "96^3 L=48 N=100 z=-0.5 salt=0.05 B=0.5 lB=1 (221 chains)",
-                   "fast": f"FP32 {args.fast_scheme} dt={args.dt_fast}, {args.replicas} replicas per launch "
-                           f"(learned preconditioner not yet in this engine)",
-                   "plain": f"FP64 {args.plain_scheme} dt={args.dt_plain}, {args.plain_replicas} replica(s)",
+                   "fast": f"FP32 {args.fast_scheme} dt={args.dt_fast}, {args.replicas} replicas per launch"
+                           + (f", learned kernel {os.path.basename(args.lrk)}, fallback threshold {args.lrk_threshold}"
+                              if lrk is not None else ""),
+                   "fp32_pc": None if fp32_pc is None else f"FP32 pc dt={args.dt_plain}, {args.replicas} replicas per launch",
+                   "plain": f"FP64 pc dt={args.dt_plain}, {args.plain_replicas} replica(s)",
                    "parallelism": "single GPU"},
         "fast": {k: fast[k] for k in keys},
+        "fp32_pc": None if fp32_pc is None else {k: fp32_pc[k] for k in keys},
         "plain_fp64": {k: plain[k] for k in keys},
+        "learned_kernel": fast["lrk"],
         "speedup_vs_plain_fp64": fast["samples_per_s"] / plain["samples_per_s"],
+        "fp32_pc_speedup_vs_plain_fp64": None if fp32_pc is None else fp32_pc["samples_per_s"] / plain["samples_per_s"],
         "H_err_same_state": float(same["H"]),
         "same_state_rel_err": {k: float(v) for k, v in same.items() if k != "H_fp64"},
-        "H_err_stream": float(H_err_stream),
+        "H_err_vs_plain": float(H_err_vs_plain),
         "H_err_sigma": float(H_err_sigma),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -251,7 +275,7 @@ def main():
                      "plain_fp64_achieved": plain_achieved},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(plain_sys[0], args.cpu_steps, plain["tau_int"], args.plain_scheme.upper())
+        out["cpu_baseline"] = cpu_baseline(plain_sys[0], args.cpu_steps, plain["tau_int"], "PC")
     print(json.dumps(out))
 
 

@@ -945,7 +945,8 @@ template <typename T> int fts_engine::solve_end(int p, bool accumulate)
     const int nc = static_cast<int>(sp.cls_type.size());
     const int N = sp.N;
     if (N > 1) {
-        hipLaunchKernelGGL(k_scale_prefix, dim3(1), dim3(64), 0, stream, logc.p, logc.p + Nmax, N, sp.mirror ? 1 : 0, rfac.p, rfac2.p,
+        hipLaunchKernelGGL(k_scale_prefix, dim3(1), dim3(256), 2 * sizeof(double) * static_cast<size_t>(N), stream, logc.p, logc.p + Nmax, N,
+                           sp.mirror ? 1 : 0, rfac.p, rfac2.p,
                            scal.p + SC_LF + p);
         HIP_TRY(hipGetLastError());
     }

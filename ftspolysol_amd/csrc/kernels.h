@@ -261,22 +261,28 @@ __global__ void k_contract_density(const V2<T>* __restrict__ qF, const V2<T>* __
 __global__ void k_scale_prefix(const double* __restrict__ logcF, const double* __restrict__ logcB, int N, int mirror, double* __restrict__ rfac,
                                double* __restrict__ rfac2, double2* __restrict__ out)
 {
-    // mirrored species: the backward exponents are the forward ones reversed
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double LFtot = 0.0;
-    for (int s = 0; s < N; ++s) {
-        LFtot += logcF[s];
-        rfac[s] = LFtot; // LF(s), replaced below
+    // one block; dynamic LDS holds LF(s) = sum_{t<=s} logcF[t] and LB(s) = sum_{t>=s} logcB[t]
+    // (mirrored species: the backward exponents are the forward ones reversed)
+    extern __shared__ double sh_prefix[];
+    double* LF = sh_prefix;
+    double* LB = sh_prefix + N;
+    for (int s = threadIdx.x; s < N; s += blockDim.x) {
+        LF[s] = logcF[s];
+        LB[s] = mirror ? logcF[N - 1 - s] : logcB[s];
     }
-    double LB = 0.0; // sum_{t > s} logcB[t]
-    for (int s = N - 1; s >= 0; --s) {
-        const double LF = rfac[s];
-        if (s < N - 1) rfac2[s] = exp2(LFtot - LF - LB);
-        LB += mirror ? logcF[N - 1 - s] : logcB[s];
-        rfac[s] = exp2(LFtot - LF - LB);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int s = 1; s < N; ++s) LF[s] += LF[s - 1];
+        for (int s = N - 2; s >= 0; --s) LB[s] += LB[s + 1];
+        out->x = LF[N - 1] * 0.69314718055994530942;
+        out->y = 0.0;
     }
-    out->x = LFtot * 0.69314718055994530942;
-    out->y = 0.0;
+    __syncthreads();
+    const double LFtot = LF[N - 1];
+    for (int s = threadIdx.x; s < N; s += blockDim.x) {
+        rfac[s] = exp2(LFtot - LF[s] - LB[s]);
+        if (s < N - 1) rfac2[s] = exp2(LFtot - LF[s] - LB[s + 1]);
+    }
 }
 
 // rho = (n/V) b / Q   (normalizedDensityKernel, common.h:554-558)
