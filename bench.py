@@ -191,6 +191,8 @@ def main():
     ap.add_argument("--dt-plain", type=float, default=0.01)
     ap.add_argument("--plain-replicas", type=int, default=1)
     ap.add_argument("--no-fp32-pc", action="store_true", help="skip the FP32 predictor-corrector comparison batch")
+    ap.add_argument("--schedule", choices=["auto", "plane", "line"], default="auto",
+                    help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -202,10 +204,11 @@ def main():
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
     schemes = {"em": SCHEME_EM, "si": SCHEME_SEMI_IMPLICIT, "pc": SCHEME_PC_REFERENCE, "si_learned": SCHEME_SI_LEARNED}
     lrk = load_params(args.lrk) if args.fast_scheme == "si_learned" else None
-    fast_sys = [make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=schemes[args.fast_scheme], seed=0xC4 + r)
-                for r in range(args.replicas)]
-    pc_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP32, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r)
-              for r in range(args.replicas)]
+    sched = {"auto": 0, "plane": 1, "line": 2}[args.schedule]
+    fast_sys = [make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=schemes[args.fast_scheme], seed=0xC4 + r,
+                            fft_schedule=sched) for r in range(args.replicas)]
+    pc_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP32, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r,
+                          fft_schedule=sched) for r in range(args.replicas)]
     plain_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r)
                  for r in range(args.plain_replicas)]
 

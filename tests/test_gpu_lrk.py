@@ -166,14 +166,20 @@ def committed_network():
     return load_params(os.path.join(ROOT, "ftspolysol_amd", "lrk_c4_48cubed.npy"))
 
 
-def learned_run(n, P, seed, steps=100):
+def learned_run(n, P, seed, burn=100, steps=100):
+    """Judge statistics of `steps` learned-scheme steps after `burn` steps (the start-up
+    transient from near-zero fields lies outside the training data)."""
     from ftspolysol_amd.system import initial_fields
     sys_l = chem_system(n, SCHEME_SI_LEARNED, seed)
     s = hip_engine(sys_l)
     s.set_fields(*initial_fields(sys_l, sigma=1e-3, seed=seed + 1000))
     s.lrk_set_network(P, 1e-3)
+    s.step(burn)
+    a = s.lrk_get_stats()
     s.step(steps)
-    return s.lrk_get_stats()
+    b = s.lrk_get_stats()
+    return {k: b[k] - a[k] for k in ("judged", "learned_steps", "fallback_steps", "sum_rel_residual_learned",
+                                      "sum_rel_residual_debye")}
 
 
 def test_trained_network_beats_debye_on_held_out_steps():
