@@ -162,6 +162,7 @@ def _bind(lib, prefix):
         "batch_replica": (C.c_int, [vp, C.c_int, P(vp), P(C.c_int)]),
         "batch_step": (C.c_int, [vp, C.c_int, C.c_int, P(Observables), C.c_int, P(C.c_int)]),
         "batch_set_timing": (C.c_int, [vp, C.c_int]),
+        "batch_set_graph": (C.c_int, [vp, C.c_int]),
         "batch_get_timing": (C.c_int, [vp, C.c_int, P(Timing)]),
         "deck_read": (C.c_int, [C.c_char_p, C.c_char_p, P(vp)]),
         "deck_destroy": (None, [vp]),
@@ -537,6 +538,9 @@ class Batch:
 
     def set_timing(self, enabled):
         self._check(self._fns["batch_set_timing"](self._h, int(enabled)))
+
+    def set_graph(self, enabled):
+        self._check(self._fns["batch_set_graph"](self._h, int(enabled)))
 
     def get_timing(self, group=0):
         t = Timing()

@@ -7,10 +7,12 @@
 // carries the jobs (sweep directions) of all its replicas at that bead index,
 // read from a device-resident job table built once.  Replicas with shorter
 // chains drop out of the later launches.  Groups run concurrently, each from
-// its own host thread on its own stream.  Everything outside the sweeps (the
-// smoothing FFTs, ions, drifts and the CL update) runs per replica on the group
-// stream; the host waits once per density evaluation for the Q checks of all
-// replicas of a group and once per step for their field checks.
+// its own host thread on its own stream.  The bead-step launches of a sweep read
+// fixed device tables, so they are captured once per table set as a hipGraph and
+// replayed with one launch.  Everything outside the sweeps (the smoothing FFTs,
+// ions, drifts and the CL update) runs per replica on the group stream; the host
+// waits once per density evaluation for the Q checks of all replicas of a group
+// and once per step for their field checks.
 #include <memory>
 #include <thread>
 
@@ -37,8 +39,14 @@ struct SweepTables {
     void* dplane = nullptr;
     void* dpencil = nullptr;
     void* dline = nullptr;
+    // the step launches [init_launches, launches.size()) captured once as a graph:
+    // their arguments are the fixed device tables above, so every replay is the same work
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
     ~SweepTables()
     {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (void* q : {dplane, dpencil, dline})
             if (q) (void)hipFree(q);
     }
@@ -52,6 +60,7 @@ struct Group {
     std::vector<int> ids; // batch index of each replica
     int sched = FTS_SCHED_PLANE;
     bool plane_ok = true;
+    bool use_graph = true; // replay captured sweep graphs (fts_batch_set_graph)
     double tune_ms[2] = {0.0, 0.0};
     std::vector<std::unique_ptr<SweepTables>> tables; // full sweeps, per species index and schedule
     double2* hQ = nullptr;                             // pinned [reps][FTS_MAX_SPECIES]
@@ -298,7 +307,24 @@ template <typename T> int group_sweeps(Group& g, int p, const std::vector<char>&
         if (!e0) return fail(FTS_DEVICE_ERROR, "hipEventCreate");
         HIP_TRY(hipEventRecord(e0, g.stream));
     }
-    if ((rc = run_tables<T>(g, *tb, tb->init_launches, tb->launches.size()))) return rc;
+    if (g.use_graph) {
+        if (!tb->exec) {
+            HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            rc = run_tables<T>(g, *tb, tb->init_launches, tb->launches.size());
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(g.stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            HIP_TRY(ec);
+            tb->graph = graph;
+            HIP_TRY(hipGraphInstantiate(&tb->exec, tb->graph, nullptr, nullptr, 0));
+        }
+        HIP_TRY(hipGraphLaunch(tb->exec, g.stream));
+    } else if ((rc = run_tables<T>(g, *tb, tb->init_launches, tb->launches.size()))) {
+        return rc;
+    }
     if (g.timing) {
         hipEvent_t e1 = group_event(g);
         if (!e1) return fail(FTS_DEVICE_ERROR, "hipEventCreate");
@@ -600,6 +626,13 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
     }
     for (auto& g : b->groups)
         if (g->rc) return fail(g->rc, g->err);
+    return FTS_OK;
+}
+
+int fts_batch_set_graph(fts_batch* b, int enabled)
+{
+    if (!b) return fail(FTS_INVALID, "null batch");
+    for (auto& g : b->groups) g->use_graph = enabled != 0;
     return FTS_OK;
 }
 

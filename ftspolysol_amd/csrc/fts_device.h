@@ -48,26 +48,16 @@ __device__ __forceinline__ double2 cexp(double2 z)
 // ---------------------------------------------------------------------------
 // Compile-time FFT plans: radix sequence and elements per thread E (a common
 // multiple of the radices, so every stage gives a thread E/R whole butterflies).
-// Length 96 has two plans: radices 4, 4, 6 with E = 12 (three LDS stages, the
-// default) or 12, 8 with E = 24 (two stages, half the threads per line).
 // ---------------------------------------------------------------------------
-#ifndef FTS_PLAN96_NST
-#define FTS_PLAN96_NST 3
-#endif
-#if FTS_PLAN96_NST == 2
-#define FTS_PLAN96_E 24
-#else
-#define FTS_PLAN96_E 12
-#endif
 __host__ __device__ constexpr int plan_e(int n)
 {
     return n == 8 ? 8 : n == 12 ? 12 : n == 16 ? 4 : n == 24 ? 12 : n == 32 ? 8 : n == 48 ? 12 : n == 64 ? 8
-         : n == 96 ? FTS_PLAN96_E : n == 128 ? 8 : n == 192 ? 12 : n == 256 ? 8 : 0;
+         : n == 96 ? 12 : n == 128 ? 8 : n == 192 ? 12 : n == 256 ? 8 : 0;
 }
 __host__ __device__ constexpr int plan_nst(int n)
 {
     return n == 8 ? 1 : n == 12 ? 2 : n == 16 ? 2 : n == 24 ? 2 : n == 32 ? 2 : n == 48 ? 3 : n == 64 ? 2
-         : n == 96 ? FTS_PLAN96_NST : n == 128 ? 3 : n == 192 ? 4 : n == 256 ? 3 : 0;
+         : n == 96 ? 3 : n == 128 ? 3 : n == 192 ? 4 : n == 256 ? 3 : 0;
 }
 __host__ __device__ constexpr int plan_radix(int n, int s)
 {
@@ -78,7 +68,7 @@ __host__ __device__ constexpr int plan_radix(int n, int s)
          : n == 32 ? (s == 0 ? 8 : 4)
          : n == 48 ? (s < 2 ? 4 : 3)
          : n == 64 ? 8
-         : n == 96 ? (FTS_PLAN96_NST == 2 ? (s == 0 ? 12 : 8) : (s < 2 ? 4 : 6))
+         : n == 96 ? (s < 2 ? 4 : 6)
          : n == 128 ? (s == 0 ? 8 : 4)
          : n == 192 ? (s < 3 ? 4 : 3)
          : n == 256 ? (s < 2 ? 8 : 4) : 0;
@@ -173,39 +163,6 @@ template <typename V, int SIGN> struct Dft<V, 8, SIGN> {
         u[6] = csub(E[2], o2);
         u[3] = cadd(E[3], o3);
         u[7] = csub(E[3], o3);
-    }
-};
-
-// radix 12 = 4 x 3 (Cooley-Tukey, n = 3 n1 + n2, k = k1 + 4 k2, twiddles w12^{n2 k1})
-template <typename V, int SIGN> struct Dft<V, 12, SIGN> {
-    __device__ __forceinline__ static void run(V* u)
-    {
-        using T = decltype(V::x);
-        const T h = T(0.86602540378443864676372317075294);
-        V y[3][4];
-#pragma unroll
-        for (int n2 = 0; n2 < 3; ++n2) {
-#pragma unroll
-            for (int n1 = 0; n1 < 4; ++n1) y[n2][n1] = u[3 * n1 + n2];
-            Dft<V, 4, SIGN>::run(y[n2]);
-        }
-        // w12^m = e^{SIGN 2 pi i m / 12}, m = n2 k1 in {1, 2, 3, 4, 6}
-        const V w1 = cmk<V>(h, T(SIGN) * T(0.5));
-        const V w2 = cmk<V>(T(0.5), T(SIGN) * h);
-        const V w4 = cmk<V>(T(-0.5), T(SIGN) * h);
-        y[1][1] = cmul(y[1][1], w1);
-        y[1][2] = cmul(y[1][2], w2);
-        y[1][3] = cmul_is<V, SIGN>(y[1][3]);
-        y[2][1] = cmul(y[2][1], w2);
-        y[2][2] = cmul(y[2][2], w4);
-        y[2][3] = cmk<V>(-y[2][3].x, -y[2][3].y);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; ++k1) {
-            V z[3] = {y[0][k1], y[1][k1], y[2][k1]};
-            Dft<V, 3, SIGN>::run(z);
-#pragma unroll
-            for (int k2 = 0; k2 < 3; ++k2) u[k1 + 4 * k2] = z[k2];
-        }
     }
 };
 

@@ -395,13 +395,14 @@ __global__ void __launch_bounds__(RED_THREADS) k_lrk_moment_sums(const double2* 
         for (int k = 0; k < 4; ++k) partials[k * RED_BLOCKS + blockIdx.x] = v[k];
 }
 
-// moments from the (mean, mean square) pairs, then the network at the bin centres
+// moments from the (mean, mean square) pairs, then the network at the bin centres:
+// workgroup q evaluates bins [q TP, (q + 1) TP); workgroup 0 stores the moments
 __global__ void __launch_bounds__(NT) k_lrk_infer(const float* __restrict__ params, const double2* __restrict__ means, LrkDevState* st)
 {
     __shared__ __align__(16) Smem s;
     __shared__ float msh[6];
     load_params(s, params);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, q = blockIdx.x;
     if (t == 0) {
         double m[6];
         auto sd = [](double2 a) { return sqrt(fmax(0.0, a.y - a.x * a.x)); };
@@ -412,19 +413,16 @@ __global__ void __launch_bounds__(NT) k_lrk_infer(const float* __restrict__ para
         m[4] = sd(means[2]);
         m[5] = sd(means[3]);
         for (int j = 0; j < 6; ++j) {
-            st->m_cur[j] = m[j];
+            if (q == 0) st->m_cur[j] = m[j];
             msh[j] = static_cast<float>(m[j]);
         }
-        st->m_cur[6] = st->m_cur[7] = 0.0;
+        if (q == 0) st->m_cur[6] = st->m_cur[7] = 0.0;
     }
     __syncthreads();
-    for (int q = 0; q < NB / TP; ++q) {
-        if (t < TP) set_features(s, t, q * TP + t, msh);
-        __syncthreads();
-        forward_tile(s);
-        if (t < TP * 2) st->r_cur[q * TP * 2 + t] = s.R[t];
-        __syncthreads();
-    }
+    if (t < TP) set_features(s, t, q * TP + t, msh);
+    __syncthreads();
+    forward_tile(s);
+    if (t < TP * 2) st->r_cur[q * TP * 2 + t] = s.R[t];
 }
 
 __global__ void __launch_bounds__(NT) k_lrk_bin_chunks(const int* __restrict__ chunk_start, const int* __restrict__ perm, const double* __restrict__ ad,
@@ -483,20 +481,34 @@ __global__ void __launch_bounds__(NT) k_lrk_judge(LrkDevState* st, int have_prev
                 S[k][t] = a;
             }
         __syncthreads();
+        // per-bin terms of the residuals R(learned) and R(A_D), summed over bins in a fixed tree order
+        __shared__ double red[5][NB]; // E_0, (RL - E)_0, (RD - E)_0 ... per field: E, learned, Debye
+        __shared__ double tot[2][3];
+        for (int c = 0; c < 2; ++c) {
+            if (t < NB) {
+                const double e = exp(static_cast<double>(st->r_prev[t * 2 + c]));
+                red[0][t] = S[3 * c + 2][t];
+                red[1][t] = -2.0 * e * S[3 * c][t] + e * e * S[3 * c + 1][t];
+                red[2][t] = -2.0 * S[3 * c][t] + S[3 * c + 1][t];
+            }
+            __syncthreads();
+            for (int off = NB / 2; off > 0; off >>= 1) {
+                if (t < off)
+                    for (int k = 0; k < 3; ++k) red[k][t] += red[k][t + off];
+                __syncthreads();
+            }
+            if (t == 0)
+                for (int k = 0; k < 3; ++k) tot[c][k] = red[k][0];
+            __syncthreads();
+        }
         if (t == 0) {
             double ex = -INFINITY;
             bool any = false;
             for (int c = 0; c < 2; ++c) {
-                double E = 0.0;
-                for (int b = 0; b < NB; ++b) E += S[3 * c + 2][b];
+                const double E = tot[c][0];
                 Esh[c] = E;
                 if (!(E > 0.0)) continue;
-                double RL = E, RD = E;
-                for (int b = 0; b < NB; ++b) {
-                    const double e = exp(static_cast<double>(st->r_prev[b * 2 + c]));
-                    RL += -2.0 * e * S[3 * c][b] + e * e * S[3 * c + 1][b];
-                    RD += -2.0 * S[3 * c][b] + S[3 * c + 1][b];
-                }
+                const double RL = E + tot[c][1], RD = E + tot[c][2];
                 st->sum_rl[c] += RL / E;
                 st->sum_rd[c] += RD / E;
                 ex = fmax(ex, (RL - RD) / E);
@@ -564,7 +576,7 @@ hipError_t lrk_moment_sums(const double2* w0, const double2* psi, int M, double2
 
 hipError_t lrk_infer(const float* params, const double2* means, LrkDevState* st, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lrk_infer, dim3(1), dim3(NT), 0, s, params, means, st);
+    hipLaunchKernelGGL(k_lrk_infer, dim3(NB / TP), dim3(NT), 0, s, params, means, st);
     return hipGetLastError();
 }
 

@@ -295,6 +295,9 @@ int fts_batch_replica(fts_batch* b, int r, fts_engine** engine, int* group);
  * records from records + r * capacity; n_records: [n_replicas].  On a failure
  * the call returns its status, and fts_last_error names the replica. */
 int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* records, int capacity, int* n_records);
+/* The bead-step launches of each batched sweep are captured once as a hipGraph and
+ * replayed (default); 0 launches them one by one (same kernels, same results). */
+int fts_batch_set_graph(fts_batch* b, int enabled);
 /* Device timing of the batched sweeps of one group (fts_timing as for an engine). */
 int fts_batch_set_timing(fts_batch* b, int enabled);
 int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out);

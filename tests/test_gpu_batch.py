@@ -95,3 +95,26 @@ def test_batch_headline_autotuned():
     rs = single.step(3, record_every=1)
     for x, y in zip(rb[2], rs):
         assert abs(x["H"] - y["H"]) <= 1e-12 * abs(y["H"])
+
+
+@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE])
+def test_graph_replays_match_eager_launches(sched):
+    """The captured sweep graphs against the same launches issued one by one: several
+    step calls (each replaying the graphs of both density evaluations many times, with
+    the per-replica eager work and host synchronisations between replays) must give
+    bit-identical fields and records."""
+    print(f"graph vs eager sched={sched}", flush=True)
+    out = []
+    for graph in (True, False):
+        systems = replica_systems(SCHEME_PC_REFERENCE, PREC_FP32, sched)
+        b = hip_batch(systems)
+        b.set_graph(graph)
+        start_fields(b.replicas, systems)
+        recs = [b.step(3, record_every=1) for _ in range(3)]
+        out.append(([e.get_field(FIELD_OMEGA) for e in b.replicas], [e.get_field(FIELD_PSI) for e in b.replicas],
+                    [[x["H"] for call in recs for x in call[r]] for r in range(len(systems))]))
+        b.close()
+    (wg, pg, hg), (we, pe, he) = out
+    for r in range(len(wg)):
+        assert np.array_equal(wg[r], we[r]) and np.array_equal(pg[r], pe[r]), r
+        assert hg[r] == he[r], r
