@@ -122,6 +122,8 @@ class RunSettings(C.Structure):
 
 
 MAX_BATCH_GROUPS = 16
+BATCH_DEFAULT_STREAMS = 4
+BATCH_MAX_STREAMS = 16
 
 
 class BatchInfo(C.Structure):
@@ -163,6 +165,7 @@ def _bind(lib, prefix):
         "batch_step": (C.c_int, [vp, C.c_int, C.c_int, P(Observables), C.c_int, P(C.c_int)]),
         "batch_set_timing": (C.c_int, [vp, C.c_int]),
         "batch_set_graph": (C.c_int, [vp, C.c_int]),
+        "batch_set_streams": (C.c_int, [vp, C.c_int]),
         "batch_get_timing": (C.c_int, [vp, C.c_int, P(Timing)]),
         "deck_read": (C.c_int, [C.c_char_p, C.c_char_p, P(vp)]),
         "deck_destroy": (None, [vp]),
@@ -541,6 +544,10 @@ class Batch:
 
     def set_graph(self, enabled):
         self._check(self._fns["batch_set_graph"](self._h, int(enabled)))
+
+    def set_streams(self, n):
+        """Side streams per group for the replicas' own work (0: all on the group stream)."""
+        self._check(self._fns["batch_set_streams"](self._h, int(n)))
 
     def get_timing(self, group=0):
         t = Timing()

@@ -63,8 +63,15 @@ struct Group {
     bool use_graph = true; // replay captured sweep graphs (fts_batch_set_graph)
     double tune_ms[2] = {0.0, 0.0};
     std::vector<std::unique_ptr<SweepTables>> tables; // full sweeps, per species index and schedule
-    double2* hQ = nullptr;                             // pinned [reps][FTS_MAX_SPECIES]
+    double2* hQ = nullptr;                             // pinned [2][reps][FTS_MAX_SPECIES]: Q, Q at (w_bar, psi_bar)
     int* hflag = nullptr;                              // pinned [reps]
+    // The work of single replicas (smoothing FFTs, ions, contractions, updates)
+    // runs on side streams forked from the group stream and joined back into it,
+    // so that the small launches of several replicas share the chip; replica r
+    // takes side[r % side.size()].  No side streams: everything on the group stream.
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joined; // per side stream
+    hipEvent_t forked = nullptr;
     // timing
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -116,11 +123,14 @@ template <typename T> RepView<T> view(fts_engine* e, int p)
     return v;
 }
 
-template <typename X> int upload(const std::vector<X>& v, void** dst)
+// copies on the group's stream: a copy on the legacy stream from one group's
+// thread while another group captures its sweeps would be refused
+template <typename X> int upload(const std::vector<X>& v, void** dst, hipStream_t stream)
 {
     if (v.empty()) return FTS_OK;
     HIP_TRY(hipMalloc(dst, sizeof(X) * v.size()));
-    HIP_TRY(hipMemcpy(*dst, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(*dst, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     return FTS_OK;
 }
 
@@ -231,7 +241,9 @@ template <typename T> int build_tables(Group& g, int p, int sched, int max_steps
         }
     }
     int rc;
-    if ((rc = upload(J.plane, &tb.dplane)) || (rc = upload(J.pencil, &tb.dpencil)) || (rc = upload(J.line, &tb.dline))) return rc;
+    if ((rc = upload(J.plane, &tb.dplane, g.stream)) || (rc = upload(J.pencil, &tb.dpencil, g.stream)) ||
+        (rc = upload(J.line, &tb.dline, g.stream)))
+        return rc;
     return FTS_OK;
 }
 
@@ -339,77 +351,134 @@ template <typename T> int group_sweeps(Group& g, int p, const std::vector<char>&
     return FTS_OK;
 }
 
+// Runs fn(r) for every replica r in mask with the replica's engine on its side
+// stream.  The side streams start after the work queued so far on the group
+// stream, and the group stream waits for all of them afterwards, so the replicas'
+// work is ordered as if it ran on the group stream.
+template <typename F> int group_each(Group& g, const std::vector<char>& mask, F&& fn)
+{
+    int rc = FTS_OK;
+    const size_t S = g.side.size();
+    if (S == 0) {
+        for (size_t r = 0; r < g.reps.size() && !rc; ++r)
+            if (mask[r]) rc = fn(r);
+        return rc;
+    }
+    HIP_TRY(hipEventRecord(g.forked, g.stream));
+    std::vector<char> used(S, 0);
+    for (size_t r = 0; r < g.reps.size() && !rc; ++r) {
+        if (!mask[r]) continue;
+        const size_t k = r % S;
+        if (!used[k]) {
+            HIP_TRY(hipStreamWaitEvent(g.side[k], g.forked, 0));
+            used[k] = 1;
+        }
+        fts_engine* e = g.reps[r];
+        e->stream = g.side[k];
+        rc = fn(r);
+        e->stream = g.stream;
+    }
+    for (size_t k = 0; k < S; ++k)
+        if (used[k]) {
+            HIP_TRY(hipEventRecord(g.joined[k], g.side[k]));
+            HIP_TRY(hipStreamWaitEvent(g.stream, g.joined[k], 0));
+        }
+    return rc;
+}
+
 // compute_densities of the replicas in mask at (w, psi) or (w_bar, psi_bar)
 template <typename T> int group_densities(Group& g, bool bar, const std::vector<char>& mask)
 {
     int rc;
     size_t max_species = 0;
-    for (size_t r = 0; r < g.reps.size(); ++r) {
-        if (!mask[r]) continue;
-        fts_engine* e = g.reps[r];
-        if ((rc = e->densities_begin(bar ? e->omega_bar.p : e->omega.p, bar ? e->psi_bar.p : e->psi.p))) return rc;
-        max_species = std::max(max_species, e->species.size());
-    }
-    for (int p = 0; p < static_cast<int>(max_species); ++p) {
-        for (size_t r = 0; r < g.reps.size(); ++r)
-            if (mask[r] && static_cast<int>(g.reps[r]->species.size()) > p && (rc = g.reps[r]->solve_begin<T>(p))) return rc;
-        if ((rc = group_sweeps<T>(g, p, mask))) return rc;
-        for (size_t r = 0; r < g.reps.size(); ++r)
-            if (mask[r] && static_cast<int>(g.reps[r]->species.size()) > p && (rc = g.reps[r]->solve_end<T>(p, true))) return rc;
-    }
     for (size_t r = 0; r < g.reps.size(); ++r)
-        if (mask[r] && (rc = g.reps[r]->densities_end())) return rc;
+        if (mask[r]) max_species = std::max(max_species, g.reps[r]->species.size());
+    // replica r's own work before the sweeps of species p (p = 0: densities_begin
+    // first) and after them (the last species: densities_end too)
+    auto has = [&](size_t r, int p) { return static_cast<int>(g.reps[r]->species.size()) > p; };
+    rc = group_each(g, mask, [&](size_t r) {
+        fts_engine* e = g.reps[r];
+        int c = e->densities_begin(bar ? e->omega_bar.p : e->omega.p, bar ? e->psi_bar.p : e->psi.p);
+        if (!c && has(r, 0)) c = e->solve_begin<T>(0);
+        return c;
+    });
+    if (rc) return rc;
+    for (int p = 0; p < static_cast<int>(max_species); ++p) {
+        if ((rc = group_sweeps<T>(g, p, mask))) return rc;
+        const bool last = p + 1 == static_cast<int>(max_species);
+        rc = group_each(g, mask, [&](size_t r) {
+            fts_engine* e = g.reps[r];
+            int c = has(r, p) ? e->solve_end<T>(p, true) : FTS_OK;
+            if (!c && has(r, p + 1)) c = e->solve_begin<T>(p + 1);
+            if (!c && last) c = e->densities_end();
+            return c;
+        });
+        if (rc) return rc;
+    }
+    if (max_species == 0)
+        return group_each(g, mask, [&](size_t r) { return g.reps[r]->densities_end(); });
     return FTS_OK;
 }
 
 bool finite_c(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
 
-// Q of the replicas in mask; a non-finite one leaves mask and is reported in why
-int group_check_Q(Group& g, const char* what, std::vector<char>& mask, std::vector<std::string>& why)
+// queue the copy of the replicas' Q into slot k of g.hQ (read after a sync)
+int group_queue_Q(Group& g, int k, const std::vector<char>& mask)
 {
     int rc;
     for (size_t r = 0; r < g.reps.size(); ++r)
-        if (mask[r] && (rc = g.reps[r]->q_queue(g.hQ + r * FTS_MAX_SPECIES))) return rc;
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    for (size_t r = 0; r < g.reps.size(); ++r) {
-        if (!mask[r]) continue;
-        for (size_t p = 0; p < g.reps[r]->species.size(); ++p)
-            if (!finite_c(g.hQ[r * FTS_MAX_SPECIES + p])) {
-                why[r] = std::string("non-finite ") + what + " at step " + std::to_string(g.reps[r]->step_counter + 1);
-                mask[r] = 0;
-                break;
-            }
-    }
+        if (mask[r] && (rc = g.reps[r]->q_queue(g.hQ + (static_cast<size_t>(k) * g.reps.size() + r) * FTS_MAX_SPECIES))) return rc;
     return FTS_OK;
+}
+
+bool group_Q_finite(const Group& g, int k, size_t r)
+{
+    for (size_t p = 0; p < g.reps[r]->species.size(); ++p)
+        if (!finite_c(g.hQ[(static_cast<size_t>(k) * g.reps.size() + r) * FTS_MAX_SPECIES + p])) return false;
+    return true;
 }
 
 // One CL step of the replicas in mask; rec[r] non-null where replica r records.
 // A replica whose step fails leaves the step with its reason in why[r] (the
 // caller restarts it or gives up); device errors return a status.
+// The whole step is queued before the host waits once: Q (and Q at the predicted
+// fields) and the finiteness of the updated fields are checked after that sync, in
+// this order, so a replica with a non-finite Q has run the rest of its step on
+// non-finite values, which the caller then discards.  The host therefore queues
+// the update work while the device runs the sweeps.
 template <typename T> int group_step(Group& g, int scheme, const std::vector<fts_observables*>& rec, std::vector<char> mask,
                                      std::vector<std::string>& why)
 {
     int rc;
     const size_t n = g.reps.size();
-    if ((rc = group_densities<T>(g, false, mask)) || (rc = group_check_Q(g, "Q", mask, why))) return rc;
-    if (scheme == FTS_SCHEME_PC_REFERENCE) {
-        for (size_t r = 0; r < n; ++r)
-            if (mask[r] && (rc = g.reps[r]->pc_predict())) return rc;
-        if ((rc = group_densities<T>(g, true, mask)) || (rc = group_check_Q(g, "Qbar", mask, why))) return rc;
-        for (size_t r = 0; r < n; ++r)
-            if (mask[r] && (rc = g.reps[r]->pc_correct(rec[r] != nullptr, rec[r]))) return rc;
-    } else {
-        for (size_t r = 0; r < n; ++r)
-            if (mask[r] && (rc = g.reps[r]->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT || scheme == FTS_SCHEME_SI_LEARNED))) return rc;
+    if ((rc = group_densities<T>(g, false, mask)) || (rc = group_queue_Q(g, 0, mask))) return rc;
+    // the engines of a batch defer measure's host part (measure_finish) to after the sync below
+    auto update = [&](size_t r) {
+        fts_engine* e = g.reps[r];
+        int c;
+        if (scheme == FTS_SCHEME_PC_REFERENCE) c = e->pc_correct(rec[r] != nullptr, rec[r]);
+        else c = e->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT || scheme == FTS_SCHEME_SI_LEARNED);
+        if (c) return c;
+        ++e->step_counter;
+        return e->finite_queue(g.hflag + r);
+    };
+    const bool pc = scheme == FTS_SCHEME_PC_REFERENCE;
+    if (pc) {
+        if ((rc = group_each(g, mask, [&](size_t r) { return g.reps[r]->pc_predict(); }))) return rc;
+        if ((rc = group_densities<T>(g, true, mask)) || (rc = group_queue_Q(g, 1, mask))) return rc;
     }
+    if ((rc = group_each(g, mask, update))) return rc;
+    HIP_TRY(hipStreamSynchronize(g.stream));
     for (size_t r = 0; r < n; ++r) {
         if (!mask[r]) continue;
-        ++g.reps[r]->step_counter;
-        if ((rc = g.reps[r]->finite_queue(g.hflag + r))) return rc;
+        const std::string at = " at step " + std::to_string(g.reps[r]->step_counter);
+        const bool q_ok = group_Q_finite(g, 0, r) && (!pc || group_Q_finite(g, 1, r));
+        if (!q_ok) --g.reps[r]->step_counter; // a step that fails at Q is not counted (as fts_step)
+        if (!group_Q_finite(g, 0, r)) why[r] = "non-finite Q" + at;
+        else if (pc && !group_Q_finite(g, 1, r)) why[r] = "non-finite Qbar" + at;
+        else if (!g.hflag[r]) why[r] = "updated fields have non-finite values" + at;
+        else if (rec[r] && (rc = g.reps[r]->measure_finish(rec[r]))) return rc;
     }
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    for (size_t r = 0; r < n; ++r)
-        if (mask[r] && !g.hflag[r]) why[r] = "updated fields have non-finite values at step " + std::to_string(g.reps[r]->step_counter);
     return FTS_OK;
 }
 
@@ -447,6 +516,26 @@ template <typename T> int group_autotune(Group& g)
     return FTS_OK;
 }
 
+// replaces the side streams of g with n new ones (n = 0: none)
+int set_side_streams(Group& g, int n)
+{
+    if (g.stream) HIP_TRY(hipStreamSynchronize(g.stream));
+    for (hipStream_t s : g.side) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : g.joined) (void)hipEventDestroy(e);
+    g.side.clear();
+    g.joined.clear();
+    for (int k = 0; k < n; ++k) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        g.side.push_back(s);
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        g.joined.push_back(e);
+    }
+    if (n > 0 && !g.forked) HIP_TRY(hipEventCreateWithFlags(&g.forked, hipEventDisableTiming));
+    return FTS_OK;
+}
+
 } // namespace
 
 struct fts_batch {
@@ -469,6 +558,8 @@ struct fts_batch {
             g->tables.clear();
             if (g->hQ) (void)hipHostFree(g->hQ);
             if (g->hflag) (void)hipHostFree(g->hflag);
+            set_side_streams(*g, 0);
+            if (g->forked) (void)hipEventDestroy(g->forked);
             if (g->stream) (void)hipStreamDestroy(g->stream);
         }
     }
@@ -497,7 +588,9 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
             g->m3 = s.m3;
             g->precision = s.precision;
             g->dL = s.dL;
-            HIP_TRY(hipStreamCreate(&g->stream));
+            // non-blocking: the group captures its sweeps on this stream, and a blocking
+            // stream under capture makes every legacy-stream call of the process fail
+            HIP_TRY(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
         }
         fts_engine* e = nullptr;
         int rc = engine_create(&s, g->stream, false, &e);
@@ -509,7 +602,7 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
     }
     for (auto& gp : b->groups) {
         Group& g = *gp;
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hQ), sizeof(double2) * FTS_MAX_SPECIES * g.reps.size()));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hQ), sizeof(double2) * FTS_MAX_SPECIES * 2 * g.reps.size()));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hflag), sizeof(int) * g.reps.size()));
         g.plane_ok = true;
         for (fts_engine* e : g.reps) g.plane_ok = g.plane_ok && e->plane_ok;
@@ -523,7 +616,9 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
             e->sched = g.sched;
             e->tune_ms[0] = g.tune_ms[0];
             e->tune_ms[1] = g.tune_ms[1];
+            e->defer_measure = true;
         }
+        if ((rc = set_side_streams(g, std::min<int>(static_cast<int>(g.reps.size()), FTS_BATCH_DEFAULT_STREAMS)))) return rc;
     }
     *out = b.release();
     return FTS_OK;
@@ -626,6 +721,16 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
     }
     for (auto& g : b->groups)
         if (g->rc) return fail(g->rc, g->err);
+    return FTS_OK;
+}
+
+int fts_batch_set_streams(fts_batch* b, int n)
+{
+    if (!b || n < 0 || n > FTS_BATCH_MAX_STREAMS) return fail(FTS_INVALID, "batch: side stream count out of range");
+    for (auto& g : b->groups) {
+        int rc = set_side_streams(*g, std::min<int>(n, static_cast<int>(g->reps.size())));
+        if (rc) return rc;
+    }
     return FTS_OK;
 }
 

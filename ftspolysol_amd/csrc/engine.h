@@ -218,7 +218,14 @@ struct fts_engine {
     int q_queue(double2* host_out);       // queue the copy of Q[n_species]; valid after a stream sync
     int psi_drift(const double2* ps, const double2* rho_src, double2* out);
     int check_finite(bool& ok);
+    // measure = measure_queue (device work and copies into pinned memory), a stream
+    // sync, measure_finish (host arithmetic).  With defer_measure set, measure()
+    // only queues and leaves the sync and measure_finish to the caller, so that a
+    // batch can queue the work of all its replicas before it waits.
     int measure(const double2* om, const double2* ps, fts_observables* rec);
+    int measure_queue(const double2* om, const double2* ps, fts_observables* rec);
+    int measure_finish(fts_observables* rec);
+    bool defer_measure = false;
     template <typename T> int stress_chain(int p);
     int contract_to(const void* A, const void* B, bool fp32, const double* sep, double2* dst);
     // measure(): device results gathered in stress_acc and copied, with the scalar

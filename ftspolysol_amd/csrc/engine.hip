@@ -1161,13 +1161,20 @@ double2 cinv_host(double2 q)
 // scalar slots and the renormalisation data come back in one copy to pinned memory.
 int fts_engine::measure(const double2* om, const double2* ps, fts_observables* rec)
 {
+    int rc = measure_queue(om, ps, rec);
+    if (rc || defer_measure) return rc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    return measure_finish(rec);
+}
+
+int fts_engine::measure_queue(const double2* om, const double2* ps, fts_observables* rec)
+{
     std::memset(rec, 0, sizeof(*rec));
     rec->step = step_counter;
     rec->n_species = static_cast<int>(species.size());
     rec->n_ions = static_cast<int>(ions.size());
     rec->stress_mask = stress_mask;
     const double invM = 1.0 / static_cast<double>(M);
-    const double invM2 = invM * invM;
     const size_t nsp = species.size();
     const bool want_stress = (stress_mask & 0x7F) != 0;
     int rc;
@@ -1221,9 +1228,21 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
     for (size_t p = 0; p < nsp; ++p)
         HIP_TRY(hipMemcpyAsync(h_shift + p * MAX_CLASSES, species[p].d_cls_shift.p, sizeof(double) * species[p].cls_shift.size(),
                                hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipStreamSynchronize(stream));
+    return FTS_OK;
+}
 
-    // ---- host arithmetic ----------------------------------------------------------
+// host arithmetic of measure over the pinned copies measure_queue made (after a sync)
+int fts_engine::measure_finish(fts_observables* rec)
+{
+    const double invM = 1.0 / static_cast<double>(M);
+    const double invM2 = invM * invM;
+    const size_t nsp = species.size();
+    const bool want_stress = (stress_mask & 0x7F) != 0;
+    const double2* h_scal = static_cast<const double2*>(pin);
+    const double2* h_acc = h_scal + SC_COUNT;
+    const double* h_rfac2 = reinterpret_cast<const double*>(h_acc + stress_acc.n);
+    const double* h_shift = h_rfac2 + nsp * static_cast<size_t>(Nmax);
+    const size_t fixed_rows = stress_chain_base(static_cast<int>(nsp));
     const double2* Q = h_scal + SC_Q;
     const double2* Qi = h_scal + SC_QION;
     for (size_t p = 0; p < nsp; ++p) {

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     __syncthreads();
 
     if constexpr (MODE == PLANE_INV || MODE == PLANE_STEP || MODE == PLANE_LAST) {
-        lds_fft<T, N, +1, N, LS, 1, false, NT>(lds, tw, tid); // along z
+        lds_fft<T, N, +1, N, LS, 1, true, NT>(lds, tw, tid); // along z
         lds_fft<T, N, +1, N, 1, LS, true, NT>(lds, tw, tid);  // along y
     }
     if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST) {
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         }
     }
     if constexpr (MODE == PLANE_FWD || MODE == PLANE_INIT || MODE == PLANE_STEP) {
-        lds_fft<T, N, -1, N, LS, 1, false, NT>(lds, tw, tid);
+        lds_fft<T, N, -1, N, LS, 1, true, NT>(lds, tw, tid);
         lds_fft<T, N, -1, N, 1, LS, true, NT>(lds, tw, tid);
     }
     if constexpr (MODE != PLANE_LAST) {

@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
     constexpr bool DO_INV = MODE == LINE_INV || MODE == LINE_INV_E_FWD || MODE == LINE_INV_E;
     constexpr bool DO_FWD_FIRST = MODE == LINE_FWD || MODE == LINE_FWD_MUL_INV;
     if constexpr (DO_FWD_FIRST) {
-        if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, false, NT>(lds, tw, tid);
+        if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, -1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
     if constexpr (MODE == LINE_FWD_MUL_INV) {
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
         __syncthreads();
     }
     if constexpr (DO_INV || MODE == LINE_FWD_MUL_INV) {
-        if constexpr (ZL) lds_fft<T, N, +1, TC, LS, 1, false, NT>(lds, tw, tid);
+        if constexpr (ZL) lds_fft<T, N, +1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, +1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
     if constexpr (MODE == LINE_INV_E_FWD || MODE == LINE_INV_E) {
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
         }
     }
     if constexpr (MODE == LINE_INV_E_FWD || MODE == LINE_E_FWD) {
-        if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, false, NT>(lds, tw, tid);
+        if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, -1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
     if constexpr (MODE != LINE_INV_E) {

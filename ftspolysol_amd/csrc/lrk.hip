@@ -471,8 +471,11 @@ __global__ void __launch_bounds__(NT) k_lrk_judge(LrkDevState* st, int have_prev
 {
     __shared__ double S[6][NB]; // S1_0 S2_0 E_0 S1_1 S2_1 E_1 per bin
     __shared__ double Esh[2];
-    __shared__ int use_sh;
+    __shared__ int use_sh, ok_sh;
     const int t = threadIdx.x;
+    // a step on non-finite fields (a batch learns of it only after the step, and
+    // then restarts the replica) leaves the statistics and the records alone
+    if (t == 0) ok_sh = 1;
     if (have_prev) {
         if (t < NB)
             for (int k = 0; k < 6; ++k) {
@@ -502,6 +505,13 @@ __global__ void __launch_bounds__(NT) k_lrk_judge(LrkDevState* st, int have_prev
             __syncthreads();
         }
         if (t == 0) {
+            for (int c = 0; c < 2; ++c)
+                for (int k = 0; k < 3; ++k)
+                    if (!isfinite(tot[c][k])) ok_sh = 0;
+        }
+        __syncthreads();
+        if (t == 0 && !ok_sh) use_sh = 0;
+        if (t == 0 && ok_sh) {
             double ex = -INFINITY;
             bool any = false;
             for (int c = 0; c < 2; ++c) {
@@ -520,7 +530,7 @@ __global__ void __launch_bounds__(NT) k_lrk_judge(LrkDevState* st, int have_prev
             use_sh = ex <= threshold ? 1 : 0;
         }
         __syncthreads();
-        if (capacity > 0) {
+        if (capacity > 0 && ok_sh) {
             double* rec = ring + static_cast<size_t>(st->write_idx % capacity) * FTS_LRK_RECORD;
             if (t < 8) rec[FTS_LRK_REC_MOMENTS + t] = t < 6 ? st->m_prev[t] : Esh[t - 6];
             if (t < NB)
@@ -540,7 +550,7 @@ __global__ void __launch_bounds__(NT) k_lrk_judge(LrkDevState* st, int have_prev
     __syncthreads();
     if (t == 0) {
         st->use = use_sh;
-        if (learned_scheme) {
+        if (learned_scheme && ok_sh) {
             if (use_sh) st->learned += 1;
             else st->fallback += 1;
         }

@@ -298,6 +298,15 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
 /* The bead-step launches of each batched sweep are captured once as a hipGraph and
  * replayed (default); 0 launches them one by one (same kernels, same results). */
 int fts_batch_set_graph(fts_batch* b, int enabled);
+/* The work of single replicas between the batched sweeps (smoothing FFTs, ions,
+ * contractions, field updates, records) runs on n side streams per group, forked
+ * from the group's stream and joined back into it (replica r on side stream
+ * r mod n), so the small launches of several replicas share the GPU.  n = 0 runs
+ * it all on the group's stream.  Default min(group size, FTS_BATCH_DEFAULT_STREAMS);
+ * the results do not depend on n. */
+#define FTS_BATCH_DEFAULT_STREAMS 4
+#define FTS_BATCH_MAX_STREAMS 16
+int fts_batch_set_streams(fts_batch* b, int n);
 /* Device timing of the batched sweeps of one group (fts_timing as for an engine). */
 int fts_batch_set_timing(fts_batch* b, int enabled);
 int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out);

@@ -118,3 +118,28 @@ def test_graph_replays_match_eager_launches(sched):
     for r in range(len(wg)):
         assert np.array_equal(wg[r], we[r]) and np.array_equal(pg[r], pe[r]), r
         assert hg[r] == he[r], r
+
+
+@pytest.mark.parametrize("scheme,precision", [(SCHEME_PC_REFERENCE, PREC_FP32), (SCHEME_EM, PREC_FP64)])
+def test_side_streams_do_not_change_results(scheme, precision):
+    """The replicas' own work on side streams (forked from and joined into the group
+    stream, records finished on the host after the step's one sync) against the same
+    work on the group stream alone: fields and every recorded observable, stress
+    included, must be bit-identical for 0, 2 and the default number of side streams."""
+    print(f"side streams scheme={scheme} precision={precision}", flush=True)
+    out = []
+    for n in (0, 2, None):
+        systems = replica_systems(scheme, precision, SCHED_PLANE)
+        b = hip_batch(systems)
+        if n is not None:
+            b.set_streams(n)
+        start_fields(b.replicas, systems)
+        recs = [b.step(2, record_every=1) for _ in range(2)]
+        out.append(([e.get_field(FIELD_OMEGA) for e in b.replicas], [e.get_field(FIELD_PSI) for e in b.replicas],
+                    [[(x["H"], tuple(x["mu_ex_poly"]), tuple(x["stress"]), x["p_iso"]) for call in recs for x in call[r]]
+                     for r in range(len(systems))]))
+        b.close()
+    for w, p, h in out[1:]:
+        for r in range(len(w)):
+            assert np.array_equal(w[r], out[0][0][r]) and np.array_equal(p[r], out[0][1][r]), r
+            assert h[r] == out[0][2][r], r
