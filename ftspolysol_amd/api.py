@@ -96,7 +96,8 @@ class Info(C.Structure):
                 ("gamma_is_imag", C.c_double * MAX_TYPES), ("mode_O", (C.c_double * MAX_TYPES) * MAX_TYPES),
                 ("ion_valence", C.c_double * MAX_IONS), ("ion_count", C.c_longlong * MAX_IONS),
                 ("ion_smearing", C.c_double * MAX_IONS), ("tune_ms", C.c_double * 2),
-                ("mirrored_species", C.c_int), ("restarts", C.c_int), ("noise_epoch", C.c_int)]
+                ("mirrored_species", C.c_int), ("restarts", C.c_int), ("noise_epoch", C.c_int),
+                ("checkpoint_step", C.c_longlong), ("failed_step", C.c_longlong)]
 
 
 class Timing(C.Structure):
@@ -122,7 +123,7 @@ class RunSettings(C.Structure):
 
 
 MAX_BATCH_GROUPS = 16
-BATCH_DEFAULT_STREAMS = 4
+BATCH_DEFAULT_STREAMS = 8
 BATCH_MAX_STREAMS = 16
 
 
@@ -154,6 +155,10 @@ def _bind(lib, prefix):
         "set_step": (C.c_int, [vp, C.c_longlong]),
         "set_restart": (C.c_int, [vp, C.c_int, C.c_int]),
         "inject_nonfinite": (C.c_int, [vp, C.c_longlong]),
+        "set_drift_stats": (C.c_int, [vp, C.c_int]),
+        "save_state": (C.c_int, [vp, C.c_char_p]),
+        "load_state": (C.c_int, [vp, C.c_char_p]),
+        "get_drift_rows": (C.c_int, [vp, P(C.c_double), C.c_int, P(C.c_int)]),
         "fft3d": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_int]),
         "bead_step": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)]),
         "set_timing": (C.c_int, [vp, C.c_int]),
@@ -294,6 +299,25 @@ class Engine:
 
     def inject_nonfinite(self, step):
         self._check(self._fns["inject_nonfinite"](self._h, step))
+
+    def save_state(self, path):
+        """Fields, step counter and noise key to a binary file (exact resume)."""
+        self._check(self._fns["save_state"](self._h, os.fsencode(path)))
+
+    def load_state(self, path):
+        self._check(self._fns["load_state"](self._h, os.fsencode(path)))
+
+    def set_drift_stats(self, enabled):
+        """Collect the drift statistics (drift_max.dat rows) of every step from now on."""
+        self._check(self._fns["set_drift_stats"](self._h, int(enabled)))
+
+    def get_drift_rows(self, capacity=100000):
+        """Rows [step, max per mode..., max psi] collected since the last call, oldest first."""
+        w = self.info.n_omega + 2
+        out = np.zeros((capacity, w))
+        n = C.c_int(0)
+        self._check(self._fns["get_drift_rows"](self._h, _dptr(out), capacity, C.byref(n)))
+        return out[:n.value].copy()
 
     def fft3d(self, x, sign):
         xin = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).reshape(-1)

@@ -701,7 +701,7 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
                 // non-finite: restart this replica from its checkpoint with new noise, or give up
                 fts_engine* e = g.reps[r];
                 g_error = why[r];
-                int rr = e->restart_from_checkpoint(e->step_counter);
+                int rr = e->restart_from_checkpoint(label[r]);
                 if (rr) {
                     g.rc = fail(rr, "replica " + std::to_string(id) + ": " + g_error);
                     break;

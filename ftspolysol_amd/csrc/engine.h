@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -107,7 +108,8 @@ enum {
     SC_H = 40,        // [8] per interaction mode: (c_l / 2) <omega_l^2>
     SC_PSI_S = 48,    // [7] k-space contraction of psi with itself
     SC_LRK = 56,      // [4] (mean, mean square) of Re w0, Im w0, Re psi, Im psi (learned kernel's moments)
-    SC_COUNT = 60
+    SC_DRIFT = 60,    // [FTS_MAX_TYPES + 1] drift statistics of the last step (.x): modes, then psi
+    SC_COUNT = 60 + FTS_MAX_TYPES + 1
 };
 
 } // namespace fts_host
@@ -241,6 +243,13 @@ struct fts_engine {
     int pc_correct(bool record, fts_observables* rec);         // after the densities at (w_bar, psi_bar)
     int em_update(bool record, fts_observables* rec, bool si); // after the densities at (w, psi)
     int finish_step();
+    // drift statistics (complexLangevin.cu:318-324, 466-497): with drift_stats set, every
+    // step queues max over sites of |lam_l/2 (u_l + u_bar_l)| per mode and |lam_psi/2
+    // (u_psi + u_psi_bar)| (predictor-corrector; EM and semi-implicit: |lam u| at the state
+    // the step starts from) into SC_DRIFT; fts_step appends [step, values] to drift_rows
+    bool drift_stats = false;
+    std::vector<double> drift_rows;
+    int drift_queue(bool pc);
 
     // timing
     hipEvent_t get_event();
@@ -251,6 +260,7 @@ struct fts_engine {
     // noise to a new Philox key (seed0 + epoch * 0x9E3779B97F4A7C15), so that the
     // retried steps do not replay the noise that led to the failure.
     int ckpt_every = 0, max_restarts = 0, restarts = 0, noise_epoch = 0;
+    long long last_failed_step = -1;
     long long ckpt_step = -1;
     long long inject_step = -1; // test hook: the step ending at this count leaves a NaN in psi (once)
     unsigned long long seed0 = 0;

@@ -1448,6 +1448,7 @@ int fts_engine::pc_correct(bool record, fts_observables* rec)
     int rc;
     const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
     if ((rc = psi_drift(psi_bar.p, rho_c.p, dpsi_bar.p))) return rc;
+    if (drift_stats && (rc = drift_queue(true))) return rc;
     if (record && (rc = measure(omega_bar.p, psi_bar.p, rec))) return rc;
     if (n_omega > 0) {
         hipLaunchKernelGGL(k_omega_correct, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, drift.p, omega_bar.p, rho_type.p, seed, t, M, omega.p);
@@ -1461,6 +1462,25 @@ int fts_engine::pc_correct(bool record, fts_observables* rec)
     } else {
         HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));
     }
+    return FTS_OK;
+}
+
+// write_drift_max / corrector_drift_max (complexLangevin.cu:318-324, 474-497): per mode
+// |lam_l / 2| max |u_l + u_bar_l| and |lam_psi / 2| max |u_psi + u_psi_bar| after the
+// corrector's drifts (pc); EM and semi-implicit: |lam| max |u| at the state the step starts from
+int fts_engine::drift_queue(bool pc)
+{
+    for (int l = 0; l < n_omega; ++l) {
+        const double lam = lambda_omega[static_cast<size_t>(l)];
+        hipLaunchKernelGGL(k_mode_drift_max2, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, mc, l, pc ? drift.p : nullptr,
+                           pc ? omega_bar.p : omega.p, rho_type.p, M, partials.p);
+        hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(RED_THREADS), 0, stream, partials.p, pc ? 0.5 * lam : lam, scal.p + SC_DRIFT + l);
+    }
+    hipLaunchKernelGGL(k_sum_max2, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, dpsi.p, pc ? static_cast<const double2*>(dpsi_bar.p) : nullptr,
+                       M, partials.p);
+    hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(RED_THREADS), 0, stream, partials.p, pc ? 0.5 * lambda_psi : lambda_psi,
+                       scal.p + SC_DRIFT + n_omega);
+    HIP_TRY(hipGetLastError());
     return FTS_OK;
 }
 
@@ -1497,6 +1517,7 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
         if ((rc = reduce_finalize(1, SC_ETA_MEAN, invM))) return rc;
     }
     if ((rc = psi_drift(psi.p, rho_c.p, dpsi.p))) return rc;
+    if (drift_stats && (rc = drift_queue(false))) return rc;
     if (si && lrk_active() && (rc = lrk_step())) return rc;
     if (n_omega > 0) {
         hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, seed, t, M, si ? 0 : 1, omega.p, drift.p);
@@ -1555,7 +1576,7 @@ int fts_engine::restart_from_checkpoint(long long failed_step)
     ++restarts;
     ++noise_epoch;
     seed = seed0 + static_cast<unsigned long long>(noise_epoch) * 0x9E3779B97F4A7C15ull;
-    (void)failed_step;
+    last_failed_step = failed_step;
     step_counter = ckpt_step;
     lrk_have_prev = false;
     return FTS_OK;
@@ -1746,6 +1767,8 @@ int fts_get_info(fts_engine* h, fts_info* o)
     o->tune_ms[1] = h->tune_ms[1];
     o->restarts = h->restarts;
     o->noise_epoch = h->noise_epoch;
+    o->checkpoint_step = h->ckpt_step;
+    o->failed_step = h->last_failed_step;
     o->mirrored_species = 0;
     for (size_t p = 0; p < h->species.size(); ++p)
         if (h->species[p].mirror) o->mirrored_species |= 1 << p;
@@ -1781,6 +1804,70 @@ int fts_set_fields(fts_engine* h, const double* om, const double* ps)
     if (!h->electro()) HIP_TRY(hipMemsetAsync(h->psi.p, 0, sizeof(double2) * Ms, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->ckpt_step = -1; // new fields: the next step takes a new checkpoint
+    h->lrk_have_prev = false;
+    return FTS_OK;
+}
+
+// Binary state file (include/fts.h): "FTSSTAT1", int32 m1 m2 m3 n_omega, int64 step,
+// uint64 seed0, int32 noise_epoch restarts, then omega [n_omega][M] and psi [M] as
+// interleaved doubles, host byte order.  Written to path.new and renamed into place.
+int fts_save_state(fts_engine* h, const char* path)
+{
+    if (!h || !path) return fail(FTS_INVALID, "null argument");
+    const size_t Ms = static_cast<size_t>(h->M), no = static_cast<size_t>(h->n_omega) * Ms;
+    std::vector<double2> w(no), p(Ms);
+    if (no) HIP_TRY(hipMemcpyAsync(w.data(), h->omega.p, sizeof(double2) * no, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(p.data(), h->psi.p, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const std::string tmp = std::string(path) + ".new";
+    FILE* fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) return fail(FTS_INVALID, std::string("cannot open ") + tmp);
+    const int32_t hdr[4] = {h->m1, h->m2, h->m3, h->n_omega};
+    const int64_t step = h->step_counter;
+    const uint64_t seed0 = h->seed0;
+    const int32_t epoch[2] = {h->noise_epoch, h->restarts};
+    bool ok = std::fwrite("FTSSTAT1", 1, 8, fp) == 8 && std::fwrite(hdr, sizeof(hdr), 1, fp) == 1 && std::fwrite(&step, sizeof(step), 1, fp) == 1 &&
+              std::fwrite(&seed0, sizeof(seed0), 1, fp) == 1 && std::fwrite(epoch, sizeof(epoch), 1, fp) == 1 &&
+              (no == 0 || std::fwrite(w.data(), sizeof(double2), no, fp) == no) && std::fwrite(p.data(), sizeof(double2), Ms, fp) == Ms;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) return fail(FTS_INVALID, std::string("cannot write ") + path);
+    return FTS_OK;
+}
+
+int fts_load_state(fts_engine* h, const char* path)
+{
+    if (!h || !path) return fail(FTS_INVALID, "null argument");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(FTS_INVALID, std::string("cannot open ") + path);
+    char magic[8];
+    int32_t hdr[4];
+    int64_t step = 0;
+    uint64_t seed0 = 0;
+    int32_t epoch[2];
+    bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, "FTSSTAT1", 8) == 0 && std::fread(hdr, sizeof(hdr), 1, fp) == 1 &&
+              std::fread(&step, sizeof(step), 1, fp) == 1 && std::fread(&seed0, sizeof(seed0), 1, fp) == 1 && std::fread(epoch, sizeof(epoch), 1, fp) == 1;
+    if (!ok) {
+        std::fclose(fp);
+        return fail(FTS_INVALID, std::string(path) + ": not a state file");
+    }
+    if (hdr[0] != h->m1 || hdr[1] != h->m2 || hdr[2] != h->m3 || hdr[3] != h->n_omega || step < 0 || epoch[0] < 0 || epoch[1] < 0) {
+        std::fclose(fp);
+        return fail(FTS_INVALID, std::string(path) + ": state of another mesh or mode count");
+    }
+    const size_t Ms = static_cast<size_t>(h->M), no = static_cast<size_t>(h->n_omega) * Ms;
+    std::vector<double2> w(no), p(Ms);
+    ok = (no == 0 || std::fread(w.data(), sizeof(double2), no, fp) == no) && std::fread(p.data(), sizeof(double2), Ms, fp) == Ms;
+    std::fclose(fp);
+    if (!ok) return fail(FTS_INVALID, std::string(path) + ": truncated state file");
+    if (no) HIP_TRY(hipMemcpyAsync(h->omega.p, w.data(), sizeof(double2) * no, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->psi.p, p.data(), sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->step_counter = step;
+    h->seed0 = seed0;
+    h->noise_epoch = epoch[0];
+    h->restarts = epoch[1];
+    h->seed = seed0 + static_cast<unsigned long long>(epoch[0]) * 0x9E3779B97F4A7C15ull;
+    h->ckpt_step = -1;
     h->lrk_have_prev = false;
     return FTS_OK;
 }
@@ -1890,16 +1977,44 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
             rec->step = step;
             ++nrec;
         }
+        if (rc == FTS_OK && h->drift_stats) {
+            double2 v[FTS_MAX_TYPES + 1];
+            if ((rc = h->read_scal(SC_DRIFT, h->n_omega + 1, v))) break;
+            h->drift_rows.push_back(static_cast<double>(step));
+            for (int k = 0; k <= h->n_omega; ++k) h->drift_rows.push_back(v[k].x);
+        }
         if (rc == FTS_NONFINITE && h->ckpt_every > 0) {
             // restart from the checkpoint with fresh noise; drop the records taken after it
             if ((rc = h->restart_from_checkpoint(step))) break;
             while (nrec > 0 && records[nrec - 1].step > h->ckpt_step) --nrec;
+            const size_t w = static_cast<size_t>(h->n_omega) + 2;
+            while (!h->drift_rows.empty() && h->drift_rows[h->drift_rows.size() - w] > static_cast<double>(h->ckpt_step))
+                h->drift_rows.resize(h->drift_rows.size() - w);
         }
         if (n_records) *n_records = nrec;
         if (rc != FTS_OK) break;
         if (h->timing && h->pending.size() > 256 && (rc = h->resolve_timing())) break;
     }
     return rc;
+}
+
+int fts_set_drift_stats(fts_engine* h, int enabled)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    h->drift_stats = enabled != 0;
+    h->drift_rows.clear();
+    return FTS_OK;
+}
+
+int fts_get_drift_rows(fts_engine* h, double* out, int capacity, int* n_rows)
+{
+    if (!h || !n_rows || capacity < 0 || (capacity > 0 && !out)) return fail(FTS_INVALID, "bad argument");
+    const size_t w = static_cast<size_t>(h->n_omega) + 2;
+    const size_t n = std::min(h->drift_rows.size() / w, static_cast<size_t>(capacity));
+    if (n) std::memcpy(out, h->drift_rows.data(), sizeof(double) * n * w);
+    h->drift_rows.erase(h->drift_rows.begin(), h->drift_rows.begin() + static_cast<std::ptrdiff_t>(n * w));
+    *n_rows = static_cast<int>(n);
+    return FTS_OK;
 }
 
 int fts_inject_nonfinite(fts_engine* h, long long step)

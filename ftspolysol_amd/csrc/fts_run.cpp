@@ -2,7 +2,7 @@
 // reference's main program (main.cu:9-93, parse.cu:3-43) over the C ABI.
 //
 //   fts_run -p deck.param [--omega w.rf] [--psi psi.rf] [--seq seq.txt]
-//           [-i run_id [-s parent_dir]] [--lrk params.npy] [--screenprint]
+//           [-i run_id [-s parent_dir]] [--lrk params.npy] [--resume state] [--screenprint]
 //
 // Reads the deck and the starting fields (missing default omega.rf -> zero fields;
 // meshes that differ are resized trilinearly; legacy A/B columns are mapped to the
@@ -13,6 +13,10 @@
 // (complexLangevin.cu:523-580), and the final fields (fieldIO.cu:716-726).
 // Non-finite steps restart from the device checkpoint with a new noise key; records
 // taken before the checkpoint are kept (the reference truncates observables.dat).
+// Every Ntmp checkpoint also writes checkpoint.state (fts_save_state: exact fields,
+// step counter and noise key); --resume continues such a state to step Nt, on the
+// trajectory the uninterrupted run takes.  drift_max.dat has a row per step after
+// N_stat (complexLangevin.cu:466-497).
 #include <sys/stat.h>
 
 #include <algorithm>
@@ -259,7 +263,7 @@ std::string step_template(const std::string& pattern)
 int main(int argc, char** argv)
 {
     const auto t_start = std::chrono::steady_clock::now();
-    std::string param_file, omega_in, psi_in, seq, run_id, run_parent, lrk_file;
+    std::string param_file, omega_in, psi_in, seq, run_id, run_parent, lrk_file, resume;
     bool omega_override = false, psi_override = false, screenprint = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -276,9 +280,10 @@ int main(int argc, char** argv)
         else if (a == "-s" || a == "--dir" || a == "--run-dir") ok = next(run_parent);
         else if (a == "--seq") ok = next(seq);
         else if (a == "--lrk") ok = next(lrk_file);
+        else if (a == "--resume") ok = next(resume);
         else if (a == "--screenprint") screenprint = true;
         else if (a == "-h" || a == "--help") {
-            std::printf("usage: %s -p param_file [--omega file] [--psi file] [--seq file] [-i run_id [-s dir]] [--lrk params.npy] [--screenprint]\n", argv[0]);
+            std::printf("usage: %s -p param_file [--omega file] [--psi file] [--seq file] [-i run_id [-s dir]] [--lrk params.npy] [--resume state] [--screenprint]\n", argv[0]);
             return EXIT_SUCCESS;
         } else ok = err("unknown option " + a);
         if (!ok) return EXIT_FAILURE;
@@ -367,7 +372,28 @@ int main(int argc, char** argv)
         if (!std::isfinite(x)) return err("input psi has non-finite values"), EXIT_FAILURE;
     if (!ok_or_report(fts_set_fields(e, omega.data(), psi.data()), "set fields")) return EXIT_FAILURE;
     if (rs.Ntmp > 0 && !ok_or_report(fts_set_restart(e, rs.Ntmp, rs.max_nan_restarts), "restart")) return EXIT_FAILURE;
-    if (rs.N_stat >= 0) std::fprintf(stderr, "note: drift_max statistics (N_stat) are not written by this driver\n");
+    if (!resume.empty()) { // after fts_set_restart, which zeroes the restart count the state carries
+        if (!ok_or_report(fts_load_state(e, resume.c_str()), "resume")) return EXIT_FAILURE;
+        fts_get_info(e, &info);
+        if (screenprint) std::printf("resuming %s at step %lld\n", resume.c_str(), info.step);
+    }
+    // drift_max.dat (complexLangevin.cu:381-408, 466-516): rows for the steps after N_stat
+    FILE* drift_fp = nullptr;
+    std::vector<std::string> drift_labels;
+    for (int a = 0; a < info.n_omega; ++a) drift_labels.push_back("u_omega" + std::to_string(a) + "_max");
+    drift_labels.push_back("u_psi_max");
+    const std::string drift_file = rs.N_stat >= 0 ? out_path(file("drift_max_file")) : std::string();
+    auto open_drift = [&]() {
+        if (drift_fp) std::fclose(drift_fp);
+        drift_fp = std::fopen(drift_file.c_str(), "w");
+        if (!drift_fp) return err("cannot open " + drift_file);
+        write_labels(drift_fp, drift_labels, "step");
+        std::fflush(drift_fp);
+        return true;
+    };
+    if (!drift_file.empty()) {
+        if (!open_drift() || !ok_or_report(fts_set_drift_stats(e, 1), "drift statistics")) return EXIT_FAILURE;
+    }
 
     // observables.dat (complexLangevin.cu:370-470)
     Out obs;
@@ -382,6 +408,7 @@ int main(int argc, char** argv)
         std::fflush(obs.fp);
     }
     const std::string tmp_omega = out_path(file("tmp_omega_file")), tmp_psi = out_path(file("tmp_psi_file"));
+    const std::string state_file = out_path("checkpoint.state");
     auto checkpoint_files = [&]() {
         std::vector<double> w(2 * M * static_cast<size_t>(info.n_omega)), p(2 * M);
         if (info.n_omega > 0 && !ok_or_report(fts_get_field(e, FTS_FIELD_OMEGA, w.data(), w.size()), "checkpoint")) return false;
@@ -390,7 +417,8 @@ int main(int argc, char** argv)
         if (!ok_or_report(fts_rf_write(nw.c_str(), m, L, info.n_omega, w.data()), "checkpoint") ||
             !ok_or_report(fts_rf_write(np.c_str(), m, L, 1, p.data()), "checkpoint"))
             return false;
-        return (std::rename(nw.c_str(), tmp_omega.c_str()) == 0 && std::rename(np.c_str(), tmp_psi.c_str()) == 0) || err("cannot replace checkpoint files");
+        if (std::rename(nw.c_str(), tmp_omega.c_str()) != 0 || std::rename(np.c_str(), tmp_psi.c_str()) != 0) return err("cannot replace checkpoint files");
+        return ok_or_report(fts_save_state(e, state_file.c_str()), "checkpoint state");
     };
     if (rs.Ntmp > 0 && !checkpoint_files()) return EXIT_FAILURE;
     const char* step_keys[8] = {"step_omega_file", "step_psi_file", "step_rho_type_file", "step_rhop_file",
@@ -406,8 +434,8 @@ int main(int argc, char** argv)
 
     const int record_every = std::max(1, rs.Nss);
     std::vector<fts_observables> recs;
-    long long step = 0;
-    int restarts_seen = 0;
+    long long step = info.step; // 0, or the resumed state's step
+    int restarts_seen = info.restarts;
     while (step < rs.Nt) {
         long long n = std::min<long long>(rs.Nt - step, 1000);
         if (rs.Ntmp > 0) n = std::min<long long>(n, rs.Ntmp - step % rs.Ntmp);
@@ -416,10 +444,31 @@ int main(int argc, char** argv)
         int nrec = 0;
         const int rc = fts_step(e, static_cast<int>(n), record_every, recs.data(), static_cast<int>(recs.size()), &nrec);
         fts_get_info(e, &info);
-        if (info.restarts != restarts_seen && obs.fp) {
+        const bool restarted = info.restarts != restarts_seen;
+        if (restarted && obs.fp)
             std::fprintf(obs.fp, "# restarted after a non-finite value from the checkpoint (restart %d); records after the checkpoint discarded\n", info.restarts);
-            restarts_seen = info.restarts;
+        if (drift_fp) {
+            if (restarted) { // the reference starts the file again (reset_drift_statistics_after_restart)
+                if (!open_drift()) return EXIT_FAILURE;
+                std::fprintf(drift_fp, "# restarted after non-finite value at step %lld from checkpoint step %lld; previous drift statistics discarded\n",
+                             info.failed_step, info.checkpoint_step);
+            }
+            const size_t w = drift_labels.size() + 1; // step, then the values
+            const int cap = static_cast<int>(n) + 1;
+            std::vector<double> rows(static_cast<size_t>(cap) * w);
+            int nr = 0;
+            if (!ok_or_report(fts_get_drift_rows(e, rows.data(), cap, &nr), "drift statistics")) return EXIT_FAILURE;
+            for (int r = 0; r < nr; ++r) {
+                const double* row = rows.data() + static_cast<size_t>(r) * w;
+                const long long st = static_cast<long long>(row[0]);
+                if (st <= rs.N_stat || (restarted && st <= info.checkpoint_step)) continue;
+                for (size_t c = 1; c < w; ++c)
+                    if (!std::isfinite(row[c])) return err("non-finite " + drift_labels[c - 1] + " at step " + std::to_string(st)), EXIT_FAILURE;
+                write_row(drift_fp, drift_labels, std::to_string(st), std::vector<double>(row + 1, row + w));
+            }
+            std::fflush(drift_fp);
         }
+        restarts_seen = info.restarts;
         for (int k = 0; k < nrec; ++k) {
             const std::vector<double> v = values_of(recs[static_cast<size_t>(k)], sys.stress_mask);
             if (obs.fp) write_row(obs.fp, obs.labels, std::to_string(recs[static_cast<size_t>(k)].step), v);
@@ -446,6 +495,7 @@ int main(int argc, char** argv)
             if (!write_fields(e, info, deck, m, L, rs.legacy_ab != 0, nm(0), nm(1), nm(2), nm(3), nm(4), nm(5), nm(6), nm(7))) return EXIT_FAILURE;
         }
     }
+    if (drift_fp) std::fclose(drift_fp);
     if (obs.fp) { // ensemble average (complexLangevin.cu:443-470)
         std::fprintf(obs.fp, "\n# ensemble_average Nss=%d Ns=%d count=%lld\n", rs.Nss, rs.Ns, obs.count);
         if (obs.count == 0) std::fprintf(obs.fp, "# no_observables\n");

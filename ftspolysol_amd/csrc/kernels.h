@@ -381,6 +381,64 @@ __global__ void k_omega_increment(ModeCoefs mc, const double2* __restrict__ rho,
     }
 }
 
+// Drift statistics (complexLangevin.cu:318-324; common.h:725-751, 1020-1050): per-block
+// max over sites of |s|^2, s = u0 + u(om, rho) of mode l (u0 the stored predictor drift, or
+// none), a non-finite |s|^2 counting as +inf.  Maxima do not depend on the order of the sites.
+__global__ void __launch_bounds__(RED_THREADS) k_mode_drift_max2(ModeCoefs mc, int l, const double2* __restrict__ u0, const double2* __restrict__ om,
+                                                                 const double2* __restrict__ rho, int M, double2* __restrict__ partials)
+{
+    __shared__ double sh[RED_THREADS];
+    double m = 0.0;
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < M; i += RED_BLOCKS * RED_THREADS) {
+        double2 s = mode_drift(mc, l, om, rho, M, i);
+        if (u0) s = cadd(u0[static_cast<size_t>(l) * M + i], s);
+        double v = s.x * s.x + s.y * s.y;
+        if (!isfinite(v)) v = INFINITY;
+        m = fmax(m, v);
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = d2(sh[0], 0.0);
+}
+
+// the same for s = a + b (b may be null): the psi drifts
+__global__ void __launch_bounds__(RED_THREADS) k_sum_max2(const double2* __restrict__ a, const double2* __restrict__ b, int M,
+                                                          double2* __restrict__ partials)
+{
+    __shared__ double sh[RED_THREADS];
+    double m = 0.0;
+    for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < M; i += RED_BLOCKS * RED_THREADS) {
+        const double2 s = b ? cadd(a[i], b[i]) : a[i];
+        double v = s.x * s.x + s.y * s.y;
+        if (!isfinite(v)) v = INFINITY;
+        m = fmax(m, v);
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = d2(sh[0], 0.0);
+}
+
+// out->x = |scale| sqrt(max over the RED_BLOCKS partials)
+__global__ void k_finalize_max(const double2* __restrict__ partials, double scale, double2* __restrict__ out)
+{
+    __shared__ double sh[RED_THREADS];
+    sh[threadIdx.x] = partials[threadIdx.x].x;
+    __syncthreads();
+    for (int off = RED_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = d2(fabs(scale) * sqrt(fmax(0.0, sh[0])), 0.0);
+}
+
 // u of interaction mode 0 (the force the learned kernel is judged against)
 __global__ void k_mode0_drift(ModeCoefs mc, const double2* __restrict__ om, const double2* __restrict__ rho, int M, double2* __restrict__ u)
 {

@@ -182,6 +182,8 @@ typedef struct fts_info {
     int mirrored_species; /* bit p set: species p runs the mirrored (forward-only) sweep */
     int restarts;         /* non-finite restarts taken (fts_set_restart) */
     int noise_epoch;      /* Philox key in use: seed + noise_epoch * 0x9E3779B97F4A7C15 */
+    long long checkpoint_step; /* step counter of the device checkpoint, -1 if none */
+    long long failed_step;     /* label of the last non-finite step a restart recovered from, -1 if none */
 } fts_info;
 
 /* Accumulated device timing of the propagator sweeps (HIP events on the engine's
@@ -232,6 +234,18 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
 /* Noise counter: the next step draws Philox counter `step`. */
 int fts_set_step(fts_engine* h, long long step);
 
+/* Binary state (no reference counterpart: the reference's checkpoint is the tmp_omega /
+ * tmp_psi .rf pair written every Ntmp steps with %.10f, complexLangevin.cu:518-556).
+ * The file holds the fields exactly with the step counter and the noise key, so a run
+ * resumed from it repeats the uninterrupted run bit for bit (FTS_SCHEME_SI_LEARNED:
+ * the first resumed step is not judged, as after fts_set_fields).  Layout, host byte
+ * order: "FTSSTAT1", int32 m1 m2 m3 n_omega, int64 step, uint64 seed, int32 noise_epoch,
+ * int32 restarts, omega [n_omega][M] and psi [M] as interleaved doubles.  Saving writes
+ * path.new and renames it into place; loading checks the mesh and mode count and takes
+ * the file's seed and noise epoch as the engine's. */
+int fts_save_state(fts_engine* h, const char* path);
+int fts_load_state(fts_engine* h, const char* path);
+
 /* Checkpoint and non-finite restart (the reference's Ntmp checkpoint and
  * max_nan_restarts, complexLangevin.cu:518-667).  With checkpoint_every > 0 the
  * engine keeps a device copy of (w, psi) taken at the first step of every
@@ -246,6 +260,18 @@ int fts_set_restart(fts_engine* h, int checkpoint_every, int max_restarts);
  * counter to `step` ends with a NaN in psi, once, so that the failure path and the
  * restart can be tested without a physical blow-up. */
 int fts_inject_nonfinite(fts_engine* h, long long step);
+
+/* Drift statistics (complexLangevin.cu:318-324, 466-497, the rows of drift_max.dat).
+ * While enabled, every step fts_step takes appends one row of n_omega + 2 doubles:
+ * the step label, then per interaction mode max over sites |lambda_l / 2 (u_l + u_bar_l)|
+ * and last max |lambda_psi / 2 (u_psi + u_psi_bar)| (predictor-corrector: the drifts at
+ * (w, psi) and at the predicted fields, as corrector_drift_max; a non-finite value is
+ * +inf).  EM and semi-implicit steps report max |lambda u| at the state the step starts
+ * from.  A restart from the checkpoint drops the rows after it.  fts_set_drift_stats
+ * clears the rows; fts_get_drift_rows moves up to `capacity` of them, oldest first, into
+ * out[capacity][n_omega + 2].  Replica batches do not collect them. */
+int fts_set_drift_stats(fts_engine* h, int enabled);
+int fts_get_drift_rows(fts_engine* h, double* out, int capacity, int* n_rows);
 
 /* Test hooks on the hot path. */
 /* 3D DFT of one field on the engine's FFT, unnormalised; sign -1 forward (e^{-ikx}), +1 inverse
@@ -304,7 +330,7 @@ int fts_batch_set_graph(fts_batch* b, int enabled);
  * r mod n), so the small launches of several replicas share the GPU.  n = 0 runs
  * it all on the group's stream.  Default min(group size, FTS_BATCH_DEFAULT_STREAMS);
  * the results do not depend on n. */
-#define FTS_BATCH_DEFAULT_STREAMS 4
+#define FTS_BATCH_DEFAULT_STREAMS 8
 #define FTS_BATCH_MAX_STREAMS 16
 int fts_batch_set_streams(fts_batch* b, int n);
 /* Device timing of the batched sweeps of one group (fts_timing as for an engine). */

@@ -29,6 +29,7 @@
 #include "philox_host.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <deque>
 #include <cmath>
 #include <limits>
@@ -130,6 +131,7 @@ struct fts_engine {
     // checkpoint and non-finite restart (complexLangevin.cu:518-667); the restart
     // moves the noise to the key seed0 + epoch * 0x9E3779B97F4A7C15
     int ckpt_every = 0, max_restarts = 0, restarts = 0, noise_epoch = 0;
+    long long last_failed_step = -1;
     long long ckpt_step = -1, inject_step = -1;
     unsigned long long seed0 = 0;
     std::vector<Cx> ckpt_omega, ckpt_psi;
@@ -174,6 +176,32 @@ struct fts_engine {
     void lrk_step(std::vector<double> (&mult)[2]);
 
     oracle::Fft3d* fft = nullptr;
+
+    // drift statistics (complexLangevin.cu:318-324, 466-497): values of the step being
+    // taken (modes, then psi) and the rows [step, values] of the steps taken
+    bool drift_stats = false;
+    std::vector<double> drift_cur, drift_rows;
+    // corrector_drift_max: |scale| sqrt(max_i |a_i + b_i|^2), a non-finite |.|^2 counting as +inf
+    double drift_max(const Cx* a, const Cx* b, double scale) const
+    {
+        double m = 0.0;
+        for (int i = 0; i < M; ++i) {
+            const Cx z = b ? Cx{a[i].x + b[i].x, a[i].y + b[i].y} : a[i];
+            double v = z.x * z.x + z.y * z.y;
+            if (!std::isfinite(v)) v = INFINITY;
+            m = std::max(m, v);
+        }
+        return std::fabs(scale) * std::sqrt(std::max(0.0, m));
+    }
+    void drift_statistics(bool pc)
+    {
+        if (!drift_stats) return;
+        drift_cur.assign(static_cast<size_t>(n_omega) + 1, 0.0);
+        const double f = pc ? 0.5 : 1.0;
+        for (int l = 0; l < n_omega; ++l)
+            drift_cur[static_cast<size_t>(l)] = drift_max(fld(drift, l), pc ? fld(drift_bar, l) : nullptr, f * lambda_omega[static_cast<size_t>(l)]);
+        drift_cur[static_cast<size_t>(n_omega)] = drift_max(dpsi.data(), pc ? dpsi_bar.data() : nullptr, f * lambda_psi);
+    }
 
     // ---- state (deviceState.h:5-53) ----------------------------------------
     Field omega, omega_bar, omega_s, drift, drift_bar;
@@ -1227,6 +1255,7 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
     compute_psi_drift(psi_bar.data(), rho_c.data(), dpsi_bar.data());
     if (record) measure(Qp, omega_bar.data(), psi_bar.data(), rec);
     compute_drifts(omega_bar.data(), drift_bar.data());
+    drift_statistics(true); // write_drift_max before corrector_step (complexLangevin.cu:690-692)
     // corrector_step (complexLangevin.cu:349-368)
     for (int l = 0; l < n_omega; ++l) {
         Cx* w = fld(omega, l);
@@ -1262,6 +1291,7 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     generate_noises(step_counter);
     compute_drifts(omega.data(), drift.data());
     compute_psi_drift(psi.data(), rho_c.data(), dpsi.data());
+    drift_statistics(false);
     std::vector<double> mult[2];
     if (semi_implicit && lrk_active()) lrk_step(mult);
     std::vector<double> P(static_cast<size_t>(M));
@@ -1455,6 +1485,8 @@ int fto_get_info(fts_engine* h, fts_info* o)
     o->step = h->step_counter;
     o->restarts = h->restarts;
     o->noise_epoch = h->noise_epoch;
+    o->checkpoint_step = h->ckpt_step;
+    o->failed_step = h->last_failed_step;
     o->n_omega = h->n_omega;
     o->monomer_types = h->nT;
     o->n_species = static_cast<int>(h->species.size());
@@ -1569,13 +1601,21 @@ int fto_step(fts_engine* h, int n, int record_every, fts_observables* records, i
             rec->step = step;
             ++nrec;
         }
+        if (rc == FTS_OK && h->drift_stats) {
+            h->drift_rows.push_back(static_cast<double>(step));
+            h->drift_rows.insert(h->drift_rows.end(), h->drift_cur.begin(), h->drift_cur.end());
+        }
         if (rc == FTS_NONFINITE && h->ckpt_every > 0) {
             if (h->ckpt_step < 0) return rc;
+            const size_t w = static_cast<size_t>(h->n_omega) + 2;
+            while (!h->drift_rows.empty() && h->drift_rows[h->drift_rows.size() - w] > static_cast<double>(h->ckpt_step))
+                h->drift_rows.resize(h->drift_rows.size() - w);
             if (h->restarts >= h->max_restarts) return fail(FTS_NONFINITE, "maximum non-finite restarts exceeded");
             h->omega = h->ckpt_omega;
             h->psi = h->ckpt_psi;
             ++h->restarts;
             ++h->noise_epoch;
+            h->last_failed_step = step;
             h->seed = h->seed0 + static_cast<unsigned long long>(h->noise_epoch) * 0x9E3779B97F4A7C15ull;
             h->step_counter = h->ckpt_step;
             h->lrk_have_prev = false;
@@ -1595,6 +1635,80 @@ int fto_set_restart(fts_engine* h, int checkpoint_every, int max_restarts)
     h->max_restarts = max_restarts;
     h->restarts = 0;
     h->ckpt_step = -1;
+    return FTS_OK;
+}
+
+// Binary state file, the engine's format: "FTSSTAT1", int32 m1 m2 m3 n_omega, int64 step,
+// uint64 seed0, int32 noise_epoch restarts, omega [n_omega][M] and psi [M] (interleaved doubles)
+int fto_save_state(fts_engine* h, const char* path)
+{
+    if (!h || !path) return fail(FTS_INVALID, "null argument");
+    const std::string tmp = std::string(path) + ".new";
+    FILE* fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) return fail(FTS_INVALID, "cannot open " + tmp);
+    const int32_t hdr[4] = {h->m1, h->m2, h->m3, h->n_omega};
+    const int64_t step = h->step_counter;
+    const uint64_t seed0 = h->seed0;
+    const int32_t epoch[2] = {h->noise_epoch, h->restarts};
+    const size_t no = static_cast<size_t>(h->n_omega) * static_cast<size_t>(h->M);
+    bool ok = std::fwrite("FTSSTAT1", 1, 8, fp) == 8 && std::fwrite(hdr, sizeof(hdr), 1, fp) == 1 && std::fwrite(&step, sizeof(step), 1, fp) == 1 &&
+              std::fwrite(&seed0, sizeof(seed0), 1, fp) == 1 && std::fwrite(epoch, sizeof(epoch), 1, fp) == 1 &&
+              (no == 0 || std::fwrite(h->omega.data(), sizeof(Cx), no, fp) == no) &&
+              std::fwrite(h->psi.data(), sizeof(Cx), static_cast<size_t>(h->M), fp) == static_cast<size_t>(h->M);
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) return fail(FTS_INVALID, std::string("cannot write ") + path);
+    return FTS_OK;
+}
+
+int fto_load_state(fts_engine* h, const char* path)
+{
+    if (!h || !path) return fail(FTS_INVALID, "null argument");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(FTS_INVALID, std::string("cannot open ") + path);
+    char magic[8];
+    int32_t hdr[4];
+    int64_t step = 0;
+    uint64_t seed0 = 0;
+    int32_t epoch[2];
+    bool ok = std::fread(magic, 1, 8, fp) == 8 && std::memcmp(magic, "FTSSTAT1", 8) == 0 && std::fread(hdr, sizeof(hdr), 1, fp) == 1 &&
+              std::fread(&step, sizeof(step), 1, fp) == 1 && std::fread(&seed0, sizeof(seed0), 1, fp) == 1 && std::fread(epoch, sizeof(epoch), 1, fp) == 1;
+    if (!ok || hdr[0] != h->m1 || hdr[1] != h->m2 || hdr[2] != h->m3 || hdr[3] != h->n_omega || step < 0 || epoch[0] < 0 || epoch[1] < 0) {
+        std::fclose(fp);
+        return fail(FTS_INVALID, std::string(path) + ": not a state file of this mesh and mode count");
+    }
+    const size_t no = static_cast<size_t>(h->n_omega) * static_cast<size_t>(h->M);
+    std::vector<Cx> w(no), p(static_cast<size_t>(h->M));
+    ok = (no == 0 || std::fread(w.data(), sizeof(Cx), no, fp) == no) && std::fread(p.data(), sizeof(Cx), p.size(), fp) == p.size();
+    std::fclose(fp);
+    if (!ok) return fail(FTS_INVALID, std::string(path) + ": truncated state file");
+    std::copy(w.begin(), w.end(), h->omega.begin());
+    h->psi = p;
+    h->step_counter = step;
+    h->seed0 = seed0;
+    h->noise_epoch = epoch[0];
+    h->restarts = epoch[1];
+    h->seed = seed0 + static_cast<unsigned long long>(epoch[0]) * 0x9E3779B97F4A7C15ull;
+    h->ckpt_step = -1;
+    h->lrk_have_prev = false;
+    return FTS_OK;
+}
+
+int fto_set_drift_stats(fts_engine* h, int enabled)
+{
+    if (!h) return fail(FTS_INVALID, "null engine");
+    h->drift_stats = enabled != 0;
+    h->drift_rows.clear();
+    return FTS_OK;
+}
+
+int fto_get_drift_rows(fts_engine* h, double* out, int capacity, int* n_rows)
+{
+    if (!h || !n_rows || capacity < 0 || (capacity > 0 && !out)) return fail(FTS_INVALID, "bad argument");
+    const size_t w = static_cast<size_t>(h->n_omega) + 2;
+    const size_t n = std::min(h->drift_rows.size() / w, static_cast<size_t>(capacity));
+    if (n) std::memcpy(out, h->drift_rows.data(), sizeof(double) * n * w);
+    h->drift_rows.erase(h->drift_rows.begin(), h->drift_rows.begin() + static_cast<std::ptrdiff_t>(n * w));
+    *n_rows = static_cast<int>(n);
     return FTS_OK;
 }
 

@@ -24,6 +24,10 @@ CLI = os.path.join(ROOT, "ftspolysol_amd", "fts_run")
 
 def test_cli_run_matches_oracle(tmp_path):
     ref, deck_path = c1_like(tmp_path)
+    # drift statistics after step 4 (complexLangevin.cu:466-497)
+    text = open(deck_path).read().replace("Ntmp = 10\n", "Ntmp = 10\nN_stat = 4\n")
+    text = text.replace('[output]\n', '[output]\ndrift_max_file = "drift.dat"\n')
+    open(deck_path, "w").write(text)
     print("[cli] running fts_run on", deck_path, flush=True)
     r = subprocess.run([CLI, "-p", deck_path, "-i", "run1", "-s", str(tmp_path / "runs")], cwd=str(tmp_path),
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
@@ -52,6 +56,16 @@ def test_cli_run_matches_oracle(tmp_path):
     H = np.array([x["H"].real for x in recs])
     np.testing.assert_allclose(vals[:, iH], H, rtol=2e-10)
     np.testing.assert_allclose(vals[:, labels.index("pressure_iso") - 1], [x["p_iso"].real for x in recs], rtol=2e-10)
+    # drift_max.dat: header, then one row per step after N_stat, against the oracle's rows
+    dl = (run / "drift.dat").read_text().splitlines()
+    assert dl[0].lstrip("#").split() == ["step", "u_omega0_max", "u_psi_max"]
+    drows = np.array([[float(v) for v in ln.split()] for ln in dl[1:]])
+    assert list(drows[:, 0]) == list(range(5, 21))
+    o2 = oracle_engine(deck.system)
+    o2.set_drift_stats(True)
+    o2.step(20)
+    orows = o2.get_drift_rows()
+    np.testing.assert_allclose(drows[:, 1:], orows[4:, 1:], rtol=2e-9)
     # final fields and checkpoint files (mesh headers, %.10f rows)
     lib = load_hip_library()
     w = rf_read(lib, str(run / "omega.rf"))
@@ -61,3 +75,33 @@ def test_cli_run_matches_oracle(tmp_path):
     for name in ("psi.rf", "rho_type.rf", "rhop.rf", "rhoc.rf", "rhoc_poly.rf", "rhoc_ion.rf", "rho_ion_0_counterion.rf",
                  "rho_ion_1_salt_anion.rf"):
         assert (run / name).exists(), name
+
+
+def test_cli_resume_repeats_the_uninterrupted_run(tmp_path):
+    """Nt = 20 in one run, against Nt = 10 and a second run resumed from its
+    checkpoint.state: the observables rows of steps 12-20 are the same text."""
+    ref, deck_path = c1_like(tmp_path)
+    full = subprocess.run([CLI, "-p", deck_path, "-i", "full", "-s", str(tmp_path / "runs")], cwd=str(tmp_path),
+                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert full.returncode == 0, full.stdout
+    half_deck = tmp_path / "half.param"
+    half_deck.write_text(open(deck_path).read().replace("Nt = 20\n", "Nt = 10\n"))
+    half = subprocess.run([CLI, "-p", str(half_deck), "-i", "half", "-s", str(tmp_path / "runs")], cwd=str(tmp_path),
+                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert half.returncode == 0, half.stdout
+    state = tmp_path / "runs" / "half" / "checkpoint.state"
+    assert state.exists()
+    print("[cli] resuming from", state, flush=True)
+    rest = subprocess.run([CLI, "-p", deck_path, "-i", "rest", "-s", str(tmp_path / "runs"), "--resume", str(state)],
+                          cwd=str(tmp_path), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert rest.returncode == 0, rest.stdout
+
+    def rows(run):
+        lines = (tmp_path / "runs" / run / "obs # not a comment.dat").read_text().splitlines()
+        return {ln.split()[0]: ln for ln in lines if ln.split() and ln.split()[0].isdigit()}
+
+    a, b = rows("full"), rows("rest")
+    assert sorted(b, key=int) == [str(k) for k in range(12, 21, 2)]
+    for k in b:
+        assert a[k] == b[k], k
+    assert (tmp_path / "runs" / "full" / "omega.rf").read_text() == (tmp_path / "runs" / "rest" / "omega.rf").read_text()
