@@ -36,7 +36,9 @@ is the standard error).  Decorrelated samples/s = (replica CL steps/s) /
 (2 tau_int), tau_int by Sokal's automatic window (c = 6) on the replica-averaged
 normalised autocorrelation of Re H.  `value` is the fast batch's whole-GPU rate.
 
-roofline: the dominant kernels are the propagator bead-step passes (one pencil
+roofline: `copy_achievable` is a 1 GiB device-to-device hipMemcpy timed on the same GPU
+(the bandwidth a streaming kernel can reach); `traffic_rate` the measured HBM traffic
+of a launch group over its duration.  The dominant kernels are the propagator bead-step passes (one pencil
 pass and one plane pass per bead index, or the four line passes, every replica's
 job in each launch).  Algorithmic bytes per bead step = read q_s + write q_{s+1}
 = 2 c M (c = 8 B for complex FP32), summed over the jobs of a launch group and
@@ -140,13 +142,52 @@ def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50, lrk
     H = np.array(H)
     tau, window = tau_int(H.real)
     tau_each = [tau_int(H.real[r:r + 1])[0] for r in range(R)]
+    # Sokal's estimate of the statistical error of tau_int: var = 2 (2W + 1) / n tau^2,
+    # n the number of samples pooled
+    tau_err = float(tau * np.sqrt(2.0 * (2 * window + 1) / H.size))
     sps = R * steps / elapsed
     stderr_H = float(H.real.std(ddof=1) * np.sqrt(2 * tau / H.size))
     return dict(replicas=R, replica_steps_per_s=sps, ms_per_step=1e3 * elapsed / steps, tau_int=tau, window=window,
-                tau_int_each=[round(float(x), 3) for x in tau_each],
+                tau_int_each=[round(float(x), 3) for x in tau_each], tau_int_err=tau_err,
                 samples_per_s=sps / (2 * tau), H_mean=float(H.real.mean()), H_imag_mean=float(H.imag.mean()),
                 H_stderr=stderr_H, H0=H[0].real, timing=timing, elapsed=elapsed, schedule=sched, tune_ms=tune,
                 restarts=int(sum(restarts)), final=final, lrk=lrk_stats)
+
+
+def copy_bandwidth(gib=1.0, reps=10):
+    """Achievable HBM bandwidth on this GPU: a device-to-device hipMemcpy of `gib` GiB
+    (read + write bytes over the copy time, HIP events), through the HIP runtime the
+    engine library already loaded (a second runtime in the process does not see the GPU)."""
+    hip = ctypes.CDLL("libamdhip64.so.7" if os.path.exists("/opt/rocm/lib/libamdhip64.so.7") else "libamdhip64.so")
+    nbytes = int(gib * (1 << 30))
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    ms = ctypes.c_float(0.0)
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc})")
+    ok(hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)), "hipMalloc")
+    try:
+        ok(hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)), "hipMalloc")
+        ok(hip.hipMemset(a, 0, ctypes.c_size_t(nbytes)), "hipMemset")
+        ok(hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3), "hipMemcpy")  # 3 = device to device
+        ok(hip.hipEventCreate(ctypes.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(ctypes.byref(e1)), "hipEventCreate")
+        ok(hip.hipEventRecord(e0, None), "hipEventRecord")
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e1, None), "hipEventRecord")
+        ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1), "hipEventElapsedTime")
+    finally:
+        for ev in (e0, e1):
+            if ev.value:
+                hip.hipEventDestroy(ev)
+        for ptr in (a, b):
+            if ptr.value:
+                hip.hipFree(ptr)
+    return 2.0 * nbytes / (ms.value / reps * 1e-3) / 1e9
 
 
 def same_state_errors(wl, fields):
@@ -233,7 +274,7 @@ def main():
                    for r in range(args.tau_replicas)]
         st = run_batch(tau_sys, args.warmup, args.steps, f"plain fp64 pc statistics x{args.tau_replicas}")
         plain["tau_int_timed_replicas"] = plain["tau_int"]
-        for k in ("tau_int", "window", "tau_int_each", "H_mean", "H_stderr"):
+        for k in ("tau_int", "window", "tau_int_each", "tau_int_err", "H_mean", "H_stderr"):
             plain[k] = st[k]
         plain["samples_per_s"] = plain["replica_steps_per_s"] / (2 * plain["tau_int"])
         plain["tau_source"] = f"pooled autocorrelation of {args.tau_replicas} FP64 replicas"
@@ -256,10 +297,15 @@ def main():
         if t.get("replicas") == args.replicas and t.get("schedule") == fast["schedule"] and t.get("precision") == "fp32":
             traffic = t["traffic_bytes"]
     _, _, plain_achieved = roof(plain["timing"])
+    try:
+        copy_gbs = copy_bandwidth()
+    except Exception as exc:  # report the peak fraction only
+        print(f"[bench] copy bandwidth not measured: {exc}", file=sys.stderr)
+        copy_gbs = None
     H_err_vs_plain = abs(fast["H_mean"] - plain["H_mean"]) / abs(plain["H_mean"])
     H_err_sigma = np.hypot(fast["H_stderr"], plain["H_stderr"]) / abs(plain["H_mean"])
-    keys = ("replicas", "replica_steps_per_s", "ms_per_step", "tau_int", "tau_int_each", "samples_per_s", "H_mean",
-            "H_stderr", "schedule", "tune_ms", "restarts")
+    keys = ("replicas", "replica_steps_per_s", "ms_per_step", "tau_int", "tau_int_err", "tau_int_each", "samples_per_s",
+            "H_mean", "H_stderr", "schedule", "tune_ms", "restarts")
     sched_name = {1: "plane", 2: "line"}
     out = {
         "metric": "decorrelated CL samples/s at 96^3 N=100, FP32+learned vs plain FP64, <H> err",
@@ -287,6 +333,7 @@ def main():
                            **({"tau_int_timed_replicas": plain["tau_int_timed_replicas"]} if "tau_int_timed_replicas" in plain else {})),
         "learned_kernel": fast["lrk"],
         "speedup_vs_plain_fp64": fast["samples_per_s"] / plain["samples_per_s"],
+        "speedup_rel_err": float(np.hypot(fast["tau_int_err"] / fast["tau_int"], plain["tau_int_err"] / plain["tau_int"])),
         "fp32_pc_speedup_vs_plain_fp64": None if fp32_pc is None else fp32_pc["samples_per_s"] / plain["samples_per_s"],
         "H_err_same_state": float(same["H"]),
         "same_state_rel_err": {k: float(v) for k, v in same.items() if k != "H_fp64"},
@@ -297,7 +344,10 @@ def main():
                      "kernel": f"bead-step launch group ({sched_name.get(fast['schedule'], '?')} schedule), FP32, "
                                f"{args.replicas} replica jobs per launch",
                      "us_per_launch_group": us, "alg_bytes_per_launch_group": byt,
-                     "plain_fp64_achieved": plain_achieved},
+                     "plain_fp64_achieved": plain_achieved,
+                     "copy_achievable": copy_gbs,
+                     "frac_of_copy": None if not copy_gbs else achieved / copy_gbs,
+                     "traffic_rate": None if traffic is None else traffic / (us * 1e-6) / 1e9},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(plain_sys[0], args.cpu_steps, plain["tau_int"], "PC")
