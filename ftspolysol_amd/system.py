@@ -121,15 +121,20 @@ CONFIGS = {
 
 
 def chemistry(n, N, rho, salt, B, lB, charge, z, dL=0.5):
-    """Derived counts for an n^3 box: returns (n_chains, num_salt, rho0, xi, charges)."""
+    """Derived counts for an n^3 box: returns (n_chains, num_salt, rho0, xi, charges).
+    n_chains = round(rho V / N), raised by one where the chains' total charge would
+    not be a whole number (auto-neutralising counterions need an integer count,
+    ion.cu:11-19): half charges on an odd number of beads in an odd number of chains."""
     V = (dL * n) ** 3
     n_chains = int(round(rho * V / N))
     num_salt = int(round(salt * V))
-    rho0 = n_chains * N / V
     if charge == "uniform":
         charges = np.full(N, z)
     else:  # charge z on every second bead (odd index)
         charges = np.where(np.arange(N) % 2 == 1, z, 0.0)
+    if lB > 0 and abs(n_chains * charges.sum() - round(n_chains * charges.sum())) > 1e-9:
+        n_chains += 1
+    rho0 = n_chains * N / V
     return n_chains, num_salt, rho0, B * rho0, charges
 
 
