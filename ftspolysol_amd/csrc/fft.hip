@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     // block gets the same value whatever the order; read by all threads at once (one
     // thread walking the m1 norms cost 11 us per 8-job launch at 96^3)
     T c = T(1);
-    if constexpr (PRODUCES_Q) {
+    auto renorm = [&]() {
         if (job.norm_in) {
             double m = 0.0;
             for (int x = tid; x < gridDim.x; x += NT) m = fmax(m, job.norm_in[x]);
@@ -66,11 +66,14 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         } else if (job.log_scale && blockIdx.x == 0 && tid == 0) {
             *job.log_scale = 0.0;
         }
-    }
+    };
+    // INIT scales the plane as it loads it; the bead step needs the factor only after
+    // the inverse transform, so it loads first and reads the norms while the loads fly
+    if constexpr (MODE == PLANE_INIT) renorm();
     double nrm = 0.0;
 
     const V* src = (MODE == PLANE_INIT) ? job.emw : job.src;
-#pragma unroll 4
+#pragma unroll
     for (int i = tid; i < N * N; i += NT) {
         V v = src[plane + i];
         if constexpr (MODE == PLANE_INIT) {
@@ -80,6 +83,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         }
         lds[(i / N) * LS + (i % N)] = v;
     }
+    if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST) renorm();
     __syncthreads();
 
     if constexpr (MODE == PLANE_INV || MODE == PLANE_STEP || MODE == PLANE_LAST) {
