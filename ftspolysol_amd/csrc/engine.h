@@ -150,6 +150,13 @@ struct fts_engine {
     DevBuf<double2> omega, omega_bar, omega_s, drift, psi, psi_bar, psi_s, dpsi, dpsi_bar;
     DevBuf<double2> W_type_s, rho_hat, rho_type, rho_c_hat, rho_c, rho_c_poly, rho_c_ion, rho_p;
     DevBuf<double2> ion_psi, ion_b, ion_rho, t1, t2, t3, mode_coef;
+    // spectrum of the psi field the last densities_begin smoothed (plane schedule,
+    // electrostatics): the ion kernels, the psi drift, the learned kernel and the
+    // measurement start from it instead of transforming psi again.  psi_hat_of names
+    // the field it belongs to and is cleared wherever psi changes.
+    DevBuf<double2> psi_hat;
+    const double2* psi_hat_of = nullptr;
+    bool psi_hat_valid(const double2* ps) const { return psi_hat_of != nullptr && psi_hat_of == ps; }
     DevBuf<double2> partials, scal;
     DevBuf<int> flag;
     // propagators (double or float complex)
@@ -189,6 +196,7 @@ struct fts_engine {
     // primitives
     KOp kop_sep(const double* sep, double c0) const { return KOp{sep, nullptr, c0, 0.0, 0.0, 0, 0}; }
     int fft_apply(const double2* src, double2* dst, const KOp& op);
+    int fft_apply_hat(const double2* hat, double2* dst, const KOp& op); // dst = IFFT[K hat] (plane schedule)
     int fft_forward(const double2* src, double2* dst);
     template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst);
     int reduce_finalize(int K, int slot, double scale);

@@ -35,7 +35,7 @@ void fts_engine::release()
     tw32_z.release();
     for (DevBuf<double2>* b : {&omega, &omega_bar, &omega_s, &drift, &psi, &psi_bar, &psi_s, &dpsi, &dpsi_bar, &W_type_s, &rho_hat,
                                &rho_type, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &ion_psi, &ion_b, &ion_rho, &t1, &t2,
-                               &t3, &mode_coef, &partials, &scal})
+                               &t3, &psi_hat, &mode_coef, &partials, &scal})
         b->release();
     flag.release();
     ckpt_omega.release();
@@ -363,6 +363,7 @@ int fts_engine::setup_device()
     for (DevBuf<double2>* b : {&omega, &omega_bar, &omega_s, &drift}) HIP_TRY(b->alloc(std::max<size_t>(1, nO) * Ms));
     for (DevBuf<double2>* b : {&psi, &psi_bar, &psi_s, &dpsi, &dpsi_bar, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &t1, &t2, &t3})
         HIP_TRY(b->alloc(Ms));
+    if (electro() && plane_ok) HIP_TRY(psi_hat.alloc(Ms));
     for (DevBuf<double2>* b : {&W_type_s, &rho_hat, &rho_type}) HIP_TRY(b->alloc(nTs * Ms));
     for (DevBuf<double2>* b : {&ion_psi, &ion_b, &ion_rho}) HIP_TRY(b->alloc(nI * Ms));
     HIP_TRY(partials.alloc(8 * RED_BLOCKS));
@@ -604,7 +605,11 @@ int fts_engine::lrk_step()
         HIP_TRY(hipGetLastError());
         if ((rc = fft_forward(omega.p, lrk_hat[0][0][cur].p)) || (rc = fft_forward(lrk_u.p, lrk_hat[0][1][cur].p))) return rc;
     }
-    if (f1 && ((rc = fft_forward(psi.p, lrk_hat[1][0][cur].p)) || (rc = fft_forward(dpsi.p, lrk_hat[1][1][cur].p)))) return rc;
+    if (f1) {
+        if (psi_hat_valid(psi.p)) HIP_TRY(hipMemcpyAsync(lrk_hat[1][0][cur].p, psi_hat.p, sizeof(double2) * M, hipMemcpyDeviceToDevice, stream));
+        else if ((rc = fft_forward(psi.p, lrk_hat[1][0][cur].p))) return rc;
+        if ((rc = fft_forward(dpsi.p, lrk_hat[1][1][cur].p))) return rc;
+    }
     HIP_TRY(lrk_moment_sums(f0 ? omega.p : nullptr, f1 ? psi.p : nullptr, M, partials.p, stream));
     if ((rc = reduce_finalize(4, SC_LRK, 1.0 / static_cast<double>(M)))) return rc;
     HIP_TRY(lrk_infer(lrk_params.p, scal.p + SC_LRK, lrk_state.p, stream));
@@ -647,6 +652,17 @@ int fts_engine::fft_apply(const double2* src, double2* dst, const KOp& op)
     HIP_TRY(launch_line<double>(g64, 0, LINE_FWD_MUL_INV, tw64_x.p, &xj, 1, stream));
     HIP_TRY(launch_line<double>(g64, 1, LINE_INV, tw64_yz.p, &j, 1, stream));
     HIP_TRY(launch_line<double>(g64, 2, LINE_INV, tw64_z.p, &j, 1, stream));
+    return FTS_OK;
+}
+
+// fft_apply from a spectrum fft_forward made: the x pass multiplies and transforms back
+// out of place, then the yz pass; the same arithmetic as fft_apply on the field
+int fts_engine::fft_apply_hat(const double2* hat, double2* dst, const KOp& op)
+{
+    PencilJob<double> xj{dst, op, hat};
+    HIP_TRY(launch_pencil<double>(g64, PENCIL_MUL_INV, &xj, 1, stream));
+    PlaneJob<double> ij{dst, dst, nullptr, nullptr};
+    HIP_TRY(launch_plane<double>(g64, PLANE_INV, &ij, 1, stream));
     return FTS_OK;
 }
 
@@ -1002,7 +1018,12 @@ int fts_engine::densities_begin(const double2* om, const double2* ps)
     hipLaunchKernelGGL(k_combine_modes, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, omega_s.p, mode_coef.p, n_omega, nT, M, W_type_s.p);
     HIP_TRY(hipGetLastError());
     if (electro()) {
-        if ((rc = fft_apply(ps, psi_s.p, kop_sep(sep_gamma.p, invM)))) return rc;
+        if (psi_hat.p) {
+            if ((rc = fft_forward(ps, psi_hat.p)) || (rc = fft_apply_hat(psi_hat.p, psi_s.p, kop_sep(sep_gamma.p, invM)))) return rc;
+            psi_hat_of = ps;
+        } else if ((rc = fft_apply(ps, psi_s.p, kop_sep(sep_gamma.p, invM)))) {
+            return rc;
+        }
     } else {
         HIP_TRY(hipMemsetAsync(psi_s.p, 0, sizeof(double2) * Ms, stream));
     }
@@ -1040,7 +1061,8 @@ int fts_engine::densities_end()
             continue;
         }
         const double* sepj = sep_ion.p + j * static_cast<size_t>(m1 + m2 + m3);
-        if ((rc = fft_apply(ps, f(ion_psi, j), kop_sep(sepj, invM)))) return rc;
+        rc = psi_hat_valid(ps) ? fft_apply_hat(psi_hat.p, f(ion_psi, j), kop_sep(sepj, invM)) : fft_apply(ps, f(ion_psi, j), kop_sep(sepj, invM));
+        if (rc) return rc;
         hipLaunchKernelGGL(k_ion_boltzmann, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, f(ion_psi, j), ions[j].z, M, f(ion_b, j), partials.p);
         HIP_TRY(hipGetLastError());
         if ((rc = reduce_finalize(1, SC_QION + static_cast<int>(j), invM))) return rc;
@@ -1061,7 +1083,8 @@ int fts_engine::psi_drift(const double2* ps, const double2* rho_src, double2* ou
         HIP_TRY(hipMemsetAsync(out, 0, sizeof(double2) * M, stream));
         return FTS_OK;
     }
-    int rc = fft_apply(ps, t2.p, KOp{nullptr, nullptr, 0.0, 1.0 / static_cast<double>(M), 0.0, 0, 0});
+    const KOp lap{nullptr, nullptr, 0.0, 1.0 / static_cast<double>(M), 0.0, 0, 0};
+    int rc = psi_hat_valid(ps) ? fft_apply_hat(psi_hat.p, t2.p, lap) : fft_apply(ps, t2.p, lap);
     if (rc) return rc;
     hipLaunchKernelGGL(k_psi_drift, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, t2.p, rho_src, 1.0 / (4.0 * Pi * lB), M, out);
     HIP_TRY(hipGetLastError());
@@ -1072,6 +1095,7 @@ int fts_engine::finite_queue(int* host_out)
 {
     if (inject_step >= 0 && step_counter == inject_step) {
         hipLaunchKernelGGL(k_fill, dim3(1), dim3(EW_THREADS), 0, stream, psi.p, double2{std::nan(""), 0.0}, 1);
+        psi_hat_of = nullptr;
         inject_step = -1;
     }
     hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(64), 0, stream, flag.p, 1);
@@ -1187,8 +1211,10 @@ int fts_engine::measure_queue(const double2* om, const double2* ps, fts_observab
         if ((rc = reduce_finalize(1, SC_H + l, 0.5 * omega_coefficient(l) * invM))) return rc;
     }
     if (electro()) {
-        if ((rc = fft_forward(ps, t3.p))) return rc;
-        if ((rc = contract_to(t3.p, t3.p, false, nullptr, scal.p + SC_PSI_S))) return rc;
+        const double2* hat = t3.p;
+        if (psi_hat_valid(ps)) hat = psi_hat.p;
+        else if ((rc = fft_forward(ps, t3.p))) return rc;
+        if ((rc = contract_to(hat, hat, false, nullptr, scal.p + SC_PSI_S))) return rc;
     }
     // pinned staging: [scal][stress_acc][rfac2: nsp x Nmax doubles][shifts: nsp x MAX_CLASSES doubles]
     double2* h_scal = static_cast<double2*>(pin);
@@ -1413,6 +1439,7 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
 // predictor (complexLangevin.cu:328-345), after the densities at (w, psi)
 int fts_engine::pc_predict()
 {
+    if (psi_hat_of == psi_bar.p) psi_hat_of = nullptr; // psi_bar is rewritten below
     const unsigned long long t = static_cast<unsigned long long>(step_counter);
     const size_t Ms = static_cast<size_t>(M);
     int rc;
@@ -1454,6 +1481,7 @@ int fts_engine::pc_correct(bool record, fts_observables* rec)
         hipLaunchKernelGGL(k_omega_correct, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, drift.p, omega_bar.p, rho_type.p, seed, t, M, omega.p);
         HIP_TRY(hipGetLastError());
     }
+    psi_hat_of = nullptr; // psi changes below
     if (electro()) {
         hipLaunchKernelGGL(k_psi_update, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, -0.5 * dt * lambda_psi,
                            static_cast<const double2*>(dpsi_bar.p), -0.5 * dt * lambda_psi, sigma_psi, scal.p + SC_ETA_MEAN, seed, t, M, psi.p);
@@ -1535,6 +1563,7 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
                            scal.p + SC_ETA_MEAN, seed, t, M, dpsi.p);
         HIP_TRY(hipGetLastError());
         if (si && (rc = fft_apply(dpsi.p, dpsi.p, KOp{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0}))) return rc;
+        psi_hat_of = nullptr; // psi changes below
         hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, Ms);
         HIP_TRY(hipGetLastError());
         if ((rc = remove_mean(psi.p))) return rc;
@@ -1578,6 +1607,7 @@ int fts_engine::restart_from_checkpoint(long long failed_step)
     seed = seed0 + static_cast<unsigned long long>(noise_epoch) * 0x9E3779B97F4A7C15ull;
     last_failed_step = failed_step;
     step_counter = ckpt_step;
+    psi_hat_of = nullptr;
     lrk_have_prev = false;
     return FTS_OK;
 }
@@ -1805,6 +1835,7 @@ int fts_set_fields(fts_engine* h, const double* om, const double* ps)
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->ckpt_step = -1; // new fields: the next step takes a new checkpoint
     h->lrk_have_prev = false;
+    h->psi_hat_of = nullptr;
     return FTS_OK;
 }
 
@@ -1869,6 +1900,7 @@ int fts_load_state(fts_engine* h, const char* path)
     h->seed = seed0 + static_cast<unsigned long long>(epoch[0]) * 0x9E3779B97F4A7C15ull;
     h->ckpt_step = -1;
     h->lrk_have_prev = false;
+    h->psi_hat_of = nullptr;
     return FTS_OK;
 }
 

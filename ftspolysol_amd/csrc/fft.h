@@ -32,7 +32,7 @@ struct KOp {
 };
 
 enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, PLANE_LAST = 4 };
-enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3 };
+enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3, PENCIL_MUL_INV = 4 };
 
 template <typename T> struct PlaneJob {
     const V2<T>* src;
@@ -53,6 +53,7 @@ template <typename T> struct PlaneJob {
 template <typename T> struct PencilJob {
     V2<T>* data;
     KOp op;
+    const V2<T>* src; // read from here instead of data when set (the result still goes to data)
 };
 
 // Line pass: a tile of LINE_TC lines of one axis, transformed along that axis in LDS.
@@ -115,9 +116,11 @@ template <typename T> __device__ __forceinline__ PencilJob<T> pick_job(PencilJob
         const PencilJob<T>* t = table + blockIdx.y;
         j.data = t->data;
         j.op = pick_kop(t->op, t->op, true);
+        j.src = t->src;
     } else {
         j.data = f ? a.data : b.data;
         j.op = pick_kop(a.op, b.op, f);
+        j.src = f ? a.src : b.src;
     }
     return j;
 }

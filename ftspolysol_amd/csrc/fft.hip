@@ -161,14 +161,15 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
     const V* twg = static_cast<const V*>(g.tw_x);
     for (int t = tid; t < N1; t += NT) tw[t] = twg[t];
     V* data = job.data;
+    const V* in = job.src ? job.src : data;
 #pragma unroll 4
     for (int i = tid; i < N1 * ZC; i += NT) {
         const int x = i / ZC, c = i % ZC;
-        lds[x * LS + c] = data[static_cast<size_t>(x) * sx + jbase + c];
+        lds[x * LS + c] = in[static_cast<size_t>(x) * sx + jbase + c];
     }
     __syncthreads();
     if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_MUL_INV) lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
-    if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL) {
+    if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL || MODE == PENCIL_MUL_INV) {
         // every element a thread touches shares its column c (NT is a multiple of ZC):
         // the (y, z) part of the kernel is one per-thread constant
         const KOp op = job.op;
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         }
         __syncthreads();
     }
-    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV) lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
+    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL_INV) lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
 #pragma unroll 4
     for (int i = tid; i < N1 * ZC; i += NT) {
         const int x = i / ZC, c = i % ZC;
@@ -246,6 +247,7 @@ hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, con
     case PENCIL_FWD: return pencil_go<T, N1, PENCIL_FWD>(g, jobs, table, njobs, s);
     case PENCIL_INV: return pencil_go<T, N1, PENCIL_INV>(g, jobs, table, njobs, s);
     case PENCIL_MUL: return pencil_go<T, N1, PENCIL_MUL>(g, jobs, table, njobs, s);
+    case PENCIL_MUL_INV: return pencil_go<T, N1, PENCIL_MUL_INV>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }
