@@ -241,7 +241,7 @@ def main():
                     help="independent FP64 replicas (one batch) whose pooled autocorrelation gives the plain scheme's "
                          "tau_int; one trajectory of `steps` steps estimates it too poorly")
     ap.add_argument("--no-fp32-pc", action="store_true", help="skip the FP32 predictor-corrector comparison batch")
-    ap.add_argument("--schedule", choices=["auto", "plane", "line"], default="auto",
+    ap.add_argument("--schedule", choices=["auto", "plane", "line", "conv"], default="auto",
                     help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -254,7 +254,7 @@ def main():
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
     schemes = {"em": SCHEME_EM, "si": SCHEME_SEMI_IMPLICIT, "pc": SCHEME_PC_REFERENCE, "si_learned": SCHEME_SI_LEARNED}
     lrk = load_params(args.lrk) if args.fast_scheme == "si_learned" else None
-    sched = {"auto": 0, "plane": 1, "line": 2}[args.schedule]
+    sched = {"auto": 0, "plane": 1, "line": 2, "conv": 3}[args.schedule]
     fast_sys = [make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=schemes[args.fast_scheme], seed=0xC4 + r,
                             fft_schedule=sched) for r in range(args.replicas)]
     pc_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP32, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r,
@@ -306,7 +306,7 @@ def main():
     H_err_sigma = np.hypot(fast["H_stderr"], plain["H_stderr"]) / abs(plain["H_mean"])
     keys = ("replicas", "replica_steps_per_s", "ms_per_step", "tau_int", "tau_int_err", "tau_int_each", "samples_per_s",
             "H_mean", "H_stderr", "schedule", "tune_ms", "restarts")
-    sched_name = {1: "plane", 2: "line"}
+    sched_name = {1: "plane", 2: "line", 3: "conv"}
     out = {
         "metric": "decorrelated CL samples/s at 96^3 N=100, FP32+learned vs plain FP64, <H> err",
         "value": fast["samples_per_s"],

@@ -163,7 +163,7 @@ template <typename T> int build_tables(Group& g, int p, int sched, int max_steps
     auto add = [&](int kind, int mode, int axis, size_t off, size_t end) {
         tb.launches.push_back(SweepLaunch{kind, mode, axis, off, static_cast<int>(end - off)});
     };
-    if (sched == FTS_SCHED_PLANE) {
+    if (sched == FTS_SCHED_PLANE || sched == FTS_SCHED_CONV) {
         size_t o = J.plane.size();
         for (const RepView<T>& r : act) {
             const int N = r.sp->N;
@@ -173,6 +173,32 @@ template <typename T> int build_tables(Group& g, int p, int sched, int max_steps
         }
         add(L_PLANE, PLANE_INIT, 0, o, J.plane.size());
         tb.init_launches = tb.launches.size();
+    }
+    if (sched == FTS_SCHED_CONV) {
+        // bead step s: pencil q_s -> tmp, plane tmp -> q_{s+1}; a replica whose chain
+        // has ended drops out of both launches
+        for (int s = 0; s < nsteps; ++s) {
+            const size_t op = J.pencil.size();
+            for (const RepView<T>& r : act) {
+                const int N = r.sp->N;
+                if (N - 1 <= s) break;
+                J.pencil.push_back(PencilJob<T>{r.tf, r.bond_op(s), r.qf + static_cast<size_t>(s) * r.Ms});
+                if (!r.sp->mirror) J.pencil.push_back(PencilJob<T>{r.tb, r.bond_op(N - 2 - s), r.qb + static_cast<size_t>(N - 1 - s) * r.Ms});
+            }
+            add(L_PENCIL, PENCIL_CONV, 0, op, J.pencil.size());
+            const size_t ol = J.plane.size();
+            for (const RepView<T>& r : act) {
+                const int N = r.sp->N;
+                if (N - 1 <= s) break;
+                const int sf = s + 1, sb = N - 2 - s;
+                J.plane.push_back(PlaneJob<T>{r.tf, nullptr, r.qf + static_cast<size_t>(sf) * r.Ms, r.E_of(sf), r.nF[s % 2], r.nF[(s + 1) % 2], r.lcF + sf, r.bond_op(s)});
+                if (!r.sp->mirror)
+                    J.plane.push_back(PlaneJob<T>{r.tb, nullptr, r.qb + static_cast<size_t>(sb) * r.Ms, r.E_of(sb), r.nB[s % 2], r.nB[(s + 1) % 2], r.lcB + sb, r.bond_op(sb)});
+            }
+            tb.jobs += static_cast<long long>(J.plane.size() - ol);
+            add(L_PLANE, PLANE_CONV_E, 0, ol, J.plane.size());
+        }
+    } else if (sched == FTS_SCHED_PLANE) {
         for (int s = 0; s < nsteps; ++s) {
             const size_t op = J.pencil.size();
             for (const RepView<T>& r : act) {
@@ -482,8 +508,9 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
     return FTS_OK;
 }
 
-// Schedule of a group's sweeps: both timed over 8 bead steps of species 0 with
-// all replicas in each launch, best of three.
+// Schedule of a group's sweeps: plane and line timed over 8 bead steps of species 0
+// with all replicas in each launch, best of three; tune_ms[k] belongs to schedule
+// k + 1 (conv only when asked for, as for single engines).
 template <typename T> int group_autotune(Group& g)
 {
     if (!g.plane_ok) {
@@ -492,8 +519,9 @@ template <typename T> int group_autotune(Group& g)
     }
     SweepTables t[2];
     const std::vector<char> all(g.reps.size(), 1);
-    int rc;
-    if ((rc = build_tables<T>(g, 0, FTS_SCHED_PLANE, 8, all, t[0])) || (rc = build_tables<T>(g, 0, FTS_SCHED_LINE, 8, all, t[1]))) return rc;
+    int rc = FTS_OK;
+    for (int k = 0; k < 2 && rc == FTS_OK; ++k) rc = build_tables<T>(g, 0, k + 1, 8, all, t[k]);
+    if (rc) return rc;
     if (t[0].steps == 0) return FTS_OK;
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(hipEventCreate(&a));
@@ -510,9 +538,12 @@ template <typename T> int group_autotune(Group& g)
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc) return rc;
-    g.tune_ms[0] = best[0];
-    g.tune_ms[1] = best[1];
-    g.sched = best[0] <= best[1] ? FTS_SCHED_PLANE : FTS_SCHED_LINE;
+    int pick = 0;
+    for (int k = 0; k < 2; ++k) {
+        g.tune_ms[k] = best[k];
+        if (best[k] < best[pick]) pick = k;
+    }
+    g.sched = pick + 1;
     return FTS_OK;
 }
 
@@ -609,13 +640,12 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
         const int want = systems[g.ids[0]].fft_schedule;
         int rc = FTS_OK;
         if (want == FTS_SCHED_AUTO) rc = g.precision == FTS_PREC_FP32 ? group_autotune<float>(g) : group_autotune<double>(g);
-        else if (want == FTS_SCHED_PLANE && !g.plane_ok) return fail(FTS_INVALID, "batch: the plane schedule does not fit this mesh");
+        else if ((want == FTS_SCHED_PLANE || want == FTS_SCHED_CONV) && !g.plane_ok) return fail(FTS_INVALID, "batch: the plane schedules do not fit this mesh");
         else g.sched = want;
         if (rc) return rc;
         for (fts_engine* e : g.reps) {
             e->sched = g.sched;
-            e->tune_ms[0] = g.tune_ms[0];
-            e->tune_ms[1] = g.tune_ms[1];
+            for (int k = 0; k < 2; ++k) e->tune_ms[k] = g.tune_ms[k];
             e->defer_measure = true;
         }
         if ((rc = set_side_streams(g, std::min<int>(static_cast<int>(g.reps.size()), FTS_BATCH_DEFAULT_STREAMS)))) return rc;
@@ -640,8 +670,7 @@ int fts_batch_get_info(fts_batch* b, fts_batch_info* o)
         o->group_mesh[k][2] = g.m3;
         o->group_precision[k] = g.precision;
         o->group_schedule[k] = g.sched;
-        o->group_tune_ms[k][0] = g.tune_ms[0];
-        o->group_tune_ms[k][1] = g.tune_ms[1];
+        for (int j = 0; j < 2; ++j) o->group_tune_ms[k][j] = g.tune_ms[j];
     }
     return FTS_OK;
 }

@@ -797,7 +797,28 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
     const void* twy = sizeof(T) == 8 ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p);
     const void* twz = sizeof(T) == 8 ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p);
     hipEvent_t ev0 = nullptr;
-    if (schedule == FTS_SCHED_PLANE) {
+    if (schedule == FTS_SCHED_CONV) {
+        // bead step s: pencil (x factor and 1/M) q_s -> tmp, plane (y, z factors,
+        // e^{-W}, renormalisation) tmp -> q_{s+1}; the first plane pass stores q_0
+        // (its transformed copy in tmp is not used)
+        PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0), nullptr, nF[0], lcF},
+                               {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), nullptr, nB[0], lcB + (N - 1)}};
+        HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, nj, stream));
+        if (timed) {
+            ev0 = get_event();
+            HIP_TRY(hipEventRecord(ev0, stream));
+        }
+        for (int s = 0; s < nsteps; ++s) {
+            const int bf = sp.bond_cls[static_cast<size_t>(s)], bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
+            const KOp of = kop_sep(sep_bond.p + static_cast<size_t>(bf) * ks, invM), ob = kop_sep(sep_bond.p + static_cast<size_t>(bb) * ks, invM);
+            PencilJob<T> pj[2] = {{tf, of, qf + static_cast<size_t>(s) * Ms}, {tb, ob, qb + static_cast<size_t>(N - 1 - s) * Ms}};
+            HIP_TRY(launch_pencil<T>(g, PENCIL_CONV, pj, nj, stream));
+            const int sf = s + 1, sb = N - 2 - s;
+            PlaneJob<T> st[2] = {{tf, nullptr, qf + static_cast<size_t>(sf) * Ms, E_of(sf), nF[s % 2], nF[(s + 1) % 2], lcF + sf, of},
+                                 {tb, nullptr, qb + static_cast<size_t>(sb) * Ms, E_of(sb), nB[s % 2], nB[(s + 1) % 2], lcB + sb, ob}};
+            HIP_TRY(launch_plane<T>(g, PLANE_CONV_E, st, nj, stream));
+        }
+    } else if (schedule == FTS_SCHED_PLANE) {
         PlaneJob<T> init[2] = {{nullptr, tf, qf, E_of(0), nullptr, nF[0], lcF},
                                {nullptr, tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), nullptr, nB[0], lcB + (N - 1)}};
         HIP_TRY(launch_plane<T>(g, PLANE_INIT, init, nj, stream));
@@ -857,6 +878,13 @@ template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T
 {
     const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
     const bool d = sizeof(T) == 8;
+    if (sched == FTS_SCHED_CONV) {
+        PencilJob<T> j1{q, op};
+        HIP_TRY(launch_pencil<T>(g, PENCIL_CONV, &j1, 1, stream));
+        PlaneJob<T> j2{q, nullptr, qout, emw, nullptr, nullptr, nullptr, op};
+        HIP_TRY(launch_plane<T>(g, PLANE_CONV_E, &j2, 1, stream));
+        return FTS_OK;
+    }
     if (sched == FTS_SCHED_PLANE) {
         PlaneJob<T> j1{q, q, nullptr, nullptr};
         HIP_TRY(launch_plane<T>(g, PLANE_FWD, &j1, 1, stream));
@@ -880,8 +908,11 @@ template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T
     return FTS_OK;
 }
 
-// Bead-step schedule chosen by timing both on this grid and precision (the FFT
-// autotuner): 2 x 8 bead steps of species 0 at zero fields, best of three.
+// Bead-step schedule chosen by timing plane and line on this grid and precision (the
+// FFT autotuner): 2 x 8 bead steps of species 0 at zero fields, best of three;
+// tune_ms[k] belongs to schedule k + 1.  The conv schedule is taken only when asked
+// for: it differs from the plane schedule by a few percent either way, so timing it
+// would make the choice, and the last bits of the results, vary from run to run.
 int fts_engine::autotune_schedule()
 {
     if (!plane_ok) {
@@ -898,9 +929,8 @@ int fts_engine::autotune_schedule()
     hipEvent_t a = get_event(), b = get_event();
     for (int rep = 0; rep < 3; ++rep)
         for (int k = 0; k < 2; ++k) {
-            const int schedule = k == 0 ? FTS_SCHED_PLANE : FTS_SCHED_LINE;
             HIP_TRY(hipEventRecord(a, stream));
-            int rc = precision == FTS_PREC_FP32 ? run_sweeps<float>(0, schedule, steps, false) : run_sweeps<double>(0, schedule, steps, false);
+            int rc = precision == FTS_PREC_FP32 ? run_sweeps<float>(0, k + 1, steps, false) : run_sweeps<double>(0, k + 1, steps, false);
             if (rc) return rc;
             HIP_TRY(hipEventRecord(b, stream));
             HIP_TRY(hipEventSynchronize(b));
@@ -910,9 +940,12 @@ int fts_engine::autotune_schedule()
         }
     event_pool.push_back(a);
     event_pool.push_back(b);
-    tune_ms[0] = best[0];
-    tune_ms[1] = best[1];
-    sched = best[0] <= best[1] ? FTS_SCHED_PLANE : FTS_SCHED_LINE;
+    int pick = 0;
+    for (int k = 0; k < 2; ++k) {
+        tune_ms[k] = best[k];
+        if (best[k] < best[pick]) pick = k;
+    }
+    sched = pick + 1;
     return FTS_OK;
 }
 
@@ -1641,9 +1674,9 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     if (!line_supported(s->m1, s->m2, s->m3))
         return fail(FTS_INVALID, "mesh not supported by the engine FFT: lengths must be in {8,12,16,24,32,48,64,96,128,192,256}");
     const bool plane_fits = fft_supported(s->m1, s->m2, s->m3, s->precision == FTS_PREC_FP32);
-    if (s->fft_schedule == FTS_SCHED_PLANE && !plane_fits)
-        return fail(FTS_INVALID, "the plane schedule needs m2 == m3 and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
-    if (s->fft_schedule < 0 || s->fft_schedule > 2) return fail(FTS_INVALID, "fft_schedule");
+    if ((s->fft_schedule == FTS_SCHED_PLANE || s->fft_schedule == FTS_SCHED_CONV) && !plane_fits)
+        return fail(FTS_INVALID, "the plane schedules need m2 == m3 and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
+    if (s->fft_schedule < 0 || s->fft_schedule > 3) return fail(FTS_INVALID, "fft_schedule");
     if (s->chain_symmetry != FTS_SYMMETRY_AUTO && s->chain_symmetry != FTS_SYMMETRY_OFF) return fail(FTS_INVALID, "chain_symmetry");
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
@@ -1793,8 +1826,7 @@ int fts_get_info(fts_engine* h, fts_info* o)
     o->precision = h->precision;
     o->scheme = h->scheme;
     o->fft_schedule = h->sched;
-    o->tune_ms[0] = h->tune_ms[0];
-    o->tune_ms[1] = h->tune_ms[1];
+    for (int k = 0; k < 2; ++k) o->tune_ms[k] = h->tune_ms[k];
     o->restarts = h->restarts;
     o->noise_epoch = h->noise_epoch;
     o->checkpoint_step = h->ckpt_step;

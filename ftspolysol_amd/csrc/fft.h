@@ -31,8 +31,10 @@ struct KOp {
     int ka, kb;
 };
 
-enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, PLANE_LAST = 4 };
-enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3, PENCIL_MUL_INV = 4 };
+enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, PLANE_LAST = 4, PLANE_CONV_E = 5 };
+// PENCIL_CONV: forward along x, x op.sep[x] op.c0 (the x factor of a separable
+// kernel), inverse along x; the plane pass PLANE_CONV_E applies the (y, z) factors
+enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3, PENCIL_MUL_INV = 4, PENCIL_CONV = 5 };
 
 template <typename T> struct PlaneJob {
     const V2<T>* src;
@@ -48,6 +50,7 @@ template <typename T> struct PlaneJob {
     const double* norm_in;
     double* norm_out;
     double* log_scale;
+    KOp op; // PLANE_CONV_E: a separable k-space kernel; its (y, z) factors are applied here
 };
 
 template <typename T> struct PencilJob {
@@ -106,6 +109,7 @@ template <typename T> __device__ __forceinline__ PlaneJob<T> pick_job(PlaneJob<T
     j.norm_in = tb ? t->norm_in : f ? a.norm_in : b.norm_in;
     j.norm_out = tb ? t->norm_out : f ? a.norm_out : b.norm_out;
     j.log_scale = tb ? t->log_scale : f ? a.log_scale : b.log_scale;
+    j.op = tb ? pick_kop(t->op, t->op, true) : pick_kop(a.op, b.op, f);
     return j;
 }
 template <typename T> __device__ __forceinline__ PencilJob<T> pick_job(PencilJob<T> a, PencilJob<T> b, const PencilJob<T>* table)

@@ -34,7 +34,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     const PlaneJob<T> job = pick_job<T>(j0, j1, table);
     const size_t plane = static_cast<size_t>(blockIdx.x) * (N * N);
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
-    constexpr bool PRODUCES_Q = MODE == PLANE_INIT || MODE == PLANE_STEP || MODE == PLANE_LAST;
+    constexpr bool PRODUCES_Q = MODE == PLANE_INIT || MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E;
 
     // renormalisation factor from the previous bead's per-plane norms: a max, so every
     // block gets the same value whatever the order; read by all threads at once (one
@@ -83,14 +83,57 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         }
         lds[(i / N) * LS + (i % N)] = v;
     }
-    if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST) renorm();
+    if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) renorm();
     __syncthreads();
 
-    if constexpr (MODE == PLANE_INV || MODE == PLANE_STEP || MODE == PLANE_LAST) {
+    // PLANE_CONV_E in FP32 loads this thread's e^{-W} values before the inverse
+    // transforms, so the loads complete under them (PER registers pairs; FP64 would
+    // lose occupancy)
+    constexpr int PER = (N * N) / NT;
+    constexpr bool PREFETCH = MODE == PLANE_CONV_E && (N * N) % NT == 0 && PER <= 16 && sizeof(T) == 4;
+    V ew[PREFETCH ? PER : 1];
+    if constexpr (MODE == PLANE_CONV_E) {
+        // the plane's share of the convolution: forward along z and y, x phi_y phi_z
+        // (the pencil pass has applied phi_x / M), inverse below
+        lds_fft<T, N, -1, N, LS, 1, true, NT>(lds, tw, tid);
+        lds_fft<T, N, -1, N, 1, LS, true, NT>(lds, tw, tid);
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) ew[k] = job.emw[plane + tid + k * NT];
+        }
+        const double* sy = job.op.sep + gridDim.x; // sep holds the m1, m2, m3 factors; gridDim.x = m1
+        const double* sz = sy + N;
+        if constexpr (NT % N == 0) {
+            // every element of this thread has z = tid % N
+            const double szt = sz[tid % N];
+#pragma unroll 4
+            for (int i = tid; i < N * N; i += NT) {
+                V* p = &lds[(i / N) * LS + (i % N)];
+                *p = cscale(*p, static_cast<T>(sy[i / N] * szt));
+            }
+        } else {
+#pragma unroll 4
+            for (int i = tid; i < N * N; i += NT) {
+                V* p = &lds[(i / N) * LS + (i % N)];
+                *p = cscale(*p, static_cast<T>(sy[i / N] * sz[i % N]));
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (MODE == PLANE_INV || MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
         lds_fft<T, N, +1, N, LS, 1, true, NT>(lds, tw, tid); // along z
         lds_fft<T, N, +1, N, 1, LS, true, NT>(lds, tw, tid);  // along y
     }
-    if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST) {
+    if constexpr (PREFETCH) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = tid + k * NT;
+            V v = cmul(lds[(i / N) * LS + (i % N)], ew[k]);
+            if (job.norm_in) v = cscale(v, c);
+            job.qout[plane + i] = v;
+            nrm = fmax(nrm, static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
+        }
+    } else if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) {
             V* p = &lds[(i / N) * LS + (i % N)];
@@ -122,7 +165,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         lds_fft<T, N, -1, N, LS, 1, true, NT>(lds, tw, tid);
         lds_fft<T, N, -1, N, 1, LS, true, NT>(lds, tw, tid);
     }
-    if constexpr (MODE != PLANE_LAST) {
+    if constexpr (MODE != PLANE_LAST && MODE != PLANE_CONV_E) {
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) job.dst[plane + i] = lds[(i / N) * LS + (i % N)];
     }
@@ -168,7 +211,17 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         lds[x * LS + c] = in[static_cast<size_t>(x) * sx + jbase + c];
     }
     __syncthreads();
-    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_MUL_INV) lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
+    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_CONV) lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
+    if constexpr (MODE == PENCIL_CONV) {
+        const double* sxf = job.op.sep;
+        const double c0 = job.op.c0;
+#pragma unroll 4
+        for (int i = tid; i < N1 * ZC; i += NT) {
+            const int x = i / ZC;
+            lds[x * LS + i % ZC] = cscale(lds[x * LS + i % ZC], static_cast<T>(sxf[x] * c0));
+        }
+        __syncthreads();
+    }
     if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL || MODE == PENCIL_MUL_INV) {
         // every element a thread touches shares its column c (NT is a multiple of ZC):
         // the (y, z) part of the kernel is one per-thread constant
@@ -197,7 +250,8 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         }
         __syncthreads();
     }
-    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL_INV) lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
+    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL_INV || MODE == PENCIL_CONV)
+        lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
 #pragma unroll 4
     for (int i = tid; i < N1 * ZC; i += NT) {
         const int x = i / ZC, c = i % ZC;
@@ -224,6 +278,7 @@ hipError_t plane_mode(const FftGeom& g, int mode, const PlaneJob<T>* jobs, const
     case PLANE_INIT: return plane_go<T, N, PLANE_INIT>(g, jobs, table, njobs, s);
     case PLANE_STEP: return plane_go<T, N, PLANE_STEP>(g, jobs, table, njobs, s);
     case PLANE_LAST: return plane_go<T, N, PLANE_LAST>(g, jobs, table, njobs, s);
+    case PLANE_CONV_E: return plane_go<T, N, PLANE_CONV_E>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -248,6 +303,7 @@ hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, con
     case PENCIL_INV: return pencil_go<T, N1, PENCIL_INV>(g, jobs, table, njobs, s);
     case PENCIL_MUL: return pencil_go<T, N1, PENCIL_MUL>(g, jobs, table, njobs, s);
     case PENCIL_MUL_INV: return pencil_go<T, N1, PENCIL_MUL_INV>(g, jobs, table, njobs, s);
+    case PENCIL_CONV: return pencil_go<T, N1, PENCIL_CONV>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -55,6 +55,8 @@ extern "C" {
 #define FTS_SCHED_AUTO 0  /* time both at creation, keep the faster */
 #define FTS_SCHED_PLANE 1 /* pencil(x: FFT, x phi/M, IFFT) + plane(yz: IFFT, x e^{-W}, store q, FFT); needs m2 == m3, plane in LDS */
 #define FTS_SCHED_LINE 2  /* four line passes x, y, z, y; any supported lengths */
+#define FTS_SCHED_CONV 3  /* plane(yz: FFT, x phi_y phi_z/M, IFFT) + pencil(x: FFT, x phi_x, IFFT, x e^{-W}, store q);
+                             separable bond kernels; needs m2 == m3, plane in LDS */
 
 /* stress_mask bits: components in the reference's fixed order (stress.cu:56-61) */
 #define FTS_STRESS_XX 0x01
@@ -167,7 +169,7 @@ typedef struct fts_info {
     int n_segment_classes; /* distinct (type, charge) bead classes: e^{-W} fields kept per species */
     int precision;
     int scheme;
-    int fft_schedule; /* FTS_SCHED_PLANE or FTS_SCHED_LINE in use */
+    int fft_schedule; /* FTS_SCHED_PLANE, FTS_SCHED_LINE or FTS_SCHED_CONV in use */
     double volume;
     double rho0;
     double short_range_self_pressure; /* param.cu:135-149 */
@@ -304,7 +306,7 @@ typedef struct fts_batch_info {
     int group_size[FTS_MAX_BATCH_GROUPS];
     int group_mesh[FTS_MAX_BATCH_GROUPS][3];
     int group_precision[FTS_MAX_BATCH_GROUPS];
-    int group_schedule[FTS_MAX_BATCH_GROUPS];        /* FTS_SCHED_PLANE or FTS_SCHED_LINE */
+    int group_schedule[FTS_MAX_BATCH_GROUPS];        /* FTS_SCHED_PLANE, FTS_SCHED_LINE or FTS_SCHED_CONV */
     double group_tune_ms[FTS_MAX_BATCH_GROUPS][2];   /* autotuner: 8 batched bead steps, plane and line */
 } fts_batch_info;
 

@@ -9,7 +9,7 @@ test_gpu_parity.py.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, PREC_FP32, PREC_FP64, SCHED_LINE, SCHED_PLANE, SCHEME_EM,
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, PREC_FP32, PREC_FP64, SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHEME_EM,
                                 SCHEME_PC_REFERENCE, SYMMETRY_OFF, hip_batch, hip_engine)
 from ftspolysol_amd.system import initial_fields, make_system
 
@@ -42,7 +42,9 @@ def start_fields(engines, systems):
 @pytest.mark.parametrize("scheme,precision,sched", [(SCHEME_PC_REFERENCE, PREC_FP64, SCHED_PLANE),
                                                     (SCHEME_EM, PREC_FP64, SCHED_LINE),
                                                     (SCHEME_EM, PREC_FP32, SCHED_PLANE),
-                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_LINE)])
+                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_LINE),
+                                                    (SCHEME_EM, PREC_FP64, SCHED_CONV),
+                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_CONV)])
 def test_batch_matches_single_engines(scheme, precision, sched):
     print(f"batch scheme={scheme} precision={precision} sched={sched}", flush=True)
     systems = replica_systems(scheme, precision, sched)
@@ -83,7 +85,8 @@ def test_batch_headline_autotuned():
     sched = b.info.group_schedule[0]
     print("batch tune ms (plane, line):", list(b.info.group_tune_ms[0]), "chosen", sched, flush=True)
     assert sched in (SCHED_PLANE, SCHED_LINE)
-    assert b.info.group_tune_ms[0][0] > 0 and b.info.group_tune_ms[0][1] > 0
+    tune = list(b.info.group_tune_ms[0])
+    assert min(tune) > 0 and tune[sched - 1] == min(tune)
     start_fields(b.replicas, systems)
     b.set_timing(True)
     rb = b.step(3, record_every=1)
@@ -97,7 +100,7 @@ def test_batch_headline_autotuned():
         assert abs(x["H"] - y["H"]) <= 1e-12 * abs(y["H"])
 
 
-@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE])
+@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE, SCHED_CONV])
 def test_graph_replays_match_eager_launches(sched):
     """The captured sweep graphs against the same launches issued one by one: several
     step calls (each replaying the graphs of both density evaluations many times, with

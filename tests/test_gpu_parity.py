@@ -8,7 +8,7 @@ engine 1e-4 relative to FP64 for H, mu and stress, 2e-5 for a bead step.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (SCHED_LINE, SCHED_PLANE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
+from ftspolysol_amd.api import (SCHED_CONV, SCHED_LINE, SCHED_PLANE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
                                 FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
                                 SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
 from ftspolysol_amd.system import System, initial_fields, make_system
@@ -48,7 +48,7 @@ def test_fft_line_path_matches_numpy(m):
     assert rel(e.fft3d(x, +1), np.fft.ifftn(x) * x.size) < 1e-13
 
 
-@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE])
+@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE, SCHED_CONV])
 @pytest.mark.parametrize("m,bond", [((32, 32, 32), 1.3), ((96, 96, 96), 1.0), ((16, 24, 24), 0.7)])
 def test_bead_step_matches_oracle(m, bond, sched):
     print(f"bead step {m} b={bond} sched={sched}", flush=True)
@@ -69,7 +69,8 @@ def test_autotuner_times_both_schedules():
     e = hip_engine(make_system(96, 100, precision=PREC_FP32))
     print("tune ms (plane, line):", list(e.info.tune_ms), "chosen", e.info.fft_schedule, flush=True)
     assert e.info.fft_schedule in (SCHED_PLANE, SCHED_LINE)
-    assert e.info.tune_ms[0] > 0 and e.info.tune_ms[1] > 0
+    tune = list(e.info.tune_ms)
+    assert min(tune) > 0 and tune[e.info.fft_schedule - 1] == min(tune)
 
 
 def c1_like(n=32, N=20, **kw):
@@ -140,7 +141,8 @@ def test_measure_matches_oracle(which):
 
 
 @pytest.mark.parametrize("scheme,sched", [(SCHEME_PC_REFERENCE, SCHED_PLANE), (SCHEME_PC_REFERENCE, SCHED_LINE),
-                                          (SCHEME_EM, SCHED_PLANE), (SCHEME_SEMI_IMPLICIT, SCHED_LINE)])
+                                          (SCHEME_EM, SCHED_PLANE), (SCHEME_SEMI_IMPLICIT, SCHED_LINE),
+                                          (SCHEME_PC_REFERENCE, SCHED_CONV)])
 def test_trajectory_matches_oracle(scheme, sched):
     print(f"trajectory scheme={scheme} sched={sched}", flush=True)
     s = c1_like(stress_mask=0x7F, scheme=scheme, fft_schedule=sched)
@@ -158,9 +160,10 @@ def test_trajectory_matches_oracle(scheme, sched):
     assert rel(g.get_field(FIELD_PSI), o.get_field(FIELD_PSI)) < 1e-10
 
 
-def test_copolymer_pc_trajectory():
-    print("copolymer pc", flush=True)
-    s = copolymer_system(stress_mask=0x7F)
+@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_CONV])
+def test_copolymer_pc_trajectory(sched):
+    print(f"copolymer pc sched={sched}", flush=True)
+    s = copolymer_system(stress_mask=0x7F, fft_schedule=sched)
     g, o = hip_engine(s), oracle_engine(s)
     set_random_fields([g, o], g.info.n_omega, s.m, 0.05, 31)
     rg, ro = g.step(3, record_every=1), o.step(3, record_every=1)
