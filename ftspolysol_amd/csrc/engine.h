@@ -198,6 +198,17 @@ struct fts_engine {
     int fft_apply(const double2* src, double2* dst, const KOp& op);
     int fft_apply_hat(const double2* hat, double2* dst, const KOp& op); // dst = IFFT[K hat] (plane schedule)
     int fft_forward(const double2* src, double2* dst);
+    // Independent FFT jobs of one replica, two to a launch on the plane schedule (twice the
+    // workgroups per pass, a third of the launches); the same arithmetic per job as the
+    // single forms, which the line schedule runs one job at a time.
+    struct FftJob {
+        const double2* src; // field (apply, forward) or spectrum (apply_hat)
+        double2* dst;
+        KOp op;
+    };
+    int fft_apply_n(const FftJob* jobs, int n);
+    int fft_apply_hat_n(const FftJob* jobs, int n);
+    int fft_forward_n(const FftJob* jobs, int n);
     template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst);
     int reduce_finalize(int K, int slot, double scale);
     int sum_field(const double2* a, int slot, double scale);

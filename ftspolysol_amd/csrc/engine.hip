@@ -600,16 +600,20 @@ int fts_engine::lrk_step()
     int rc;
     const int cur = lrk_par, prv = lrk_par ^ 1;
     const bool f0 = n_omega > 0, f1 = electro();
+    FftJob fj[4];
+    int nf = 0;
     if (f0) {
         hipLaunchKernelGGL(k_mode0_drift, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, omega.p, rho_type.p, M, lrk_u.p);
         HIP_TRY(hipGetLastError());
-        if ((rc = fft_forward(omega.p, lrk_hat[0][0][cur].p)) || (rc = fft_forward(lrk_u.p, lrk_hat[0][1][cur].p))) return rc;
+        fj[nf++] = FftJob{omega.p, lrk_hat[0][0][cur].p, KOp{}};
+        fj[nf++] = FftJob{lrk_u.p, lrk_hat[0][1][cur].p, KOp{}};
     }
     if (f1) {
         if (psi_hat_valid(psi.p)) HIP_TRY(hipMemcpyAsync(lrk_hat[1][0][cur].p, psi_hat.p, sizeof(double2) * M, hipMemcpyDeviceToDevice, stream));
-        else if ((rc = fft_forward(psi.p, lrk_hat[1][0][cur].p))) return rc;
-        if ((rc = fft_forward(dpsi.p, lrk_hat[1][1][cur].p))) return rc;
+        else fj[nf++] = FftJob{psi.p, lrk_hat[1][0][cur].p, KOp{}};
+        fj[nf++] = FftJob{dpsi.p, lrk_hat[1][1][cur].p, KOp{}};
     }
+    if ((rc = fft_forward_n(fj, nf))) return rc;
     HIP_TRY(lrk_moment_sums(f0 ? omega.p : nullptr, f1 ? psi.p : nullptr, M, partials.p, stream));
     if ((rc = reduce_finalize(4, SC_LRK, 1.0 / static_cast<double>(M)))) return rc;
     HIP_TRY(lrk_infer(lrk_params.p, scal.p + SC_LRK, lrk_state.p, stream));
@@ -686,6 +690,68 @@ template <typename T> int fts_engine::fft_forward_t(const V2<T>* src, V2<T>* dst
 }
 
 int fts_engine::fft_forward(const double2* src, double2* dst) { return fft_forward_t<double>(src, dst); }
+
+int fts_engine::fft_apply_n(const FftJob* jobs, int n)
+{
+    for (int k = 0; k < n; k += 2) {
+        const int c = std::min(2, n - k);
+        if (!plane_ok) {
+            for (int u = 0; u < c; ++u)
+                if (int rc = fft_apply(jobs[k + u].src, jobs[k + u].dst, jobs[k + u].op)) return rc;
+            continue;
+        }
+        PlaneJob<double> pj[2], ij[2];
+        PencilJob<double> xj[2];
+        for (int u = 0; u < c; ++u) {
+            const FftJob& j = jobs[k + u];
+            pj[u] = PlaneJob<double>{j.src, j.dst, nullptr, nullptr};
+            xj[u] = PencilJob<double>{j.dst, j.op};
+            ij[u] = PlaneJob<double>{j.dst, j.dst, nullptr, nullptr};
+        }
+        HIP_TRY(launch_plane<double>(g64, PLANE_FWD, pj, c, stream));
+        HIP_TRY(launch_pencil<double>(g64, PENCIL_FWD_MUL_INV, xj, c, stream));
+        HIP_TRY(launch_plane<double>(g64, PLANE_INV, ij, c, stream));
+    }
+    return FTS_OK;
+}
+
+int fts_engine::fft_apply_hat_n(const FftJob* jobs, int n)
+{
+    for (int k = 0; k < n; k += 2) {
+        const int c = std::min(2, n - k);
+        PencilJob<double> xj[2];
+        PlaneJob<double> ij[2];
+        for (int u = 0; u < c; ++u) {
+            const FftJob& j = jobs[k + u];
+            xj[u] = PencilJob<double>{j.dst, j.op, j.src};
+            ij[u] = PlaneJob<double>{j.dst, j.dst, nullptr, nullptr};
+        }
+        HIP_TRY(launch_pencil<double>(g64, PENCIL_MUL_INV, xj, c, stream));
+        HIP_TRY(launch_plane<double>(g64, PLANE_INV, ij, c, stream));
+    }
+    return FTS_OK;
+}
+
+int fts_engine::fft_forward_n(const FftJob* jobs, int n)
+{
+    for (int k = 0; k < n; k += 2) {
+        const int c = std::min(2, n - k);
+        if (!plane_ok) {
+            for (int u = 0; u < c; ++u)
+                if (int rc = fft_forward(jobs[k + u].src, jobs[k + u].dst)) return rc;
+            continue;
+        }
+        PlaneJob<double> pj[2];
+        PencilJob<double> xj[2];
+        for (int u = 0; u < c; ++u) {
+            pj[u] = PlaneJob<double>{jobs[k + u].src, jobs[k + u].dst, nullptr, nullptr};
+            xj[u] = PencilJob<double>{jobs[k + u].dst, KOp{}};
+        }
+        HIP_TRY(launch_plane<double>(g64, PLANE_FWD, pj, c, stream));
+        HIP_TRY(launch_pencil<double>(g64, PENCIL_FWD, xj, c, stream));
+    }
+    return FTS_OK;
+}
 
 int fts_engine::reduce_finalize(int Kc, int slot, double scale)
 {
@@ -1072,12 +1138,15 @@ int fts_engine::densities_end()
     const double invM = 1.0 / static_cast<double>(M);
     int rc;
     const double2* ps = dens_psi;
-    for (int al = 0; al < nT; ++al)
-        if ((rc = fft_apply(f(rho_hat, al), f(rho_type, al), kop_sep(sep_gamma.p, invM)))) return rc;
+    // the smoothed type densities and the polymer charge density: one convolution each
+    FftJob jb[FTS_MAX_TYPES + 1];
+    int nj = 0;
+    for (int al = 0; al < nT; ++al) jb[nj++] = FftJob{f(rho_hat, al), f(rho_type, al), kop_sep(sep_gamma.p, invM)};
+    if (electro()) jb[nj++] = FftJob{rho_c_hat.p, rho_c_poly.p, kop_sep(sep_gamma.p, invM)};
+    if ((rc = fft_apply_n(jb, nj))) return rc;
     hipLaunchKernelGGL(k_sum_fields, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho_type.p, nT, M, rho_p.p);
     HIP_TRY(hipGetLastError());
     if (electro()) {
-        if ((rc = fft_apply(rho_c_hat.p, rho_c_poly.p, kop_sep(sep_gamma.p, invM)))) return rc;
         HIP_TRY(hipMemcpyAsync(rho_c.p, rho_c_poly.p, sizeof(double2) * Ms, hipMemcpyDeviceToDevice, stream));
     } else {
         HIP_TRY(hipMemsetAsync(rho_c_poly.p, 0, sizeof(double2) * Ms, stream));
@@ -1085,26 +1154,39 @@ int fts_engine::densities_end()
     }
     HIP_TRY(hipMemsetAsync(rho_c_ion.p, 0, sizeof(double2) * Ms, stream));
     // ion::compute_densities (ion.cu:54-72)
-    for (size_t j = 0; j < ions.size(); ++j) {
-        const double nd = static_cast<double>(ions[j].n) / V;
-        double2* rho = f(ion_rho, j);
-        if (!electro()) {
-            hipLaunchKernelGGL(k_fill, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho, double2{nd, 0.0}, M);
+    const size_t nI = ions.size();
+    auto sep_of = [&](size_t j) { return sep_ion.p + j * static_cast<size_t>(m1 + m2 + m3); };
+    if (!electro()) {
+        for (size_t j = 0; j < nI; ++j) {
+            hipLaunchKernelGGL(k_fill, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, f(ion_rho, j), double2{static_cast<double>(ions[j].n) / V, 0.0}, M);
             HIP_TRY(hipGetLastError());
-            continue;
         }
-        const double* sepj = sep_ion.p + j * static_cast<size_t>(m1 + m2 + m3);
-        rc = psi_hat_valid(ps) ? fft_apply_hat(psi_hat.p, f(ion_psi, j), kop_sep(sepj, invM)) : fft_apply(ps, f(ion_psi, j), kop_sep(sepj, invM));
-        if (rc) return rc;
+        return FTS_OK;
+    }
+    // the smoothed potential of every ion species (all from psi)
+    FftJob ij[FTS_MAX_IONS];
+    const bool hat = psi_hat_valid(ps);
+    for (size_t j = 0; j < nI; ++j) ij[j] = FftJob{hat ? psi_hat.p : ps, f(ion_psi, j), kop_sep(sep_of(j), invM)};
+    if ((rc = hat ? fft_apply_hat_n(ij, static_cast<int>(nI)) : fft_apply_n(ij, static_cast<int>(nI)))) return rc;
+    for (size_t j = 0; j < nI; ++j) {
         hipLaunchKernelGGL(k_ion_boltzmann, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, f(ion_psi, j), ions[j].z, M, f(ion_b, j), partials.p);
         HIP_TRY(hipGetLastError());
         if ((rc = reduce_finalize(1, SC_QION + static_cast<int>(j), invM))) return rc;
-        hipLaunchKernelGGL(k_ion_density, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, f(ion_b, j), scal.p + SC_QION + j, nd, M, rho);
+        hipLaunchKernelGGL(k_ion_density, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, f(ion_b, j), scal.p + SC_QION + j,
+                           static_cast<double>(ions[j].n) / V, M, f(ion_rho, j));
         HIP_TRY(hipGetLastError());
-        // ion charge z S_{Gamma_j} rho_j (fft_apply_real_kernel_scaled, common.h:1087-1091)
-        if ((rc = fft_apply(rho, t1.p, kop_sep(sepj, ions[j].z * invM)))) return rc;
-        hipLaunchKernelGGL(k_add2, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho_c_ion.p, rho_c.p, t1.p, M);
-        HIP_TRY(hipGetLastError());
+    }
+    // ion charges z S_{Gamma_j} rho_j (fft_apply_real_kernel_scaled, common.h:1087-1091), two
+    // species at a time through t1 and t2, added in species order
+    double2* tmp[2] = {t1.p, t2.p};
+    for (size_t j0 = 0; j0 < nI; j0 += 2) {
+        const int c = static_cast<int>(std::min<size_t>(2, nI - j0));
+        for (int u = 0; u < c; ++u) ij[u] = FftJob{f(ion_rho, j0 + u), tmp[u], kop_sep(sep_of(j0 + u), ions[j0 + u].z * invM)};
+        if ((rc = fft_apply_n(ij, c))) return rc;
+        for (int u = 0; u < c; ++u) {
+            hipLaunchKernelGGL(k_add2, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, rho_c_ion.p, rho_c.p, tmp[u], M);
+            HIP_TRY(hipGetLastError());
+        }
     }
     return FTS_OK;
 }
@@ -1269,17 +1351,17 @@ int fts_engine::measure_queue(const double2* om, const double2* ps, fts_observab
         }
         // polymer field coupling (one row per type) and, with electrostatics, the psi row and one row per ion
         double2* fixed = stress_acc.p + fixed_rows * 7;
-        for (int al = 0; al < nT; ++al) {
-            if ((rc = fft_forward(f(W_type_s, al), t1.p)) || (rc = fft_forward(f(rho_hat, al), t2.p))) return rc;
-            if ((rc = contract_to(t1.p, t2.p, false, nullptr, fixed + static_cast<size_t>(al) * 7))) return rc;
-        }
+        auto pair = [&](const double2* a, const double2* b, size_t row) {
+            const FftJob jb[2] = {{a, t1.p, KOp{}}, {b, t2.p, KOp{}}};
+            int r = fft_forward_n(jb, 2);
+            return r ? r : contract_to(t1.p, t2.p, false, nullptr, fixed + row * 7);
+        };
+        for (int al = 0; al < nT; ++al)
+            if ((rc = pair(f(W_type_s, al), f(rho_hat, al), static_cast<size_t>(al)))) return rc;
         if (electro()) {
-            if ((rc = fft_forward(psi_s.p, t1.p)) || (rc = fft_forward(rho_c_hat.p, t2.p))) return rc;
-            if ((rc = contract_to(t1.p, t2.p, false, nullptr, fixed + static_cast<size_t>(nT) * 7))) return rc;
-            for (size_t j = 0; j < ions.size(); ++j) {
-                if ((rc = fft_forward(f(ion_psi, j), t1.p)) || (rc = fft_forward(f(ion_b, j), t2.p))) return rc;
-                if ((rc = contract_to(t1.p, t2.p, false, nullptr, fixed + (static_cast<size_t>(nT) + 1 + j) * 7))) return rc;
-            }
+            if ((rc = pair(psi_s.p, rho_c_hat.p, static_cast<size_t>(nT)))) return rc;
+            for (size_t j = 0; j < ions.size(); ++j)
+                if ((rc = pair(f(ion_psi, j), f(ion_b, j), static_cast<size_t>(nT) + 1 + j))) return rc;
         }
         HIP_TRY(hipMemcpyAsync(h_acc, stress_acc.p, sizeof(double2) * stress_acc.n, hipMemcpyDeviceToHost, stream));
     }
@@ -1583,19 +1665,26 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
     if (n_omega > 0) {
         hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, seed, t, M, si ? 0 : 1, omega.p, drift.p);
         HIP_TRY(hipGetLastError());
-        if (si) {
-            for (int l = 0; l < n_omega; ++l)
-                if ((rc = fft_apply(f(drift, l), f(drift, l), KOp{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0}))) return rc;
+    }
+    if (electro()) {
+        hipLaunchKernelGGL(k_psi_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, dpsi.p, -dt * lambda_psi, sigma_psi,
+                           scal.p + SC_ETA_MEAN, seed, t, M, dpsi.p);
+        HIP_TRY(hipGetLastError());
+    }
+    if (si) {
+        // the semi-implicit operator on every increment (independent fields: shared launches)
+        FftJob jb[FTS_MAX_TYPES + 1];
+        int nj = 0;
+        for (int l = 0; l < n_omega; ++l) jb[nj++] = FftJob{f(drift, l), f(drift, l), KOp{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0}};
+        if (electro()) jb[nj++] = FftJob{dpsi.p, dpsi.p, KOp{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0}};
+        if ((rc = fft_apply_n(jb, nj))) return rc;
+        if (n_omega > 0) {
             const size_t n = static_cast<size_t>(n_omega) * Ms;
             hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream, omega.p, drift.p, n);
             HIP_TRY(hipGetLastError());
         }
     }
     if (electro()) {
-        hipLaunchKernelGGL(k_psi_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, dpsi.p, -dt * lambda_psi, sigma_psi,
-                           scal.p + SC_ETA_MEAN, seed, t, M, dpsi.p);
-        HIP_TRY(hipGetLastError());
-        if (si && (rc = fft_apply(dpsi.p, dpsi.p, KOp{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0}))) return rc;
         psi_hat_of = nullptr; // psi changes below
         hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, Ms);
         HIP_TRY(hipGetLastError());
