@@ -29,7 +29,12 @@ struct KOp {
     const double* table;
     double c0, c1, c2;
     int ka, kb;
+    int table_noise; // multiply by sqrt(t (1 + t) / 2) instead of the table value t
 };
+
+// the table factor of a KOp: t, or sqrt(t (1 + t) / 2) (the noise factor that goes with a
+// semi-implicit table t = 1 / (1 + a))
+__host__ __device__ __forceinline__ double kop_table(const KOp& op, double t) { return op.table_noise ? sqrt(0.5 * t * (1.0 + t)) : t; }
 
 enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, PLANE_LAST = 4, PLANE_CONV_E = 5 };
 // PENCIL_CONV: forward along x, x op.sep[x] op.c0 (the x factor of a separable
@@ -95,6 +100,7 @@ __device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
     k.c2 = f ? a.c2 : b.c2;
     k.ka = f ? a.ka : b.ka;
     k.kb = f ? a.kb : b.kb;
+    k.table_noise = f ? a.table_noise : b.table_noise;
     return k;
 }
 template <typename T> __device__ __forceinline__ PlaneJob<T> pick_job(PlaneJob<T> a, PlaneJob<T> b, const PlaneJob<T>* table)

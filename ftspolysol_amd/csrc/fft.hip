@@ -175,7 +175,7 @@ __device__ __forceinline__ double kop_value(const KOp& op, const double* kvec, i
 {
     double v = 1.0;
     if (op.sep) v = op.sep[x] * op.sep[m1 + y] * op.sep[m1 + m2 + z];
-    if (op.table) v *= op.table[idx];
+    if (op.table) v *= kop_table(op, op.table[idx]);
     if (op.c1 != 0.0 || op.c2 != 0.0) {
         const double k[3] = {kvec[x], kvec[m1 + y], kvec[m1 + m2 + z]};
         double poly = op.c0;
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         for (int i = tid; i < N1 * ZC; i += NT) {
             const int x = i / ZC;
             double kv = op.sep ? op.sep[x] * syz : 1.0;
-            if (op.table) kv *= op.table[static_cast<size_t>(x) * sx + j];
+            if (op.table) kv *= kop_table(op, op.table[static_cast<size_t>(x) * sx + j]);
             if (op.c1 != 0.0 || op.c2 != 0.0) {
                 const double k[3] = {g.kvec[x], ky, kz};
                 double poly = op.c0;

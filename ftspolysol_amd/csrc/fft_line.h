@@ -64,7 +64,7 @@ __device__ __forceinline__ double line_kop_value(const KOp& op, const FftGeom& g
 {
     const int off = lk.axis == 0 ? 0 : g.m1;
     double kv = op.sep ? op.sep[off + t] * lk.fixed : 1.0;
-    if (op.table) kv *= op.table[gi];
+    if (op.table) kv *= kop_table(op, op.table[gi]);
     if (op.c1 != 0.0 || op.c2 != 0.0) {
         double k[3] = {lk.kf[0], lk.kf[1], lk.kf[2]};
         k[lk.axis] = g.kvec[off + t];

@@ -52,11 +52,11 @@ extern "C" {
 #define FTS_SYMMETRY_OFF 1  /* always run both sweeps */
 
 /* bead-step FFT schedules (fts_system.fft_schedule, fts_info.fft_schedule) */
-#define FTS_SCHED_AUTO 0  /* time both at creation, keep the faster */
+#define FTS_SCHED_AUTO 0  /* time plane and line at creation, keep the faster */
 #define FTS_SCHED_PLANE 1 /* pencil(x: FFT, x phi/M, IFFT) + plane(yz: IFFT, x e^{-W}, store q, FFT); needs m2 == m3, plane in LDS */
 #define FTS_SCHED_LINE 2  /* four line passes x, y, z, y; any supported lengths */
-#define FTS_SCHED_CONV 3  /* plane(yz: FFT, x phi_y phi_z/M, IFFT) + pencil(x: FFT, x phi_x, IFFT, x e^{-W}, store q);
-                             separable bond kernels; needs m2 == m3, plane in LDS */
+#define FTS_SCHED_CONV 3  /* pencil(x: FFT of q_s, x phi_x/M, IFFT) + plane(yz: FFT, x phi_y phi_z, IFFT, x e^{-W},
+                             store q_{s+1}); separable bond kernels; needs m2 == m3, plane in LDS; used only when asked for */
 
 /* stress_mask bits: components in the reference's fixed order (stress.cu:56-61) */
 #define FTS_STRESS_XX 0x01
@@ -382,9 +382,12 @@ int fts_rf_write(const char* path, const int* m, const double* L, int n_fields, 
  * No counterpart in the reference, which has no semi-implicit scheme at all.
  *
  * The semi-implicit update (FTS_SCHEME_SEMI_IMPLICIT) is
- *     w <- w + S_P[-dt lam u(w) + eta],   P(k) = 1 / (1 + dt lam A(k)),
+ *     w <- w + S_P[-dt lam u(w)] + S_Pn[eta],   P(k) = 1 / (1 + dt lam A(k)),
+ *     Pn(k) = sqrt(P (1 + P) / 2) = P sqrt(1 + dt lam A / 2),
  * with A = A_D, the Debye (discrete-chain RPA) linear response of the
- * homogeneous state.  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
+ * homogeneous state.  The noise factor Pn makes the stationary variance of a
+ * linear mode (u = A w) exactly 1 / A at any dt; P alone would give
+ * 2 / (A (2 + dt lam A)), an O(dt) deficit of the short-wavelength fluctuations.  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
  *     A(k) = A_D(k) exp(r_c(bin(|k|))),   r_c = 4 tanh(z_c / 4),
  * where z = MLP(|k|, moments of the current fields): field c = 0 is interaction
  * mode 0, c = 1 is psi.  The MLP has FTS_LRK_FEATURES inputs (padded to

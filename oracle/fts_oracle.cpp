@@ -1294,7 +1294,21 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     drift_statistics(false);
     std::vector<double> mult[2];
     if (semi_implicit && lrk_active()) lrk_step(mult);
-    std::vector<double> P(static_cast<size_t>(M));
+    // Semi-implicit: w += S_P[-dt lam u] + S_Pn[eta] with P = 1 / (1 + a), a = dt lam A(k),
+    // and the noise factor Pn = sqrt(P (1 + P) / 2) = P sqrt(1 + a / 2): for a linear drift
+    // u = A w the stationary variance of every mode is then exactly 1 / A at any dt
+    // (eta transformed by P would give 2 / (A (2 + a))).  EM adds eta to the drift.
+    const bool split = semi_implicit && noise;
+    std::vector<double> P(static_cast<size_t>(M)), Pn(split ? static_cast<size_t>(M) : 0);
+    std::vector<Cx> nz(split ? static_cast<size_t>(M) : 0);
+    auto add_noise = [&](Cx* w, const double* eta) {
+        for (int i = 0; i < M; ++i) {
+            nz[static_cast<size_t>(i)] = Cx{eta[i], 0.0};
+            Pn[static_cast<size_t>(i)] = std::sqrt(0.5 * P[static_cast<size_t>(i)] * (1.0 + P[static_cast<size_t>(i)]));
+        }
+        apply_kernel(nz.data(), nz.data(), Pn.data());
+        axpy(w, nz.data(), 1.0);
+    };
     for (int l = 0; l < n_omega; ++l) {
         const double lam = lambda_omega[static_cast<size_t>(l)];
         Cx* w = fld(omega, l);
@@ -1302,7 +1316,7 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
         const double* eta = eta_omega.data() + static_cast<size_t>(l) * M;
         for (int i = 0; i < M; ++i) {
             r[i] = cscale(r[i], -dt * lam);
-            r[i].x += eta[i];
+            if (!split) r[i].x += eta[i];
         }
         if (semi_implicit) {
             for (int i = 0; i < M; ++i) {
@@ -1312,12 +1326,13 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
             apply_kernel(r, r, P.data());
         }
         axpy(w, r, 1.0);
+        if (split) add_noise(w, eta);
     }
     if (electrostatics()) {
         Cx* r = dpsi.data();
         for (int i = 0; i < M; ++i) {
             r[i] = cscale(r[i], -dt * lambda_psi);
-            r[i].x += etaPsi[static_cast<size_t>(i)];
+            if (!split) r[i].x += etaPsi[static_cast<size_t>(i)];
         }
         if (semi_implicit) {
             for (int i = 0; i < M; ++i) {
@@ -1327,6 +1342,7 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
             apply_kernel(r, r, P.data());
         }
         axpy(psi.data(), r, 1.0);
+        if (split) add_noise(psi.data(), etaPsi.data());
     }
     remove_psi_zero_mode(psi.data());
     ++step_counter;
