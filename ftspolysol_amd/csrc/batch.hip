@@ -65,6 +65,7 @@ struct Group {
     std::vector<std::unique_ptr<SweepTables>> tables; // full sweeps, per species index and schedule
     double2* hQ = nullptr;                             // pinned [2][reps][FTS_MAX_SPECIES]: Q, Q at (w_bar, psi_bar)
     int* hflag = nullptr;                              // pinned [reps]
+    double2* hdrift = nullptr;                         // pinned [reps][FTS_MAX_TYPES + 1]: drift statistics of the step
     // The work of single replicas (smoothing FFTs, ions, contractions, updates)
     // runs on side streams forked from the group stream and joined back into it,
     // so that the small launches of several replicas share the chip; replica r
@@ -486,6 +487,9 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
         else c = e->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT || scheme == FTS_SCHEME_SI_LEARNED);
         if (c) return c;
         ++e->step_counter;
+        if (e->drift_stats)
+            HIP_TRY(hipMemcpyAsync(g.hdrift + r * (FTS_MAX_TYPES + 1), e->scal.p + SC_DRIFT, sizeof(double2) * (static_cast<size_t>(e->n_omega) + 1),
+                                   hipMemcpyDeviceToHost, e->stream));
         return e->finite_queue(g.hflag + r);
     };
     const bool pc = scheme == FTS_SCHEME_PC_REFERENCE;
@@ -588,6 +592,7 @@ struct fts_batch {
         for (auto& g : groups) {
             g->tables.clear();
             if (g->hQ) (void)hipHostFree(g->hQ);
+            if (g->hdrift) (void)hipHostFree(g->hdrift);
             if (g->hflag) (void)hipHostFree(g->hflag);
             set_side_streams(*g, 0);
             if (g->forked) (void)hipEventDestroy(g->forked);
@@ -635,6 +640,7 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
         Group& g = *gp;
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hQ), sizeof(double2) * FTS_MAX_SPECIES * 2 * g.reps.size()));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hflag), sizeof(int) * g.reps.size()));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hdrift), sizeof(double2) * (FTS_MAX_TYPES + 1) * g.reps.size()));
         g.plane_ok = true;
         for (fts_engine* e : g.reps) g.plane_ok = g.plane_ok && e->plane_ok;
         const int want = systems[g.ids[0]].fft_schedule;
@@ -722,13 +728,20 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
             if (g.rc) break;
             for (size_t r = 0; r < n_rep; ++r) {
                 const int id = g.ids[r];
+                fts_engine* e = g.reps[r];
                 if (rec[r]) {
                     rec[r]->step = label[r];
                     ++n_records[id];
                 }
-                if (why[r].empty()) continue;
+                if (why[r].empty()) {
+                    // drift statistics of an accepted step: [step, modes..., psi], as fts_step
+                    if (mask[r] && e->drift_stats) {
+                        e->drift_rows.push_back(static_cast<double>(label[r]));
+                        for (int k = 0; k <= e->n_omega; ++k) e->drift_rows.push_back(g.hdrift[r * (FTS_MAX_TYPES + 1) + static_cast<size_t>(k)].x);
+                    }
+                    continue;
+                }
                 // non-finite: restart this replica from its checkpoint with new noise, or give up
-                fts_engine* e = g.reps[r];
                 g_error = why[r];
                 int rr = e->restart_from_checkpoint(label[r]);
                 if (rr) {
@@ -737,6 +750,9 @@ int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* recor
                 }
                 if (n_records)
                     while (n_records[id] > 0 && records[static_cast<size_t>(id) * capacity + n_records[id] - 1].step > e->ckpt_step) --n_records[id];
+                const size_t w = static_cast<size_t>(e->n_omega) + 2;
+                while (!e->drift_rows.empty() && e->drift_rows[e->drift_rows.size() - w] > static_cast<double>(e->ckpt_step))
+                    e->drift_rows.resize(e->drift_rows.size() - w);
             }
         }
         if (g.rc) g.err = g_error;

@@ -7,18 +7,19 @@ removal after the update changes nothing), so the row of a step must equal
 max |w_(t+1) - w_t| / dt and max |psi_(t+1) - psi_t| / dt, read from the fields
 before and after it; an EM step moves them by -dt lambda u, the statistic
 reported for EM.  A restart from the checkpoint drops the rows after it.
-GPU: the engine's rows against the oracle's on the same fields and noise.
+GPU: the engine's rows against the oracle's on the same fields and noise, and the
+rows of a batch's replicas against single engines of the same systems.
 """
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, SCHEME_EM, SCHEME_PC_REFERENCE, hip_engine
+from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, SCHEME_EM, SCHEME_PC_REFERENCE, hip_batch, hip_engine
 from ftspolysol_amd.system import initial_fields, make_system
 from oracle_lib import oracle_engine
 
 
-def small(scheme, **kw):
-    return make_system(12, 6, rho=0.2, salt=0.05, B=0.5, lB=1.0, dt=0.01, scheme=scheme, seed=91, **kw)
+def small(scheme, seed=91, **kw):
+    return make_system(12, 6, rho=0.2, salt=0.05, B=0.5, lB=1.0, dt=0.01, scheme=scheme, seed=seed, **kw)
 
 
 def fields(e):
@@ -81,3 +82,26 @@ def test_engine_drift_rows_match_oracle(scheme):
     assert a.shape == b.shape == (4, 3)
     assert np.array_equal(a[:, 0], b[:, 0])
     np.testing.assert_allclose(a[:, 1:], b[:, 1:], rtol=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM])
+def test_batch_drift_rows_match_single_engines(scheme):
+    """Replicas of a batch collect the same rows as single engines of their systems."""
+    print(f"batch drift rows scheme={scheme}", flush=True)
+    systems = [small(scheme, seed=91 + r) for r in range(3)]
+    b = hip_batch(systems)
+    singles = [hip_engine(s) for s in systems]
+    for r, s in enumerate(systems):
+        for e in (b.replicas[r], singles[r]):
+            e.set_fields(*initial_fields(s, sigma=0.05, seed=40 + r))
+            e.set_drift_stats(True)
+    b.step(3)
+    b.step(2)
+    for r, e in enumerate(singles):
+        e.step(5)
+        a, c = b.replicas[r].get_drift_rows(), e.get_drift_rows()
+        assert a.shape == c.shape == (5, 3)
+        assert list(a[:, 0]) == [1, 2, 3, 4, 5]
+        np.testing.assert_allclose(a[:, 1:], c[:, 1:], rtol=1e-12)
+    b.close()
