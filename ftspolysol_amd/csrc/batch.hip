@@ -58,7 +58,7 @@ struct Group {
     hipStream_t stream = nullptr;
     std::vector<fts_engine*> reps;
     std::vector<int> ids; // batch index of each replica
-    int sched = FTS_SCHED_PLANE;
+    int sched = FTS_SCHED_CONV;
     bool plane_ok = true;
     bool use_graph = true; // replay captured sweep graphs (fts_batch_set_graph)
     double tune_ms[2] = {0.0, 0.0};
@@ -512,9 +512,9 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
     return FTS_OK;
 }
 
-// Schedule of a group's sweeps: plane and line timed over 8 bead steps of species 0
-// with all replicas in each launch, best of three; tune_ms[k] belongs to schedule
-// k + 1 (conv only when asked for, as for single engines).
+// Schedule of a group's sweeps: conv and line timed over 8 bead steps of species 0
+// with all replicas in each launch, best of three; tune_ms = {conv, line} (plane only
+// when asked for, as for single engines).
 template <typename T> int group_autotune(Group& g)
 {
     if (!g.plane_ok) {
@@ -524,7 +524,8 @@ template <typename T> int group_autotune(Group& g)
     SweepTables t[2];
     const std::vector<char> all(g.reps.size(), 1);
     int rc = FTS_OK;
-    for (int k = 0; k < 2 && rc == FTS_OK; ++k) rc = build_tables<T>(g, 0, k + 1, 8, all, t[k]);
+    const int cand[2] = {FTS_SCHED_CONV, FTS_SCHED_LINE};
+    for (int k = 0; k < 2 && rc == FTS_OK; ++k) rc = build_tables<T>(g, 0, cand[k], 8, all, t[k]);
     if (rc) return rc;
     if (t[0].steps == 0) return FTS_OK;
     hipEvent_t a = nullptr, b = nullptr;
@@ -547,7 +548,7 @@ template <typename T> int group_autotune(Group& g)
         g.tune_ms[k] = best[k];
         if (best[k] < best[pick]) pick = k;
     }
-    g.sched = pick + 1;
+    g.sched = cand[pick];
     return FTS_OK;
 }
 

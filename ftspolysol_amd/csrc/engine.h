@@ -221,7 +221,7 @@ struct fts_engine {
     template <typename T> int run_sweeps(int p, int schedule, int nsteps, bool timed);
     int autotune_schedule();
     template <typename T> int bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op);
-    int sched = FTS_SCHED_PLANE;
+    int sched = FTS_SCHED_CONV;
     bool plane_ok = true;
     size_t norm_stride = 0;
     double tune_ms[2] = {0.0, 0.0};

@@ -977,9 +977,9 @@ template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T
 
 // Bead-step schedule chosen by timing plane and line on this grid and precision (the
 // FFT autotuner): 2 x 8 bead steps of species 0 at zero fields, best of three;
-// tune_ms[k] belongs to schedule k + 1.  The conv schedule is taken only when asked
-// for: it differs from the plane schedule by a few percent either way, so timing it
-// would make the choice, and the last bits of the results, vary from run to run.
+// tune_ms = {conv, line}.  The plane schedule is taken only when asked for: it is
+// within a few percent of conv either way, so timing it too would make the choice,
+// and the last bits of the results, vary from run to run.
 int fts_engine::autotune_schedule()
 {
     if (!plane_ok) {
@@ -988,16 +988,17 @@ int fts_engine::autotune_schedule()
     }
     const HostSpecies& sp = species[0];
     if (sp.N < 2) {
-        sched = FTS_SCHED_PLANE;
+        sched = FTS_SCHED_CONV;
         return FTS_OK;
     }
     const int steps = std::min(8, sp.N - 1);
+    const int cand[2] = {FTS_SCHED_CONV, FTS_SCHED_LINE};
     double best[2] = {1e30, 1e30};
     hipEvent_t a = get_event(), b = get_event();
     for (int rep = 0; rep < 3; ++rep)
         for (int k = 0; k < 2; ++k) {
             HIP_TRY(hipEventRecord(a, stream));
-            int rc = precision == FTS_PREC_FP32 ? run_sweeps<float>(0, k + 1, steps, false) : run_sweeps<double>(0, k + 1, steps, false);
+            int rc = precision == FTS_PREC_FP32 ? run_sweeps<float>(0, cand[k], steps, false) : run_sweeps<double>(0, cand[k], steps, false);
             if (rc) return rc;
             HIP_TRY(hipEventRecord(b, stream));
             HIP_TRY(hipEventSynchronize(b));
@@ -1012,7 +1013,7 @@ int fts_engine::autotune_schedule()
         tune_ms[k] = best[k];
         if (best[k] < best[pick]) pick = k;
     }
-    sched = pick + 1;
+    sched = cand[pick];
     return FTS_OK;
 }
 
@@ -1820,7 +1821,7 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     e->seed0 = s->seed;
     e->stress_mask = s->stress_mask;
     e->plane_ok = plane_fits;
-    e->sched = s->fft_schedule == FTS_SCHED_AUTO ? (plane_fits ? FTS_SCHED_PLANE : FTS_SCHED_LINE) : s->fft_schedule;
+    e->sched = s->fft_schedule == FTS_SCHED_AUTO ? (plane_fits ? FTS_SCHED_CONV : FTS_SCHED_LINE) : s->fft_schedule;
     int rc = e->build_modes(s);
     if (rc) return bail(rc);
     e->total_type_segments.assign(static_cast<size_t>(e->nT), 0);

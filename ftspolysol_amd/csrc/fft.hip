@@ -19,6 +19,24 @@ template <int N> constexpr int plane_threads()
 }
 template <int N1> constexpr int pencil_threads() { return PENCIL_ZC * (N1 / plan_e(N1)); }
 
+// two consecutive complex FP32 values as one 16-byte access (16-byte aligned: every
+// field starts at a multiple of an even point count)
+template <typename V> __device__ __forceinline__ void ld_pair(const V* p, V& a, V& b)
+{
+    const float4 w = *reinterpret_cast<const float4*>(p);
+    a = cmk<V>(w.x, w.y);
+    b = cmk<V>(w.z, w.w);
+}
+template <typename V> __device__ __forceinline__ void st_pair(V* p, V a, V b)
+{
+    float4 w;
+    w.x = a.x;
+    w.y = a.y;
+    w.z = b.x;
+    w.w = b.y;
+    *reinterpret_cast<float4*>(p) = w;
+}
+
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j0, PlaneJob<T> j1, const PlaneJob<T>* __restrict__ table, const V2<T>* __restrict__ twg)
 {
@@ -73,34 +91,45 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     double nrm = 0.0;
 
     const V* src = (MODE == PLANE_INIT) ? job.emw : job.src;
+    // FP32: pairs of points per 16-byte access
+    constexpr bool PAIRS = sizeof(T) == 4 && N % 2 == 0;
+    if constexpr (PAIRS) {
 #pragma unroll
-    for (int i = tid; i < N * N; i += NT) {
-        V v = src[plane + i];
-        if constexpr (MODE == PLANE_INIT) {
-            v = cscale(v, c);
-            job.qout[plane + i] = v;
-            nrm = fmax(nrm, static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
+        for (int j = tid; j < N * N / 2; j += NT) {
+            const int i = 2 * j;
+            V v0, v1;
+            ld_pair(src + plane + i, v0, v1);
+            if constexpr (MODE == PLANE_INIT) {
+                v0 = cscale(v0, c);
+                v1 = cscale(v1, c);
+                st_pair(job.qout + plane + i, v0, v1);
+                nrm = fmax(nrm, fmax(static_cast<double>(v0.x) * v0.x + static_cast<double>(v0.y) * v0.y,
+                                     static_cast<double>(v1.x) * v1.x + static_cast<double>(v1.y) * v1.y));
+            }
+            V* l = &lds[(i / N) * LS + (i % N)];
+            l[0] = v0;
+            l[1] = v1;
         }
-        lds[(i / N) * LS + (i % N)] = v;
+    } else {
+#pragma unroll
+        for (int i = tid; i < N * N; i += NT) {
+            V v = src[plane + i];
+            if constexpr (MODE == PLANE_INIT) {
+                v = cscale(v, c);
+                job.qout[plane + i] = v;
+                nrm = fmax(nrm, static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
+            }
+            lds[(i / N) * LS + (i % N)] = v;
+        }
     }
     if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) renorm();
     __syncthreads();
 
-    // PLANE_CONV_E in FP32 loads this thread's e^{-W} values before the inverse
-    // transforms, so the loads complete under them (PER registers pairs; FP64 would
-    // lose occupancy)
-    constexpr int PER = (N * N) / NT;
-    constexpr bool PREFETCH = MODE == PLANE_CONV_E && (N * N) % NT == 0 && PER <= 16 && sizeof(T) == 4;
-    V ew[PREFETCH ? PER : 1];
     if constexpr (MODE == PLANE_CONV_E) {
         // the plane's share of the convolution: forward along z and y, x phi_y phi_z
         // (the pencil pass has applied phi_x / M), inverse below
         lds_fft<T, N, -1, N, LS, 1, true, NT>(lds, tw, tid);
         lds_fft<T, N, -1, N, 1, LS, true, NT>(lds, tw, tid);
-        if constexpr (PREFETCH) {
-#pragma unroll
-            for (int k = 0; k < PER; ++k) ew[k] = job.emw[plane + tid + k * NT];
-        }
         const double* sy = job.op.sep + gridDim.x; // sep holds the m1, m2, m3 factors; gridDim.x = m1
         const double* sz = sy + N;
         if constexpr (NT % N == 0) {
@@ -124,15 +153,25 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         lds_fft<T, N, +1, N, LS, 1, true, NT>(lds, tw, tid); // along z
         lds_fft<T, N, +1, N, 1, LS, true, NT>(lds, tw, tid);  // along y
     }
-    if constexpr (PREFETCH) {
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = tid + k * NT;
-            V v = cmul(lds[(i / N) * LS + (i % N)], ew[k]);
-            if (job.norm_in) v = cscale(v, c);
-            job.qout[plane + i] = v;
-            nrm = fmax(nrm, static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
+    if constexpr (PAIRS && (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E)) {
+#pragma unroll 2
+        for (int j = tid; j < N * N / 2; j += NT) {
+            const int i = 2 * j;
+            V* p = &lds[(i / N) * LS + (i % N)];
+            V e0, e1;
+            ld_pair(job.emw + plane + i, e0, e1);
+            V v0 = cmul(p[0], e0), v1 = cmul(p[1], e1);
+            if (job.norm_in) {
+                v0 = cscale(v0, c);
+                v1 = cscale(v1, c);
+            }
+            p[0] = v0;
+            p[1] = v1;
+            st_pair(job.qout + plane + i, v0, v1);
+            nrm = fmax(nrm, fmax(static_cast<double>(v0.x) * v0.x + static_cast<double>(v0.y) * v0.y,
+                                 static_cast<double>(v1.x) * v1.x + static_cast<double>(v1.y) * v1.y));
         }
+        __syncthreads();
     } else if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) {
@@ -166,8 +205,17 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         lds_fft<T, N, -1, N, 1, LS, true, NT>(lds, tw, tid);
     }
     if constexpr (MODE != PLANE_LAST && MODE != PLANE_CONV_E) {
+        if constexpr (PAIRS) {
+#pragma unroll 2
+            for (int j = tid; j < N * N / 2; j += NT) {
+                const int i = 2 * j;
+                const V* l = &lds[(i / N) * LS + (i % N)];
+                st_pair(job.dst + plane + i, l[0], l[1]);
+            }
+        } else {
 #pragma unroll 4
-        for (int i = tid; i < N * N; i += NT) job.dst[plane + i] = lds[(i / N) * LS + (i % N)];
+            for (int i = tid; i < N * N; i += NT) job.dst[plane + i] = lds[(i / N) * LS + (i % N)];
+        }
     }
 }
 
@@ -205,10 +253,21 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
     for (int t = tid; t < N1; t += NT) tw[t] = twg[t];
     V* data = job.data;
     const V* in = job.src ? job.src : data;
+    // FP32: pairs of columns per 16-byte access (ZC and sx are even)
+    constexpr bool PAIRS = sizeof(T) == 4 && ZC % 2 == 0;
+    if constexpr (PAIRS) {
 #pragma unroll 4
-    for (int i = tid; i < N1 * ZC; i += NT) {
-        const int x = i / ZC, c = i % ZC;
-        lds[x * LS + c] = in[static_cast<size_t>(x) * sx + jbase + c];
+        for (int j = tid; j < N1 * ZC / 2; j += NT) {
+            const int i = 2 * j;
+            const int x = i / ZC, c = i % ZC;
+            ld_pair(in + static_cast<size_t>(x) * sx + jbase + c, lds[x * LS + c], lds[x * LS + c + 1]);
+        }
+    } else {
+#pragma unroll 4
+        for (int i = tid; i < N1 * ZC; i += NT) {
+            const int x = i / ZC, c = i % ZC;
+            lds[x * LS + c] = in[static_cast<size_t>(x) * sx + jbase + c];
+        }
     }
     __syncthreads();
     if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_CONV) lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
@@ -252,10 +311,19 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
     }
     if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL_INV || MODE == PENCIL_CONV)
         lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
+    if constexpr (PAIRS) {
 #pragma unroll 4
-    for (int i = tid; i < N1 * ZC; i += NT) {
-        const int x = i / ZC, c = i % ZC;
-        data[static_cast<size_t>(x) * sx + jbase + c] = lds[x * LS + c];
+        for (int j = tid; j < N1 * ZC / 2; j += NT) {
+            const int i = 2 * j;
+            const int x = i / ZC, c = i % ZC;
+            st_pair(data + static_cast<size_t>(x) * sx + jbase + c, lds[x * LS + c], lds[x * LS + c + 1]);
+        }
+    } else {
+#pragma unroll 4
+        for (int i = tid; i < N1 * ZC; i += NT) {
+            const int x = i / ZC, c = i % ZC;
+            data[static_cast<size_t>(x) * sx + jbase + c] = lds[x * LS + c];
+        }
     }
 }
 

@@ -83,10 +83,10 @@ def test_batch_headline_autotuned():
     systems = [make_system(96, 100, scheme=SCHEME_EM, precision=PREC_FP32, seed=7 + r) for r in range(4)]
     b = hip_batch(systems)
     sched = b.info.group_schedule[0]
-    print("batch tune ms (plane, line):", list(b.info.group_tune_ms[0]), "chosen", sched, flush=True)
-    assert sched in (SCHED_PLANE, SCHED_LINE)
+    print("batch tune ms (conv, line):", list(b.info.group_tune_ms[0]), "chosen", sched, flush=True)
+    assert sched in (SCHED_CONV, SCHED_LINE)
     tune = list(b.info.group_tune_ms[0])
-    assert min(tune) > 0 and tune[sched - 1] == min(tune)
+    assert min(tune) > 0 and tune[0 if sched == SCHED_CONV else 1] == min(tune)
     start_fields(b.replicas, systems)
     b.set_timing(True)
     rb = b.step(3, record_every=1)
