@@ -207,6 +207,11 @@ struct fts_engine {
         KOp op;
     };
     int fft_apply_n(const FftJob* jobs, int n);
+    // An FP32 engine on the plane passes transforms its FP64 fields in complex FP32
+    // (the passes read and write the FP64 fields, the spectra in between live in
+    // tmpF / tmpB, free outside the sweeps); an FP64 engine in FP64.
+    bool fields_fp32() const { return precision == FTS_PREC_FP32 && plane_ok; }
+    int fft32_n(int kind, const FftJob* jobs, int n); // kind: 0 apply, 1 apply from a spectrum, 2 forward
     int fft_apply_hat_n(const FftJob* jobs, int n);
     int fft_forward_n(const FftJob* jobs, int n);
     template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst);

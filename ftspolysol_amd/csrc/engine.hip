@@ -640,6 +640,10 @@ int fts_engine::lrk_step()
 // dst = IFFT[K(k) FFT[src]]; callers fold 1/M into K (fft_apply_real_kernel, common.h:1079-1085)
 int fts_engine::fft_apply(const double2* src, double2* dst, const KOp& op)
 {
+    if (fields_fp32()) {
+        const FftJob j{src, dst, op};
+        return fft32_n(0, &j, 1);
+    }
     if (plane_ok) {
         PlaneJob<double> pj{src, dst, nullptr, nullptr};
         HIP_TRY(launch_plane<double>(g64, PLANE_FWD, &pj, 1, stream));
@@ -664,6 +668,10 @@ int fts_engine::fft_apply(const double2* src, double2* dst, const KOp& op)
 // out of place, then the yz pass; the same arithmetic as fft_apply on the field
 int fts_engine::fft_apply_hat(const double2* hat, double2* dst, const KOp& op)
 {
+    if (fields_fp32()) {
+        const FftJob j{hat, dst, op};
+        return fft32_n(1, &j, 1);
+    }
     PencilJob<double> xj{dst, op, hat};
     HIP_TRY(launch_pencil<double>(g64, PENCIL_MUL_INV, &xj, 1, stream));
     PlaneJob<double> ij{dst, dst, nullptr, nullptr};
@@ -690,10 +698,51 @@ template <typename T> int fts_engine::fft_forward_t(const V2<T>* src, V2<T>* dst
     return FTS_OK;
 }
 
-int fts_engine::fft_forward(const double2* src, double2* dst) { return fft_forward_t<double>(src, dst); }
+int fts_engine::fft_forward(const double2* src, double2* dst)
+{
+    if (fields_fp32()) {
+        const FftJob j{src, dst, KOp{}};
+        return fft32_n(2, &j, 1);
+    }
+    return fft_forward_t<double>(src, dst);
+}
+
+// FP64 fields through FP32 plane / pencil passes, two jobs per launch
+int fts_engine::fft32_n(int kind, const FftJob* jobs, int n)
+{
+    float2* scr[2] = {static_cast<float2*>(tmpF), static_cast<float2*>(tmpB)};
+    for (int k = 0; k < n; k += 2) {
+        const int c = std::min(2, n - k);
+        PlaneJob<float> pj[2] = {};
+        PencilJob<float> xj[2] = {};
+        for (int u = 0; u < c; ++u) {
+            const FftJob& j = jobs[k + u];
+            xj[u].data = scr[u];
+            xj[u].op = j.op;
+            if (kind == 0 || kind == 2) {
+                pj[u].dst = scr[u];
+                pj[u].src64 = j.src;
+            } else {
+                xj[u].src64 = j.src;
+            }
+            if (kind == 2) xj[u].dst64 = j.dst;
+        }
+        if (kind == 0 || kind == 2) HIP_TRY(launch_plane<float>(g32, PLANE_FWD_D2F, pj, c, stream));
+        HIP_TRY(launch_pencil<float>(g32, kind == 0 ? PENCIL_FWD_MUL_INV : kind == 1 ? PENCIL_MUL_INV_D2F : PENCIL_FWD_F2D, xj, c, stream));
+        if (kind == 2) continue;
+        for (int u = 0; u < c; ++u) {
+            pj[u] = PlaneJob<float>{};
+            pj[u].src = scr[u];
+            pj[u].dst64 = jobs[k + u].dst;
+        }
+        HIP_TRY(launch_plane<float>(g32, PLANE_INV_F2D, pj, c, stream));
+    }
+    return FTS_OK;
+}
 
 int fts_engine::fft_apply_n(const FftJob* jobs, int n)
 {
+    if (fields_fp32()) return fft32_n(0, jobs, n);
     for (int k = 0; k < n; k += 2) {
         const int c = std::min(2, n - k);
         if (!plane_ok) {
@@ -718,6 +767,7 @@ int fts_engine::fft_apply_n(const FftJob* jobs, int n)
 
 int fts_engine::fft_apply_hat_n(const FftJob* jobs, int n)
 {
+    if (fields_fp32()) return fft32_n(1, jobs, n);
     for (int k = 0; k < n; k += 2) {
         const int c = std::min(2, n - k);
         PencilJob<double> xj[2];
@@ -735,6 +785,7 @@ int fts_engine::fft_apply_hat_n(const FftJob* jobs, int n)
 
 int fts_engine::fft_forward_n(const FftJob* jobs, int n)
 {
+    if (fields_fp32()) return fft32_n(2, jobs, n);
     for (int k = 0; k < n; k += 2) {
         const int c = std::min(2, n - k);
         if (!plane_ok) {

@@ -36,10 +36,14 @@ struct KOp {
 // semi-implicit table t = 1 / (1 + a))
 __host__ __device__ __forceinline__ double kop_table(const KOp& op, double t) { return op.table_noise ? sqrt(0.5 * t * (1.0 + t)) : t; }
 
-enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, PLANE_LAST = 4, PLANE_CONV_E = 5 };
+// PLANE_FWD_D2F / PLANE_INV_F2D: PLANE_FWD reading src64, PLANE_INV writing dst64 (complex
+// FP64 fields transformed in the pass's precision)
+enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, PLANE_LAST = 4, PLANE_CONV_E = 5, PLANE_FWD_D2F = 6, PLANE_INV_F2D = 7 };
 // PENCIL_CONV: forward along x, x op.sep[x] op.c0 (the x factor of a separable
 // kernel), inverse along x; the plane pass PLANE_CONV_E applies the (y, z) factors
-enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3, PENCIL_MUL_INV = 4, PENCIL_CONV = 5 };
+// PENCIL_FWD_F2D: PENCIL_FWD writing dst64; PENCIL_MUL_INV_D2F: PENCIL_MUL_INV reading src64
+enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3, PENCIL_MUL_INV = 4, PENCIL_CONV = 5, PENCIL_FWD_F2D = 6,
+                  PENCIL_MUL_INV_D2F = 7 };
 
 template <typename T> struct PlaneJob {
     const V2<T>* src;
@@ -56,12 +60,16 @@ template <typename T> struct PlaneJob {
     double* norm_out;
     double* log_scale;
     KOp op; // PLANE_CONV_E: a separable k-space kernel; its (y, z) factors are applied here
+    const double2* src64; // PLANE_FWD_D2F
+    double2* dst64;       // PLANE_INV_F2D
 };
 
 template <typename T> struct PencilJob {
     V2<T>* data;
     KOp op;
     const V2<T>* src; // read from here instead of data when set (the result still goes to data)
+    const double2* src64; // PENCIL_MUL_INV_D2F
+    double2* dst64;       // PENCIL_FWD_F2D
 };
 
 // Line pass: a tile of LINE_TC lines of one axis, transformed along that axis in LDS.
@@ -116,6 +124,8 @@ template <typename T> __device__ __forceinline__ PlaneJob<T> pick_job(PlaneJob<T
     j.norm_out = tb ? t->norm_out : f ? a.norm_out : b.norm_out;
     j.log_scale = tb ? t->log_scale : f ? a.log_scale : b.log_scale;
     j.op = tb ? pick_kop(t->op, t->op, true) : pick_kop(a.op, b.op, f);
+    j.src64 = tb ? t->src64 : f ? a.src64 : b.src64;
+    j.dst64 = tb ? t->dst64 : f ? a.dst64 : b.dst64;
     return j;
 }
 template <typename T> __device__ __forceinline__ PencilJob<T> pick_job(PencilJob<T> a, PencilJob<T> b, const PencilJob<T>* table)
@@ -127,10 +137,14 @@ template <typename T> __device__ __forceinline__ PencilJob<T> pick_job(PencilJob
         j.data = t->data;
         j.op = pick_kop(t->op, t->op, true);
         j.src = t->src;
+        j.src64 = t->src64;
+        j.dst64 = t->dst64;
     } else {
         j.data = f ? a.data : b.data;
         j.op = pick_kop(a.op, b.op, f);
         j.src = f ? a.src : b.src;
+        j.src64 = f ? a.src64 : b.src64;
+        j.dst64 = f ? a.dst64 : b.dst64;
     }
     return j;
 }

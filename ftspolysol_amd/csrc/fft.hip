@@ -93,7 +93,13 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     const V* src = (MODE == PLANE_INIT) ? job.emw : job.src;
     // FP32: pairs of points per 16-byte access
     constexpr bool PAIRS = sizeof(T) == 4 && N % 2 == 0;
-    if constexpr (PAIRS) {
+    if constexpr (MODE == PLANE_FWD_D2F) {
+#pragma unroll 4
+        for (int i = tid; i < N * N; i += NT) {
+            const double2 d = job.src64[plane + i];
+            lds[(i / N) * LS + (i % N)] = cmk<V>(static_cast<T>(d.x), static_cast<T>(d.y));
+        }
+    } else if constexpr (PAIRS) {
 #pragma unroll
         for (int j = tid; j < N * N / 2; j += NT) {
             const int i = 2 * j;
@@ -149,7 +155,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         }
         __syncthreads();
     }
-    if constexpr (MODE == PLANE_INV || MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
+    if constexpr (MODE == PLANE_INV || MODE == PLANE_INV_F2D || MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
         lds_fft<T, N, +1, N, LS, 1, true, NT>(lds, tw, tid); // along z
         lds_fft<T, N, +1, N, 1, LS, true, NT>(lds, tw, tid);  // along y
     }
@@ -200,11 +206,17 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
             }
         }
     }
-    if constexpr (MODE == PLANE_FWD || MODE == PLANE_INIT || MODE == PLANE_STEP) {
+    if constexpr (MODE == PLANE_FWD || MODE == PLANE_FWD_D2F || MODE == PLANE_INIT || MODE == PLANE_STEP) {
         lds_fft<T, N, -1, N, LS, 1, true, NT>(lds, tw, tid);
         lds_fft<T, N, -1, N, 1, LS, true, NT>(lds, tw, tid);
     }
-    if constexpr (MODE != PLANE_LAST && MODE != PLANE_CONV_E) {
+    if constexpr (MODE == PLANE_INV_F2D) {
+#pragma unroll 4
+        for (int i = tid; i < N * N; i += NT) {
+            const V v = lds[(i / N) * LS + (i % N)];
+            job.dst64[plane + i] = make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
+        }
+    } else if constexpr (MODE != PLANE_LAST && MODE != PLANE_CONV_E) {
         if constexpr (PAIRS) {
 #pragma unroll 2
             for (int j = tid; j < N * N / 2; j += NT) {
@@ -255,7 +267,14 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
     const V* in = job.src ? job.src : data;
     // FP32: pairs of columns per 16-byte access (ZC and sx are even)
     constexpr bool PAIRS = sizeof(T) == 4 && ZC % 2 == 0;
-    if constexpr (PAIRS) {
+    if constexpr (MODE == PENCIL_MUL_INV_D2F) {
+#pragma unroll 4
+        for (int i = tid; i < N1 * ZC; i += NT) {
+            const int x = i / ZC, c = i % ZC;
+            const double2 d = job.src64[static_cast<size_t>(x) * sx + jbase + c];
+            lds[x * LS + c] = cmk<V>(static_cast<T>(d.x), static_cast<T>(d.y));
+        }
+    } else if constexpr (PAIRS) {
 #pragma unroll 4
         for (int j = tid; j < N1 * ZC / 2; j += NT) {
             const int i = 2 * j;
@@ -270,7 +289,8 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         }
     }
     __syncthreads();
-    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_CONV) lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
+    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_F2D || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_CONV)
+        lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
     if constexpr (MODE == PENCIL_CONV) {
         const double* sxf = job.op.sep;
         const double c0 = job.op.c0;
@@ -281,7 +301,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         }
         __syncthreads();
     }
-    if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL || MODE == PENCIL_MUL_INV) {
+    if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL || MODE == PENCIL_MUL_INV || MODE == PENCIL_MUL_INV_D2F) {
         // every element a thread touches shares its column c (NT is a multiple of ZC):
         // the (y, z) part of the kernel is one per-thread constant
         const KOp op = job.op;
@@ -309,9 +329,16 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         }
         __syncthreads();
     }
-    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL_INV || MODE == PENCIL_CONV)
+    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL_INV || MODE == PENCIL_MUL_INV_D2F || MODE == PENCIL_CONV)
         lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
-    if constexpr (PAIRS) {
+    if constexpr (MODE == PENCIL_FWD_F2D) {
+#pragma unroll 4
+        for (int i = tid; i < N1 * ZC; i += NT) {
+            const int x = i / ZC, c = i % ZC;
+            const V v = lds[x * LS + c];
+            job.dst64[static_cast<size_t>(x) * sx + jbase + c] = make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
+        }
+    } else if constexpr (PAIRS) {
 #pragma unroll 4
         for (int j = tid; j < N1 * ZC / 2; j += NT) {
             const int i = 2 * j;
@@ -343,6 +370,8 @@ hipError_t plane_mode(const FftGeom& g, int mode, const PlaneJob<T>* jobs, const
     switch (mode) {
     case PLANE_FWD: return plane_go<T, N, PLANE_FWD>(g, jobs, table, njobs, s);
     case PLANE_INV: return plane_go<T, N, PLANE_INV>(g, jobs, table, njobs, s);
+    case PLANE_FWD_D2F: return plane_go<T, N, PLANE_FWD_D2F>(g, jobs, table, njobs, s);
+    case PLANE_INV_F2D: return plane_go<T, N, PLANE_INV_F2D>(g, jobs, table, njobs, s);
     case PLANE_INIT: return plane_go<T, N, PLANE_INIT>(g, jobs, table, njobs, s);
     case PLANE_STEP: return plane_go<T, N, PLANE_STEP>(g, jobs, table, njobs, s);
     case PLANE_LAST: return plane_go<T, N, PLANE_LAST>(g, jobs, table, njobs, s);
@@ -372,6 +401,8 @@ hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, con
     case PENCIL_MUL: return pencil_go<T, N1, PENCIL_MUL>(g, jobs, table, njobs, s);
     case PENCIL_MUL_INV: return pencil_go<T, N1, PENCIL_MUL_INV>(g, jobs, table, njobs, s);
     case PENCIL_CONV: return pencil_go<T, N1, PENCIL_CONV>(g, jobs, table, njobs, s);
+    case PENCIL_FWD_F2D: return pencil_go<T, N1, PENCIL_FWD_F2D>(g, jobs, table, njobs, s);
+    case PENCIL_MUL_INV_D2F: return pencil_go<T, N1, PENCIL_MUL_INV_D2F>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }

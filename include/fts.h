@@ -38,7 +38,8 @@ extern "C" {
 #define FTS_SCHEME_SI_LEARNED 3    /* semi-implicit EM with the learned linear-response kernel (no reference counterpart) */
 
 #define FTS_PREC_FP64 0 /* complex FP64 everywhere, as the reference (common.h:23) */
-#define FTS_PREC_FP32 1 /* complex FP32 propagators/FFTs, per-bead log-renormalisation, FP64 reductions */
+#define FTS_PREC_FP32 1 /* complex FP32 propagators and FFTs (the FP64 fields are transformed in FP32 on the plane
+                           schedules), per-bead log-renormalisation, FP64 reductions and field updates */
 
 #define FTS_MAX_SPECIES 8
 #define FTS_MAX_IONS 8
