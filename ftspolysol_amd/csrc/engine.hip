@@ -609,12 +609,24 @@ int fts_engine::lrk_step()
         fj[nf++] = FftJob{omega.p, lrk_hat[0][0][cur].p, KOp{}};
         fj[nf++] = FftJob{lrk_u.p, lrk_hat[0][1][cur].p, KOp{}};
     }
+    // an FP32 engine keeps the judge's spectra in complex FP32 (its transforms are FP32):
+    // half the bytes for k_lrk_bin_chunks' gather
+    const bool s32 = fields_fp32();
     if (f1) {
-        if (psi_hat_valid(psi.p)) HIP_TRY(hipMemcpyAsync(lrk_hat[1][0][cur].p, psi_hat.p, sizeof(double2) * M, hipMemcpyDeviceToDevice, stream));
-        else fj[nf++] = FftJob{psi.p, lrk_hat[1][0][cur].p, KOp{}};
+        if (psi_hat_valid(psi.p)) {
+            if (s32) {
+                hipLaunchKernelGGL((k_cast<double, float>), dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi_hat.p,
+                                   reinterpret_cast<float2*>(lrk_hat[1][0][cur].p), 1.0, static_cast<size_t>(M));
+                HIP_TRY(hipGetLastError());
+            } else {
+                HIP_TRY(hipMemcpyAsync(lrk_hat[1][0][cur].p, psi_hat.p, sizeof(double2) * M, hipMemcpyDeviceToDevice, stream));
+            }
+        } else {
+            fj[nf++] = FftJob{psi.p, lrk_hat[1][0][cur].p, KOp{}};
+        }
         fj[nf++] = FftJob{dpsi.p, lrk_hat[1][1][cur].p, KOp{}};
     }
-    if ((rc = fft_forward_n(fj, nf))) return rc;
+    if ((rc = s32 ? fft32_n(3, fj, nf) : fft_forward_n(fj, nf))) return rc;
     HIP_TRY(lrk_moment_sums(f0 ? omega.p : nullptr, f1 ? psi.p : nullptr, M, partials.p, stream));
     if ((rc = reduce_finalize(4, SC_LRK, 1.0 / static_cast<double>(M)))) return rc;
     HIP_TRY(lrk_infer(lrk_params.p, scal.p + SC_LRK, lrk_state.p, stream));
@@ -622,7 +634,7 @@ int fts_engine::lrk_step()
         HIP_TRY(lrk_bin_chunks(lrk_nchunks, lrk_chunk_start.p, lrk_perm.p, lrk_ad_perm.p, lrk_nperm, f0 ? lrk_hat[0][0][cur].p : nullptr,
                                f0 ? lrk_hat[0][0][prv].p : nullptr, f0 ? lrk_hat[0][1][cur].p : nullptr, f0 ? lrk_hat[0][1][prv].p : nullptr,
                                f1 ? lrk_hat[1][0][cur].p : nullptr, f1 ? lrk_hat[1][0][prv].p : nullptr, f1 ? lrk_hat[1][1][cur].p : nullptr,
-                               f1 ? lrk_hat[1][1][prv].p : nullptr, lrk_chunk_sums.p, stream));
+                               f1 ? lrk_hat[1][1][prv].p : nullptr, s32 ? 1 : 0, lrk_chunk_sums.p, stream));
     HIP_TRY(lrk_judge(lrk_state.p, lrk_have_prev ? 1 : 0, lrk_bin_chunk.p, lrk_chunk_sums.p, lrk_threshold,
                       scheme == FTS_SCHEME_SI_LEARNED ? 1 : 0, lrk_ring.p, lrk_capacity, stream));
     if (scheme == FTS_SCHEME_SI_LEARNED)
@@ -707,9 +719,26 @@ int fts_engine::fft_forward(const double2* src, double2* dst)
     return fft_forward_t<double>(src, dst);
 }
 
-// FP64 fields through FP32 plane / pencil passes, two jobs per launch
+// FP64 fields through FP32 plane / pencil passes, two jobs per launch.  kind 3: forward
+// into a complex FP32 spectrum (dst holds float2 values; no scratch)
 int fts_engine::fft32_n(int kind, const FftJob* jobs, int n)
 {
+    if (kind == 3) {
+        for (int k = 0; k < n; k += 2) {
+            const int c = std::min(2, n - k);
+            PlaneJob<float> pj[2] = {};
+            PencilJob<float> xj[2] = {};
+            for (int u = 0; u < c; ++u) {
+                float2* d = reinterpret_cast<float2*>(jobs[k + u].dst);
+                pj[u].dst = d;
+                pj[u].src64 = jobs[k + u].src;
+                xj[u].data = d;
+            }
+            HIP_TRY(launch_plane<float>(g32, PLANE_FWD_D2F, pj, c, stream));
+            HIP_TRY(launch_pencil<float>(g32, PENCIL_FWD, xj, c, stream));
+        }
+        return FTS_OK;
+    }
     float2* scr[2] = {static_cast<float2*>(tmpF), static_cast<float2*>(tmpB)};
     for (int k = 0; k < n; k += 2) {
         const int c = std::min(2, n - k);
@@ -1804,13 +1833,6 @@ int fts_engine::restart_from_checkpoint(long long failed_step)
     psi_hat_of = nullptr;
     lrk_have_prev = false;
     return FTS_OK;
-}
-
-template <typename S, typename D>
-__global__ void k_cast(const V2<S>* __restrict__ src, V2<D>* __restrict__ dst, double scale, size_t n)
-{
-    const size_t i = static_cast<size_t>(blockIdx.x) * EW_THREADS + threadIdx.x;
-    if (i < n) dst[i] = cmk<V2<D>>(static_cast<D>(scale * src[i].x), static_cast<D>(scale * src[i].y));
 }
 
 // ---------------------------------------------------------------------------

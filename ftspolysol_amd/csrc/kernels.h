@@ -51,6 +51,14 @@ __global__ void k_finalize(const double2* __restrict__ partials, int K, double2*
     }
 }
 
+// dst = scale * src, converting the complex precision
+template <typename S, typename D>
+__global__ void k_cast(const V2<S>* __restrict__ src, V2<D>* __restrict__ dst, double scale, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * EW_THREADS + threadIdx.x;
+    if (i < n) dst[i] = cmk<V2<D>>(static_cast<D>(scale * src[i].x), static_cast<D>(scale * src[i].y));
+}
+
 // ---- plain sums -------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS) k_sum(const V2<T>* __restrict__ a, int M, double2* __restrict__ partials)

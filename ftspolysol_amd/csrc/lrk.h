@@ -37,10 +37,11 @@ struct LrkDevState {
 hipError_t lrk_moment_sums(const double2* w0, const double2* psi, int M, double2* partials, hipStream_t s);
 // means: the 4 finalized (mean, mean square) pairs.
 hipError_t lrk_infer(const float* params, const double2* means, LrkDevState* st, hipStream_t s);
-// chunk partials: out[chunk][6] = S1_0, S2_0, E_0, S1_1, S2_1, E_1
+// chunk partials: out[chunk][6] = S1_0, S2_0, E_0, S1_1, S2_1, E_1.  The spectra are
+// complex FP64, or complex FP32 (fp32 != 0: the pointers hold float2 values).
 hipError_t lrk_bin_chunks(int n_chunks, const int* chunk_start, const int* perm, const double* ad_perm, int n_perm,
-                          const double2* Wc0, const double2* Wp0, const double2* Uc0, const double2* Up0,
-                          const double2* Wc1, const double2* Wp1, const double2* Uc1, const double2* Up1, double* out,
+                          const void* Wc0, const void* Wp0, const void* Uc0, const void* Up0,
+                          const void* Wc1, const void* Wp1, const void* Uc1, const void* Up1, int fp32, double* out,
                           hipStream_t s);
 // bin_chunk[b] .. bin_chunk[b + 1]: the chunks of bin b.  have_prev = 0 skips the judgement.
 hipError_t lrk_judge(LrkDevState* st, int have_prev, const int* bin_chunk, const double* chunk_sums, double threshold,

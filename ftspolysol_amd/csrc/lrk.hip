@@ -425,11 +425,12 @@ __global__ void __launch_bounds__(NT) k_lrk_infer(const float* __restrict__ para
     if (t < TP * 2) st->r_cur[q * TP * 2 + t] = s.R[t];
 }
 
+template <typename S>
 __global__ void __launch_bounds__(NT) k_lrk_bin_chunks(const int* __restrict__ chunk_start, const int* __restrict__ perm, const double* __restrict__ ad,
-                                                       int n_perm, const double2* __restrict__ Wc0, const double2* __restrict__ Wp0,
-                                                       const double2* __restrict__ Uc0, const double2* __restrict__ Up0,
-                                                       const double2* __restrict__ Wc1, const double2* __restrict__ Wp1,
-                                                       const double2* __restrict__ Uc1, const double2* __restrict__ Up1, double* __restrict__ out)
+                                                       int n_perm, const S* __restrict__ Wc0, const S* __restrict__ Wp0,
+                                                       const S* __restrict__ Uc0, const S* __restrict__ Up0,
+                                                       const S* __restrict__ Wc1, const S* __restrict__ Wp1,
+                                                       const S* __restrict__ Uc1, const S* __restrict__ Up1, double* __restrict__ out)
 {
     double2 v[4];
     for (int k = 0; k < 4; ++k) v[k] = make_double2(0.0, 0.0);
@@ -438,16 +439,16 @@ __global__ void __launch_bounds__(NT) k_lrk_bin_chunks(const int* __restrict__ c
         const int i = perm[j];
         if (Wc0) {
             const double a = ad[j];
-            const double2 dW = make_double2(Wc0[i].x - Wp0[i].x, Wc0[i].y - Wp0[i].y);
-            const double2 dU = make_double2(Uc0[i].x - Up0[i].x, Uc0[i].y - Up0[i].y);
+            const double2 dW = make_double2(static_cast<double>(Wc0[i].x) - Wp0[i].x, static_cast<double>(Wc0[i].y) - Wp0[i].y);
+            const double2 dU = make_double2(static_cast<double>(Uc0[i].x) - Up0[i].x, static_cast<double>(Uc0[i].y) - Up0[i].y);
             v[0].x += a * (dU.x * dW.x + dU.y * dW.y);
             v[0].y += a * a * (dW.x * dW.x + dW.y * dW.y);
             v[1].x += dU.x * dU.x + dU.y * dU.y;
         }
         if (Wc1) {
             const double a = ad[n_perm + j];
-            const double2 dW = make_double2(Wc1[i].x - Wp1[i].x, Wc1[i].y - Wp1[i].y);
-            const double2 dU = make_double2(Uc1[i].x - Up1[i].x, Uc1[i].y - Up1[i].y);
+            const double2 dW = make_double2(static_cast<double>(Wc1[i].x) - Wp1[i].x, static_cast<double>(Wc1[i].y) - Wp1[i].y);
+            const double2 dU = make_double2(static_cast<double>(Uc1[i].x) - Up1[i].x, static_cast<double>(Uc1[i].y) - Up1[i].y);
             v[2].x += a * (dU.x * dW.x + dU.y * dW.y);
             v[2].y += a * a * (dW.x * dW.x + dW.y * dW.y);
             v[3].x += dU.x * dU.x + dU.y * dU.y;
@@ -590,13 +591,22 @@ hipError_t lrk_infer(const float* params, const double2* means, LrkDevState* st,
     return hipGetLastError();
 }
 
-hipError_t lrk_bin_chunks(int n_chunks, const int* chunk_start, const int* perm, const double* ad_perm, int n_perm, const double2* Wc0,
-                          const double2* Wp0, const double2* Uc0, const double2* Up0, const double2* Wc1, const double2* Wp1,
-                          const double2* Uc1, const double2* Up1, double* out, hipStream_t s)
+hipError_t lrk_bin_chunks(int n_chunks, const int* chunk_start, const int* perm, const double* ad_perm, int n_perm, const void* Wc0,
+                          const void* Wp0, const void* Uc0, const void* Up0, const void* Wc1, const void* Wp1,
+                          const void* Uc1, const void* Up1, int fp32, double* out, hipStream_t s)
 {
     if (n_chunks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lrk_bin_chunks, dim3(n_chunks), dim3(NT), 0, s, chunk_start, perm, ad_perm, n_perm, Wc0, Wp0, Uc0, Up0, Wc1, Wp1, Uc1,
-                       Up1, out);
+    if (fp32) {
+        using S = float2;
+        hipLaunchKernelGGL(k_lrk_bin_chunks<S>, dim3(n_chunks), dim3(NT), 0, s, chunk_start, perm, ad_perm, n_perm, static_cast<const S*>(Wc0),
+                           static_cast<const S*>(Wp0), static_cast<const S*>(Uc0), static_cast<const S*>(Up0), static_cast<const S*>(Wc1),
+                           static_cast<const S*>(Wp1), static_cast<const S*>(Uc1), static_cast<const S*>(Up1), out);
+    } else {
+        using S = double2;
+        hipLaunchKernelGGL(k_lrk_bin_chunks<S>, dim3(n_chunks), dim3(NT), 0, s, chunk_start, perm, ad_perm, n_perm, static_cast<const S*>(Wc0),
+                           static_cast<const S*>(Wp0), static_cast<const S*>(Uc0), static_cast<const S*>(Up0), static_cast<const S*>(Wc1),
+                           static_cast<const S*>(Wp1), static_cast<const S*>(Uc1), static_cast<const S*>(Up1), out);
+    }
     return hipGetLastError();
 }
 
