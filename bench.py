@@ -18,7 +18,9 @@ Three engines run the workload:
           predictor-corrector step (complexLangevin.cu:669-698).
   plain   one complex-FP64 system, predictor-corrector, as the reference runs it.
 Replica 0 of each batch has the plain engine's seed and initial fields.  Each
-takes `warmup` untimed steps, then `steps` timed steps recording Re H every step.
+takes `warmup` untimed steps, then timed steps recording Re H every step: `steps`
+(2000) for the fast leg, whose tau_int sets `value`, `--aux-steps` (1000) for the
+others.
 tau_int of one 1000-step trajectory scatters by a factor of ten between seeds
 (0.5 .. 5.8 measured), so the plain scheme's tau_int and <H> come from the pooled
 statistics of `--tau-replicas` independent FP64 replicas of the same dynamics
@@ -228,7 +230,9 @@ def cpu_baseline(system, n_steps, tau_plain, scheme_name):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=2000, help="timed steps of the fast leg (the measured one)")
+    ap.add_argument("--aux-steps", type=int, default=1000,
+                    help="timed steps of the comparison legs (FP32 PC, plain FP64 and its tau_int replicas), at most --steps")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--replicas", type=int, default=8, help="replicas in the fast batch")
     ap.add_argument("--dt-fast", type=float, default=0.01)
@@ -264,15 +268,16 @@ def main():
 
     fast = run_batch(fast_sys, args.warmup, args.steps, f"fast fp32 {args.fast_scheme} x{args.replicas}", lrk=lrk,
                      lrk_threshold=args.lrk_threshold)
-    fp32_pc = None if args.no_fp32_pc else run_batch(pc_sys, args.warmup, args.steps, f"fp32 pc x{args.replicas}")
-    plain = run_batch(plain_sys, args.warmup, args.steps, f"plain fp64 pc x{args.plain_replicas}")
+    aux = min(args.aux_steps, args.steps)
+    fp32_pc = None if args.no_fp32_pc else run_batch(pc_sys, args.warmup, aux, f"fp32 pc x{args.replicas}")
+    plain = run_batch(plain_sys, args.warmup, aux, f"plain fp64 pc x{args.plain_replicas}")
     plain["tau_source"] = f"{args.plain_replicas} replica(s) timed"
     if args.tau_replicas > args.plain_replicas:
         # the same dynamics in more independent replicas: tau_int (and <H>) from their pooled
         # statistics; the throughput stays that of the timed plain engine
         tau_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r)
                    for r in range(args.tau_replicas)]
-        st = run_batch(tau_sys, args.warmup, args.steps, f"plain fp64 pc statistics x{args.tau_replicas}")
+        st = run_batch(tau_sys, args.warmup, aux, f"plain fp64 pc statistics x{args.tau_replicas}")
         plain["tau_int_timed_replicas"] = plain["tau_int"]
         for k in ("tau_int", "window", "tau_int_each", "tau_int_err", "H_mean", "H_stderr"):
             plain[k] = st[k]
