@@ -414,7 +414,9 @@ template <typename F> int group_each(Group& g, const std::vector<char>& mask, F&
 }
 
 // compute_densities of the replicas in mask at (w, psi) or (w_bar, psi_bar)
-template <typename T> int group_densities(Group& g, bool bar, const std::vector<char>& mask)
+// tail (optional): replica r's work after its densities, in the same fork of side
+// streams (one join fewer per step)
+template <typename T, typename Tail> int group_densities(Group& g, bool bar, const std::vector<char>& mask, Tail&& tail)
 {
     int rc;
     size_t max_species = 0;
@@ -438,13 +440,21 @@ template <typename T> int group_densities(Group& g, bool bar, const std::vector<
             int c = has(r, p) ? e->solve_end<T>(p, true) : FTS_OK;
             if (!c && has(r, p + 1)) c = e->solve_begin<T>(p + 1);
             if (!c && last) c = e->densities_end();
+            if (!c && last) c = tail(r);
             return c;
         });
         if (rc) return rc;
     }
     if (max_species == 0)
-        return group_each(g, mask, [&](size_t r) { return g.reps[r]->densities_end(); });
+        return group_each(g, mask, [&](size_t r) {
+            int c = g.reps[r]->densities_end();
+            return c ? c : tail(r);
+        });
     return FTS_OK;
+}
+template <typename T> int group_densities(Group& g, bool bar, const std::vector<char>& mask)
+{
+    return group_densities<T>(g, bar, mask, [](size_t) { return FTS_OK; });
 }
 
 bool finite_c(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
@@ -478,7 +488,6 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
 {
     int rc;
     const size_t n = g.reps.size();
-    if ((rc = group_densities<T>(g, false, mask)) || (rc = group_queue_Q(g, 0, mask))) return rc;
     // the engines of a batch defer measure's host part (measure_finish) to after the sync below
     auto update = [&](size_t r) {
         fts_engine* e = g.reps[r];
@@ -494,10 +503,18 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
     };
     const bool pc = scheme == FTS_SCHEME_PC_REFERENCE;
     if (pc) {
+        if ((rc = group_densities<T>(g, false, mask)) || (rc = group_queue_Q(g, 0, mask))) return rc;
         if ((rc = group_each(g, mask, [&](size_t r) { return g.reps[r]->pc_predict(); }))) return rc;
         if ((rc = group_densities<T>(g, true, mask)) || (rc = group_queue_Q(g, 1, mask))) return rc;
+        if ((rc = group_each(g, mask, update))) return rc;
+    } else {
+        // one fork for each replica's densities, Q copy and update
+        rc = group_densities<T>(g, false, mask, [&](size_t r) {
+            int c = g.reps[r]->q_queue(g.hQ + r * FTS_MAX_SPECIES);
+            return c ? c : update(r);
+        });
+        if (rc) return rc;
     }
-    if ((rc = group_each(g, mask, update))) return rc;
     HIP_TRY(hipStreamSynchronize(g.stream));
     for (size_t r = 0; r < n; ++r) {
         if (!mask[r]) continue;

@@ -242,6 +242,9 @@ def test_fp32_engine_tracks_fp64_at_96():
     assert abs(g64.get_field(FIELD_RHO_C).mean()) < 1e-10
     r32 = g32.get_field(FIELD_RHO_P)
     assert rel(r32, rho_p) < 1e-3
+    # the fields the FP32 form transforms in FP32 (smoothing, ion potentials and charges)
+    for fid in (FIELD_RHO_C, FIELD_RHO_C_ION, FIELD_ION_RHO):
+        assert rel(g32.get_field(fid), g64.get_field(fid)) < 1e-3, fid
 
 
 def test_noncubic_line_schedule_measure():
