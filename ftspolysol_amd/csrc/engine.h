@@ -297,10 +297,6 @@ struct fts_engine {
     // vectors sorted by bin (with A_D in that order) and their partial-sum chunks;
     // k-space w0, u0, psi, u_psi of the current and previous step (ping-pong).
     DevBuf<double> si_A_omega, si_A_psi;
-    // semi-implicit steps transform the noise apart from the drift, by sqrt(P (1 + P) / 2)
-    // instead of P: the stationary variance of every linear mode is then exact (1 / A)
-    // at any dt, where P alone gives 2 / (A (2 + dt lam A)).  n_omega + 1 fields.
-    DevBuf<double2> si_noise;
     DevBuf<unsigned char> lrk_bin;
     DevBuf<int> lrk_perm, lrk_chunk_start, lrk_bin_chunk;
     DevBuf<double> lrk_ad_perm, lrk_chunk_sums, lrk_ring;

@@ -34,7 +34,7 @@ void fts_engine::release()
     tw64_z.release();
     tw32_z.release();
     for (DevBuf<double2>* b : {&omega, &omega_bar, &omega_s, &drift, &psi, &psi_bar, &psi_s, &dpsi, &dpsi_bar, &W_type_s, &rho_hat,
-                               &rho_type, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &ion_psi, &ion_b, &ion_rho, &si_noise, &t1, &t2,
+                               &rho_type, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &ion_psi, &ion_b, &ion_rho, &t1, &t2,
                                &t3, &psi_hat, &mode_coef, &partials, &scal})
         b->release();
     flag.release();
@@ -496,7 +496,6 @@ int fts_engine::build_si_tables()
     HIP_TRY(si_table_psi.alloc(Pp.size()));
     HIP_TRY(hipMemcpy(si_table_omega.p, Pw.data(), sizeof(double) * Pw.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(si_table_psi.p, Pp.data(), sizeof(double) * Pp.size(), hipMemcpyHostToDevice));
-    HIP_TRY(si_noise.alloc((static_cast<size_t>(n_omega) + 1) * static_cast<size_t>(M)));
     HIP_TRY(si_A_omega.alloc(Aw.size()));
     HIP_TRY(si_A_psi.alloc(Ap.size()));
     if (!Aw.empty()) HIP_TRY(hipMemcpy(si_A_omega.p, Aw.data(), sizeof(double) * Aw.size(), hipMemcpyHostToDevice));
@@ -1736,7 +1735,8 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
     int rc;
     if (record && (rc = measure(omega.p, psi.p, rec))) return rc;
     const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
-    if (electro()) {
+    // EM adds real-space noise (psi's mean-free); the semi-implicit step adds it in k-space
+    if (electro() && !si) {
         hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, seed, t, 0x100u, sigma_psi, M, partials.p);
         HIP_TRY(hipGetLastError());
         if ((rc = reduce_finalize(1, SC_ETA_MEAN, invM))) return rc;
@@ -1744,51 +1744,51 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
     if ((rc = psi_drift(psi.p, rho_c.p, dpsi.p))) return rc;
     if (drift_stats && (rc = drift_queue(false))) return rc;
     if (si && lrk_active() && (rc = lrk_step())) return rc;
-    // semi-implicit: the noise apart from the drift (n_omega mode fields, then psi), for
-    // its own factor sqrt(P (1 + P) / 2)
-    const bool split = si && noise;
-    double2* nw = split ? si_noise.p : nullptr;
-    double2* np = split ? si_noise.p + static_cast<size_t>(n_omega) * Ms : nullptr;
     if (n_omega > 0) {
         hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, seed, t, M, si ? 0 : 1, omega.p, drift.p,
-                           nw);
+                           si ? 0 : 1);
         HIP_TRY(hipGetLastError());
     }
     if (electro()) {
         hipLaunchKernelGGL(k_psi_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, dpsi.p, -dt * lambda_psi, sigma_psi,
-                           scal.p + SC_ETA_MEAN, seed, t, M, dpsi.p, np);
+                           scal.p + SC_ETA_MEAN, seed, t, M, dpsi.p, si ? 0 : 1);
         HIP_TRY(hipGetLastError());
     }
     if (si) {
-        // the semi-implicit operator on every increment (independent fields: shared launches)
-        FftJob jb[2 * (FTS_MAX_TYPES + 1)];
+        // w += S_P[-dt lam u] + S_Pn[eta]: the multiply pass of each increment's transform
+        // adds sigma sqrt(M) Pn(k) g(k) / M (kop_noise), g the spectrum of the step's white
+        // noise; psi's drops k = 0 (mean-free noise).  Independent fields share launches.
+        const double rM = 1.0 / std::sqrt(static_cast<double>(M));
+        FftJob jb[FTS_MAX_TYPES + 1];
         int nj = 0;
-        for (int l = 0; l < n_omega; ++l) jb[nj++] = FftJob{f(drift, l), f(drift, l), KOp{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0}};
-        if (electro()) jb[nj++] = FftJob{dpsi.p, dpsi.p, KOp{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0}};
-        if (split) {
-            for (int l = 0; l < n_omega; ++l)
-                jb[nj++] = FftJob{nw + l * Ms, nw + l * Ms, KOp{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0, 1}};
-            if (electro()) jb[nj++] = FftJob{np, np, KOp{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0, 1}};
+        for (int l = 0; l < n_omega; ++l) {
+            KOp op{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0};
+            op.nseed = seed;
+            op.nstep = t;
+            op.nfield = SI_NOISE_FIELD + 2u * static_cast<unsigned>(l);
+            op.namp = mc.sigma[l] * rM;
+            jb[nj++] = FftJob{f(drift, l), f(drift, l), op};
+        }
+        if (electro()) {
+            KOp op{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0};
+            op.nseed = seed;
+            op.nstep = t;
+            op.nfield = SI_NOISE_FIELD_PSI;
+            op.namp = sigma_psi * rM;
+            op.nzero = 1;
+            jb[nj++] = FftJob{dpsi.p, dpsi.p, op};
         }
         if ((rc = fft_apply_n(jb, nj))) return rc;
         if (n_omega > 0) {
             const size_t n = static_cast<size_t>(n_omega) * Ms;
             hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream, omega.p, drift.p, n);
             HIP_TRY(hipGetLastError());
-            if (split) {
-                hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream, omega.p, nw, n);
-                HIP_TRY(hipGetLastError());
-            }
         }
     }
     if (electro()) {
         psi_hat_of = nullptr; // psi changes below
         hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, Ms);
         HIP_TRY(hipGetLastError());
-        if (split) {
-            hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, np, Ms);
-            HIP_TRY(hipGetLastError());
-        }
         if ((rc = remove_mean(psi.p))) return rc;
     } else {
         HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));

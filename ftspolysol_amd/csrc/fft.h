@@ -30,6 +30,13 @@ struct KOp {
     double c0, c1, c2;
     int ka, kb;
     int table_noise; // multiply by sqrt(t (1 + t) / 2) instead of the table value t
+    // k-space noise of a semi-implicit step (namp != 0): the multiply pass adds
+    // namp sqrt(t (1 + t) / 2) g(k), g the spectrum of real white noise of unit
+    // variance per site divided by sqrt(M) (kop_noise); nzero drops k = 0
+    unsigned long long nseed, nstep;
+    unsigned nfield;
+    double namp;
+    int nzero;
 };
 
 // the table factor of a KOp: t, or sqrt(t (1 + t) / 2) (the noise factor that goes with a
@@ -42,8 +49,11 @@ enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, P
 // PENCIL_CONV: forward along x, x op.sep[x] op.c0 (the x factor of a separable
 // kernel), inverse along x; the plane pass PLANE_CONV_E applies the (y, z) factors
 // PENCIL_FWD_F2D: PENCIL_FWD writing dst64; PENCIL_MUL_INV_D2F: PENCIL_MUL_INV reading src64
+// PENCIL_FWD_MUL_INV_NOISE / LINE_FWD_MUL_INV_NOISE: the multiply adds the k-space
+// noise of its KOp (the launch functions pick them when a job's op has namp != 0; the
+// Philox rounds stay out of the registers of the plain multiply)
 enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3, PENCIL_MUL_INV = 4, PENCIL_CONV = 5, PENCIL_FWD_F2D = 6,
-                  PENCIL_MUL_INV_D2F = 7 };
+                  PENCIL_MUL_INV_D2F = 7, PENCIL_FWD_MUL_INV_NOISE = 8 };
 
 template <typename T> struct PlaneJob {
     const V2<T>* src;
@@ -82,7 +92,7 @@ template <typename T> struct PencilJob {
 // with line passes is x(FWD_MUL_INV) -> y(INV) -> z(INV_E_FWD) -> y(FWD); it moves
 // more bytes than the plane/pencil pair but every pass has m^2/16 workgroups.
 constexpr int LINE_TC = 16;
-enum LineMode { LINE_FWD = 0, LINE_INV = 1, LINE_FWD_MUL_INV = 2, LINE_INV_E_FWD = 3, LINE_INV_E = 4, LINE_E_FWD = 5 };
+enum LineMode { LINE_FWD = 0, LINE_INV = 1, LINE_FWD_MUL_INV = 2, LINE_INV_E_FWD = 3, LINE_INV_E = 4, LINE_E_FWD = 5, LINE_FWD_MUL_INV_NOISE = 6 };
 
 template <typename T> struct LineJob {
     V2<T>* data;          // transformed in place (E_FWD writes its result here)
@@ -109,7 +119,40 @@ __device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
     k.ka = f ? a.ka : b.ka;
     k.kb = f ? a.kb : b.kb;
     k.table_noise = f ? a.table_noise : b.table_noise;
+    k.nseed = f ? a.nseed : b.nseed;
+    k.nstep = f ? a.nstep : b.nstep;
+    k.nfield = f ? a.nfield : b.nfield;
+    k.namp = f ? a.namp : b.namp;
+    k.nzero = f ? a.nzero : b.nzero;
     return k;
+}
+
+// g(k) at grid index gi = (x m2 + y) m3 + z: Hermitian, so that its inverse transform is
+// real.  The pair {k, -k} draws two normals a, b from the Philox stream (nseed, nstep,
+// nfield / nfield + 1, min index of the pair): g = (a +- i b) / sqrt 2, + at the smaller
+// index; a self-conjugate k (every component 0 or m/2) draws a alone: g = a.  This is
+// the distribution of FFT[eta] / sqrt(M) for eta real white noise of unit variance.
+__device__ __forceinline__ double2 kop_noise(const KOp& op, int m1, int m2, int m3, size_t gi)
+{
+    double2 r = make_double2(0.0, 0.0);
+    if (op.namp == 0.0 || !op.table) return r;
+    const size_t s23 = static_cast<size_t>(m2) * m3;
+    const int x = static_cast<int>(gi / s23), y = static_cast<int>((gi / m3) % m2), z = static_cast<int>(gi % m3);
+    const size_t pi = (static_cast<size_t>((m1 - x) % m1) * m2 + (m2 - y) % m2) * m3 + (m3 - z) % m3;
+    if (op.nzero && gi == 0) return r;
+    const double t = op.table[gi];
+    const double amp = op.namp * sqrt(0.5 * t * (1.0 + t));
+    if (pi == gi) {
+        r.x = amp * cl_normal(op.nseed, op.nstep, op.nfield, static_cast<uint32_t>(gi));
+        return r;
+    }
+    const uint32_t site = static_cast<uint32_t>(gi < pi ? gi : pi);
+    const double a = cl_normal(op.nseed, op.nstep, op.nfield, site);
+    const double b = cl_normal(op.nseed, op.nstep, op.nfield + 1u, site);
+    const double h = amp * 0.70710678118654752440;
+    r.x = h * a;
+    r.y = gi < pi ? h * b : -h * b;
+    return r;
 }
 template <typename T> __device__ __forceinline__ PlaneJob<T> pick_job(PlaneJob<T> a, PlaneJob<T> b, const PlaneJob<T>* table)
 {

@@ -289,7 +289,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         }
     }
     __syncthreads();
-    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_F2D || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_CONV)
+    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_F2D || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD_MUL_INV_NOISE || MODE == PENCIL_CONV)
         lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
     if constexpr (MODE == PENCIL_CONV) {
         const double* sxf = job.op.sep;
@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
         }
         __syncthreads();
     }
-    if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL || MODE == PENCIL_MUL_INV || MODE == PENCIL_MUL_INV_D2F) {
+    if constexpr (MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD_MUL_INV_NOISE || MODE == PENCIL_MUL || MODE == PENCIL_MUL_INV || MODE == PENCIL_MUL_INV_D2F) {
         // every element a thread touches shares its column c (NT is a multiple of ZC):
         // the (y, z) part of the kernel is one per-thread constant
         const KOp op = job.op;
@@ -327,9 +327,21 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
             }
             lds[x * LS + c] = cscale(lds[x * LS + c], static_cast<T>(kv));
         }
+        // k-space noise (semi-implicit steps), one element at a time: the Philox rounds
+        // would otherwise multiply the registers of the unrolled loop above
+        if constexpr (MODE == PENCIL_FWD_MUL_INV_NOISE) {
+#pragma unroll 1
+            for (int i = tid; i < N1 * ZC; i += NT) {
+                const int x = i / ZC;
+                const double2 nz = kop_noise(op, g.m1, g.m2, g.m3, static_cast<size_t>(x) * sx + j);
+                lds[x * LS + c].x += static_cast<T>(nz.x);
+                lds[x * LS + c].y += static_cast<T>(nz.y);
+            }
+        }
         __syncthreads();
     }
-    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_MUL_INV || MODE == PENCIL_MUL_INV_D2F || MODE == PENCIL_CONV)
+    if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD_MUL_INV_NOISE || MODE == PENCIL_MUL_INV || MODE == PENCIL_MUL_INV_D2F ||
+                  MODE == PENCIL_CONV)
         lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
     if constexpr (MODE == PENCIL_FWD_F2D) {
 #pragma unroll 4
@@ -403,6 +415,7 @@ hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, con
     case PENCIL_CONV: return pencil_go<T, N1, PENCIL_CONV>(g, jobs, table, njobs, s);
     case PENCIL_FWD_F2D: return pencil_go<T, N1, PENCIL_FWD_F2D>(g, jobs, table, njobs, s);
     case PENCIL_MUL_INV_D2F: return pencil_go<T, N1, PENCIL_MUL_INV_D2F>(g, jobs, table, njobs, s);
+    case PENCIL_FWD_MUL_INV_NOISE: return pencil_go<T, N1, PENCIL_FWD_MUL_INV_NOISE>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -455,6 +468,7 @@ hipError_t plane_any(const FftGeom& g, int mode, const PlaneJob<T>* jobs, const 
 template <typename T>
 hipError_t pencil_any(const FftGeom& g, int mode, const PencilJob<T>* jobs, const PencilJob<T>* table, int njobs, hipStream_t s)
 {
+    if (mode == PENCIL_FWD_MUL_INV && jobs && (jobs[0].op.namp != 0.0 || (njobs > 1 && jobs[1].op.namp != 0.0))) mode = PENCIL_FWD_MUL_INV_NOISE;
     switch (g.m1) {
     case 8: return pencil_mode<T, 8>(g, mode, jobs, table, njobs, s);
     case 12: return pencil_mode<T, 12>(g, mode, jobs, table, njobs, s);

@@ -130,12 +130,13 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
     __syncthreads();
 
     constexpr bool DO_INV = MODE == LINE_INV || MODE == LINE_INV_E_FWD || MODE == LINE_INV_E;
-    constexpr bool DO_FWD_FIRST = MODE == LINE_FWD || MODE == LINE_FWD_MUL_INV;
+    constexpr bool MUL = MODE == LINE_FWD_MUL_INV || MODE == LINE_FWD_MUL_INV_NOISE;
+    constexpr bool DO_FWD_FIRST = MODE == LINE_FWD || MUL;
     if constexpr (DO_FWD_FIRST) {
         if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, -1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
-    if constexpr (MODE == LINE_FWD_MUL_INV) {
+    if constexpr (MUL) {
         static_assert(!ZL, "the k-space multiply runs on x or y lines");
         static_assert(NT % TC == 0, "line threads must cover whole rows");
         const KOp op = job.op;
@@ -148,9 +149,20 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
             V* p = &lds[t * LS + cc];
             *p = cscale(*p, static_cast<T>(line_kop_value(op, g, lk, t, cb + t * S)));
         }
+        // k-space noise (semi-implicit steps), one element at a time: the Philox rounds
+        // would otherwise multiply the registers of the unrolled loop above
+        if constexpr (MODE == LINE_FWD_MUL_INV_NOISE) {
+#pragma unroll 1
+            for (int i = tid; i < N * TC; i += NT) {
+                const int t = i / TC;
+                const double2 nz = kop_noise(op, g.m1, g.m2, g.m3, cb + t * S);
+                lds[t * LS + cc].x += static_cast<T>(nz.x);
+                lds[t * LS + cc].y += static_cast<T>(nz.y);
+            }
+        }
         __syncthreads();
     }
-    if constexpr (DO_INV || MODE == LINE_FWD_MUL_INV) {
+    if constexpr (DO_INV || MUL) {
         if constexpr (ZL) lds_fft<T, N, +1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, +1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
@@ -326,6 +338,7 @@ hipError_t line_dispatch(const FftGeom& g, int axis, int mode, const void* tw, c
         if (mode == LINE_FWD) return line_go<T, N, 0, LINE_FWD>(g, tw, jobs, table, njobs, s);
         if (mode == LINE_INV) return line_go<T, N, 0, LINE_INV>(g, tw, jobs, table, njobs, s);
         if (mode == LINE_FWD_MUL_INV) return line_go<T, N, 0, LINE_FWD_MUL_INV>(g, tw, jobs, table, njobs, s);
+        if (mode == LINE_FWD_MUL_INV_NOISE) return line_go<T, N, 0, LINE_FWD_MUL_INV_NOISE>(g, tw, jobs, table, njobs, s);
     } else if (axis == 1) {
         if (mode == LINE_FWD) return line_go<T, N, 1, LINE_FWD>(g, tw, jobs, table, njobs, s);
         if (mode == LINE_INV) return line_go<T, N, 1, LINE_INV>(g, tw, jobs, table, njobs, s);
@@ -347,6 +360,7 @@ hipError_t line_any(const FftGeom& g, int axis, int mode, const void* tw, const 
 {
     using namespace line_detail;
     const int n = axis == 0 ? g.m1 : axis == 1 ? g.m2 : g.m3;
+    if (mode == LINE_FWD_MUL_INV && jobs && (jobs[0].op.namp != 0.0 || (njobs > 1 && jobs[1].op.namp != 0.0))) mode = LINE_FWD_MUL_INV_NOISE;
     switch (n) {
     case 8: return line_dispatch<T, 8>(g, axis, mode, tw, jobs, table, njobs, s);
     case 12: return line_dispatch<T, 12>(g, axis, mode, tw, jobs, table, njobs, s);

@@ -375,7 +375,7 @@ __global__ void k_omega_correct(ModeCoefs mc, const double2* __restrict__ drift,
 // EM / semi-implicit increment: r = -dt lam u(omega, rho) + eta.  EM adds it to omega
 // directly (add = 1); the semi-implicit scheme filters r in k-space first (add = 0).
 __global__ void k_omega_increment(ModeCoefs mc, const double2* __restrict__ rho, unsigned long long seed, unsigned long long step, int M,
-                                  int add, double2* __restrict__ om, double2* __restrict__ r_out, double2* __restrict__ n_out)
+                                  int add, double2* __restrict__ om, double2* __restrict__ r_out, int with_noise)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
@@ -383,10 +383,8 @@ __global__ void k_omega_increment(ModeCoefs mc, const double2* __restrict__ rho,
         const size_t o = static_cast<size_t>(l) * M + i;
         const double2 u = mode_drift(mc, l, om, rho, M, i);
         double2 r = cscale(u, -mc.lam_dt[l]);
-        const double n = mc.sigma[l] * cl_normal(seed, step, static_cast<uint32_t>(l), static_cast<uint32_t>(i));
-        // n_out set: the noise goes there, apart from the drift
-        if (n_out) n_out[o] = d2(n, 0.0);
-        else r.x += n;
+        // without noise: the semi-implicit step adds its noise in k-space
+        if (with_noise) r.x += mc.sigma[l] * cl_normal(seed, step, static_cast<uint32_t>(l), static_cast<uint32_t>(i));
         if (add) om[o] = cadd(om[o], r);
         else r_out[o] = r;
     }
@@ -474,15 +472,13 @@ __global__ void k_psi_update(const double2* __restrict__ base, const double2* __
 
 // psi increment for EM / SI: r = -dt lamPsi dpsi + eta
 __global__ void k_psi_increment(const double2* __restrict__ dpsi, double c1, double sigma, const double2* __restrict__ eta_mean,
-                                unsigned long long seed, unsigned long long step, int M, double2* __restrict__ r, double2* __restrict__ n_out)
+                                unsigned long long seed, unsigned long long step, int M, double2* __restrict__ r, int with_noise)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
     double2 v = cscale(dpsi[i], c1);
-    const double n = sigma * cl_normal(seed, step, 0x100u, static_cast<uint32_t>(i)) + (-eta_mean[0].x);
-    // n_out set: the (mean-free) noise goes there, apart from the drift
-    if (n_out) n_out[i] = d2(n, 0.0);
-    else v.x += n;
+    // without noise: the semi-implicit step adds its (mean-free) noise in k-space
+    if (with_noise) v.x += sigma * cl_normal(seed, step, 0x100u, static_cast<uint32_t>(i)) + (-eta_mean[0].x);
     r[i] = v;
 }
 

@@ -389,7 +389,15 @@ int fts_rf_write(const char* path, const int* m, const double* L, int n_fields, 
  * with A = A_D, the Debye (discrete-chain RPA) linear response of the
  * homogeneous state.  The noise factor Pn makes the stationary variance of a
  * linear mode (u = A w) exactly 1 / A at any dt; P alone would give
- * 2 / (A (2 + dt lam A)), an O(dt) deficit of the short-wavelength fluctuations.  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
+ * 2 / (A (2 + dt lam A)), an O(dt) deficit of the short-wavelength fluctuations.
+ * eta, real white noise of variance 2 lam dt / dV per site, is drawn in k-space:
+ * FFT[eta](k) = sqrt(2 lam dt M / dV) g(k), g Hermitian; the pair {k, -k} takes
+ * normals a, b from the Philox stream (seed, step, field F or F + 1, the pair's
+ * smaller grid index), g = (a + i b) / sqrt 2 at the smaller index and its
+ * conjugate at the other, a self-conjugate k takes a alone; F = 0x400 + 2 l for
+ * mode l and 0x500 for psi, whose k = 0 term is dropped (mean-free noise).  The
+ * increment's transform adds it in its multiply pass, so the noise costs no
+ * transform of its own.  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
  *     A(k) = A_D(k) exp(r_c(bin(|k|))),   r_c = 4 tanh(z_c / 4),
  * where z = MLP(|k|, moments of the current fields): field c = 0 is interaction
  * mode 0, c = 1 is psi.  The MLP has FTS_LRK_FEATURES inputs (padded to

@@ -1297,16 +1297,37 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     // Semi-implicit: w += S_P[-dt lam u] + S_Pn[eta] with P = 1 / (1 + a), a = dt lam A(k),
     // and the noise factor Pn = sqrt(P (1 + P) / 2) = P sqrt(1 + a / 2): for a linear drift
     // u = A w the stationary variance of every mode is then exactly 1 / A at any dt
-    // (eta transformed by P would give 2 / (A (2 + a))).  EM adds eta to the drift.
+    // (eta transformed by P would give 2 / (A (2 + a))).  The noise is drawn in k-space:
+    // FFT[eta] = sigma sqrt(M) g(k), g Hermitian (the pair {k, -k} takes normals a, b of
+    // Philox fields F, F + 1 at the pair's smaller index: g = (a +- i b) / sqrt 2, + at the
+    // smaller index; a self-conjugate k takes a alone), psi's without k = 0.  EM adds the
+    // real-space eta to the drift.
     const bool split = semi_implicit && noise;
-    std::vector<double> P(static_cast<size_t>(M)), Pn(split ? static_cast<size_t>(M) : 0);
+    std::vector<double> P(static_cast<size_t>(M));
     std::vector<Cx> nz(split ? static_cast<size_t>(M) : 0);
-    auto add_noise = [&](Cx* w, const double* eta) {
-        for (int i = 0; i < M; ++i) {
-            nz[static_cast<size_t>(i)] = Cx{eta[i], 0.0};
-            Pn[static_cast<size_t>(i)] = std::sqrt(0.5 * P[static_cast<size_t>(i)] * (1.0 + P[static_cast<size_t>(i)]));
-        }
-        apply_kernel(nz.data(), nz.data(), Pn.data());
+    auto add_noise = [&](Cx* w, double sigma, unsigned field, bool drop_zero) {
+        const double amp0 = sigma / std::sqrt(static_cast<double>(M));
+        for (int x = 0; x < m1; ++x)
+            for (int y = 0; y < m2; ++y)
+                for (int z = 0; z < m3; ++z) {
+                    const int gi = (x * m2 + y) * m3 + z;
+                    const int pi = (((m1 - x) % m1) * m2 + (m2 - y) % m2) * m3 + (m3 - z) % m3;
+                    const double t = P[static_cast<size_t>(gi)];
+                    const double amp = amp0 * std::sqrt(0.5 * t * (1.0 + t));
+                    Cx v{0.0, 0.0};
+                    if (drop_zero && gi == 0) {
+                    } else if (pi == gi) {
+                        v.x = amp * oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field, static_cast<uint32_t>(gi));
+                    } else {
+                        const uint32_t site = static_cast<uint32_t>(std::min(gi, pi));
+                        const double a = oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field, site);
+                        const double b = oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field + 1u, site);
+                        const double h = amp * 0.70710678118654752440;
+                        v = Cx{h * a, gi < pi ? h * b : -h * b};
+                    }
+                    nz[static_cast<size_t>(gi)] = v;
+                }
+        fft->run(nz.data(), +1);
         axpy(w, nz.data(), 1.0);
     };
     for (int l = 0; l < n_omega; ++l) {
@@ -1316,7 +1337,7 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
         const double* eta = eta_omega.data() + static_cast<size_t>(l) * M;
         for (int i = 0; i < M; ++i) {
             r[i] = cscale(r[i], -dt * lam);
-            if (!split) r[i].x += eta[i];
+            if (!semi_implicit) r[i].x += eta[i];
         }
         if (semi_implicit) {
             for (int i = 0; i < M; ++i) {
@@ -1326,13 +1347,13 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
             apply_kernel(r, r, P.data());
         }
         axpy(w, r, 1.0);
-        if (split) add_noise(w, eta);
+        if (split) add_noise(w, std::sqrt(2.0 * lam * dt / dV), 0x400u + 2u * static_cast<unsigned>(l), false);
     }
     if (electrostatics()) {
         Cx* r = dpsi.data();
         for (int i = 0; i < M; ++i) {
             r[i] = cscale(r[i], -dt * lambda_psi);
-            if (!split) r[i].x += etaPsi[static_cast<size_t>(i)];
+            if (!semi_implicit) r[i].x += etaPsi[static_cast<size_t>(i)];
         }
         if (semi_implicit) {
             for (int i = 0; i < M; ++i) {
@@ -1342,7 +1363,7 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
             apply_kernel(r, r, P.data());
         }
         axpy(psi.data(), r, 1.0);
-        if (split) add_noise(psi.data(), etaPsi.data());
+        if (split) add_noise(psi.data(), std::sqrt(2.0 * lambda_psi * dt / dV), 0x500u, true);
     }
     remove_psi_zero_mode(psi.data());
     ++step_counter;

@@ -283,43 +283,76 @@ def test_linear_mode_damping(scheme):
     assert abs(amp - expected) < 1e-4, (amp, expected)
 
 
+def si_noise_spectrum(seed, step, field, m, P, drop_zero):
+    """g(k) Pn(k) of the semi-implicit noise, restated from its definition: the pair
+    {k, -k} draws normals a, b (Philox fields `field`, `field` + 1, site = the pair's
+    smaller grid index), g = (a + i b) / sqrt 2 at the smaller index and its conjugate at
+    the other; a self-conjugate k draws a alone; Pn = sqrt(P (1 + P) / 2)."""
+    g = oracle_library().fto_cl_normal
+    g.restype = ctypes.c_double
+    g.argtypes = [ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint]
+    m1, m2, m3 = m
+    out = np.zeros(m, dtype=complex)
+    for x in range(m1):
+        for y in range(m2):
+            for z in range(m3):
+                gi = (x * m2 + y) * m3 + z
+                px, py, pz = (-x) % m1, (-y) % m2, (-z) % m3
+                pi = (px * m2 + py) * m3 + pz
+                if drop_zero and gi == 0:
+                    continue
+                if pi == gi:
+                    v = g(seed, step, field, gi)
+                else:
+                    site = min(gi, pi)
+                    a, b = g(seed, step, field, site), g(seed, step, field + 1, site)
+                    v = (a + 1j * (b if gi < pi else -b)) / np.sqrt(2)
+                out[x, y, z] = v
+    return out * np.sqrt(P * (1 + P) / 2)
+
+
 def test_semi_implicit_noise_factor():
-    """The noise of a semi-implicit step goes through Pn = sqrt(P (1 + P) / 2), the
-    drift through P = 1 / (1 + dt lam A(k)).  One step from the same state with the
-    noise on and off gives the step's noise increment: sigma eta for EM (the raw noise,
-    same stream), IFFT[Pn FFT[sigma eta]] for the semi-implicit scheme.  A(k) from the
-    closed forms: for omega the chain's linear response (as above); for psi of a
-    neutral chain without ions k^2 / (4 pi lB).  For a linear mode, Pn keeps the
-    stationary variance at 1 / A: Var = Pn^2 2 lam dt / (1 - (1 - P a)^2) with
-    a = dt lam A, and 1 - P a = P, gives 2 lam dt P (1 + P) / (2 (1 - P^2)) =
-    lam dt / (1 - P) = 1 / A."""
-    m, N, n, xi, dt, lB, lam_psi = (16, 16, 16), 10, 6, 0.05, 0.2, 1.0, 0.7
+    """The noise of a semi-implicit step is drawn in k-space and transformed with
+    Pn = sqrt(P (1 + P) / 2), P = 1 / (1 + dt lam A(k)) the drift's factor.  One step
+    from the same state with the noise on and off gives the step's noise increment,
+    which must equal sigma sqrt(M) IFFT[Pn g], g restated here from its definition and
+    the exported Philox generator, with A(k) from the closed forms: for omega the
+    chain's linear response (as above), for psi of a neutral chain without ions
+    k^2 / (4 pi lB) (k = 0 dropped: mean-free noise).  g is the spectrum of real white
+    noise / sqrt(M): the increment is real, and E|g|^2 = 1 at every k.  For a linear
+    mode Pn keeps the stationary variance at 1 / A: Var = Pn^2 2 lam dt / (1 - (1 - P a)^2)
+    with a = dt lam A and 1 - P a = P gives lam dt / (1 - P) = 1 / A."""
+    m, N, n, xi, dt, lB, lam_psi, seed = (16, 16, 16), 10, 6, 0.05, 0.2, 1.0, 0.7, 77
     rng = np.random.default_rng(12)
     om0 = np.full((1,) + m, -1j * xi) + 1e-3 * (rng.standard_normal((1,) + m) + 1j * rng.standard_normal((1,) + m))
     ps0 = 1e-3 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
 
-    def step(scheme, noise):
-        e = oracle_engine(homopolymer(m, N, n, xi=xi, lB=lB, dt=dt, scheme=scheme, noise=noise, seed=77,
+    def step(noise):
+        e = oracle_engine(homopolymer(m, N, n, xi=xi, lB=lB, dt=dt, scheme=SCHEME_SEMI_IMPLICIT, noise=noise, seed=seed,
                                       lambda_omega=(1.0,), lambda_psi=lam_psi))
         e.set_fields(om0, ps0)
         e.step(1)
-        return e.get_field(FIELD_OMEGA)[0], e.get_field(FIELD_PSI)[0]
+        return e.get_field(FIELD_OMEGA)[0], e.get_field(FIELD_PSI)[0], e.info.volume
 
-    si, si0, em, em0 = step(SCHEME_SEMI_IMPLICIT, True), step(SCHEME_SEMI_IMPLICIT, False), step(SCHEME_EM, True), step(SCHEME_EM, False)
+    (w1, p1, V), (w0, p0, _) = step(True), step(False)
     kx, ky, kz = kvecs(m, 0.5)
     k2 = kx ** 2 + ky ** 2 + kz ** 2
-    V = np.prod(m) * 0.5 ** 3
+    M = np.prod(m)
+    dV = V / M
     j = np.arange(1, N)
     phi = np.exp(-k2 / 6)[..., None]
     pair = N + 2 * np.sum((N - j) * phi ** j, axis=-1)
     A_w = n * N / V / xi + np.exp(-k2) * (n / V) * pair
     A_p = k2 / (4 * np.pi * lB)
-    for f, A, lam in ((0, A_w, 1.0), (1, A_p, lam_psi)):
-        d_si, d_em = si[f] - si0[f], em[f] - em0[f]
+    for d, A, lam, field, drop in ((w1 - w0, A_w, 1.0, 0x400, False), (p1 - p0, A_p, lam_psi, 0x500, True)):
         P = 1 / (1 + dt * lam * A)
-        Pn = np.sqrt(P * (1 + P) / 2)
-        expected = np.fft.ifftn(Pn * np.fft.fftn(d_em))
+        sigma = np.sqrt(2 * lam * dt / dV)
+        expected = sigma * np.sqrt(M) * np.fft.ifftn(si_noise_spectrum(seed, 0, field, m, P, drop))
         scale = np.abs(expected).max()
-        assert np.abs(d_si - expected).max() < 1e-10 * scale, f
-        # the drift's factor alone is a different increment
-        assert np.abs(np.fft.ifftn(P * np.fft.fftn(d_em)) - expected).max() > 1e-2 * scale, f
+        assert np.abs(expected.imag).max() < 1e-12 * scale, field   # Hermitian: a real increment
+        assert np.abs(d - expected).max() < 1e-10 * scale, field
+        # unit spectrum: |g|^2 averaged over the grid is 1 (within sampling error)
+        gk = np.fft.fftn(d) / (sigma * np.sqrt(M)) / np.sqrt(P * (1 + P) / 2)
+        if drop:
+            gk.flat[0] = 1.0
+        assert abs(np.mean(np.abs(gk) ** 2) - 1) < 5 * np.sqrt(2 / M), field
