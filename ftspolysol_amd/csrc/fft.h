@@ -128,10 +128,10 @@ __device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
 }
 
 // g(k) at grid index gi = (x m2 + y) m3 + z: Hermitian, so that its inverse transform is
-// real.  The pair {k, -k} draws two normals a, b from the Philox stream (nseed, nstep,
-// nfield / nfield + 1, min index of the pair): g = (a +- i b) / sqrt 2, + at the smaller
-// index; a self-conjugate k (every component 0 or m/2) draws a alone: g = a.  This is
-// the distribution of FFT[eta] / sqrt(M) for eta real white noise of unit variance.
+// real.  The pair {k, -k} takes the two Box-Muller normals a, b of one Philox draw
+// (nseed, nstep, nfield, site = the pair's smaller index): g = (a +- i b) / sqrt 2, + at
+// the smaller index; a self-conjugate k (every component 0 or m/2) takes a alone: g = a.
+// This is the distribution of FFT[eta] / sqrt(M) for eta real white noise of unit variance.
 __device__ __forceinline__ double2 kop_noise(const KOp& op, int m1, int m2, int m3, size_t gi)
 {
     double2 r = make_double2(0.0, 0.0);
@@ -146,12 +146,10 @@ __device__ __forceinline__ double2 kop_noise(const KOp& op, int m1, int m2, int 
         r.x = amp * cl_normal(op.nseed, op.nstep, op.nfield, static_cast<uint32_t>(gi));
         return r;
     }
-    const uint32_t site = static_cast<uint32_t>(gi < pi ? gi : pi);
-    const double a = cl_normal(op.nseed, op.nstep, op.nfield, site);
-    const double b = cl_normal(op.nseed, op.nstep, op.nfield + 1u, site);
+    const double2 ab = cl_normal_pair(op.nseed, op.nstep, op.nfield, static_cast<uint32_t>(gi < pi ? gi : pi));
     const double h = amp * 0.70710678118654752440;
-    r.x = h * a;
-    r.y = gi < pi ? h * b : -h * b;
+    r.x = h * ab.x;
+    r.y = gi < pi ? h * ab.y : -h * ab.y;
     return r;
 }
 template <typename T> __device__ __forceinline__ PlaneJob<T> pick_job(PlaneJob<T> a, PlaneJob<T> b, const PlaneJob<T>* table)

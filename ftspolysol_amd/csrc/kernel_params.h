@@ -9,9 +9,9 @@ namespace fts {
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 256;
 
-// Philox field ids of the semi-implicit k-space noise: mode l draws from fields
-// SI_NOISE_FIELD + 2l and + 2l + 1, psi from SI_NOISE_FIELD_PSI and + 1 (the real-space
-// noise of EM and PC uses fields l and 0x100)
+// Philox field ids of the semi-implicit k-space noise: mode l draws from field
+// SI_NOISE_FIELD + 2l, psi from SI_NOISE_FIELD_PSI (the real-space noise of EM and PC
+// uses fields l and 0x100)
 constexpr unsigned SI_NOISE_FIELD = 0x400u;
 constexpr unsigned SI_NOISE_FIELD_PSI = 0x500u;
 constexpr int EW_THREADS = 256;

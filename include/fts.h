@@ -392,10 +392,11 @@ int fts_rf_write(const char* path, const int* m, const double* L, int n_fields, 
  * 2 / (A (2 + dt lam A)), an O(dt) deficit of the short-wavelength fluctuations.
  * eta, real white noise of variance 2 lam dt / dV per site, is drawn in k-space:
  * FFT[eta](k) = sqrt(2 lam dt M / dV) g(k), g Hermitian; the pair {k, -k} takes
- * normals a, b from the Philox stream (seed, step, field F or F + 1, the pair's
- * smaller grid index), g = (a + i b) / sqrt 2 at the smaller index and its
- * conjugate at the other, a self-conjugate k takes a alone; F = 0x400 + 2 l for
- * mode l and 0x500 for psi, whose k = 0 term is dropped (mean-free noise).  The
+ * the two Box-Muller normals a (cosine) and b (sine) of one Philox draw (seed,
+ * step, field F, the pair's smaller grid index), g = (a + i b) / sqrt 2 at the
+ * smaller index and its conjugate at the other, a self-conjugate k takes a alone;
+ * F = 0x400 + 2 l for mode l and 0x500 for psi, whose k = 0 term is dropped
+ * (mean-free noise).  The
  * increment's transform adds it in its multiply pass, so the noise costs no
  * transform of its own.  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
  *     A(k) = A_D(k) exp(r_c(bin(|k|))),   r_c = 4 tanh(z_c / 4),

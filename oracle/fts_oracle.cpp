@@ -1298,9 +1298,10 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     // and the noise factor Pn = sqrt(P (1 + P) / 2) = P sqrt(1 + a / 2): for a linear drift
     // u = A w the stationary variance of every mode is then exactly 1 / A at any dt
     // (eta transformed by P would give 2 / (A (2 + a))).  The noise is drawn in k-space:
-    // FFT[eta] = sigma sqrt(M) g(k), g Hermitian (the pair {k, -k} takes normals a, b of
-    // Philox fields F, F + 1 at the pair's smaller index: g = (a +- i b) / sqrt 2, + at the
-    // smaller index; a self-conjugate k takes a alone), psi's without k = 0.  EM adds the
+    // FFT[eta] = sigma sqrt(M) g(k), g Hermitian (the pair {k, -k} takes the two
+    // Box-Muller normals a, b of one Philox draw, field F, site = the pair's smaller index:
+    // g = (a +- i b) / sqrt 2, + at the smaller index; a self-conjugate k takes a alone),
+    // psi's without k = 0.  EM adds the
     // real-space eta to the drift.
     const bool split = semi_implicit && noise;
     std::vector<double> P(static_cast<size_t>(M));
@@ -1319,11 +1320,10 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
                     } else if (pi == gi) {
                         v.x = amp * oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field, static_cast<uint32_t>(gi));
                     } else {
-                        const uint32_t site = static_cast<uint32_t>(std::min(gi, pi));
-                        const double a = oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field, site);
-                        const double b = oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field + 1u, site);
+                        double ab[2];
+                        oracle::cl_normal_pair(seed, static_cast<uint64_t>(step_counter), field, static_cast<uint32_t>(std::min(gi, pi)), ab);
                         const double h = amp * 0.70710678118654752440;
-                        v = Cx{h * a, gi < pi ? h * b : -h * b};
+                        v = Cx{h * ab[0], gi < pi ? h * ab[1] : -h * ab[1]};
                     }
                     nz[static_cast<size_t>(gi)] = v;
                 }
@@ -1936,6 +1936,11 @@ void fto_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out)
 double fto_cl_normal(unsigned long long seed, unsigned long long step, unsigned field, unsigned site)
 {
     return oracle::cl_normal(seed, step, field, site);
+}
+
+void fto_cl_normal_pair(unsigned long long seed, unsigned long long step, unsigned field, unsigned site, double* out)
+{
+    oracle::cl_normal_pair(seed, step, field, site, out);
 }
 
 } // extern "C"

@@ -56,4 +56,20 @@ inline double cl_normal(uint64_t seed, uint64_t step, uint32_t field, uint32_t s
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
 }
 
+// both Box-Muller normals of one draw: cl_normal's value (cos) and its sine partner
+inline void cl_normal_pair(uint64_t seed, uint64_t step, uint32_t field, uint32_t site, double* out)
+{
+    const uint32_t ctr[4] = {site, field, static_cast<uint32_t>(step), static_cast<uint32_t>(step >> 32)};
+    const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+    uint32_t r[4];
+    philox4x32_10(ctr, key, r);
+    const uint64_t a = ((static_cast<uint64_t>(r[0]) << 32) | r[1]) >> 11;
+    const uint64_t b = ((static_cast<uint64_t>(r[2]) << 32) | r[3]) >> 11;
+    const double u1 = (static_cast<double>(a) + 0.5) * 0x1.0p-53;
+    const double u2 = (static_cast<double>(b) + 0.5) * 0x1.0p-53;
+    const double rad = std::sqrt(-2.0 * std::log(u1));
+    out[0] = rad * std::cos(6.283185307179586 * u2);
+    out[1] = rad * std::sin(6.283185307179586 * u2);
+}
+
 } // namespace oracle
