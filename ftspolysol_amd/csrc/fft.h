@@ -132,6 +132,8 @@ __device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
 // (nseed, nstep, nfield, site = the pair's smaller index): g = (a +- i b) / sqrt 2, + at
 // the smaller index; a self-conjugate k (every component 0 or m/2) takes a alone: g = a.
 // This is the distribution of FFT[eta] / sqrt(M) for eta real white noise of unit variance.
+// F32 (FP32 transforms): the normals' transcendentals in FP32.
+template <bool F32 = false>
 __device__ __forceinline__ double2 kop_noise(const KOp& op, int m1, int m2, int m3, size_t gi)
 {
     double2 r = make_double2(0.0, 0.0);
@@ -142,11 +144,11 @@ __device__ __forceinline__ double2 kop_noise(const KOp& op, int m1, int m2, int 
     if (op.nzero && gi == 0) return r;
     const double t = op.table[gi];
     const double amp = op.namp * sqrt(0.5 * t * (1.0 + t));
+    const double2 ab = cl_normal_pair<F32>(op.nseed, op.nstep, op.nfield, static_cast<uint32_t>(gi < pi ? gi : pi));
     if (pi == gi) {
-        r.x = amp * cl_normal(op.nseed, op.nstep, op.nfield, static_cast<uint32_t>(gi));
+        r.x = amp * ab.x;
         return r;
     }
-    const double2 ab = cl_normal_pair(op.nseed, op.nstep, op.nfield, static_cast<uint32_t>(gi < pi ? gi : pi));
     const double h = amp * 0.70710678118654752440;
     r.x = h * ab.x;
     r.y = gi < pi ? h * ab.y : -h * ab.y;

@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<
 #pragma unroll 1
             for (int i = tid; i < N1 * ZC; i += NT) {
                 const int x = i / ZC;
-                const double2 nz = kop_noise(op, g.m1, g.m2, g.m3, static_cast<size_t>(x) * sx + j);
+                const double2 nz = kop_noise<sizeof(T) == 4>(op, g.m1, g.m2, g.m3, static_cast<size_t>(x) * sx + j);
                 lds[x * LS + c].x += static_cast<T>(nz.x);
                 lds[x * LS + c].y += static_cast<T>(nz.y);
             }

@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
 #pragma unroll 1
             for (int i = tid; i < N * TC; i += NT) {
                 const int t = i / TC;
-                const double2 nz = kop_noise(op, g.m1, g.m2, g.m3, cb + t * S);
+                const double2 nz = kop_noise<sizeof(T) == 4>(op, g.m1, g.m2, g.m3, cb + t * S);
                 lds[t * LS + cc].x += static_cast<T>(nz.x);
                 lds[t * LS + cc].y += static_cast<T>(nz.y);
             }

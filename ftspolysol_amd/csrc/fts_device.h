@@ -278,7 +278,9 @@ __device__ __forceinline__ double cl_normal(uint64_t seed, uint64_t step, uint32
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
-// both Box-Muller normals of one draw: cl_normal's value (cos) and its sine partner
+// both Box-Muller normals of one draw: cl_normal's value (cos) and its sine partner;
+// F32: the logarithm, root and sine/cosine in FP32 (for FP32 transforms)
+template <bool F32 = false>
 __device__ __forceinline__ double2 cl_normal_pair(uint64_t seed, uint64_t step, uint32_t field, uint32_t site)
 {
     uint32_t c[4] = {site, field, (uint32_t)step, (uint32_t)(step >> 32)};
@@ -287,6 +289,12 @@ __device__ __forceinline__ double2 cl_normal_pair(uint64_t seed, uint64_t step, 
     const uint64_t b = (((uint64_t)c[2] << 32) | c[3]) >> 11;
     const double u1 = ((double)a + 0.5) * 0x1.0p-53;
     const double u2 = ((double)b + 0.5) * 0x1.0p-53;
+    if constexpr (F32) {
+        const float r = sqrtf(-2.0f * logf(static_cast<float>(u1)));
+        float sn, cs;
+        sincospif(2.0f * static_cast<float>(u2), &sn, &cs);
+        return make_double2(r * cs, r * sn);
+    }
     const double r = sqrt(-2.0 * log(u1));
     return make_double2(r * cos(6.283185307179586 * u2), r * sin(6.283185307179586 * u2));
 }
