@@ -251,8 +251,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; runs on several GPUs are not part of this build "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs each system (or replica batch) on a single GPU, "
+              "and runs on several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
 
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)

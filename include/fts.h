@@ -398,8 +398,9 @@ int fts_rf_write(const char* path, const int* m, const double* L, int n_fields, 
  * F = 0x400 + 2 l for mode l and 0x500 for psi, whose k = 0 term is dropped
  * (mean-free noise).  The
  * increment's transform adds it in its multiply pass, so the noise costs no
- * transform of its own.  FP32 engines evaluate the Box-Muller logarithm, root and
- * sine/cosine in FP32 (the draws themselves are the same).  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
+ * transform of its own.  FP32 transforms (an FP32 engine on the plane schedules)
+ * evaluate the Box-Muller logarithm, root and sine/cosine in FP32 (the draws
+ * themselves are the same).  FTS_SCHEME_SI_LEARNED replaces it by the field-adapted
  *     A(k) = A_D(k) exp(r_c(bin(|k|))),   r_c = 4 tanh(z_c / 4),
  * where z = MLP(|k|, moments of the current fields): field c = 0 is interaction
  * mode 0, c = 1 is psi.  The MLP has FTS_LRK_FEATURES inputs (padded to
