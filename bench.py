@@ -234,7 +234,7 @@ def main():
     ap.add_argument("--aux-steps", type=int, default=1000,
                     help="timed steps of the comparison legs (FP32 PC, plain FP64 and its tau_int replicas), at most --steps")
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--replicas", type=int, default=8, help="replicas in the fast batch")
+    ap.add_argument("--replicas", type=int, default=16, help="replicas in the fast batch")
     ap.add_argument("--dt-fast", type=float, default=0.01)
     ap.add_argument("--fast-scheme", choices=["si_learned", "si", "em", "pc"], default="si_learned")
     ap.add_argument("--lrk", default=LRK_PARAMS, help="parameters of the learned kernel (.npy)")
@@ -298,9 +298,9 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "pmc_traffic_96cubed_N100.json")
     if os.path.exists(tpath):
-        t = json.load(open(tpath))
-        if t.get("replicas") == args.replicas and t.get("schedule") == fast["schedule"] and t.get("precision") == "fp32":
-            traffic = t["traffic_bytes"]
+        for t in json.load(open(tpath)).get("entries", []):
+            if t.get("replicas") == args.replicas and t.get("schedule") == fast["schedule"] and t.get("precision") == "fp32":
+                traffic = t["traffic_bytes"]
     _, _, plain_achieved = roof(plain["timing"])
     try:
         copy_gbs = copy_bandwidth()
