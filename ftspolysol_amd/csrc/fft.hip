@@ -17,7 +17,7 @@ template <int N> constexpr int plane_threads()
 {
     return (N * N / plan_e(N)) > 1024 ? 1024 : (N * N / plan_e(N));
 }
-template <int N1> constexpr int pencil_threads() { return PENCIL_ZC * (N1 / plan_e(N1)); }
+template <int N1, int ZC = PENCIL_ZC> constexpr int pencil_threads() { return ZC * (N1 / plan_e(N1)); }
 
 // two consecutive complex FP32 values as one 16-byte access (16-byte aligned: every
 // field starts at a multiple of an even point count)
@@ -248,12 +248,14 @@ __device__ __forceinline__ double kop_value(const KOp& op, const double* kvec, i
     return v;
 }
 
-template <typename T, int N1, int MODE>
-__global__ void __launch_bounds__(pencil_threads<N1>()) pencil_kernel(PencilJob<T> j0, PencilJob<T> j1, const PencilJob<T>* __restrict__ table, FftGeom g)
+// ZCW: (y, z) columns per workgroup; the bead-step multiplies of batches take 32
+// (256-byte row segments), everything else PENCIL_ZC
+template <typename T, int N1, int MODE, int ZCW>
+__global__ void __launch_bounds__((pencil_threads<N1, ZCW>())) pencil_kernel(PencilJob<T> j0, PencilJob<T> j1, const PencilJob<T>* __restrict__ table, FftGeom g)
 {
     using V = V2<T>;
-    constexpr int NT = pencil_threads<N1>();
-    constexpr int ZC = PENCIL_ZC;
+    constexpr int NT = pencil_threads<N1, ZCW>();
+    constexpr int ZC = ZCW;
     constexpr int LS = ZC + 1;
     __shared__ V lds[N1 * LS];
     __shared__ V tw[N1];
@@ -392,15 +394,27 @@ hipError_t plane_mode(const FftGeom& g, int mode, const PlaneJob<T>* jobs, const
     }
 }
 
+template <typename T, int N1, int MODE, int ZC>
+hipError_t pencil_launch(const FftGeom& g, const PencilJob<T>* jobs, const PencilJob<T>* table, int njobs, hipStream_t s)
+{
+    const size_t tiles = static_cast<size_t>(g.m2) * g.m3 / ZC;
+    dim3 grid(static_cast<unsigned>(tiles), njobs);
+    const PencilJob<T> none{};
+    hipLaunchKernelGGL((pencil_kernel<T, N1, MODE, ZC>), grid, dim3(pencil_threads<N1, ZC>()), 0, s, jobs ? jobs[0] : none,
+                       jobs ? jobs[njobs > 1 ? 1 : 0] : none, table, g);
+    return hipGetLastError();
+}
+
+// Launches of four or more jobs (replica batches) run the bead-step multiplies on
+// 32-column tiles when the (y, z) plane divides: measured 1.7 % (FP32) and 4 % (FP64)
+// faster per 8-job group, 2 % slower for a single system.  Same arithmetic per column.
 template <typename T, int N1, int MODE>
 hipError_t pencil_go(const FftGeom& g, const PencilJob<T>* jobs, const PencilJob<T>* table, int njobs, hipStream_t s)
 {
-    const size_t tiles = static_cast<size_t>(g.m2) * g.m3 / PENCIL_ZC;
-    dim3 grid(static_cast<unsigned>(tiles), njobs);
-    const PencilJob<T> none{};
-    hipLaunchKernelGGL((pencil_kernel<T, N1, MODE>), grid, dim3(pencil_threads<N1>()), 0, s, jobs ? jobs[0] : none,
-                       jobs ? jobs[njobs > 1 ? 1 : 0] : none, table, g);
-    return hipGetLastError();
+    if constexpr (MODE == PENCIL_CONV || MODE == PENCIL_FWD_MUL_INV) {
+        if (njobs >= 4 && (static_cast<size_t>(g.m2) * g.m3) % 32 == 0) return pencil_launch<T, N1, MODE, 32>(g, jobs, table, njobs, s);
+    }
+    return pencil_launch<T, N1, MODE, PENCIL_ZC>(g, jobs, table, njobs, s);
 }
 
 template <typename T, int N1>
