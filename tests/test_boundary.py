@@ -131,3 +131,14 @@ def test_tau_int_ar1():
         tau, _ = bench.tau_int(x)
         exact = 0.5 * (1 + rho) / (1 - rho)
         assert abs(tau - exact) < 0.08 * exact + 0.02, (rho, tau, exact)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the engine runs a system (or a replica batch) on one GPU; bench.py says so for --gpus 2
+    # and exits before it touches a device
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and not r.stdout.strip()
