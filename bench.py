@@ -5,46 +5,56 @@ b = 1, charge -0.5 on every bead, counterions, 1:1 salt 0.05 b^-3, bead density
 0.2 b^-3 (221 chains), excluded volume B = 0.5 b^3, lB = 1 b, a = 1 b,
 lambda = 1.  Synthetic initial fields (w, psi = 1e-3 complex Gaussian).
 
-Three engines run the workload:
-  fast    FP32 + learned: a batch of R independent replicas (own seeds and
-          initial fields) with complex-FP32 propagators and FFTs, per-bead
-          renormalisation, FP64 reductions and field accumulation, stepping the
-          semi-implicit scheme with the learned linear-response kernel
-          (FTS_SCHEME_SI_LEARNED; network trained by ftspolysol_amd.train_lrk on
-          48^3 data, ftspolysol_amd/lrk_c4_48cubed.npy), judged every step by the
-          exact force, falling back to the Debye kernel when its excess force
-          residual passes 1e-3.  Every bead-step launch carries all replicas.
-  fp32_pc the same batch without the learned part: FP32, the reference's
-          predictor-corrector step (complexLangevin.cu:669-698).
-  plain   one complex-FP64 system, predictor-corrector, as the reference runs it.
-Replica 0 of each batch has the plain engine's seed and initial fields.  Each
-takes `warmup` untimed steps, then timed steps recording Re H every step: `steps`
-(2000) for the fast leg, whose tau_int sets `value`, `--aux-steps` (1000) for the
-others.
-tau_int of one 1000-step trajectory scatters by a factor of ten between seeds
-(0.5 .. 5.8 measured), so the plain scheme's tau_int and <H> come from the pooled
-statistics of `--tau-replicas` independent FP64 replicas of the same dynamics
-(one batch, untimed for the rate); its throughput is the timed plain engine's.
-Every replica keeps a device checkpoint every 50 steps; a replica whose step
-turns non-finite is restored from it with a new noise key and the repeated steps
-count in the elapsed time; the restarts are reported.
+Legs (engines stepping the workload; every batch carries all its replicas in each
+bead-step launch, each replica with its own seed and initial fields; replica 0 of
+every leg has the same seed and fields):
+  fast        FP32 + learned: R replicas with complex-FP32 propagators and FFTs,
+              per-bead renormalisation, FP64 reductions and field accumulation,
+              stepping the semi-implicit scheme with the learned linear-response
+              kernel (FTS_SCHEME_SI_LEARNED; network trained by
+              ftspolysol_amd.train_lrk on 48^3 data, lrk_c4_48cubed.npy), judged
+              every step by the exact force, falling back to the Debye kernel when
+              its excess force residual passes 1e-3.  `value` is this leg.
+  si_debye    the same batch, seeds and dt with the Debye kernel only
+              (FTS_SCHEME_SEMI_IMPLICIT): what the learned kernel gains or loses.
+  plain_fp64_batched  FP64, the reference's predictor-corrector
+              (complexLangevin.cu:669-698), a batch of --plain-replicas: the plain
+              engine at its own best throughput, and the plain scheme's tau_int and
+              <H>.
+  plain_fp64  one FP64 predictor-corrector system, as the reference runs it (its
+              rate; its tau_int and <H> are the batched leg's, the same dynamics).
+  fp32_pc     (--with-fp32-pc) FP32 predictor-corrector batch.
 
-<H> error: `H_err_same_state` (the precision error) measures H, mu and the
+Each leg: max(warmup, 200) untimed equilibration steps; then exactly `steps` timed
+steps (bracketed by device synchronisation, recording H every step as the
+reference measures its observables every step, complexLangevin.cu:684), which set
+the leg's rate; then untimed steps that extend the record of H to --record-steps
+(2000 for the FP32 legs, --plain-record-steps 1000 for the FP64 batch) whatever
+`steps` is.  tau_int and <H> come from that record, so they do not depend on the
+driver's --steps.  A leg whose record is shorter than 50 Sokal windows reports
+samples_per_s null.
+
+Decorrelated samples/s = (replica CL steps/s) / (2 tau_int), tau_int by Sokal's
+automatic window (c = 6) on the replica-averaged normalised autocorrelation of Re H,
+with Sokal's error tau sqrt(2 (2W + 1) / n).  Every replica keeps a device checkpoint
+every 50 steps; a replica whose step turns non-finite is restored from it with a new
+noise key (the repeated steps count in the elapsed time); restarts are reported.
+
+<H> error: `same_state_rel_err` (the precision error) measures H, mu and the
 stress tensor with an FP32 engine and an FP64 engine on the same fields (fast
 replica 0's final state); `H_err_vs_plain` compares the time averages of the fast
-batch and the plain engine (statistical once trajectories decorrelate, plus the
-time-step bias of the fast scheme against the predictor-corrector; `H_err_sigma`
-is the standard error).  Decorrelated samples/s = (replica CL steps/s) /
-(2 tau_int), tau_int by Sokal's automatic window (c = 6) on the replica-averaged
-normalised autocorrelation of Re H.  `value` is the fast batch's whole-GPU rate.
+batch and the plain FP64 batch (statistical once trajectories decorrelate, plus
+the time-step bias of the semi-implicit scheme against the predictor-corrector;
+`H_err_sigma` is the combined standard error).
 
-roofline: `copy_achievable` is a 1 GiB device-to-device hipMemcpy timed on the same GPU
-(the bandwidth a streaming kernel can reach); `traffic_rate` the measured HBM traffic
-of a launch group over its duration.  The dominant kernels are the propagator bead-step passes (one pencil
+roofline: the dominant kernels are the propagator bead-step passes (one pencil
 pass and one plane pass per bead index, or the four line passes, every replica's
 job in each launch).  Algorithmic bytes per bead step = read q_s + write q_{s+1}
 = 2 c M (c = 8 B for complex FP32), summed over the jobs of a launch group and
-divided by the group's average duration from HIP events on the batch stream.
+divided by the group's average duration from HIP events on the batch stream over
+the timed steps.  `copy_achievable` is a 1 GiB device-to-device hipMemcpy timed on
+the same GPU; `traffic_rate` the measured HBM traffic of a launch group (rocprofv3
+FETCH_SIZE + WRITE_SIZE, committed under profiles/) over its duration.
 
 cpu_baseline: the FP64 CPU restatement (oracle/) running the plain scheme on the
 host cores for a bounded sample of steps of the same workload.
@@ -61,7 +71,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, Engine, PREC_FP32, PREC_FP64, SCHEME_EM,  # noqa: E402
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, Engine, PREC_FP32, PREC_FP64,  # noqa: E402
                                 SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, hip_batch, hip_engine)
 from ftspolysol_amd.lrk import load_params  # noqa: E402
 from ftspolysol_amd.system import initial_fields, make_system  # noqa: E402
@@ -99,7 +109,22 @@ def tau_int(x, c=6.0):
     return float(tau), n - 1
 
 
-def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50, lrk=None, lrk_threshold=1e-3):
+MIN_EQUILIBRATION = 200
+
+
+def lrk_totals(stats):
+    return {"judged": sum(x["judged"] for x in stats), "learned": sum(x["learned_steps"] for x in stats),
+            "fallback": sum(x["fallback_steps"] for x in stats),
+            "res_learned": [sum(x["sum_rel_residual_learned"][c] for x in stats) for c in range(2)],
+            "res_debye": [sum(x["sum_rel_residual_debye"][c] for x in stats) for c in range(2)]}
+
+
+def run_leg(systems, label, warmup, steps, record_steps, ckpt_every=50, max_restarts=50, lrk=None,
+            lrk_threshold=1e-3, stats=True):
+    """One leg: equilibrate max(warmup, MIN_EQUILIBRATION) untimed steps, time exactly
+    `steps` steps (recording H every step), then extend the record of H untimed to
+    `record_steps` steps when it is shorter.  Returns the rate of the timed steps and the
+    statistics of the record."""
     b = hip_batch(systems)
     for r, (e, s) in enumerate(zip(b.replicas, systems)):
         om, ps = initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r)
@@ -108,52 +133,73 @@ def run_batch(systems, warmup, steps, label, ckpt_every=50, max_restarts=50, lrk
         if lrk is not None:
             e.lrk_set_network(lrk, lrk_threshold)
     R = len(systems)
+    equil = max(warmup, MIN_EQUILIBRATION)
     t0 = time.perf_counter()
     done = 0
-    while done < warmup:
-        k = min(100, warmup - done)
+    while done < equil:
+        k = min(100, equil - done)
         b.step(k)
         done += k
-    print(f"[bench] {label}: warmup {warmup} steps in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    b.set_timing(True)
+    print(f"[bench] {label}: {equil} equilibration steps in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    lrk0 = lrk_totals([e.lrk_get_stats() for e in b.replicas]) if lrk is not None else None
+    restarts0 = sum(e.get_info().restarts for e in b.replicas)
     H = [[] for _ in range(R)]
+    # timed region: exactly `steps` steps; fts_batch_step returns after its last device
+    # synchronisation, so the wall clock brackets the device work
+    b.set_timing(True)
     t0 = time.perf_counter()
-    done = 0
-    while done < steps:
-        k = min(200, steps - done)
+    recs = b.step(steps, record_every=1)
+    elapsed = time.perf_counter() - t0
+    timing = b.get_timing(0)   # before set_timing, which clears the accumulated events
+    b.set_timing(False)
+    for r in range(R):
+        H[r].extend(x["H"] for x in recs[r])
+    print(f"[bench] {label}: {steps} timed steps in {elapsed:.2f} s", file=sys.stderr, flush=True)
+    done = steps
+    t1 = time.perf_counter()
+    while stats and done < record_steps:
+        k = min(250, record_steps - done)
         recs = b.step(k, record_every=1)
         for r in range(R):
             H[r].extend(x["H"] for x in recs[r])
         done += k
-        print(f"[bench] {label}: {done}/{steps} timed steps", file=sys.stderr, flush=True)
-    elapsed = time.perf_counter() - t0
-    restarts = [e.get_info().restarts for e in b.replicas]
+        print(f"[bench] {label}: record {done}/{record_steps} steps", file=sys.stderr, flush=True)
+    record_s = time.perf_counter() - t1
+    restarts = sum(e.get_info().restarts for e in b.replicas) - restarts0
     lrk_stats = None
     if lrk is not None:
-        st = [e.lrk_get_stats() for e in b.replicas]
-        judged = max(1, sum(x["judged"] for x in st))
-        lrk_stats = {"learned_steps": int(sum(x["learned_steps"] for x in st)),
-                     "fallback_steps": int(sum(x["fallback_steps"] for x in st)),
-                     "rel_force_residual_learned": [float(sum(x["sum_rel_residual_learned"][c] for x in st) / judged) for c in range(2)],
-                     "rel_force_residual_debye": [float(sum(x["sum_rel_residual_debye"][c] for x in st) / judged) for c in range(2)]}
+        t = lrk_totals([e.lrk_get_stats() for e in b.replicas])
+        judged = max(1, t["judged"] - lrk0["judged"])
+        lrk_stats = {"judged_steps": int(t["judged"] - lrk0["judged"]),
+                     "learned_steps": int(t["learned"] - lrk0["learned"]),
+                     "fallback_steps": int(t["fallback"] - lrk0["fallback"]),
+                     "learned_fraction": (t["learned"] - lrk0["learned"]) / judged,
+                     "rel_force_residual_learned": [(t["res_learned"][c] - lrk0["res_learned"][c]) / judged for c in range(2)],
+                     "rel_force_residual_debye": [(t["res_debye"][c] - lrk0["res_debye"][c]) / judged for c in range(2)]}
     final = (b.replicas[0].get_field(FIELD_OMEGA), b.replicas[0].get_field(FIELD_PSI)[0])
-    timing = b.get_timing(0)
     sched = b.info.group_schedule[0]
     tune = list(b.info.group_tune_ms[0])
     b.close()
+    out = dict(replicas=R, replica_steps_per_s=R * steps / elapsed, ms_per_step=1e3 * elapsed / steps,
+               equilibration_steps=equil, timed_steps=steps, timing=timing, elapsed=elapsed, schedule=sched,
+               tune_ms=tune, restarts=int(restarts), final=final, lrk=lrk_stats)
+    if not stats:
+        return out
     H = np.array(H)
     tau, window = tau_int(H.real)
-    tau_each = [tau_int(H.real[r:r + 1])[0] for r in range(R)]
+    n = H.shape[1]
     # Sokal's estimate of the statistical error of tau_int: var = 2 (2W + 1) / n tau^2,
     # n the number of samples pooled
     tau_err = float(tau * np.sqrt(2.0 * (2 * window + 1) / H.size))
-    sps = R * steps / elapsed
-    stderr_H = float(H.real.std(ddof=1) * np.sqrt(2 * tau / H.size))
-    return dict(replicas=R, replica_steps_per_s=sps, ms_per_step=1e3 * elapsed / steps, tau_int=tau, window=window,
-                tau_int_each=[round(float(x), 3) for x in tau_each], tau_int_err=tau_err,
-                samples_per_s=sps / (2 * tau), H_mean=float(H.real.mean()), H_imag_mean=float(H.imag.mean()),
-                H_stderr=stderr_H, H0=H[0].real, timing=timing, elapsed=elapsed, schedule=sched, tune_ms=tune,
-                restarts=int(sum(restarts)), final=final, lrk=lrk_stats)
+    valid = n >= 50 * max(window, 1)
+    out.update(record_steps=n, tau_record_samples=int(H.size), record_replica_steps_per_s=R * (n - steps) / record_s
+               if n > steps and record_s > 0 else None,
+               tau_int=tau, window=window, tau_int_err=tau_err,
+               tau_int_each=[round(float(tau_int(H.real[r:r + 1])[0]), 3) for r in range(R)],
+               samples_per_s=(out["replica_steps_per_s"] / (2 * tau)) if valid else None,
+               H_mean=float(H.real.mean()), H_imag_mean=float(H.imag.mean()),
+               H_stderr=float(H.real.std(ddof=1) * np.sqrt(2 * tau / H.size)))
+    return out
 
 
 def copy_bandwidth(gib=1.0, reps=10):
@@ -227,24 +273,37 @@ def cpu_baseline(system, n_steps, tau_plain, scheme_name):
                       f"({el:.1f} s, own mixed-radix FFT, OpenMP); samples/s uses the GPU plain engine's tau_int"}
 
 
+def ratio_err(x, y):
+    """Relative error of x/y's decorrelated-sample ratio from the two tau_int errors."""
+    return float(np.hypot(x["tau_int_err"] / x["tau_int"], y["tau_int_err"] / y["tau_int"]))
+
+
+LEG_KEYS = ("replicas", "replica_steps_per_s", "ms_per_step", "equilibration_steps", "timed_steps", "record_steps",
+            "tau_record_samples", "record_replica_steps_per_s", "tau_int", "tau_int_err", "window", "tau_int_each",
+            "samples_per_s", "H_mean", "H_stderr", "schedule", "tune_ms", "restarts")
+
+
+def leg_summary(leg):
+    return {k: leg[k] for k in LEG_KEYS if k in leg}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000, help="timed steps of the fast leg (the measured one)")
-    ap.add_argument("--aux-steps", type=int, default=1000,
-                    help="timed steps of the comparison legs (FP32 PC, plain FP64 and its tau_int replicas), at most --steps")
-    ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--replicas", type=int, default=16, help="replicas in the fast batch")
+    ap.add_argument("--steps", type=int, default=200, help="timed steps of every leg (their rate)")
+    ap.add_argument("--warmup", type=int, default=200,
+                    help=f"untimed steps before the timed ones (at least {MIN_EQUILIBRATION} are taken)")
+    ap.add_argument("--record-steps", type=int, default=2000,
+                    help="steps of the H record of the FP32 legs (tau_int, <H>), timed steps included")
+    ap.add_argument("--plain-record-steps", type=int, default=1000, help="the same for the plain FP64 batch")
+    ap.add_argument("--replicas", type=int, default=16, help="replicas in the FP32 batches")
+    ap.add_argument("--plain-replicas", type=int, default=8, help="replicas in the plain FP64 batch")
     ap.add_argument("--dt-fast", type=float, default=0.01)
-    ap.add_argument("--fast-scheme", choices=["si_learned", "si", "em", "pc"], default="si_learned")
+    ap.add_argument("--dt-plain", type=float, default=0.01)
     ap.add_argument("--lrk", default=LRK_PARAMS, help="parameters of the learned kernel (.npy)")
     ap.add_argument("--lrk-threshold", type=float, default=1e-3)
-    ap.add_argument("--dt-plain", type=float, default=0.01)
-    ap.add_argument("--plain-replicas", type=int, default=1)
-    ap.add_argument("--tau-replicas", type=int, default=8,
-                    help="independent FP64 replicas (one batch) whose pooled autocorrelation gives the plain scheme's "
-                         "tau_int; one trajectory of `steps` steps estimates it too poorly")
-    ap.add_argument("--no-fp32-pc", action="store_true", help="skip the FP32 predictor-corrector comparison batch")
+    ap.add_argument("--with-fp32-pc", action="store_true", help="add the FP32 predictor-corrector batch")
+    ap.add_argument("--no-debye", action="store_true", help="skip the FP32 Debye semi-implicit batch")
     ap.add_argument("--schedule", choices=["auto", "plane", "line", "conv"], default="auto",
                     help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
@@ -254,35 +313,36 @@ def main():
         print("bench.py measures one GPU: the engine runs each system (or replica batch) on a single GPU, "
               "and runs on several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1:
+        print("--steps must be at least 1", file=sys.stderr)
+        sys.exit(2)
 
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
-    schemes = {"em": SCHEME_EM, "si": SCHEME_SEMI_IMPLICIT, "pc": SCHEME_PC_REFERENCE, "si_learned": SCHEME_SI_LEARNED}
-    lrk = load_params(args.lrk) if args.fast_scheme == "si_learned" else None
+    lrk = load_params(args.lrk)
     sched = {"auto": 0, "plane": 1, "line": 2, "conv": 3}[args.schedule]
-    fast_sys = [make_system(**wl, dt=args.dt_fast, precision=PREC_FP32, scheme=schemes[args.fast_scheme], seed=0xC4 + r,
-                            fft_schedule=sched) for r in range(args.replicas)]
-    pc_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP32, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r,
-                          fft_schedule=sched) for r in range(args.replicas)]
-    plain_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r)
-                 for r in range(args.plain_replicas)]
+    fp32 = lambda scheme, dt: [make_system(**wl, dt=dt, precision=PREC_FP32, scheme=scheme, seed=0xC4 + r,
+                                           fft_schedule=sched) for r in range(args.replicas)]
+    plain = lambda n: [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE,
+                                   seed=0xC4 + r) for r in range(n)]
+    rec = max(args.record_steps, args.steps)
+    prec = max(args.plain_record_steps, args.steps)
+    W, K = args.warmup, args.steps
 
-    fast = run_batch(fast_sys, args.warmup, args.steps, f"fast fp32 {args.fast_scheme} x{args.replicas}", lrk=lrk,
-                     lrk_threshold=args.lrk_threshold)
-    aux = min(args.aux_steps, args.steps)
-    fp32_pc = None if args.no_fp32_pc else run_batch(pc_sys, args.warmup, aux, f"fp32 pc x{args.replicas}")
-    plain = run_batch(plain_sys, args.warmup, aux, f"plain fp64 pc x{args.plain_replicas}")
-    plain["tau_source"] = f"{args.plain_replicas} replica(s) timed"
-    if args.tau_replicas > args.plain_replicas:
-        # the same dynamics in more independent replicas: tau_int (and <H>) from their pooled
-        # statistics; the throughput stays that of the timed plain engine
-        tau_sys = [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE, seed=0xC4 + r)
-                   for r in range(args.tau_replicas)]
-        st = run_batch(tau_sys, args.warmup, aux, f"plain fp64 pc statistics x{args.tau_replicas}")
-        plain["tau_int_timed_replicas"] = plain["tau_int"]
-        for k in ("tau_int", "window", "tau_int_each", "tau_int_err", "H_mean", "H_stderr"):
-            plain[k] = st[k]
-        plain["samples_per_s"] = plain["replica_steps_per_s"] / (2 * plain["tau_int"])
-        plain["tau_source"] = f"pooled autocorrelation of {args.tau_replicas} FP64 replicas"
+    fast = run_leg(fp32(SCHEME_SI_LEARNED, args.dt_fast), f"fast fp32 si_learned x{args.replicas}", W, K, rec, lrk=lrk,
+                   lrk_threshold=args.lrk_threshold)
+    debye = None if args.no_debye else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_fast),
+                                               f"si_debye fp32 x{args.replicas}", W, K, rec)
+    fp32_pc = run_leg(fp32(SCHEME_PC_REFERENCE, args.dt_plain), f"fp32 pc x{args.replicas}", W, K, rec) \
+        if args.with_fp32_pc else None
+    plain_b = run_leg(plain(args.plain_replicas), f"plain fp64 pc x{args.plain_replicas}", W, K, prec)
+    plain_1 = run_leg(plain(1), "plain fp64 pc x1", W, K, 0, stats=False)
+    # one system's rate; its tau_int and <H> are those of the same dynamics in the batch
+    for key in ("tau_int", "tau_int_err", "window", "H_mean", "H_stderr", "record_steps", "tau_record_samples"):
+        plain_1[key] = plain_b[key]
+    plain_1["samples_per_s"] = plain_1["replica_steps_per_s"] / (2 * plain_b["tau_int"]) \
+        if plain_b["samples_per_s"] is not None else None
+    plain_1["tau_source"] = f"the plain FP64 batch's record ({args.plain_replicas} replicas)"
+
     same = same_state_errors(wl, fast["final"])
 
     def roof(t):
@@ -301,24 +361,29 @@ def main():
         for t in json.load(open(tpath)).get("entries", []):
             if t.get("replicas") == args.replicas and t.get("schedule") == fast["schedule"] and t.get("precision") == "fp32":
                 traffic = t["traffic_bytes"]
-    _, _, plain_achieved = roof(plain["timing"])
+    _, _, plain_achieved = roof(plain_1["timing"])
+    _, _, plain_b_achieved = roof(plain_b["timing"])
     try:
         copy_gbs = copy_bandwidth()
     except Exception as exc:  # report the peak fraction only
         print(f"[bench] copy bandwidth not measured: {exc}", file=sys.stderr)
         copy_gbs = None
-    H_err_vs_plain = abs(fast["H_mean"] - plain["H_mean"]) / abs(plain["H_mean"])
-    H_err_sigma = np.hypot(fast["H_stderr"], plain["H_stderr"]) / abs(plain["H_mean"])
-    keys = ("replicas", "replica_steps_per_s", "ms_per_step", "tau_int", "tau_int_err", "tau_int_each", "samples_per_s",
-            "H_mean", "H_stderr", "schedule", "tune_ms", "restarts")
+    H_err_vs_plain = abs(fast["H_mean"] - plain_b["H_mean"]) / abs(plain_b["H_mean"])
+    H_err_sigma = np.hypot(fast["H_stderr"], plain_b["H_stderr"]) / abs(plain_b["H_mean"])
+
+    def speedup(x, y):
+        if x is None or y is None or x.get("samples_per_s") is None or y.get("samples_per_s") is None:
+            return None
+        return {"ratio": x["samples_per_s"] / y["samples_per_s"], "rel_err": ratio_err(x, y)}
+
     sched_name = {1: "plane", 2: "line", 3: "conv"}
     out = {
         "metric": "decorrelated CL samples/s at 96^3 N=100, FP32+learned vs plain FP64, <H> err",
         "value": fast["samples_per_s"],
         "unit": "decorrelated CL samples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": K,
+        "warmup": W,
         "ms_per_step": fast["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
@@ -326,36 +391,47 @@ def main():
         "dtype": "c64 propagators/FFT, f64 reductions",
         "data": "synthetic",
         "config": {"workload": "96^3 L=48 N=100 z=-0.5 salt=0.05 B=0.5 lB=1 (221 chains)",
-                   "fast": f"FP32 {args.fast_scheme} dt={args.dt_fast}, {args.replicas} replicas per launch"
-                           + (f", learned kernel {os.path.basename(args.lrk)}, fallback threshold {args.lrk_threshold}"
-                              if lrk is not None else ""),
-                   "fp32_pc": None if fp32_pc is None else f"FP32 pc dt={args.dt_plain}, {args.replicas} replicas per launch",
-                   "plain": f"FP64 pc dt={args.dt_plain}, {args.plain_replicas} replica(s)",
+                   "fast": f"FP32 si_learned dt={args.dt_fast}, {args.replicas} replicas per launch, learned kernel "
+                           f"{os.path.basename(args.lrk)}, fallback threshold {args.lrk_threshold}",
+                   "si_debye": None if debye is None else f"FP32 si (Debye kernel) dt={args.dt_fast}, {args.replicas} replicas",
+                   "plain_fp64_batched": f"FP64 pc dt={args.dt_plain}, {args.plain_replicas} replicas per launch",
+                   "plain_fp64": f"FP64 pc dt={args.dt_plain}, 1 system",
+                   "statistics": f"tau_int and <H> from an H record of {rec} steps (FP32 legs) / {prec} (FP64 batch) "
+                                 f"after {max(W, MIN_EQUILIBRATION)} equilibration steps; rate from the {K} timed steps",
                    "parallelism": "single GPU"},
-        "fast": {k: fast[k] for k in keys},
-        "fp32_pc": None if fp32_pc is None else {k: fp32_pc[k] for k in keys},
-        "plain_fp64": dict({k: plain[k] for k in keys}, tau_source=plain["tau_source"],
-                           **({"tau_int_timed_replicas": plain["tau_int_timed_replicas"]} if "tau_int_timed_replicas" in plain else {})),
+        "equilibration_steps": fast["equilibration_steps"],
+        "tau_record_samples": fast["tau_record_samples"],
+        "learned_fraction": fast["lrk"]["learned_fraction"],
+        "fast": leg_summary(fast),
+        "si_debye": None if debye is None else leg_summary(debye),
+        "fp32_pc": None if fp32_pc is None else leg_summary(fp32_pc),
+        "plain_fp64_batched": leg_summary(plain_b),
+        "plain_fp64": dict(leg_summary(plain_1), tau_source=plain_1["tau_source"]),
         "learned_kernel": fast["lrk"],
-        "speedup_vs_plain_fp64": fast["samples_per_s"] / plain["samples_per_s"],
-        "speedup_rel_err": float(np.hypot(fast["tau_int_err"] / fast["tau_int"], plain["tau_int_err"] / plain["tau_int"])),
-        "fp32_pc_speedup_vs_plain_fp64": None if fp32_pc is None else fp32_pc["samples_per_s"] / plain["samples_per_s"],
+        "speedup_vs_plain_fp64": speedup(fast, plain_1),
+        "speedup_vs_plain_fp64_batched": speedup(fast, plain_b),
+        "learned_vs_debye": speedup(fast, debye),
+        "debye_speedup_vs_plain_fp64": speedup(debye, plain_1),
+        "fp32_pc_speedup_vs_plain_fp64": speedup(fp32_pc, plain_1),
         "H_err_same_state": float(same["H"]),
         "same_state_rel_err": {k: float(v) for k, v in same.items() if k != "H_fp64"},
         "H_err_vs_plain": float(H_err_vs_plain),
         "H_err_sigma": float(H_err_sigma),
+        "H_mean": {"fast": fast["H_mean"], "si_debye": None if debye is None else debye["H_mean"],
+                   "plain_fp64_batched": plain_b["H_mean"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"bead-step launch group ({sched_name.get(fast['schedule'], '?')} schedule), FP32, "
                                f"{args.replicas} replica jobs per launch",
                      "us_per_launch_group": us, "alg_bytes_per_launch_group": byt,
                      "plain_fp64_achieved": plain_achieved,
+                     "plain_fp64_batched_achieved": plain_b_achieved,
                      "copy_achievable": copy_gbs,
                      "frac_of_copy": None if not copy_gbs else achieved / copy_gbs,
                      "traffic_rate": None if traffic is None else traffic / (us * 1e-6) / 1e9},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(plain_sys[0], args.cpu_steps, plain["tau_int"], "PC")
+        out["cpu_baseline"] = cpu_baseline(plain(1)[0], args.cpu_steps, plain_b["tau_int"], "PC")
     print(json.dumps(out))
 
 
