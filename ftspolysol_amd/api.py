@@ -161,6 +161,7 @@ def _bind(lib, prefix):
         "get_drift_rows": (C.c_int, [vp, P(C.c_double), C.c_int, P(C.c_int)]),
         "fft3d": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_int]),
         "bead_step": (C.c_int, [vp, P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)]),
+        "bead_step_noalloc": (C.c_int, [vp, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
         "set_timing": (C.c_int, [vp, C.c_int]),
         "get_timing": (C.c_int, [vp, P(Timing)]),
         "batch_create": (C.c_int, [P(SystemDesc), C.c_int, P(vp)]),
@@ -332,6 +333,11 @@ class Engine:
         self._check(self._fns["bead_step"](self._h, _dptr(qi.view(np.float64)), _dptr(ei.view(np.float64)),
                                            float(bond), _dptr(out.view(np.float64))))
         return out.reshape(self.shape)
+
+    def bead_step_noalloc(self, q_dev, exp_minus_w_dev, bond, q_out_dev, stream=None):
+        """fts_bead_step_noalloc on device addresses (ints); enqueued on `stream` (None: the
+        engine's stream) without waiting."""
+        self._check(self._fns["bead_step_noalloc"](self._h, q_dev, exp_minus_w_dev, float(bond), q_out_dev, stream))
 
     def set_timing(self, enabled):
         self._check(self._fns["set_timing"](self._h, int(enabled)))

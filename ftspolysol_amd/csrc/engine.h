@@ -140,6 +140,8 @@ struct fts_engine {
 
     // ---- device ---------------------------------------------------------------
     hipStream_t stream = nullptr;
+    int device = 0;                  // the GPU current at creation
+    void* noalloc_scratch = nullptr; // this GPU's address of the no-alloc scratch array (noalloc.hip)
     bool own_stream = true; // false: the stream belongs to a replica batch
     FftGeom g64{}, g32{};
     DevBuf<double> kvec, sep_gamma, sep_bond, sep_ion, si_table_omega, si_table_psi;
@@ -225,7 +227,7 @@ struct fts_engine {
     template <typename T> int solve_species(int p, bool accumulate = true);
     template <typename T> int run_sweeps(int p, int schedule, int nsteps, bool timed);
     int autotune_schedule();
-    template <typename T> int bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op);
+    template <typename T> int bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op, hipStream_t s);
     int sched = FTS_SCHED_CONV;
     bool plane_ok = true;
     size_t norm_stride = 0;
@@ -317,3 +319,6 @@ struct fts_engine {
 // s->fft_schedule is FTS_SCHED_AUTO.
 int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine** out);
 
+// The no-alloc scratch array of the current GPU (noalloc.hip): looks its address up
+// once per device (hipGetSymbolAddress with that device current) and returns it.
+int noalloc_setup(void** scratch_out);

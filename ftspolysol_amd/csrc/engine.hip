@@ -13,6 +13,8 @@
 // followed by Gaussian smoothing, ions, drifts and the fused CL update kernels.
 // Reductions stay on the device; the host reads a scalar only where it needs
 // one (finiteness checks, records).
+#include <mutex>
+
 #include "engine.h"
 #include "kernels.h"
 
@@ -1020,7 +1022,7 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
 
 // One bead step on a single field with the engine's schedule (fts_bead_step): q is
 // overwritten with intermediates, qout = e^{-W} S_phi[q].
-template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op)
+template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T>* qout, const KOp& op, hipStream_t stream)
 {
     const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
     const bool d = sizeof(T) == 8;
@@ -1053,6 +1055,9 @@ template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T
     HIP_TRY(launch_line<T>(g, 2, LINE_INV_E, twz, &zj, 1, stream));
     return FTS_OK;
 }
+
+template int fts_engine::bead_hook<float>(V2<float>*, const V2<float>*, V2<float>*, const KOp&, hipStream_t);
+template int fts_engine::bead_hook<double>(V2<double>*, const V2<double>*, V2<double>*, const KOp&, hipStream_t);
 
 // Bead-step schedule chosen by timing plane and line on this grid and precision (the
 // FFT autotuner): 2 x 8 bead steps of species 0 at zero fields, best of three;
@@ -1846,6 +1851,55 @@ int fts_create(const fts_system* s, fts_engine** out) { return engine_create(s, 
 
 } // extern "C"
 
+// ---- the no-alloc scratch of fts_bead_step_noalloc -------------------------------
+// fts_bead_step_noalloc (below fts_bead_step) is the bead step of fts_bead_step on
+// device arrays, for callers that cannot allocate device memory (code under HIP
+// graph capture, serving loops, harnesses that forbid hipMalloc).
+//
+// Its intermediates (the bond kernel's axis factors and the work fields of the
+// transform passes) live in one file-scope device array, `scratch`, of
+// NOALLOC_BYTES bytes.  Host code never uses the array's name as an address: the
+// name is a host-side stand-in.  noalloc_setup looks up the array's address on the
+// current GPU once per device with hipGetSymbolAddress (each GPU holds its own copy
+// until the process ends) and every engine keeps the address of its GPU; the kernels
+// receive pointers derived from it.  Every call on a GPU shares that GPU's array and
+// finds the previous call's contents there: each call writes every scratch byte it
+// reads before reading it, so nothing needs clearing.  Calls that overlap on one GPU
+// (two streams, two host threads) race on the array.
+namespace {
+
+// grids of at most 2^20 points (96^3 = 884 736 fits): the FP32 form stages q, e^{-W}
+// and q' in complex FP32 (3 x 8 B per point), the FP64 form the work field
+// (16 B per point); the bond factors take m1 + m2 + m3 <= 768 doubles
+constexpr size_t NOALLOC_MAX_POINTS = size_t(1) << 20;
+constexpr size_t NOALLOC_TABLE_BYTES = 8192;
+constexpr size_t NOALLOC_BYTES = NOALLOC_TABLE_BYTES + 3 * sizeof(float2) * NOALLOC_MAX_POINTS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+int noalloc_setup(void** scratch_out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(FTS_DEVICE_ERROR, "device index beyond the no-alloc table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(FTS_DEVICE_ERROR, "hipGetSymbolAddress returned no address for the no-alloc scratch");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *scratch_out = g_scratch_ptr[dev];
+    return FTS_OK;
+}
+
+
 int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine** out)
 {
     if (!s || !out) return fail(FTS_INVALID, "null argument");
@@ -1982,6 +2036,11 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     for (double x : e->lambda_omega)
         if (!std::isfinite(x) || x < 0.0) return bail(fail(FTS_INVALID, "lambda_omega"));
     if ((rc = e->build_ions(s))) return bail(rc);
+    {
+        const hipError_t he = hipGetDevice(&e->device);
+        if (he != hipSuccess) return bail(fail(FTS_DEVICE_ERROR, std::string("hipGetDevice: ") + hipGetErrorString(he)));
+    }
+    if ((rc = noalloc_setup(&e->noalloc_scratch))) return bail(rc);
     if ((rc = e->setup_device())) return bail(rc);
     const bool semi = e->scheme == FTS_SCHEME_SEMI_IMPLICIT || e->scheme == FTS_SCHEME_SI_LEARNED;
     if (semi && (rc = e->build_si_tables())) return bail(rc);
@@ -2316,13 +2375,10 @@ int fts_bead_step(fts_engine* h, const double* q, const double* emw, double bond
 {
     if (!h || !q || !emw || !q_out || !(bond > 0.0)) return fail(FTS_INVALID, "bad argument");
     const size_t Ms = static_cast<size_t>(h->M);
-    std::vector<double> sep;
-    for (int x = 0; x < h->m1; ++x) sep.push_back(std::exp(-bond * bond * std::pow(wave_number(x, h->m1, h->L1), 2) / 6.0));
-    for (int y = 0; y < h->m2; ++y) sep.push_back(std::exp(-bond * bond * std::pow(wave_number(y, h->m2, h->L2), 2) / 6.0));
-    for (int z = 0; z < h->m3; ++z) sep.push_back(std::exp(-bond * bond * std::pow(wave_number(z, h->m3, h->L3), 2) / 6.0));
+    const int nk = h->m1 + h->m2 + h->m3;
     DevBuf<double> dsep;
-    HIP_TRY(dsep.alloc(sep.size()));
-    HIP_TRY(hipMemcpyAsync(dsep.p, sep.data(), sizeof(double) * sep.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(dsep.alloc(static_cast<size_t>(nk)));
+    hipLaunchKernelGGL(k_bond_table, dim3((nk + 255) / 256), dim3(256), 0, h->stream, h->kvec.p, bond, dsep.p, nk);
     HIP_TRY(hipMemcpyAsync(h->t1.p, q, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipMemcpyAsync(h->t2.p, emw, sizeof(double2) * Ms, hipMemcpyHostToDevice, h->stream));
     const KOp op = h->kop_sep(dsep.p, 1.0 / static_cast<double>(h->M));
@@ -2334,16 +2390,61 @@ int fts_bead_step(fts_engine* h, const double* q, const double* emw, double bond
         float2* o = static_cast<float2*>(h->qF);
         hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, h->stream, h->t1.p, a, 1.0, Ms);
         hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, h->stream, h->t2.p, e, 1.0, Ms);
-        rc = h->bead_hook<float>(a, e, o, op);
+        rc = h->bead_hook<float>(a, e, o, op, h->stream);
         hipLaunchKernelGGL((k_cast<float, double>), dim3(nb), dim3(EW_THREADS), 0, h->stream, o, h->t3.p, 1.0, Ms);
     } else {
-        rc = h->bead_hook<double>(h->t1.p, h->t2.p, h->t3.p, op);
+        rc = h->bead_hook<double>(h->t1.p, h->t2.p, h->t3.p, op, h->stream);
     }
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(q_out, h->t3.p, sizeof(double2) * Ms, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     dsep.release();
+    return FTS_OK;
+}
+
+int fts_bead_step_noalloc(fts_engine* h, const double* q, const double* exp_minus_w, double bond, double* q_out,
+                                     void* stream)
+{
+    if (!h || !q || !exp_minus_w || !q_out || !(bond > 0.0)) return fail(FTS_INVALID, "bead_step_noalloc: bad argument");
+    const size_t Ms = static_cast<size_t>(h->M);
+    if (Ms > NOALLOC_MAX_POINTS)
+        return fail(FTS_INVALID, "bead_step_noalloc: a grid of " + std::to_string(Ms) + " points exceeds the no-alloc scratch (at most " +
+                                     std::to_string(NOALLOC_MAX_POINTS) + " points)");
+    const int nk = h->m1 + h->m2 + h->m3;
+    if (static_cast<size_t>(nk) * sizeof(double) > NOALLOC_TABLE_BYTES) return fail(FTS_INVALID, "bead_step_noalloc: axis lengths");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != h->device) return fail(FTS_INVALID, "bead_step_noalloc: the engine's GPU must be the current device");
+    if (!h->noalloc_scratch) return fail(FTS_DEVICE_ERROR, "bead_step_noalloc: scratch address not set up");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+    unsigned char* base = static_cast<unsigned char*>(h->noalloc_scratch);
+    double* table = reinterpret_cast<double*>(base);
+    unsigned char* fields = base + NOALLOC_TABLE_BYTES;
+    hipLaunchKernelGGL(k_bond_table, dim3((nk + 255) / 256), dim3(256), 0, s, h->kvec.p, bond, table, nk);
+    const KOp op = h->kop_sep(table, 1.0 / static_cast<double>(h->M));
+    const int nb = ew_blocks(Ms);
+    const double2* qd = reinterpret_cast<const double2*>(q);
+    const double2* ed = reinterpret_cast<const double2*>(exp_minus_w);
+    double2* od = reinterpret_cast<double2*>(q_out);
+    int rc = FTS_OK;
+    if (h->precision == FTS_PREC_FP32) {
+        float2* a = reinterpret_cast<float2*>(fields);
+        float2* e = a + Ms;
+        float2* o = e + Ms;
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, s, qd, a, 1.0, Ms);
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(nb), dim3(EW_THREADS), 0, s, ed, e, 1.0, Ms);
+        rc = h->bead_hook<float>(a, e, o, op, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL((k_cast<float, double>), dim3(nb), dim3(EW_THREADS), 0, s, o, od, 1.0, Ms);
+    } else {
+        // the passes overwrite their input with intermediates: work on a copy of q
+        double2* a = reinterpret_cast<double2*>(fields);
+        hipLaunchKernelGGL((k_cast<double, double>), dim3(nb), dim3(EW_THREADS), 0, s, qd, a, 1.0, Ms);
+        rc = h->bead_hook<double>(a, ed, od, op, s);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipGetLastError());
     return FTS_OK;
 }
 

@@ -59,6 +59,15 @@ __global__ void k_cast(const V2<S>* __restrict__ src, V2<D>* __restrict__ dst, d
     if (i < n) dst[i] = cmk<V2<D>>(static_cast<D>(scale * src[i].x), static_cast<D>(scale * src[i].y));
 }
 
+// separable bond kernel of one bond length: out[i] = exp(-b^2 k_i^2 / 6) for the
+// m1 + m2 + m3 axis wave numbers k (x, then y, then z), the factors of
+// phi_b(k) = exp(-b^2 |k|^2 / 6) (spectral.cu:22-30)
+__global__ void k_bond_table(const double* __restrict__ kvec, double bond, double* __restrict__ out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = exp(-bond * bond * (kvec[i] * kvec[i]) / 6.0);
+}
+
 // ---- plain sums -------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(RED_THREADS) k_sum(const V2<T>* __restrict__ a, int M, double2* __restrict__ partials)

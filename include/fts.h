@@ -284,6 +284,19 @@ int fts_fft3d(fts_engine* h, const double* in, double* out, int sign);
 /* One bead step q' = e^{-W} * IFFT[phi_b(k) FFT[q]/M] (polymer.cu:396-400) with bond length b. */
 int fts_bead_step(fts_engine* h, const double* q, const double* exp_minus_w, double bond, double* q_out);
 
+/* fts_bead_step on device arrays without allocating.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  q, exp_minus_w and q_out are device arrays of
+ * M interleaved complex doubles on the engine's GPU (q_out may not alias the inputs); the
+ * work is enqueued on `stream` (a hipStream_t; NULL: the engine's stream) and the call
+ * returns without waiting, never allocating or freeing device memory and never
+ * synchronising.  The result equals fts_bead_step's bit for bit.  The intermediates live
+ * in one scratch array per GPU (about 24 MiB, held until the process ends), so grids are
+ * limited to 2^20 points (larger ones fail with FTS_INVALID), and calls that overlap on
+ * one GPU (two streams or host threads) race on it.  The engine's GPU must be the current
+ * device.  The engine's own buffers and state are not touched. */
+int fts_bead_step_noalloc(fts_engine* h, const double* q, const double* exp_minus_w, double bond, double* q_out, void* stream);
+
 /* Device timing of the propagator sweeps (engine only; the oracle ignores it). */
 int fts_set_timing(fts_engine* h, int enabled);
 int fts_get_timing(fts_engine* h, fts_timing* out);

@@ -45,8 +45,9 @@ def test_oracle_exports_the_same_interface():
     # timing and replica batches are device-scheduling concepts: the oracle checks a
     # batch replica by replica through the single-system interface.  The front end
     # (decks, .rf files, resize) is host I/O that every caller shares; its own tests
-    # pin it, and there is no device arithmetic for the oracle to check.
-    engine_only = {"fts_set_timing", "fts_get_timing", "fts_field_resize"}
+    # pin it, and there is no device arithmetic for the oracle to check.  The no-alloc
+    # bead step takes device arrays; its test compares it with fts_bead_step.
+    engine_only = {"fts_set_timing", "fts_get_timing", "fts_field_resize", "fts_bead_step_noalloc"}
     missing = [n for n in declared_functions()
                if n not in engine_only and not n.startswith(("fts_batch_", "fts_deck_", "fts_rf_"))
                and not hasattr(lib, "fto_" + n[4:])]
