@@ -287,6 +287,188 @@ def leg_summary(leg):
     return {k: leg[k] for k in LEG_KEYS if k in leg}
 
 
+def obs_vector(rec):
+    """The observables a record reports (real parts, as observables.dat prints them,
+    complexLangevin.cu:72-91): H, mu_ex of the polymer, the six stress components, p_iso."""
+    return np.concatenate([[rec["H"].real], rec["mu_ex_poly"].real, rec["stress"].real, [rec["p_iso"].real]])
+
+
+OBS_NAMES = ["H", "mu_ex_poly", "sigma_xx", "sigma_yy", "sigma_zz", "sigma_xy", "sigma_xz", "sigma_yz", "p_iso"]
+
+
+def same_state(sys64, sys32, om, ps):
+    """Observables of an FP32 and an FP64 engine on the same fields."""
+    e64, e32 = hip_engine(sys64), hip_engine(sys32)
+    for e in (e64, e32):
+        e.set_fields(om, ps)
+    a, b = obs_vector(e32.measure()), obs_vector(e64.measure())
+    e64.close()
+    e32.close()
+    return a, b
+
+
+def run_c2(args):
+    """Precision form, single system: 64^3, L = 32, N = 50, salt 0.1, semi-implicit (Debye)
+    at dt = 0.05; FP32 (propagators, FFTs; per-bead renormalisation; FP64 reductions and
+    field updates) against FP64 on the same noise stream.  Three measurements:
+      same_state  the precision error: FP32 and FP64 observables on identical fields, at
+                  --c2-states states along an FP64 trajectory (max relative error, per
+                  observable; mu compared absolutely where |mu| < 1, the stress components
+                  relative to the largest |sigma_ab|);
+      divergence  the same-noise FP32 and FP64 trajectories step by step from identical
+                  initial fields: relative difference of H at every step (they separate
+                  as the CL dynamics amplify the rounding differences);
+      ensemble    time averages over R replicas x --c2-steps steps (records every
+                  --c2-every, the first tenth dropped) in FP32 and in FP64 batches with the
+                  same seeds and initial fields: relative difference of the averages with
+                  the combined standard error (Sokal tau_int of each record)."""
+    from ftspolysol_amd.system import config
+    R, steps, every = args.c2_replicas, args.c2_steps, args.c2_every
+    mk = lambda prec, seed, mask=0x7F: config(2, precision=prec, scheme=SCHEME_SEMI_IMPLICIT, stress_mask=mask, seed=seed)
+    # divergence of one same-noise pair
+    pair = {}
+    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32)):
+        e = hip_engine(mk(prec, 0xC2))
+        e.set_fields(*initial_fields(mk(prec, 0xC2), sigma=1e-3, cfg=2))
+        pair[name] = np.array([r["H"].real for r in e.step(args.c2_div_steps, record_every=1)])
+        e.close()
+    div = np.abs(pair["fp32"] - pair["fp64"]) / np.abs(pair["fp64"])
+    print(f"[bench c2] divergence over {args.c2_div_steps} steps measured", file=sys.stderr, flush=True)
+    # ensemble batches, with same-state probes of replica 0 along the FP64 run
+    recs, rate, restarts = {}, {}, {}
+    probes = []
+    chunk = max(every, steps // max(1, args.c2_states))
+    for name, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32)):
+        systems = [mk(prec, 0xC2 + r) for r in range(R)]
+        b = hip_batch(systems)
+        for r, (e, sy) in enumerate(zip(b.replicas, systems)):
+            e.set_fields(*initial_fields(sy, sigma=1e-3, seed=0x5EED0002 + r))
+            e.set_restart(50, 50)
+        rr = [[] for _ in range(R)]
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            out = b.step(k, record_every=every)
+            for r in range(R):
+                rr[r] += out[r]
+            done += k
+            if name == "fp64" and len(probes) < args.c2_states:
+                om = b.replicas[0].get_field(FIELD_OMEGA)
+                ps = b.replicas[0].get_field(FIELD_PSI)[0]
+                probes.append((om, ps))
+            print(f"[bench c2] {name} x{R}: {done}/{steps} steps", file=sys.stderr, flush=True)
+        rate[name] = R * steps / (time.perf_counter() - t0)
+        restarts[name] = sum(e.get_info().restarts for e in b.replicas)
+        recs[name] = rr
+        b.close()
+    errs = []
+    for om, ps in probes:
+        a, b64 = same_state(mk(PREC_FP64, 0xC2), mk(PREC_FP32, 0xC2), om, ps)
+        e = np.abs(a - b64) / np.maximum(np.abs(b64), 1e-300)
+        e[1] = abs(a[1] - b64[1]) / max(1.0, abs(b64[1]))   # mu = -ln Q: absolute where |mu| < 1
+        # stress components relative to the tensor's scale (off-diagonal ones are near 0)
+        e[2:8] = np.abs(a[2:8] - b64[2:8]) / np.abs(b64[2:8]).max()
+        errs.append(e)
+    errs = np.array(errs)
+    per = {}
+    for i, nm in enumerate(OBS_NAMES):
+        stats = {}
+        for name in ("fp64", "fp32"):
+            X = np.array([[obs_vector(x)[i] for x in rr] for rr in recs[name]])
+            X = X[:, X.shape[1] // 10:]
+            tau, _ = tau_int(X)
+            stats[name] = (float(X.mean()), float(X.std(ddof=1) * np.sqrt(2 * tau / X.size)), float(tau))
+        m64, se64, _ = stats["fp64"]
+        m32, se32, _ = stats["fp32"]
+        per[nm] = {"same_state_max_rel_err": float(errs[:, i].max()),
+                   "fp64_mean": m64, "fp64_stderr": se64, "fp32_mean": m32, "fp32_stderr": se32,
+                   "tau_int_records": stats["fp64"][2],
+                   "ensemble_rel_diff": abs(m32 - m64) / abs(m64), "ensemble_rel_sigma": float(np.hypot(se32, se64) / abs(m64))}
+    first = lambda thr: next((int(i) + 1 for i, v in enumerate(div) if v > thr), None)
+    line = {"metric": "C2 precision form: FP32 vs FP64 per-observable relative error on the same states",
+            "value": float(errs.max()), "unit": "max relative error, same fields", "higher_is_better": False, "n_gpus": 1,
+            "steps": steps,
+            "config": {"workload": "64^3 L=32 N=50 z=-0.5 salt=0.1 B=0.5 lB=1, semi-implicit (Debye) dt=0.05",
+                       "ensemble": f"{R} replicas per precision, {steps} steps, records every {every}, first tenth dropped",
+                       "same_state": f"{len(probes)} states of FP64 replica 0, one every {chunk} steps"},
+            "replica_steps_per_s": rate, "restarts": restarts, "observables": per,
+            "divergence": {"steps_to_1e-6": first(1e-6), "steps_to_1e-4": first(1e-4), "steps_to_1e-2": first(1e-2),
+                           "H_rel_diff": [float(f"{v:.3g}") for v in div]}}
+    print(json.dumps(line))
+
+
+def run_c3(args):
+    """Batched form: 64 replicas (n in {32, 48, 64, 96}, L = n/2, N in [20, 100], salt
+    log-uniform in [1e-3, 0.5], lB in [0.5, 2], own seeds), FP32 propagators, FP64
+    reductions, semi-implicit (Debye) steps at dt = 0.01 in one replica batch: replica
+    CL steps/s over exactly `steps` timed steps after max(warmup, 50) untimed ones."""
+    from ftspolysol_amd.system import replica_params
+    systems = [make_system(**replica_params(r), dt=0.01, precision=PREC_FP32, scheme=SCHEME_SEMI_IMPLICIT)
+               for r in range(64)]
+    b = hip_batch(systems)
+    for r, (e, s) in enumerate(zip(b.replicas, systems)):
+        e.set_fields(*initial_fields(s, sigma=1e-3, seed=900 + r))
+        e.set_restart(50, 50)
+    t0 = time.perf_counter()
+    b.step(max(args.warmup, 50))
+    print(f"[bench c3] warmup in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    b.set_timing(True)
+    t0 = time.perf_counter()
+    recs = b.step(args.steps, record_every=1)
+    el = time.perf_counter() - t0
+    groups = []
+    for g in range(b.info.n_groups):
+        t = b.get_timing(g)
+        us = 1e3 * t["sweep_ms"] / max(1, t["step_launches"])
+        groups.append({"mesh": b.info.group_mesh[g][0], "replicas": b.info.group_size[g],
+                       "schedule": b.info.group_schedule[g], "us_per_launch_group": us,
+                       "alg_GBps": t["algorithmic_bytes"] / max(1e-30, t["sweep_ms"] * 1e-3) / 1e9})
+    restarts = sum(e.get_info().restarts for e in b.replicas)
+    finite = all(np.isfinite(x["H"]) for rr in recs for x in rr)
+    b.close()
+    print(json.dumps({"metric": "C3 batched form: replica CL steps/s, 64 mixed replicas", "value": 64 * args.steps / el,
+                      "unit": "replica CL steps/s", "higher_is_better": True, "n_gpus": 1, "steps": args.steps,
+                      "ms_per_step": 1e3 * el / args.steps, "config": {"workload": "64 replicas, n in {32,48,64,96}, "
+                      "N in [20,100], salt log-uniform [1e-3,0.5], lB in [0.5,2]; FP32 semi-implicit dt=0.01"},
+                      "groups": groups, "restarts": restarts, "records_finite": finite}))
+
+
+def run_c5(args):
+    """Large single system: 256^3, L = 128, N = 200, charge -1 on every second bead, salt
+    0.02, B = 1, lB = 0.7; FP32 propagators (both sweeps), FP64 reductions, the learned
+    preconditioner; `steps` timed steps with the full stress tensor and p_iso in a record
+    every --c5-stress-every steps, then one fts_measure (densities + all observables) timed
+    on its own."""
+    from ftspolysol_amd.system import config
+    s = config(5, precision=PREC_FP32, scheme=SCHEME_SI_LEARNED, stress_mask=0x7F, seed=0xC5)
+    e = hip_engine(s)
+    e.lrk_set_network(load_params(args.lrk), args.lrk_threshold)
+    e.set_fields(*initial_fields(s, sigma=1e-3, cfg=5))
+    e.set_restart(50, 20)
+    e.step(max(args.warmup, 5))
+    print("[bench c5] warmed up", file=sys.stderr, flush=True)
+    e.set_timing(True)
+    t0 = time.perf_counter()
+    recs = e.step(args.steps, record_every=args.c5_stress_every)
+    el = time.perf_counter() - t0
+    t = e.get_timing()
+    e.set_timing(False)
+    print(f"[bench c5] {args.steps} timed steps in {el:.2f} s", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    rec = e.measure()
+    measure_s = time.perf_counter() - t1
+    e.close()
+    us = 1e3 * t["sweep_ms"] / max(1, t["step_launches"])
+    print(json.dumps({"metric": "C5 large system: CL steps/s at 256^3 N=200", "value": args.steps / el, "unit": "CL steps/s",
+                      "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "ms_per_step": 1e3 * el / args.steps,
+                      "config": {"workload": "256^3 L=128 N=200 z=-1 on odd beads salt=0.02 B=1 lB=0.7; FP32 + learned SI",
+                                 "stress_every": args.c5_stress_every},
+                      "bead_step_us": us, "bead_step_alg_GBps": t["algorithmic_bytes"] / max(1e-30, t["sweep_ms"] * 1e-3) / 1e9,
+                      "stress_records": len(recs), "measure_with_stress_s": measure_s,
+                      "stress_finite": bool(np.all(np.isfinite(rec["stress"])))}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,11 +490,22 @@ def main():
                     help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", choices=["c4", "c2", "c3", "c5"], default="c4",
+                    help="c4: the headline (default); c2: FP32 vs FP64 errors at 64^3; c3: 64 mixed replicas; c5: 256^3")
+    ap.add_argument("--c2-steps", type=int, default=10000)
+    ap.add_argument("--c2-every", type=int, default=10)
+    ap.add_argument("--c2-replicas", type=int, default=8)
+    ap.add_argument("--c2-states", type=int, default=10)
+    ap.add_argument("--c2-div-steps", type=int, default=100)
+    ap.add_argument("--c5-stress-every", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine runs each system (or replica batch) on a single GPU, "
               "and runs on several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
+    if args.config != "c4":
+        {"c2": run_c2, "c3": run_c3, "c5": run_c5}[args.config](args)
+        return
     if args.steps < 1:
         print("--steps must be at least 1", file=sys.stderr)
         sys.exit(2)

@@ -1,4 +1,10 @@
-"""GPU: the two configurations the other tests do not reach at full size.
+"""GPU: the configurations the other tests do not reach at full size.
+
+Precision form (64^3, L = 32, N = 50, salt 0.1, semi-implicit at dt = 0.05): FP32
+against FP64 on the same noise stream.  At identical fields (states along an FP64
+trajectory) H, mu, the stress tensor and p_iso agree within the precision form's bar
+of 1e-4; the same-noise trajectories agree step by step (H within 1e-5) for the
+first steps, before the CL dynamics amplify the rounding differences.
 
 Large single system (256^3, L = 128, N = 200, charge -1 on every second bead, salt
 0.02, B = 1, lB = 0.7): FP32 propagators (both sweeps: the chain is not
@@ -16,7 +22,7 @@ import numpy as np
 import pytest
 
 from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, FIELD_RHO_C, FIELD_RHO_P, PREC_FP32, PREC_FP64, SCHED_LINE, SCHED_PLANE,
-                                SCHEME_PC_REFERENCE, SCHEME_SI_LEARNED, hip_batch, hip_engine)
+                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, hip_batch, hip_engine)
 from ftspolysol_amd.lrk import load_params
 from ftspolysol_amd.system import config, initial_fields, make_system, replica_params
 
@@ -90,3 +96,33 @@ def test_batched_form_64_replicas():
         assert np.abs(e.get_field(FIELD_OMEGA) - w).max() <= 1e-12 * np.abs(w).max(), (n, r)
         e.close()
     b.close()
+
+
+def test_precision_form_64():
+    print("64^3 N=50 FP32 vs FP64, semi-implicit dt=0.05", flush=True)
+    mk = lambda prec: config(2, precision=prec, scheme=SCHEME_SEMI_IMPLICIT, stress_mask=0x7F, seed=0xC2)
+    s64, s32 = mk(PREC_FP64), mk(PREC_FP32)
+    assert s64.m == (64, 64, 64) and s64.desc.species[0].N == 50 and s64.desc.dt == 0.05
+    g64, g32 = hip_engine(s64), hip_engine(s32)
+    init = initial_fields(s64, sigma=1e-3, cfg=2)
+    for e in (g64, g32):
+        e.set_fields(*init)
+    r64, r32 = g64.step(8, record_every=1), g32.step(8, record_every=1)
+    # the same noise: the trajectories agree step by step at first (record 0 is the
+    # initial state, where H is near 0)
+    for a, b in zip(r32[1:], r64[1:]):
+        assert abs(a["H"] - b["H"]) <= 1e-5 * abs(b["H"]), (a["step"], a["H"], b["H"])
+    # identical fields: the precision error of every observable
+    probe = hip_engine(mk(PREC_FP32))
+    g64.set_restart(20, 10)   # dt = 0.05 sees rare non-finite excursions (restarted from a checkpoint)
+    for k in range(3):
+        g64.step(40)
+        om, ps = g64.get_field(FIELD_OMEGA), g64.get_field(FIELD_PSI)[0]
+        probe.set_fields(om, ps)
+        a, b = probe.measure(), g64.measure()
+        assert abs(a["H"] - b["H"]) <= 1e-4 * abs(b["H"]), (k, a["H"], b["H"])
+        assert abs(a["p_iso"] - b["p_iso"]) <= 1e-4 * abs(b["p_iso"])
+        assert np.abs(a["stress"] - b["stress"]).max() <= 1e-4 * np.abs(b["stress"]).max()
+        assert np.abs(a["mu_ex_poly"] - b["mu_ex_poly"]).max() <= 1e-4 * max(1.0, np.abs(b["mu_ex_poly"]).max())
+    for e in (g64, g32, probe):
+        e.close()
