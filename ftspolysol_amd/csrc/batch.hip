@@ -7,12 +7,14 @@
 // carries the jobs (sweep directions) of all its replicas at that bead index,
 // read from a device-resident job table built once.  Replicas with shorter
 // chains drop out of the later launches.  Groups run concurrently, each from
-// its own host thread on its own stream.  The bead-step launches of a sweep read
-// fixed device tables, so they are captured once per table set as a hipGraph and
-// replayed with one launch.  Everything outside the sweeps (the smoothing FFTs,
-// ions, drifts and the CL update) runs per replica on the group stream; the host
-// waits once per density evaluation for the Q checks of all replicas of a group
-// and once per step for their field checks.
+// its own host thread on its own stream.  Everything outside the sweeps (the
+// smoothing FFTs, ions, drifts and the CL update) is each replica's own work, on
+// side streams forked from the group stream.  A whole CL step of a group (its
+// sweeps, every replica's work and the copies of the Q values and finiteness flags
+// to pinned memory) is captured once as a hipGraph and replayed: every argument of
+// its launches is a fixed device address except the noise key {seed, step}, which
+// the kernels read from device memory that a captured copy refreshes from pinned
+// host memory on every replay.  The host waits once per step.
 #include <memory>
 #include <thread>
 
@@ -52,6 +54,30 @@ struct SweepTables {
     }
 };
 
+// One captured CL step of a group: valid while the participating replicas, the
+// replicas that record, the learned kernel's ping-pong state and every replica's
+// configuration are those of the capture.
+struct StepGraph {
+    std::vector<char> mask, rec;
+    std::vector<int> lrk_par, lrk_prev;
+    std::vector<unsigned long long> epoch;
+    bool timing = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // host state of each replica after the captured step, restored after every replay
+    std::vector<const double2*> psi_hat_of, dens_psi;
+    std::vector<int> last_species, par_after, prev_after;
+    fts_timing dt{}; // timing counters one step adds
+    int nstamp = 0;
+    ~StepGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+constexpr int MAX_STAMPS = 2 * 2 * FTS_MAX_SPECIES; // two density evaluations (PC) x species x {before, after}
+
 struct Group {
     int m1 = 0, m2 = 0, m3 = 0, precision = 0;
     double dL = 0;
@@ -60,7 +86,9 @@ struct Group {
     std::vector<int> ids; // batch index of each replica
     int sched = FTS_SCHED_CONV;
     bool plane_ok = true;
-    bool use_graph = true; // replay captured sweep graphs (fts_batch_set_graph)
+    int graph_level = 2;    // fts_batch_set_graph: 0 eager launches, 1 sweep graphs, 2 whole-step graphs
+    bool capturing = false; // inside a whole-step capture: sweeps launch their kernels into it
+    std::vector<std::unique_ptr<StepGraph>> step_graphs;
     double tune_ms[2] = {0.0, 0.0};
     std::vector<std::unique_ptr<SweepTables>> tables; // full sweeps, per species index and schedule
     double2* hQ = nullptr;                             // pinned [2][reps][FTS_MAX_SPECIES]: Q, Q at (w_bar, psi_bar)
@@ -73,10 +101,14 @@ struct Group {
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> joined; // per side stream
     hipEvent_t forked = nullptr;
-    // timing
+    // timing: device wall-clock stamps before and after each sweep span of a step (HIP
+    // events recorded inside a captured graph cannot be timed), copied to pinned memory
+    // with the step's other results and summed after its sync
     bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-    std::vector<hipEvent_t> pool;
+    unsigned long long* dstamps = nullptr;
+    unsigned long long* hstamps = nullptr;
+    int nstamp = 0;
+    double clock_khz = 0.0;
     fts_timing tstat{};
     int rc = FTS_OK;
     std::string err;
@@ -294,34 +326,31 @@ template <typename T> int run_tables(Group& g, const SweepTables& tb, size_t fir
     return FTS_OK;
 }
 
-hipEvent_t group_event(Group& g)
+// device wall-clock stamp (constant-rate counter of hipDeviceAttributeWallClockRate kHz)
+__global__ void k_stamp(unsigned long long* __restrict__ out)
 {
-    if (!g.pool.empty()) {
-        hipEvent_t e = g.pool.back();
-        g.pool.pop_back();
-        return e;
-    }
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
+    if (threadIdx.x == 0) out[0] = wall_clock64();
 }
 
-int resolve_group_timing(Group& g)
+// sweep time of the stamps written by the step just synchronised
+void resolve_stamps(Group& g)
 {
-    if (g.pending.empty()) return FTS_OK;
-    HIP_TRY(hipEventSynchronize(g.pending.back().second));
-    for (auto& pr : g.pending) {
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, pr.first, pr.second));
-        g.tstat.sweep_ms += ms;
-        g.pool.push_back(pr.first);
-        g.pool.push_back(pr.second);
-    }
-    g.pending.clear();
+    for (int k = 0; k + 1 < g.nstamp; k += 2)
+        g.tstat.sweep_ms += static_cast<double>(g.hstamps[k + 1] - g.hstamps[k]) / g.clock_khz;
+    g.nstamp = 0;
+}
+
+int stamp(Group& g)
+{
+    if (g.nstamp >= MAX_STAMPS) return fail(FTS_DEVICE_ERROR, "batch: too many timed sweep spans in one step");
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, g.stream, g.dstamps + g.nstamp);
+    HIP_TRY(hipGetLastError());
+    ++g.nstamp;
     return FTS_OK;
 }
 
-template <typename T> int group_sweeps(Group& g, int p, const std::vector<char>& mask)
+// the job tables of species p over the replicas in mask (built once per participant set)
+template <typename T> int ensure_tables(Group& g, int p, const std::vector<char>& mask, SweepTables** out)
 {
     SweepTables* tb = nullptr;
     for (auto& t : g.tables)
@@ -337,16 +366,19 @@ template <typename T> int group_sweeps(Group& g, int p, const std::vector<char>&
         int rc = build_tables<T>(g, p, g.sched, -1, mask, *tb);
         if (rc) return rc;
     }
-    if (tb->steps == 0) return FTS_OK;
-    int rc = run_tables<T>(g, *tb, 0, tb->init_launches);
+    *out = tb;
+    return FTS_OK;
+}
+
+template <typename T> int group_sweeps(Group& g, int p, const std::vector<char>& mask)
+{
+    SweepTables* tb = nullptr;
+    int rc = ensure_tables<T>(g, p, mask, &tb);
     if (rc) return rc;
-    hipEvent_t e0 = nullptr;
-    if (g.timing) {
-        e0 = group_event(g);
-        if (!e0) return fail(FTS_DEVICE_ERROR, "hipEventCreate");
-        HIP_TRY(hipEventRecord(e0, g.stream));
-    }
-    if (g.use_graph) {
+    if (tb->steps == 0) return FTS_OK;
+    if ((rc = run_tables<T>(g, *tb, 0, tb->init_launches))) return rc;
+    if (g.timing && (rc = stamp(g))) return rc;
+    if (g.graph_level >= 1 && !g.capturing) {
         if (!tb->exec) {
             HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
             rc = run_tables<T>(g, *tb, tb->init_launches, tb->launches.size());
@@ -365,15 +397,11 @@ template <typename T> int group_sweeps(Group& g, int p, const std::vector<char>&
         return rc;
     }
     if (g.timing) {
-        hipEvent_t e1 = group_event(g);
-        if (!e1) return fail(FTS_DEVICE_ERROR, "hipEventCreate");
-        HIP_TRY(hipEventRecord(e1, g.stream));
-        g.pending.emplace_back(e0, e1);
+        if ((rc = stamp(g))) return rc;
         g.tstat.sweeps += 1;
         g.tstat.bead_steps += tb->jobs;
         g.tstat.step_launches += tb->steps;
         g.tstat.algorithmic_bytes += static_cast<double>(tb->jobs) * 2.0 * sizeof(V2<T>) * static_cast<double>(g.reps[0]->M);
-        if (g.pending.size() > 256 && (rc = resolve_group_timing(g))) return rc;
     }
     return FTS_OK;
 }
@@ -476,19 +504,15 @@ bool group_Q_finite(const Group& g, int k, size_t r)
 }
 
 // One CL step of the replicas in mask; rec[r] non-null where replica r records.
-// A replica whose step fails leaves the step with its reason in why[r] (the
-// caller restarts it or gives up); device errors return a status.
 // The whole step is queued before the host waits once: Q (and Q at the predicted
 // fields) and the finiteness of the updated fields are checked after that sync, in
 // this order, so a replica with a non-finite Q has run the rest of its step on
 // non-finite values, which the caller then discards.  The host therefore queues
 // the update work while the device runs the sweeps.
-template <typename T> int group_step(Group& g, int scheme, const std::vector<fts_observables*>& rec, std::vector<char> mask,
-                                     std::vector<std::string>& why)
+template <typename T> int group_step_queue(Group& g, int scheme, const std::vector<fts_observables*>& rec, const std::vector<char>& mask)
 {
     int rc;
-    const size_t n = g.reps.size();
-    // the engines of a batch defer measure's host part (measure_finish) to after the sync below
+    // the engines of a batch defer measure's host part (measure_finish) to after the sync
     auto update = [&](size_t r) {
         fts_engine* e = g.reps[r];
         int c;
@@ -501,8 +525,7 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
                                    hipMemcpyDeviceToHost, e->stream));
         return e->finite_queue(g.hflag + r);
     };
-    const bool pc = scheme == FTS_SCHEME_PC_REFERENCE;
-    if (pc) {
+    if (scheme == FTS_SCHEME_PC_REFERENCE) {
         if ((rc = group_densities<T>(g, false, mask)) || (rc = group_queue_Q(g, 0, mask))) return rc;
         if ((rc = group_each(g, mask, [&](size_t r) { return g.reps[r]->pc_predict(); }))) return rc;
         if ((rc = group_densities<T>(g, true, mask)) || (rc = group_queue_Q(g, 1, mask))) return rc;
@@ -515,8 +538,20 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
         });
         if (rc) return rc;
     }
-    HIP_TRY(hipStreamSynchronize(g.stream));
-    for (size_t r = 0; r < n; ++r) {
+    if (g.timing && g.nstamp > 0)
+        HIP_TRY(hipMemcpyAsync(g.hstamps, g.dstamps, sizeof(unsigned long long) * static_cast<size_t>(g.nstamp), hipMemcpyDeviceToHost, g.stream));
+    return FTS_OK;
+}
+
+// After the step's sync: a replica whose step fails leaves it with its reason in why[r]
+// (the caller restarts it or gives up); records finish on the host.
+int group_step_check(Group& g, int scheme, const std::vector<fts_observables*>& rec, const std::vector<char>& mask,
+                     std::vector<std::string>& why)
+{
+    int rc;
+    const bool pc = scheme == FTS_SCHEME_PC_REFERENCE;
+    if (g.timing) resolve_stamps(g);
+    for (size_t r = 0; r < g.reps.size(); ++r) {
         if (!mask[r]) continue;
         const std::string at = " at step " + std::to_string(g.reps[r]->step_counter);
         const bool q_ok = group_Q_finite(g, 0, r) && (!pc || group_Q_finite(g, 1, r));
@@ -527,6 +562,127 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
         else if (rec[r] && (rc = g.reps[r]->measure_finish(rec[r]))) return rc;
     }
     return FTS_OK;
+}
+
+// Whether this step may run as a captured graph: level 2, and no replica with the
+// per-step host work a replay would skip (drift statistics, the NaN test hook)
+bool step_graph_ok(const Group& g, const std::vector<char>& mask)
+{
+    if (g.graph_level < 2) return false;
+    for (size_t r = 0; r < g.reps.size(); ++r)
+        if (mask[r] && (g.reps[r]->drift_stats || g.reps[r]->inject_step >= 0)) return false;
+    return true;
+}
+
+StepGraph* find_step_graph(Group& g, const std::vector<char>& mask, const std::vector<fts_observables*>& rec)
+{
+    for (auto& sg : g.step_graphs) {
+        if (sg->mask != mask || sg->timing != g.timing) continue;
+        bool same = true;
+        for (size_t r = 0; r < g.reps.size() && same; ++r) {
+            const fts_engine* e = g.reps[r];
+            same = sg->rec[r] == (rec[r] != nullptr ? 1 : 0) && sg->epoch[r] == e->config_epoch &&
+                   (!mask[r] || (sg->lrk_par[r] == e->lrk_par && sg->lrk_prev[r] == (e->lrk_have_prev ? 1 : 0)));
+        }
+        if (same) return sg.get();
+    }
+    return nullptr;
+}
+
+// One CL step of the group as a replayed graph (captured at its first use), or eagerly
+template <typename T> int group_step(Group& g, int scheme, const std::vector<fts_observables*>& rec, std::vector<char> mask,
+                                     std::vector<std::string>& why)
+{
+    int rc;
+    g.nstamp = 0;
+    if (!step_graph_ok(g, mask)) {
+        if ((rc = group_step_queue<T>(g, scheme, rec, mask))) return rc;
+        HIP_TRY(hipStreamSynchronize(g.stream));
+        return group_step_check(g, scheme, rec, mask, why);
+    }
+    const size_t n = g.reps.size();
+    StepGraph* sg = find_step_graph(g, mask, rec);
+    if (!sg) {
+        // the sweeps' job tables exist before the capture (building them uploads and waits)
+        size_t max_species = 0;
+        for (size_t r = 0; r < n; ++r)
+            if (mask[r]) max_species = std::max(max_species, g.reps[r]->species.size());
+        for (size_t p = 0; p < max_species; ++p) {
+            SweepTables* tb = nullptr;
+            if ((rc = ensure_tables<T>(g, static_cast<int>(p), mask, &tb))) return rc;
+        }
+        if (g.step_graphs.size() >= 16) {
+            HIP_TRY(hipStreamSynchronize(g.stream));
+            g.step_graphs.clear();
+        }
+        auto fresh = std::make_unique<StepGraph>();
+        fresh->mask = mask;
+        fresh->timing = g.timing;
+        for (size_t r = 0; r < n; ++r) {
+            const fts_engine* e = g.reps[r];
+            fresh->rec.push_back(rec[r] != nullptr ? 1 : 0);
+            fresh->epoch.push_back(e->config_epoch);
+            fresh->lrk_par.push_back(e->lrk_par);
+            fresh->lrk_prev.push_back(e->lrk_have_prev ? 1 : 0);
+        }
+        const fts_timing before = g.tstat;
+        HIP_TRY(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+        g.capturing = true;
+        rc = group_step_queue<T>(g, scheme, rec, mask);
+        g.capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(g.stream, &graph);
+        fresh->graph = graph;
+        if (rc) return rc;
+        HIP_TRY(ec);
+        HIP_TRY(hipGraphInstantiate(&fresh->exec, fresh->graph, nullptr, nullptr, 0));
+        // the capture ran the step's host code: the replicas' host state is that after the step
+        for (size_t r = 0; r < n; ++r) {
+            const fts_engine* e = g.reps[r];
+            fresh->psi_hat_of.push_back(e->psi_hat_of);
+            fresh->dens_psi.push_back(e->dens_psi);
+            fresh->last_species.push_back(e->last_species);
+            fresh->par_after.push_back(e->lrk_par);
+            fresh->prev_after.push_back(e->lrk_have_prev ? 1 : 0);
+        }
+        fresh->dt.sweeps = g.tstat.sweeps - before.sweeps;
+        fresh->dt.bead_steps = g.tstat.bead_steps - before.bead_steps;
+        fresh->dt.step_launches = g.tstat.step_launches - before.step_launches;
+        fresh->dt.algorithmic_bytes = g.tstat.algorithmic_bytes - before.algorithmic_bytes;
+        fresh->nstamp = g.nstamp;
+        g.step_graphs.push_back(std::move(fresh));
+        HIP_TRY(hipGraphLaunch(g.step_graphs.back()->exec, g.stream));
+    } else {
+        // replay: the step's noise keys and record headers, then the host state the
+        // captured step leaves behind
+        for (size_t r = 0; r < n; ++r) {
+            if (!mask[r]) continue;
+            fts_engine* e = g.reps[r];
+            e->hkey[0] = e->seed;
+            e->hkey[1] = static_cast<unsigned long long>(e->step_counter);
+            if (rec[r]) e->measure_header(rec[r]);
+        }
+        HIP_TRY(hipGraphLaunch(sg->exec, g.stream));
+        for (size_t r = 0; r < n; ++r) {
+            if (!mask[r]) continue;
+            fts_engine* e = g.reps[r];
+            ++e->step_counter;
+            e->psi_hat_of = sg->psi_hat_of[r];
+            e->dens_psi = sg->dens_psi[r];
+            e->last_species = sg->last_species[r];
+            e->lrk_par = sg->par_after[r];
+            e->lrk_have_prev = sg->prev_after[r] != 0;
+        }
+        if (g.timing) {
+            g.tstat.sweeps += sg->dt.sweeps;
+            g.tstat.bead_steps += sg->dt.bead_steps;
+            g.tstat.step_launches += sg->dt.step_launches;
+            g.tstat.algorithmic_bytes += sg->dt.algorithmic_bytes;
+        }
+        g.nstamp = sg->nstamp;
+    }
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return group_step_check(g, scheme, rec, mask, why);
 }
 
 // Schedule of a group's sweeps: conv and line timed over 8 bead steps of species 0
@@ -598,17 +754,14 @@ struct fts_batch {
     int scheme = 0;
     ~fts_batch()
     {
-        for (auto& g : groups) {
+        for (auto& g : groups)
             if (g->stream) (void)hipStreamSynchronize(g->stream);
-            for (auto& pr : g->pending) {
-                (void)hipEventDestroy(pr.first);
-                (void)hipEventDestroy(pr.second);
-            }
-            for (hipEvent_t e : g->pool) (void)hipEventDestroy(e);
-        }
         for (fts_engine* e : reps) delete e; // engines do not own the group streams
         for (auto& g : groups) {
+            g->step_graphs.clear();
             g->tables.clear();
+            if (g->dstamps) (void)hipFree(g->dstamps);
+            if (g->hstamps) (void)hipHostFree(g->hstamps);
             if (g->hQ) (void)hipHostFree(g->hQ);
             if (g->hdrift) (void)hipHostFree(g->hdrift);
             if (g->hflag) (void)hipHostFree(g->hflag);
@@ -659,6 +812,15 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hQ), sizeof(double2) * FTS_MAX_SPECIES * 2 * g.reps.size()));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hflag), sizeof(int) * g.reps.size()));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hdrift), sizeof(double2) * (FTS_MAX_TYPES + 1) * g.reps.size()));
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&g.dstamps), sizeof(unsigned long long) * MAX_STAMPS));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hstamps), sizeof(unsigned long long) * MAX_STAMPS));
+        {
+            int dev = 0, khz = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+            if (khz <= 0) return fail(FTS_DEVICE_ERROR, "batch: no device wall-clock rate");
+            g.clock_khz = static_cast<double>(khz);
+        }
         g.plane_ok = true;
         for (fts_engine* e : g.reps) g.plane_ok = g.plane_ok && e->plane_ok;
         const int want = systems[g.ids[0]].fft_schedule;
@@ -793,14 +955,19 @@ int fts_batch_set_streams(fts_batch* b, int n)
     for (auto& g : b->groups) {
         int rc = set_side_streams(*g, std::min<int>(n, static_cast<int>(g->reps.size())));
         if (rc) return rc;
+        g->step_graphs.clear(); // captured with the previous fork structure
     }
     return FTS_OK;
 }
 
-int fts_batch_set_graph(fts_batch* b, int enabled)
+int fts_batch_set_graph(fts_batch* b, int level)
 {
-    if (!b) return fail(FTS_INVALID, "null batch");
-    for (auto& g : b->groups) g->use_graph = enabled != 0;
+    if (!b || level < 0 || level > 2) return fail(FTS_INVALID, "batch: graph level must be 0, 1 or 2");
+    for (auto& g : b->groups) {
+        if (g->stream) HIP_TRY(hipStreamSynchronize(g->stream));
+        g->graph_level = level;
+        g->step_graphs.clear();
+    }
     return FTS_OK;
 }
 
@@ -808,8 +975,6 @@ int fts_batch_set_timing(fts_batch* b, int enabled)
 {
     if (!b) return fail(FTS_INVALID, "null batch");
     for (auto& g : b->groups) {
-        int rc = resolve_group_timing(*g);
-        if (rc) return rc;
         g->timing = enabled != 0;
         g->tstat = fts_timing{};
     }
@@ -819,10 +984,7 @@ int fts_batch_set_timing(fts_batch* b, int enabled)
 int fts_batch_get_timing(fts_batch* b, int group, fts_timing* out)
 {
     if (!b || !out || group < 0 || group >= static_cast<int>(b->groups.size())) return fail(FTS_INVALID, "batch: bad argument");
-    Group& g = *b->groups[static_cast<size_t>(group)];
-    int rc = resolve_group_timing(g);
-    if (rc) return rc;
-    *out = g.tstat;
+    *out = b->groups[static_cast<size_t>(group)]->tstat;
     return FTS_OK;
 }
 

@@ -253,6 +253,7 @@ struct fts_engine {
     int measure(const double2* om, const double2* ps, fts_observables* rec);
     int measure_queue(const double2* om, const double2* ps, fts_observables* rec);
     int measure_finish(fts_observables* rec);
+    void measure_header(fts_observables* rec) const;
     bool defer_measure = false;
     template <typename T> int stress_chain(int p);
     int contract_to(const void* A, const void* B, bool fp32, const double* sep, double2* dst);
@@ -261,6 +262,18 @@ struct fts_engine {
     DevBuf<double2> stress_acc;
     size_t stress_chain_base(int p) const; // first stress_acc row of species p's chain
     void* pin = nullptr;                   // pinned staging
+    // the noise key {seed, step} of the step being taken: kernels that draw CL noise read
+    // it from nkey, which key_queue refreshes from the pinned hkey on the stream (a copy a
+    // captured step graph repeats on every replay, reading hkey as the host left it)
+    DevBuf<unsigned long long> nkey;
+    unsigned long long* hkey = nullptr;
+    int key_queue();
+    // fill kernels for the clears of the step (no memset nodes in captured graphs)
+    int zero_async(double2* p, size_t n);
+    // bumped by every call that changes values a captured step graph holds (the learned
+    // kernel's threshold and ring, drift statistics, the test hook): the batch drops its
+    // step graphs of this replica when it changes
+    unsigned long long config_epoch = 0;
     size_t pin_bytes = 0;
     int step_pc(bool record, fts_observables* rec);
     int step_em(bool record, fts_observables* rec, bool si);

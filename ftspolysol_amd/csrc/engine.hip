@@ -45,6 +45,9 @@ void fts_engine::release()
     stress_acc.release();
     if (pin) (void)hipHostFree(pin);
     pin = nullptr;
+    nkey.release();
+    if (hkey) (void)hipHostFree(hkey);
+    hkey = nullptr;
     for (HostSpecies& sp : species) {
         sp.d_cls_type.release();
         sp.d_cls_start.release();
@@ -373,6 +376,8 @@ int fts_engine::setup_device()
     HIP_TRY(stress_acc.alloc((stress_chain_base(static_cast<int>(species.size())) + nTs + 1 + nI) * 7));
     pin_bytes = sizeof(double2) * (SC_COUNT + stress_acc.n) + sizeof(double) * species.size() * (static_cast<size_t>(Nmax) + MAX_CLASSES);
     HIP_TRY(hipHostMalloc(&pin, pin_bytes));
+    HIP_TRY(nkey.alloc(2));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&hkey), 2 * sizeof(unsigned long long)));
     HIP_TRY(flag.alloc(1));
     HIP_TRY(mode_coef.alloc(std::max<size_t>(1, nO * nTs)));
     mode_coef_host.assign(std::max<size_t>(1, nO * nTs), double2{0.0, 0.0});
@@ -1210,10 +1215,9 @@ int fts_engine::densities_begin(const double2* om, const double2* ps)
             return rc;
         }
     } else {
-        HIP_TRY(hipMemsetAsync(psi_s.p, 0, sizeof(double2) * Ms, stream));
+        if ((rc = zero_async(psi_s.p, Ms))) return rc;
     }
-    HIP_TRY(hipMemsetAsync(rho_hat.p, 0, sizeof(double2) * rho_hat.n, stream));
-    HIP_TRY(hipMemsetAsync(rho_c_hat.p, 0, sizeof(double2) * Ms, stream));
+    if ((rc = zero_async(rho_hat.p, rho_hat.n)) || (rc = zero_async(rho_c_hat.p, Ms))) return rc;
     return FTS_OK;
 }
 
@@ -1235,10 +1239,9 @@ int fts_engine::densities_end()
     if (electro()) {
         HIP_TRY(hipMemcpyAsync(rho_c.p, rho_c_poly.p, sizeof(double2) * Ms, hipMemcpyDeviceToDevice, stream));
     } else {
-        HIP_TRY(hipMemsetAsync(rho_c_poly.p, 0, sizeof(double2) * Ms, stream));
-        HIP_TRY(hipMemsetAsync(rho_c.p, 0, sizeof(double2) * Ms, stream));
+        if ((rc = zero_async(rho_c_poly.p, Ms)) || (rc = zero_async(rho_c.p, Ms))) return rc;
     }
-    HIP_TRY(hipMemsetAsync(rho_c_ion.p, 0, sizeof(double2) * Ms, stream));
+    if ((rc = zero_async(rho_c_ion.p, Ms))) return rc;
     // ion::compute_densities (ion.cu:54-72)
     const size_t nI = ions.size();
     auto sep_of = [&](size_t j) { return sep_ion.p + j * static_cast<size_t>(m1 + m2 + m3); };
@@ -1280,16 +1283,29 @@ int fts_engine::densities_end()
 // u_psi = S_{k^2} psi / (4 pi lB) + i rho, minus its mean (complexLangevin.cu:299-311)
 int fts_engine::psi_drift(const double2* ps, const double2* rho_src, double2* out)
 {
-    if (!electro()) {
-        HIP_TRY(hipMemsetAsync(out, 0, sizeof(double2) * M, stream));
-        return FTS_OK;
-    }
+    if (!electro()) return zero_async(out, static_cast<size_t>(M));
     const KOp lap{nullptr, nullptr, 0.0, 1.0 / static_cast<double>(M), 0.0, 0, 0};
     int rc = psi_hat_valid(ps) ? fft_apply_hat(psi_hat.p, t2.p, lap) : fft_apply(ps, t2.p, lap);
     if (rc) return rc;
     hipLaunchKernelGGL(k_psi_drift, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, t2.p, rho_src, 1.0 / (4.0 * Pi * lB), M, out);
     HIP_TRY(hipGetLastError());
     return remove_mean(out);
+}
+
+int fts_engine::key_queue()
+{
+    hkey[0] = seed;
+    hkey[1] = static_cast<unsigned long long>(step_counter);
+    HIP_TRY(hipMemcpyAsync(nkey.p, hkey, 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, stream));
+    return FTS_OK;
+}
+
+int fts_engine::zero_async(double2* p, size_t n)
+{
+    if (n == 0) return FTS_OK;
+    hipLaunchKernelGGL(k_fill, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream, p, double2{0.0, 0.0}, static_cast<int>(n));
+    HIP_TRY(hipGetLastError());
+    return FTS_OK;
 }
 
 int fts_engine::finite_queue(int* host_out)
@@ -1392,13 +1408,19 @@ int fts_engine::measure(const double2* om, const double2* ps, fts_observables* r
     return measure_finish(rec);
 }
 
-int fts_engine::measure_queue(const double2* om, const double2* ps, fts_observables* rec)
+// the host fields of a record, set when its device work is queued
+void fts_engine::measure_header(fts_observables* rec) const
 {
     std::memset(rec, 0, sizeof(*rec));
     rec->step = step_counter;
     rec->n_species = static_cast<int>(species.size());
     rec->n_ions = static_cast<int>(ions.size());
     rec->stress_mask = stress_mask;
+}
+
+int fts_engine::measure_queue(const double2* om, const double2* ps, fts_observables* rec)
+{
+    measure_header(rec);
     const double invM = 1.0 / static_cast<double>(M);
     const size_t nsp = species.size();
     const bool want_stress = (stress_mask & 0x7F) != 0;
@@ -1641,28 +1663,28 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
 int fts_engine::pc_predict()
 {
     if (psi_hat_of == psi_bar.p) psi_hat_of = nullptr; // psi_bar is rewritten below
-    const unsigned long long t = static_cast<unsigned long long>(step_counter);
     const size_t Ms = static_cast<size_t>(M);
     int rc;
+    if ((rc = key_queue())) return rc; // the step's noise key (the corrector draws the same eta)
     const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
     if (electro()) {
-        hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, seed, t, 0x100u, sigma_psi, M, partials.p);
+        hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, nkey.p, 0x100u, sigma_psi, M, partials.p);
         HIP_TRY(hipGetLastError());
         if ((rc = reduce_finalize(1, SC_ETA_MEAN, 1.0 / static_cast<double>(M)))) return rc;
     }
     if (n_omega > 0) {
-        hipLaunchKernelGGL(k_omega_predict, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, omega.p, rho_type.p, seed, t, M, drift.p, omega_bar.p);
+        hipLaunchKernelGGL(k_omega_predict, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, omega.p, rho_type.p, nkey.p, M, drift.p, omega_bar.p);
         HIP_TRY(hipGetLastError());
     }
     // predictor psi drift uses rho_c of the current fields (complexLangevin.cu:331, 674)
     if ((rc = psi_drift(psi.p, rho_c.p, dpsi.p))) return rc;
     if (electro()) {
         hipLaunchKernelGGL(k_psi_update, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, -dt * lambda_psi,
-                           static_cast<const double2*>(nullptr), 0.0, sigma_psi, scal.p + SC_ETA_MEAN, seed, t, M, psi_bar.p);
+                           static_cast<const double2*>(nullptr), 0.0, sigma_psi, scal.p + SC_ETA_MEAN, nkey.p, M, psi_bar.p);
         HIP_TRY(hipGetLastError());
         if ((rc = remove_mean(psi_bar.p))) return rc;
-    } else {
-        HIP_TRY(hipMemsetAsync(psi_bar.p, 0, sizeof(double2) * Ms, stream));
+    } else if ((rc = zero_async(psi_bar.p, Ms))) {
+        return rc;
     }
     return FTS_OK;
 }
@@ -1671,7 +1693,6 @@ int fts_engine::pc_predict()
 // the record is taken at the predictor state (complexLangevin.cu:680-684)
 int fts_engine::pc_correct(bool record, fts_observables* rec)
 {
-    const unsigned long long t = static_cast<unsigned long long>(step_counter);
     const size_t Ms = static_cast<size_t>(M);
     int rc;
     const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
@@ -1679,17 +1700,17 @@ int fts_engine::pc_correct(bool record, fts_observables* rec)
     if (drift_stats && (rc = drift_queue(true))) return rc;
     if (record && (rc = measure(omega_bar.p, psi_bar.p, rec))) return rc;
     if (n_omega > 0) {
-        hipLaunchKernelGGL(k_omega_correct, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, drift.p, omega_bar.p, rho_type.p, seed, t, M, omega.p);
+        hipLaunchKernelGGL(k_omega_correct, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, drift.p, omega_bar.p, rho_type.p, nkey.p, M, omega.p);
         HIP_TRY(hipGetLastError());
     }
     psi_hat_of = nullptr; // psi changes below
     if (electro()) {
         hipLaunchKernelGGL(k_psi_update, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, -0.5 * dt * lambda_psi,
-                           static_cast<const double2*>(dpsi_bar.p), -0.5 * dt * lambda_psi, sigma_psi, scal.p + SC_ETA_MEAN, seed, t, M, psi.p);
+                           static_cast<const double2*>(dpsi_bar.p), -0.5 * dt * lambda_psi, sigma_psi, scal.p + SC_ETA_MEAN, nkey.p, M, psi.p);
         HIP_TRY(hipGetLastError());
         if ((rc = remove_mean(psi.p))) return rc;
-    } else {
-        HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));
+    } else if ((rc = zero_async(psi.p, Ms))) {
+        return rc;
     }
     return FTS_OK;
 }
@@ -1734,15 +1755,15 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool si)
 
 int fts_engine::em_update(bool record, fts_observables* rec, bool si)
 {
-    const unsigned long long t = static_cast<unsigned long long>(step_counter);
     const size_t Ms = static_cast<size_t>(M);
     const double invM = 1.0 / static_cast<double>(M);
     int rc;
     if (record && (rc = measure(omega.p, psi.p, rec))) return rc;
+    if ((rc = key_queue())) return rc; // the step's noise key
     const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
     // EM adds real-space noise (psi's mean-free); the semi-implicit step adds it in k-space
     if (electro() && !si) {
-        hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, seed, t, 0x100u, sigma_psi, M, partials.p);
+        hipLaunchKernelGGL(k_sum_noise, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, nkey.p, 0x100u, sigma_psi, M, partials.p);
         HIP_TRY(hipGetLastError());
         if ((rc = reduce_finalize(1, SC_ETA_MEAN, invM))) return rc;
     }
@@ -1750,13 +1771,13 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
     if (drift_stats && (rc = drift_queue(false))) return rc;
     if (si && lrk_active() && (rc = lrk_step())) return rc;
     if (n_omega > 0) {
-        hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, seed, t, M, si ? 0 : 1, omega.p, drift.p,
+        hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, nkey.p, M, si ? 0 : 1, omega.p, drift.p,
                            si ? 0 : 1);
         HIP_TRY(hipGetLastError());
     }
     if (electro()) {
         hipLaunchKernelGGL(k_psi_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, dpsi.p, -dt * lambda_psi, sigma_psi,
-                           scal.p + SC_ETA_MEAN, seed, t, M, dpsi.p, si ? 0 : 1);
+                           scal.p + SC_ETA_MEAN, nkey.p, M, dpsi.p, si ? 0 : 1);
         HIP_TRY(hipGetLastError());
     }
     if (si) {
@@ -1768,16 +1789,14 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
         int nj = 0;
         for (int l = 0; l < n_omega; ++l) {
             KOp op{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0};
-            op.nseed = seed;
-            op.nstep = t;
+            op.nkey = nkey.p;
             op.nfield = SI_NOISE_FIELD + 2u * static_cast<unsigned>(l);
             op.namp = mc.sigma[l] * rM;
             jb[nj++] = FftJob{f(drift, l), f(drift, l), op};
         }
         if (electro()) {
             KOp op{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0};
-            op.nseed = seed;
-            op.nstep = t;
+            op.nkey = nkey.p;
             op.nfield = SI_NOISE_FIELD_PSI;
             op.namp = sigma_psi * rM;
             op.nzero = 1;
@@ -1795,8 +1814,8 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
         hipLaunchKernelGGL(k_axpy1, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, Ms);
         HIP_TRY(hipGetLastError());
         if ((rc = remove_mean(psi.p))) return rc;
-    } else {
-        HIP_TRY(hipMemsetAsync(psi.p, 0, sizeof(double2) * Ms, stream));
+    } else if ((rc = zero_async(psi.p, Ms))) {
+        return rc;
     }
     return FTS_OK;
 }
@@ -2096,6 +2115,7 @@ int fts_get_info(fts_engine* h, fts_info* o)
 int fts_set_fields(fts_engine* h, const double* om, const double* ps)
 {
     if (!h) return fail(FTS_INVALID, "null engine");
+    ++h->config_epoch;
     const size_t Ms = static_cast<size_t>(h->M);
     const size_t no = static_cast<size_t>(h->n_omega) * Ms;
     if (om) HIP_TRY(hipMemcpyAsync(h->omega.p, om, sizeof(double2) * no, hipMemcpyHostToDevice, h->stream));
@@ -2142,6 +2162,7 @@ int fts_save_state(fts_engine* h, const char* path)
 int fts_load_state(fts_engine* h, const char* path)
 {
     if (!h || !path) return fail(FTS_INVALID, "null argument");
+    ++h->config_epoch;
     FILE* fp = std::fopen(path, "rb");
     if (!fp) return fail(FTS_INVALID, std::string("cannot open ") + path);
     char magic[8];
@@ -2307,6 +2328,7 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
 int fts_set_drift_stats(fts_engine* h, int enabled)
 {
     if (!h) return fail(FTS_INVALID, "null engine");
+    ++h->config_epoch;
     h->drift_stats = enabled != 0;
     h->drift_rows.clear();
     return FTS_OK;
@@ -2326,6 +2348,7 @@ int fts_get_drift_rows(fts_engine* h, double* out, int capacity, int* n_rows)
 int fts_inject_nonfinite(fts_engine* h, long long step)
 {
     if (!h) return fail(FTS_INVALID, "null engine");
+    ++h->config_epoch;
     h->inject_step = step;
     return FTS_OK;
 }
@@ -2343,6 +2366,7 @@ int fts_set_restart(fts_engine* h, int checkpoint_every, int max_restarts)
 int fts_set_step(fts_engine* h, long long step)
 {
     if (!h || step < 0) return fail(FTS_INVALID, "bad step");
+    ++h->config_epoch;
     h->step_counter = step;
     h->ckpt_step = -1;
     return FTS_OK;
@@ -2477,6 +2501,7 @@ extern "C" {
 int fts_lrk_set_network(fts_engine* h, const float* params, double threshold)
 {
     if (!h || !params) return fail(FTS_INVALID, "lrk_set_network: null argument");
+    ++h->config_epoch;
     if (std::isnan(threshold)) return fail(FTS_INVALID, "lrk_set_network: threshold is NaN");
     for (int j = 0; j < FTS_LRK_NPARAMS; ++j)
         if (!std::isfinite(params[j])) return fail(FTS_INVALID, "lrk_set_network: non-finite parameter");
@@ -2490,6 +2515,7 @@ int fts_lrk_set_network(fts_engine* h, const float* params, double threshold)
 int fts_lrk_set_recording(fts_engine* h, int capacity)
 {
     if (!h || capacity < 0) return fail(FTS_INVALID, "lrk_set_recording: bad argument");
+    ++h->config_epoch;
     if (capacity > 0 && h->scheme != FTS_SCHEME_SEMI_IMPLICIT && h->scheme != FTS_SCHEME_SI_LEARNED)
         return fail(FTS_INVALID, "training records need a semi-implicit scheme");
     int rc;

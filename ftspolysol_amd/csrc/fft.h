@@ -32,8 +32,10 @@ struct KOp {
     int table_noise; // multiply by sqrt(t (1 + t) / 2) instead of the table value t
     // k-space noise of a semi-implicit step (namp != 0): the multiply pass adds
     // namp sqrt(t (1 + t) / 2) g(k), g the spectrum of real white noise of unit
-    // variance per site divided by sqrt(M) (kop_noise); nzero drops k = 0
-    unsigned long long nseed, nstep;
+    // variance per site divided by sqrt(M) (kop_noise); nzero drops k = 0.  nkey: the
+    // device noise key {seed, step} of the step (read when the pass runs, so a
+    // replayed graph draws the noise of the step it replays)
+    const unsigned long long* nkey;
     unsigned nfield;
     double namp;
     int nzero;
@@ -119,8 +121,7 @@ __device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
     k.ka = f ? a.ka : b.ka;
     k.kb = f ? a.kb : b.kb;
     k.table_noise = f ? a.table_noise : b.table_noise;
-    k.nseed = f ? a.nseed : b.nseed;
-    k.nstep = f ? a.nstep : b.nstep;
+    k.nkey = f ? a.nkey : b.nkey;
     k.nfield = f ? a.nfield : b.nfield;
     k.namp = f ? a.namp : b.namp;
     k.nzero = f ? a.nzero : b.nzero;
@@ -129,7 +130,7 @@ __device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
 
 // g(k) at grid index gi = (x m2 + y) m3 + z: Hermitian, so that its inverse transform is
 // real.  The pair {k, -k} takes the two Box-Muller normals a, b of one Philox draw
-// (nseed, nstep, nfield, site = the pair's smaller index): g = (a +- i b) / sqrt 2, + at
+// (seed and step from nkey, nfield, site = the pair's smaller index): g = (a +- i b) / sqrt 2, + at
 // the smaller index; a self-conjugate k (every component 0 or m/2) takes a alone: g = a.
 // This is the distribution of FFT[eta] / sqrt(M) for eta real white noise of unit variance.
 // F32 (FP32 transforms): the normals' transcendentals in FP32.
@@ -144,7 +145,7 @@ __device__ __forceinline__ double2 kop_noise(const KOp& op, int m1, int m2, int 
     if (op.nzero && gi == 0) return r;
     const double t = op.table[gi];
     const double amp = op.namp * sqrt(0.5 * t * (1.0 + t));
-    const double2 ab = cl_normal_pair<F32>(op.nseed, op.nstep, op.nfield, static_cast<uint32_t>(gi < pi ? gi : pi));
+    const double2 ab = cl_normal_pair<F32>(op.nkey[0], op.nkey[1], op.nfield, static_cast<uint32_t>(gi < pi ? gi : pi));
     if (pi == gi) {
         r.x = amp * ab.x;
         return r;

@@ -92,10 +92,12 @@ __global__ void __launch_bounds__(RED_THREADS) k_sum_product(const double2* __re
     if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
 }
 
-// sum of the real CL noise sigma * n(seed, step, field, i)
-__global__ void __launch_bounds__(RED_THREADS) k_sum_noise(unsigned long long seed, unsigned long long step, unsigned field, double sigma, int M,
+// sum of the real CL noise sigma * n(seed, step, field, i); key = {seed, step} on the device
+// (the noise key of the step, refreshed from the host before every step)
+__global__ void __launch_bounds__(RED_THREADS) k_sum_noise(const unsigned long long* __restrict__ key, unsigned field, double sigma, int M,
                                                            double2* __restrict__ partials)
 {
+    const unsigned long long seed = key[0], step = key[1];
     double2 v[1] = {d2(0.0, 0.0)};
     for (int i = blockIdx.x * RED_THREADS + threadIdx.x; i < M; i += RED_BLOCKS * RED_THREADS)
         v[0].x += sigma * cl_normal(seed, step, field, static_cast<uint32_t>(i));
@@ -349,11 +351,12 @@ __device__ __forceinline__ double2 mode_drift(const ModeCoefs& mc, int l, const 
 }
 
 // predictor (complexLangevin.cu:272-285, 328-340): u = drift(omega, rho); omega_bar = omega - dt lam u + eta
-__global__ void k_omega_predict(ModeCoefs mc, const double2* __restrict__ om, const double2* __restrict__ rho, unsigned long long seed,
-                                unsigned long long step, int M, double2* __restrict__ drift, double2* __restrict__ om_bar)
+__global__ void k_omega_predict(ModeCoefs mc, const double2* __restrict__ om, const double2* __restrict__ rho,
+                                const unsigned long long* __restrict__ key, int M, double2* __restrict__ drift, double2* __restrict__ om_bar)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
+    const unsigned long long seed = key[0], step = key[1];
     for (int l = 0; l < mc.nO; ++l) {
         const size_t o = static_cast<size_t>(l) * M + i;
         const double2 u = mode_drift(mc, l, om, rho, M, i);
@@ -366,11 +369,12 @@ __global__ void k_omega_predict(ModeCoefs mc, const double2* __restrict__ om, co
 
 // corrector (complexLangevin.cu:349-360): omega += -dt lam/2 (u + u_bar) + eta (the same eta)
 __global__ void k_omega_correct(ModeCoefs mc, const double2* __restrict__ drift, const double2* __restrict__ om_bar,
-                                const double2* __restrict__ rho_bar, unsigned long long seed, unsigned long long step, int M,
+                                const double2* __restrict__ rho_bar, const unsigned long long* __restrict__ key, int M,
                                 double2* __restrict__ om)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
+    const unsigned long long seed = key[0], step = key[1];
     for (int l = 0; l < mc.nO; ++l) {
         const size_t o = static_cast<size_t>(l) * M + i;
         const double2 ub = mode_drift(mc, l, om_bar, rho_bar, M, i);
@@ -383,11 +387,12 @@ __global__ void k_omega_correct(ModeCoefs mc, const double2* __restrict__ drift,
 
 // EM / semi-implicit increment: r = -dt lam u(omega, rho) + eta.  EM adds it to omega
 // directly (add = 1); the semi-implicit scheme filters r in k-space first (add = 0).
-__global__ void k_omega_increment(ModeCoefs mc, const double2* __restrict__ rho, unsigned long long seed, unsigned long long step, int M,
+__global__ void k_omega_increment(ModeCoefs mc, const double2* __restrict__ rho, const unsigned long long* __restrict__ key, int M,
                                   int add, double2* __restrict__ om, double2* __restrict__ r_out, int with_noise)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
+    const unsigned long long seed = key[0], step = key[1];
     for (int l = 0; l < mc.nO; ++l) {
         const size_t o = static_cast<size_t>(l) * M + i;
         const double2 u = mode_drift(mc, l, om, rho, M, i);
@@ -467,11 +472,12 @@ __global__ void k_mode0_drift(ModeCoefs mc, const double2* __restrict__ om, cons
 
 // psi update: out = base + c1 d1 + c2 d2 + (sigma n_i - eta_mean) on the real part
 __global__ void k_psi_update(const double2* __restrict__ base, const double2* __restrict__ d1, double c1, const double2* __restrict__ dd2,
-                             double c2, double sigma, const double2* __restrict__ eta_mean, unsigned long long seed, unsigned long long step,
+                             double c2, double sigma, const double2* __restrict__ eta_mean, const unsigned long long* __restrict__ key,
                              int M, double2* __restrict__ out)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
+    const unsigned long long seed = key[0], step = key[1];
     double2 v = cadd(base[i], cscale(d1[i], c1));
     if (dd2) v = cadd(v, cscale(dd2[i], c2));
     const double eta = sigma * cl_normal(seed, step, 0x100u, static_cast<uint32_t>(i)) + (-eta_mean[0].x);
@@ -481,10 +487,11 @@ __global__ void k_psi_update(const double2* __restrict__ base, const double2* __
 
 // psi increment for EM / SI: r = -dt lamPsi dpsi + eta
 __global__ void k_psi_increment(const double2* __restrict__ dpsi, double c1, double sigma, const double2* __restrict__ eta_mean,
-                                unsigned long long seed, unsigned long long step, int M, double2* __restrict__ r, int with_noise)
+                                const unsigned long long* __restrict__ key, int M, double2* __restrict__ r, int with_noise)
 {
     const int i = blockIdx.x * EW_THREADS + threadIdx.x;
     if (i >= M) return;
+    const unsigned long long seed = key[0], step = key[1];
     double2 v = cscale(dpsi[i], c1);
     // without noise: the semi-implicit step adds its (mean-free) noise in k-space
     if (with_noise) v.x += sigma * cl_normal(seed, step, 0x100u, static_cast<uint32_t>(i)) + (-eta_mean[0].x);

@@ -338,9 +338,18 @@ int fts_batch_replica(fts_batch* b, int r, fts_engine** engine, int* group);
  * records from records + r * capacity; n_records: [n_replicas].  On a failure
  * the call returns its status, and fts_last_error names the replica. */
 int fts_batch_step(fts_batch* b, int n, int record_every, fts_observables* records, int capacity, int* n_records);
-/* The bead-step launches of each batched sweep are captured once as a hipGraph and
- * replayed (default); 0 launches them one by one (same kernels, same results). */
-int fts_batch_set_graph(fts_batch* b, int enabled);
+/* Graph level of the batch's steps (same kernels, same results at every level):
+ *   2 (default) each CL step of a group -- its sweeps, every replica's own work on the
+ *     side streams and the copies of Q and the finiteness flags -- is captured once as
+ *     a hipGraph and replayed; a step is captured again when the replicas taking part or
+ *     recording, the learned kernel's state or a replica's configuration change.  The
+ *     noise key {seed, step} reaches the kernels through device memory that a captured
+ *     copy refreshes from pinned host memory on every replay.  Steps of replicas with
+ *     drift statistics or an armed fts_inject_nonfinite run as at level 1;
+ *   1 the bead-step launches of each batched sweep are captured once and replayed, the
+ *     rest is launched one by one;
+ *   0 everything is launched one by one. */
+int fts_batch_set_graph(fts_batch* b, int level);
 /* The work of single replicas between the batched sweeps (smoothing FFTs, ions,
  * contractions, field updates, records) runs on n side streams per group, forked
  * from the group's stream and joined back into it (replica r on side stream

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, PREC_FP32, PREC_FP64, SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHEME_EM,
-                                SCHEME_PC_REFERENCE, SYMMETRY_OFF, hip_batch, hip_engine)
+                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SYMMETRY_OFF, hip_batch, hip_engine)
 from ftspolysol_amd.system import initial_fields, make_system
 
 pytestmark = pytest.mark.gpu
@@ -100,27 +100,33 @@ def test_batch_headline_autotuned():
         assert abs(x["H"] - y["H"]) <= 1e-12 * abs(y["H"])
 
 
-@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE, SCHED_CONV])
-def test_graph_replays_match_eager_launches(sched):
-    """The captured sweep graphs against the same launches issued one by one: several
-    step calls (each replaying the graphs of both density evaluations many times, with
-    the per-replica eager work and host synchronisations between replays) must give
-    bit-identical fields and records."""
-    print(f"graph vs eager sched={sched}", flush=True)
+@pytest.mark.parametrize("scheme,sched", [(SCHEME_PC_REFERENCE, SCHED_PLANE), (SCHEME_PC_REFERENCE, SCHED_LINE),
+                                          (SCHEME_SEMI_IMPLICIT, SCHED_CONV), (SCHEME_EM, SCHED_CONV)])
+def test_graph_replays_match_eager_launches(scheme, sched):
+    """Whole-step graphs (level 2), sweep graphs (1) and launches issued one by one (0):
+    several step calls with records on every step, on none and on every second step (so
+    the captured steps change between calls and replays follow captures) must give
+    bit-identical fields and records, and the same timing counters."""
+    print(f"graph levels scheme={scheme} sched={sched}", flush=True)
     out = []
-    for graph in (True, False):
-        systems = replica_systems(SCHEME_PC_REFERENCE, PREC_FP32, sched)
+    for level in (2, 1, 0):
+        systems = replica_systems(scheme, PREC_FP32, sched)
         b = hip_batch(systems)
-        b.set_graph(graph)
+        b.set_graph(level)
+        b.set_timing(True)
         start_fields(b.replicas, systems)
-        recs = [b.step(3, record_every=1) for _ in range(3)]
+        recs = [b.step(3, record_every=1), b.step(4), b.step(4, record_every=2), b.step(3, record_every=1)]
+        t = b.get_timing(0)
+        assert t["sweep_ms"] > 0.0
         out.append(([e.get_field(FIELD_OMEGA) for e in b.replicas], [e.get_field(FIELD_PSI) for e in b.replicas],
-                    [[x["H"] for call in recs for x in call[r]] for r in range(len(systems))]))
+                    [[(x["step"], x["H"], tuple(x["stress"])) for call in recs for x in call[r]] for r in range(len(systems))],
+                    (t["sweeps"], t["bead_steps"], t["step_launches"], t["algorithmic_bytes"])))
         b.close()
-    (wg, pg, hg), (we, pe, he) = out
-    for r in range(len(wg)):
-        assert np.array_equal(wg[r], we[r]) and np.array_equal(pg[r], pe[r]), r
-        assert hg[r] == he[r], r
+    for level, (w, p, h, t) in zip((1, 0), out[1:]):
+        for r in range(len(w)):
+            assert np.array_equal(w[r], out[0][0][r]) and np.array_equal(p[r], out[0][1][r]), (level, r)
+            assert h[r] == out[0][2][r], (level, r)
+        assert t == out[0][3], (level, t, out[0][3])
 
 
 @pytest.mark.parametrize("scheme,precision", [(SCHEME_PC_REFERENCE, PREC_FP32), (SCHEME_EM, PREC_FP64)])

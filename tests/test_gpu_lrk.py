@@ -139,6 +139,32 @@ def test_batch_learned_matches_single_engines():
     b.close()
 
 
+def test_batch_learned_graph_matches_eager():
+    """The learned scheme in replayed whole-step graphs (the judge's ping-pong spectra
+    alternate between two captured steps) against launches issued one by one: bit-
+    identical fields, records and judge statistics."""
+    P = small_network(41)
+    out = []
+    for level in (2, 0):
+        systems = [charged_system(SCHEME_SI_LEARNED, m=12, seed=50 + r) for r in range(3)]
+        b = hip_batch(systems)
+        b.set_graph(level)
+        for r in range(3):
+            b.replicas[r].set_fields(*start_fields(12, seed=60 + r))
+            b.replicas[r].lrk_set_network(P, 1e300 if r != 1 else 1e-3)
+        recs = [b.step(5, record_every=1), b.step(4), b.step(3, record_every=1)]
+        out.append(([b.replicas[r].get_field(FIELD_OMEGA) for r in range(3)],
+                    [[x["H"] for call in recs for x in call[r]] for r in range(3)],
+                    [b.replicas[r].lrk_get_stats() for r in range(3)]))
+        b.close()
+    (wg, hg, sg), (we, he, se) = out
+    for r in range(3):
+        assert np.array_equal(wg[r], we[r]) and hg[r] == he[r], r
+        for k in ("judged", "learned_steps", "fallback_steps"):
+            assert sg[r][k] == se[r][k], (r, k)
+        assert np.array_equal(sg[r]["sum_rel_residual_learned"], se[r]["sum_rel_residual_learned"]), r
+
+
 def chem_system(n, scheme, seed):
     """The headline chemistry (N = 100, rho = 0.2, salt 0.05, lB = 1, dL = 0.5) on an n^3 grid, FP32."""
     from ftspolysol_amd.system import make_system
