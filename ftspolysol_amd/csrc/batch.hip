@@ -133,6 +133,8 @@ template <typename T> struct RepView {
         const size_t ks = static_cast<size_t>(e->m1 + e->m2 + e->m3);
         return e->kop_sep(e->sep_bond.p + static_cast<size_t>(sp->bond_cls[static_cast<size_t>(bond)]) * ks, 1.0 / static_cast<double>(e->M));
     }
+    KOp bond_unit(int bond) const { return e->kop_sep(bond_op(bond).sep, 1.0); } // the factors without 1/M
+
 };
 
 template <typename T> RepView<T> view(fts_engine* e, int p)
@@ -252,6 +254,49 @@ template <typename T> int build_tables(Group& g, int p, int sched, int max_steps
             }
             tb.jobs += static_cast<long long>(J.plane.size() - ol);
             add(L_PLANE, s == maxN - 2 ? PLANE_LAST : PLANE_STEP, 0, ol, J.plane.size());
+        }
+    } else if (sched == FTS_SCHED_SEPARABLE) {
+        size_t o = J.line.size();
+        for (const RepView<T>& r : act) {
+            const int N = r.sp->N;
+            J.line.push_back(LineJob<T>{nullptr, r.qf, r.E_of(0), KOp{}, nullptr, r.nF[0], r.lcF});
+            if (!r.sp->mirror)
+                J.line.push_back(LineJob<T>{nullptr, r.qb + static_cast<size_t>(N - 1) * r.Ms, r.E_of(N - 1), KOp{}, nullptr, r.nB[0], r.lcB + (N - 1)});
+        }
+        add(L_LINE, LINE_E, 1, o, J.line.size());
+        tb.init_launches = tb.launches.size();
+        // bead step s: x pass q_s -> tmp, z pass in place, y pass tmp -> q_{s+1}; a
+        // replica whose chain has ended drops out of all three
+        for (int s = 0; s < nsteps; ++s) {
+            size_t nact = 0;
+            while (nact < act.size() && act[nact].sp->N - 1 > s) ++nact;
+            const size_t ox = J.line.size();
+            for (size_t k = 0; k < nact; ++k) {
+                const RepView<T>& r = act[k];
+                const int N = r.sp->N;
+                J.line.push_back(LineJob<T>{r.tf, nullptr, nullptr, r.bond_op(s), nullptr, nullptr, nullptr, r.qf + static_cast<size_t>(s) * r.Ms});
+                if (!r.sp->mirror)
+                    J.line.push_back(LineJob<T>{r.tb, nullptr, nullptr, r.bond_op(N - 2 - s), nullptr, nullptr, nullptr, r.qb + static_cast<size_t>(N - 1 - s) * r.Ms});
+            }
+            add(L_LINE, LINE_CONV, 0, ox, J.line.size());
+            const size_t oz = J.line.size();
+            for (size_t k = 0; k < nact; ++k) {
+                const RepView<T>& r = act[k];
+                J.line.push_back(LineJob<T>{r.tf, nullptr, nullptr, r.bond_unit(s)});
+                if (!r.sp->mirror) J.line.push_back(LineJob<T>{r.tb, nullptr, nullptr, r.bond_unit(r.sp->N - 2 - s)});
+            }
+            add(L_LINE, LINE_CONV, 2, oz, J.line.size());
+            const size_t oy = J.line.size();
+            for (size_t k = 0; k < nact; ++k) {
+                const RepView<T>& r = act[k];
+                const int N = r.sp->N;
+                const int sf = s + 1, sb = N - 2 - s;
+                J.line.push_back(LineJob<T>{r.tf, r.qf + static_cast<size_t>(sf) * r.Ms, r.E_of(sf), r.bond_unit(s), r.nF[s % 2], r.nF[(s + 1) % 2], r.lcF + sf});
+                if (!r.sp->mirror)
+                    J.line.push_back(LineJob<T>{r.tb, r.qb + static_cast<size_t>(sb) * r.Ms, r.E_of(sb), r.bond_unit(sb), r.nB[s % 2], r.nB[(s + 1) % 2], r.lcB + sb});
+            }
+            tb.jobs += static_cast<long long>(J.line.size() - oy);
+            add(L_LINE, LINE_CONV_E, 1, oy, J.line.size());
         }
     } else {
         size_t o = J.line.size();
@@ -685,19 +730,19 @@ template <typename T> int group_step(Group& g, int scheme, const std::vector<fts
     return group_step_check(g, scheme, rec, mask, why);
 }
 
-// Schedule of a group's sweeps: conv and line timed over 8 bead steps of species 0
-// with all replicas in each launch, best of three; tune_ms = {conv, line} (plane only
-// when asked for, as for single engines).
+// Schedule of a group's sweeps: conv and separable timed over 8 bead steps of species
+// 0 with all replicas in each launch, best of three; tune_ms = {conv, separable}
+// (plane and line only when asked for, as for single engines).
 template <typename T> int group_autotune(Group& g)
 {
     if (!g.plane_ok) {
-        g.sched = FTS_SCHED_LINE;
+        g.sched = FTS_SCHED_SEPARABLE;
         return FTS_OK;
     }
     SweepTables t[2];
     const std::vector<char> all(g.reps.size(), 1);
     int rc = FTS_OK;
-    const int cand[2] = {FTS_SCHED_CONV, FTS_SCHED_LINE};
+    const int cand[2] = {FTS_SCHED_CONV, FTS_SCHED_SEPARABLE};
     for (int k = 0; k < 2 && rc == FTS_OK; ++k) rc = build_tables<T>(g, 0, cand[k], 8, all, t[k]);
     if (rc) return rc;
     if (t[0].steps == 0) return FTS_OK;

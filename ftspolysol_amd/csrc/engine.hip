@@ -422,7 +422,7 @@ int fts_engine::setup_device()
     }
     q_shift_total.assign(species.size(), 0.0);
     q_log_gain.assign(species.size(), 0.0);
-    norm_stride = std::max(static_cast<size_t>(m1), static_cast<size_t>(m1) * m2 / LINE_TC + 1);
+    norm_stride = std::max(static_cast<size_t>(m1), static_cast<size_t>(m1) * std::max(m2, m3) / LINE_TC + 1);
     HIP_TRY(norms.alloc(4 * norm_stride));
     HIP_TRY(logc.alloc(2 * static_cast<size_t>(Nmax)));
     HIP_TRY(rfac.alloc(static_cast<size_t>(Nmax)));
@@ -989,6 +989,32 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
                                  {tb, tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), nB[s % 2], nB[(s + 1) % 2], lcB + sb}};
             HIP_TRY(launch_plane<T>(g, s < nsteps - 1 ? PLANE_STEP : PLANE_LAST, st, nj, stream));
         }
+    } else if (schedule == FTS_SCHED_SEPARABLE) {
+        // bead step s, one axis of the separable bond kernel per pass: x line pass (x
+        // factor and 1/M) q_s -> tmp, z line pass (z factor) in place, y line pass (y
+        // factor, e^{-W}, renormalisation) tmp -> q_{s+1}; the first y pass stores
+        // q_0 = e^{-W_0} alone
+        LineJob<T> init[2] = {{nullptr, qf, E_of(0), KOp{}, nullptr, nF[0], lcF},
+                              {nullptr, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), KOp{}, nullptr, nB[0], lcB + (N - 1)}};
+        HIP_TRY(launch_line<T>(g, 1, LINE_E, twy, init, nj, stream));
+        if (timed) {
+            ev0 = get_event();
+            HIP_TRY(hipEventRecord(ev0, stream));
+        }
+        for (int s = 0; s < nsteps; ++s) {
+            const int bf = sp.bond_cls[static_cast<size_t>(s)], bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
+            const KOp of = kop_sep(sep_bond.p + static_cast<size_t>(bf) * ks, invM), ob = kop_sep(sep_bond.p + static_cast<size_t>(bb) * ks, invM);
+            const KOp of1 = kop_sep(of.sep, 1.0), ob1 = kop_sep(ob.sep, 1.0);
+            LineJob<T> xj[2] = {{tf, nullptr, nullptr, of, nullptr, nullptr, nullptr, qf + static_cast<size_t>(s) * Ms},
+                                {tb, nullptr, nullptr, ob, nullptr, nullptr, nullptr, qb + static_cast<size_t>(N - 1 - s) * Ms}};
+            HIP_TRY(launch_line<T>(g, 0, LINE_CONV, twx, xj, nj, stream));
+            LineJob<T> zj[2] = {{tf, nullptr, nullptr, of1}, {tb, nullptr, nullptr, ob1}};
+            HIP_TRY(launch_line<T>(g, 2, LINE_CONV, twz, zj, nj, stream));
+            const int sf = s + 1, sb = N - 2 - s;
+            LineJob<T> yj[2] = {{tf, qf + static_cast<size_t>(sf) * Ms, E_of(sf), of1, nF[s % 2], nF[(s + 1) % 2], lcF + sf},
+                                {tb, qb + static_cast<size_t>(sb) * Ms, E_of(sb), ob1, nB[s % 2], nB[(s + 1) % 2], lcB + sb}};
+            HIP_TRY(launch_line<T>(g, 1, LINE_CONV_E, twy, yj, nj, stream));
+        }
     } else {
         LineJob<T> init[2] = {{tf, qf, E_of(0), KOp{}, nullptr, nF[0], lcF},
                               {tb, qb + static_cast<size_t>(N - 1) * Ms, E_of(N - 1), KOp{}, nullptr, nB[0], lcB + (N - 1)}};
@@ -1050,6 +1076,15 @@ template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T
     const void* twx = d ? static_cast<const void*>(tw64_x.p) : static_cast<const void*>(tw32_x.p);
     const void* twy = d ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p);
     const void* twz = d ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p);
+    if (sched == FTS_SCHED_SEPARABLE) {
+        LineJob<T> xj{q, nullptr, nullptr, op};
+        HIP_TRY(launch_line<T>(g, 0, LINE_CONV, twx, &xj, 1, stream));
+        LineJob<T> zj{q, nullptr, nullptr, kop_sep(op.sep, 1.0)};
+        HIP_TRY(launch_line<T>(g, 2, LINE_CONV, twz, &zj, 1, stream));
+        LineJob<T> yj{q, qout, emw, kop_sep(op.sep, 1.0)};
+        HIP_TRY(launch_line<T>(g, 1, LINE_CONV_E, twy, &yj, 1, stream));
+        return FTS_OK;
+    }
     LineJob<T> j{q};
     HIP_TRY(launch_line<T>(g, 2, LINE_FWD, twz, &j, 1, stream));
     HIP_TRY(launch_line<T>(g, 1, LINE_FWD, twy, &j, 1, stream));
@@ -1064,15 +1099,16 @@ template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T
 template int fts_engine::bead_hook<float>(V2<float>*, const V2<float>*, V2<float>*, const KOp&, hipStream_t);
 template int fts_engine::bead_hook<double>(V2<double>*, const V2<double>*, V2<double>*, const KOp&, hipStream_t);
 
-// Bead-step schedule chosen by timing plane and line on this grid and precision (the
-// FFT autotuner): 2 x 8 bead steps of species 0 at zero fields, best of three;
-// tune_ms = {conv, line}.  The plane schedule is taken only when asked for: it is
+// Bead-step schedule chosen by timing conv and separable on this grid and precision
+// (the FFT autotuner): 2 x 8 bead steps of species 0 at zero fields, best of three;
+// tune_ms = {conv, separable}.  The plane schedule is taken only when asked for: it is
 // within a few percent of conv either way, so timing it too would make the choice,
-// and the last bits of the results, vary from run to run.
+// and the last bits of the results, vary from run to run.  The line schedule moves
+// more bytes than the separable one for the same transforms and is never chosen.
 int fts_engine::autotune_schedule()
 {
     if (!plane_ok) {
-        sched = FTS_SCHED_LINE;
+        sched = FTS_SCHED_SEPARABLE;
         return FTS_OK;
     }
     const HostSpecies& sp = species[0];
@@ -1081,7 +1117,7 @@ int fts_engine::autotune_schedule()
         return FTS_OK;
     }
     const int steps = std::min(8, sp.N - 1);
-    const int cand[2] = {FTS_SCHED_CONV, FTS_SCHED_LINE};
+    const int cand[2] = {FTS_SCHED_CONV, FTS_SCHED_SEPARABLE};
     double best[2] = {1e30, 1e30};
     hipEvent_t a = get_event(), b = get_event();
     for (int rep = 0; rep < 3; ++rep)
@@ -1932,7 +1968,7 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     const bool plane_fits = fft_supported(s->m1, s->m2, s->m3, s->precision == FTS_PREC_FP32);
     if ((s->fft_schedule == FTS_SCHED_PLANE || s->fft_schedule == FTS_SCHED_CONV) && !plane_fits)
         return fail(FTS_INVALID, "the plane schedules need m2 == m3 and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
-    if (s->fft_schedule < 0 || s->fft_schedule > 3) return fail(FTS_INVALID, "fft_schedule");
+    if (s->fft_schedule < 0 || s->fft_schedule > 4) return fail(FTS_INVALID, "fft_schedule");
     if (s->chain_symmetry != FTS_SYMMETRY_AUTO && s->chain_symmetry != FTS_SYMMETRY_OFF) return fail(FTS_INVALID, "chain_symmetry");
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
@@ -1967,7 +2003,7 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     e->seed0 = s->seed;
     e->stress_mask = s->stress_mask;
     e->plane_ok = plane_fits;
-    e->sched = s->fft_schedule == FTS_SCHED_AUTO ? (plane_fits ? FTS_SCHED_CONV : FTS_SCHED_LINE) : s->fft_schedule;
+    e->sched = s->fft_schedule == FTS_SCHED_AUTO ? (plane_fits ? FTS_SCHED_CONV : FTS_SCHED_SEPARABLE) : s->fft_schedule;
     int rc = e->build_modes(s);
     if (rc) return bail(rc);
     e->total_type_segments.assign(static_cast<size_t>(e->nT), 0);

@@ -93,8 +93,12 @@ template <typename T> struct PencilJob {
 // forward transform) and E_FWD (q = e^{-W} (x 2^k), store q, forward).  The bead step
 // with line passes is x(FWD_MUL_INV) -> y(INV) -> z(INV_E_FWD) -> y(FWD); it moves
 // more bytes than the plane/pencil pair but every pass has m^2/16 workgroups.
+// The separable schedule convolves one axis per pass with that axis' factor of the bond
+// kernel: x(CONV, q_s from src) -> z(CONV) -> y(CONV_E: x e^{-W} (x 2^k), store q); E
+// stores q = e^{-W} (x 2^k) alone (the first bead of a sweep).
 constexpr int LINE_TC = 16;
-enum LineMode { LINE_FWD = 0, LINE_INV = 1, LINE_FWD_MUL_INV = 2, LINE_INV_E_FWD = 3, LINE_INV_E = 4, LINE_E_FWD = 5, LINE_FWD_MUL_INV_NOISE = 6 };
+enum LineMode { LINE_FWD = 0, LINE_INV = 1, LINE_FWD_MUL_INV = 2, LINE_INV_E_FWD = 3, LINE_INV_E = 4, LINE_E_FWD = 5, LINE_FWD_MUL_INV_NOISE = 6,
+                LINE_CONV = 7, LINE_CONV_E = 8, LINE_E = 9 };
 
 template <typename T> struct LineJob {
     V2<T>* data;          // transformed in place (E_FWD writes its result here)
@@ -104,6 +108,7 @@ template <typename T> struct LineJob {
     const double* norm_in;
     double* norm_out;     // per-block max |q|^2 (gridDim.x entries)
     double* log_scale;
+    const V2<T>* src;     // CONV: read from here instead of data when set (the result still goes to data)
 };
 
 // The job of a workgroup: row blockIdx.y of a device job table, or (no table) j0
@@ -205,6 +210,7 @@ template <typename T> __device__ __forceinline__ LineJob<T> pick_job(LineJob<T> 
         j.norm_in = t->norm_in;
         j.norm_out = t->norm_out;
         j.log_scale = t->log_scale;
+        j.src = t->src;
     } else {
         j.data = f ? a.data : b.data;
         j.qout = f ? a.qout : b.qout;
@@ -213,6 +219,7 @@ template <typename T> __device__ __forceinline__ LineJob<T> pick_job(LineJob<T> 
         j.norm_in = f ? a.norm_in : b.norm_in;
         j.norm_out = f ? a.norm_out : b.norm_out;
         j.log_scale = f ? a.log_scale : b.log_scale;
+        j.src = f ? a.src : b.src;
     }
     return j;
 }

@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
     constexpr bool ZL = AXIS == 2; // z lines: LDS rows are lines ([c][t]); otherwise [t][c]
     constexpr int LS = ZL ? N + 1 : TC + 1;
     constexpr int NRED = NT % 64 == 0 ? NT / 64 : NT;
-    constexpr bool PRODUCES_Q = MODE == LINE_INV_E_FWD || MODE == LINE_INV_E || MODE == LINE_E_FWD;
+    constexpr bool PRODUCES_Q = MODE == LINE_INV_E_FWD || MODE == LINE_INV_E || MODE == LINE_E_FWD || MODE == LINE_CONV_E || MODE == LINE_E;
     __shared__ V lds[ZL ? TC * (N + 1) : N * (TC + 1)];
     __shared__ V tw[N];
     __shared__ double red[NRED];
@@ -113,14 +113,14 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
         }
     }
     double nrm = 0.0;
-    const V* src = MODE == LINE_E_FWD ? job.emw : job.data;
+    const V* src = (MODE == LINE_E_FWD || MODE == LINE_E) ? job.emw : (job.src ? job.src : job.data);
 #pragma unroll
     for (int i = tid; i < N * TC; i += NT) {
         const int cc = ZL ? i / N : i % TC;
         const int t = ZL ? i % N : i / TC;
         const size_t gi = line_base(AXIS, col0 + cc, g.m2, g.m3) + t * S;
         V v = src[gi];
-        if constexpr (MODE == LINE_E_FWD) {
+        if constexpr (MODE == LINE_E_FWD || MODE == LINE_E) {
             v = cscale(v, c);
             job.qout[gi] = v;
             nrm = fmax(nrm, static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y);
@@ -129,12 +129,28 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
     }
     __syncthreads();
 
+    constexpr bool CONV = MODE == LINE_CONV || MODE == LINE_CONV_E;
     constexpr bool DO_INV = MODE == LINE_INV || MODE == LINE_INV_E_FWD || MODE == LINE_INV_E;
     constexpr bool MUL = MODE == LINE_FWD_MUL_INV || MODE == LINE_FWD_MUL_INV_NOISE;
-    constexpr bool DO_FWD_FIRST = MODE == LINE_FWD || MUL;
+    constexpr bool DO_FWD_FIRST = MODE == LINE_FWD || MUL || CONV;
     if constexpr (DO_FWD_FIRST) {
         if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, -1, TC, 1, LS, true, NT>(lds, tw, tid);
+    }
+    if constexpr (CONV) {
+        // x the line axis' factor of the separable kernel (op.sep holds the m1, m2, m3
+        // factors) and op.c0
+        const int off = AXIS == 0 ? 0 : AXIS == 1 ? g.m1 : g.m1 + g.m2;
+        const double* f = job.op.sep + off;
+        const double c0 = job.op.c0;
+#pragma unroll
+        for (int i = tid; i < N * TC; i += NT) {
+            const int cc = ZL ? i / N : i % TC;
+            const int t = ZL ? i % N : i / TC;
+            V* p = &lds[ZL ? cc * LS + t : t * LS + cc];
+            *p = cscale(*p, static_cast<T>(f[t] * c0));
+        }
+        __syncthreads();
     }
     if constexpr (MUL) {
         static_assert(!ZL, "the k-space multiply runs on x or y lines");
@@ -162,11 +178,11 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
         }
         __syncthreads();
     }
-    if constexpr (DO_INV || MUL) {
+    if constexpr (DO_INV || MUL || CONV) {
         if constexpr (ZL) lds_fft<T, N, +1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, +1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
-    if constexpr (MODE == LINE_INV_E_FWD || MODE == LINE_INV_E) {
+    if constexpr (MODE == LINE_INV_E_FWD || MODE == LINE_INV_E || MODE == LINE_CONV_E) {
 #pragma unroll
         for (int i = tid; i < N * TC; i += NT) {
             const int cc = ZL ? i / N : i % TC;
@@ -191,7 +207,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
         if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, -1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
-    if constexpr (MODE != LINE_INV_E) {
+    if constexpr (MODE != LINE_INV_E && MODE != LINE_CONV_E && MODE != LINE_E) {
 #pragma unroll
         for (int i = tid; i < N * TC; i += NT) {
             const int cc = ZL ? i / N : i % TC;
@@ -334,6 +350,13 @@ hipError_t line_dispatch(const FftGeom& g, int axis, int mode, const void* tw, c
             if (mode == LINE_INV) return rline_go<T, N, 1, LINE_INV>(g, tw, jobs, table, njobs, s);
         }
     }
+    if (mode == LINE_CONV) {
+        if (axis == 0) return line_go<T, N, 0, LINE_CONV>(g, tw, jobs, table, njobs, s);
+        if (axis == 1) return line_go<T, N, 1, LINE_CONV>(g, tw, jobs, table, njobs, s);
+        if (axis == 2) return line_go<T, N, 2, LINE_CONV>(g, tw, jobs, table, njobs, s);
+    }
+    if (mode == LINE_CONV_E && axis == 1) return line_go<T, N, 1, LINE_CONV_E>(g, tw, jobs, table, njobs, s);
+    if (mode == LINE_E && axis == 1) return line_go<T, N, 1, LINE_E>(g, tw, jobs, table, njobs, s);
     if (axis == 0) {
         if (mode == LINE_FWD) return line_go<T, N, 0, LINE_FWD>(g, tw, jobs, table, njobs, s);
         if (mode == LINE_INV) return line_go<T, N, 0, LINE_INV>(g, tw, jobs, table, njobs, s);

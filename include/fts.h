@@ -53,12 +53,14 @@ extern "C" {
 #define FTS_SYMMETRY_OFF 1  /* always run both sweeps */
 
 /* bead-step FFT schedules (fts_system.fft_schedule, fts_info.fft_schedule) */
-#define FTS_SCHED_AUTO 0  /* time conv and line at creation, keep the faster */
+#define FTS_SCHED_AUTO 0  /* time conv and separable at creation, keep the faster (separable when conv does not fit) */
 #define FTS_SCHED_PLANE 1 /* pencil(x: FFT, x phi/M, IFFT) + plane(yz: IFFT, x e^{-W}, store q, FFT); needs m2 == m3, plane in LDS;
                              used only when asked for */
 #define FTS_SCHED_LINE 2  /* four line passes x, y, z, y; any supported lengths */
 #define FTS_SCHED_CONV 3  /* pencil(x: FFT of q_s, x phi_x/M, IFFT) + plane(yz: FFT, x phi_y phi_z, IFFT, x e^{-W},
                              store q_{s+1}); separable bond kernels; needs m2 == m3, plane in LDS */
+#define FTS_SCHED_SEPARABLE 4 /* three line passes, one bond-kernel factor each: x (FFT of q_s, x phi_x/M, IFFT),
+                                 z (FFT, x phi_z, IFFT), y (FFT, x phi_y, IFFT, x e^{-W}, store q_{s+1}); any supported lengths */
 
 /* stress_mask bits: components in the reference's fixed order (stress.cu:56-61) */
 #define FTS_STRESS_XX 0x01
@@ -171,7 +173,7 @@ typedef struct fts_info {
     int n_segment_classes; /* distinct (type, charge) bead classes: e^{-W} fields kept per species */
     int precision;
     int scheme;
-    int fft_schedule; /* FTS_SCHED_PLANE, FTS_SCHED_LINE or FTS_SCHED_CONV in use */
+    int fft_schedule; /* FTS_SCHED_PLANE, _LINE, _CONV or _SEPARABLE in use */
     double volume;
     double rho0;
     double short_range_self_pressure; /* param.cu:135-149 */
@@ -182,7 +184,7 @@ typedef struct fts_info {
     double ion_valence[FTS_MAX_IONS];
     long long ion_count[FTS_MAX_IONS];
     double ion_smearing[FTS_MAX_IONS];
-    double tune_ms[2]; /* autotuner: time of 8 bead steps, conv and line schedule (0 if not timed) */
+    double tune_ms[2]; /* autotuner: time of 8 bead steps, conv and separable schedule (0 if not timed) */
     int mirrored_species; /* bit p set: species p runs the mirrored (forward-only) sweep */
     int restarts;         /* non-finite restarts taken (fts_set_restart) */
     int noise_epoch;      /* Philox key in use: seed + noise_epoch * 0x9E3779B97F4A7C15 */
@@ -321,8 +323,8 @@ typedef struct fts_batch_info {
     int group_size[FTS_MAX_BATCH_GROUPS];
     int group_mesh[FTS_MAX_BATCH_GROUPS][3];
     int group_precision[FTS_MAX_BATCH_GROUPS];
-    int group_schedule[FTS_MAX_BATCH_GROUPS];        /* FTS_SCHED_PLANE, FTS_SCHED_LINE or FTS_SCHED_CONV */
-    double group_tune_ms[FTS_MAX_BATCH_GROUPS][2];   /* autotuner: 8 batched bead steps, conv and line */
+    int group_schedule[FTS_MAX_BATCH_GROUPS];        /* FTS_SCHED_PLANE, _LINE, _CONV or _SEPARABLE */
+    double group_tune_ms[FTS_MAX_BATCH_GROUPS][2];   /* autotuner: 8 batched bead steps, conv and separable */
 } fts_batch_info;
 
 /* Build n replicas from systems[0..n-1] (each with its own seed); fields start at 0.

@@ -9,7 +9,7 @@ test_gpu_parity.py.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, PREC_FP32, PREC_FP64, SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHEME_EM,
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, PREC_FP32, PREC_FP64, SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SCHEME_EM,
                                 SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SYMMETRY_OFF, hip_batch, hip_engine)
 from ftspolysol_amd.system import initial_fields, make_system
 
@@ -44,7 +44,9 @@ def start_fields(engines, systems):
                                                     (SCHEME_EM, PREC_FP32, SCHED_PLANE),
                                                     (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_LINE),
                                                     (SCHEME_EM, PREC_FP64, SCHED_CONV),
-                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_CONV)])
+                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_CONV),
+                                                    (SCHEME_SEMI_IMPLICIT, PREC_FP64, SCHED_SEPARABLE),
+                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_SEPARABLE)])
 def test_batch_matches_single_engines(scheme, precision, sched):
     print(f"batch scheme={scheme} precision={precision} sched={sched}", flush=True)
     systems = replica_systems(scheme, precision, sched)
@@ -83,8 +85,8 @@ def test_batch_headline_autotuned():
     systems = [make_system(96, 100, scheme=SCHEME_EM, precision=PREC_FP32, seed=7 + r) for r in range(4)]
     b = hip_batch(systems)
     sched = b.info.group_schedule[0]
-    print("batch tune ms (conv, line):", list(b.info.group_tune_ms[0]), "chosen", sched, flush=True)
-    assert sched in (SCHED_CONV, SCHED_LINE)
+    print("batch tune ms (conv, separable):", list(b.info.group_tune_ms[0]), "chosen", sched, flush=True)
+    assert sched in (SCHED_CONV, SCHED_SEPARABLE)
     tune = list(b.info.group_tune_ms[0])
     assert min(tune) > 0 and tune[0 if sched == SCHED_CONV else 1] == min(tune)
     start_fields(b.replicas, systems)
@@ -101,7 +103,8 @@ def test_batch_headline_autotuned():
 
 
 @pytest.mark.parametrize("scheme,sched", [(SCHEME_PC_REFERENCE, SCHED_PLANE), (SCHEME_PC_REFERENCE, SCHED_LINE),
-                                          (SCHEME_SEMI_IMPLICIT, SCHED_CONV), (SCHEME_EM, SCHED_CONV)])
+                                          (SCHEME_SEMI_IMPLICIT, SCHED_CONV), (SCHEME_EM, SCHED_CONV),
+                                          (SCHEME_EM, SCHED_SEPARABLE)])
 def test_graph_replays_match_eager_launches(scheme, sched):
     """Whole-step graphs (level 2), sweep graphs (1) and launches issued one by one (0):
     several step calls with records on every step, on none and on every second step (so

@@ -21,7 +21,7 @@ means (to FP32 rounding), and one replica of each mesh reproduces its single-eng
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, FIELD_RHO_C, FIELD_RHO_P, PREC_FP32, PREC_FP64, SCHED_LINE, SCHED_PLANE,
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, FIELD_RHO_C, FIELD_RHO_P, PREC_FP32, PREC_FP64, SCHED_PLANE, SCHED_SEPARABLE,
                                 SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, hip_batch, hip_engine)
 from ftspolysol_amd.lrk import load_params
 from ftspolysol_amd.system import config, initial_fields, make_system, replica_params
@@ -45,7 +45,7 @@ def test_large_system_all_forms():
     print("256^3 N=200 FP32 + learned", flush=True)
     s32 = config(5, precision=PREC_FP32, scheme=SCHEME_SI_LEARNED, stress_mask=0x7F, seed=0xC5)
     g = hip_engine(s32)
-    assert g.info.fft_schedule == SCHED_LINE and g.info.mirrored_species == 0
+    assert g.info.fft_schedule == SCHED_SEPARABLE and g.info.mirrored_species == 0
     g.lrk_set_network(load_params("ftspolysol_amd/lrk_c4_48cubed.npy"), 1e-3)
     g.set_fields(*initial_fields(s32, sigma=1e-3, cfg=5))
     g.set_timing(True)

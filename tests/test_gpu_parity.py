@@ -8,7 +8,7 @@ engine 1e-4 relative to FP64 for H, mu and stress, 2e-5 for a bead step.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (SCHED_CONV, SCHED_LINE, SCHED_PLANE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
+from ftspolysol_amd.api import (SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
                                 FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
                                 SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
 from ftspolysol_amd.system import System, initial_fields, make_system
@@ -38,17 +38,18 @@ def test_fft_matches_numpy(m):
 
 @pytest.mark.parametrize("m", [(32, 24, 48), (128, 128, 128), (16, 192, 8), (256, 16, 32)])
 def test_fft_line_path_matches_numpy(m):
-    """Grids the plane pass cannot hold run every FFT as line passes."""
+    """Grids the plane pass cannot hold run every FFT as line passes (and their bead
+    steps on the separable line schedule)."""
     print(f"fft line path {m}", flush=True)
     e = hip_engine(bare_system(m))
-    assert e.info.fft_schedule == SCHED_LINE
+    assert e.info.fft_schedule == SCHED_SEPARABLE
     rng = np.random.default_rng(3)
     x = rng.standard_normal(m) + 1j * rng.standard_normal(m)
     assert rel(e.fft3d(x, -1), np.fft.fftn(x)) < 1e-13
     assert rel(e.fft3d(x, +1), np.fft.ifftn(x) * x.size) < 1e-13
 
 
-@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE, SCHED_CONV])
+@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_LINE, SCHED_CONV, SCHED_SEPARABLE])
 @pytest.mark.parametrize("m,bond", [((32, 32, 32), 1.3), ((96, 96, 96), 1.0), ((16, 24, 24), 0.7)])
 def test_bead_step_matches_oracle(m, bond, sched):
     print(f"bead step {m} b={bond} sched={sched}", flush=True)
@@ -65,10 +66,28 @@ def test_bead_step_matches_oracle(m, bond, sched):
     assert rel(hip_engine(s32).bead_step(q, emw, bond), ref) < 2e-5
 
 
+@pytest.mark.parametrize("sched", [SCHED_LINE, SCHED_SEPARABLE])
+@pytest.mark.parametrize("m,bond", [((32, 24, 48), 1.1), ((16, 192, 8), 0.9), ((128, 128, 128), 1.2), ((256, 16, 32), 0.8)])
+def test_bead_step_line_schedules_match_oracle(m, bond, sched):
+    """The line schedules on grids the plane pass cannot hold."""
+    print(f"bead step {m} b={bond} sched={sched}", flush=True)
+    s = bare_system(m, fft_schedule=sched)
+    g, o = hip_engine(s), oracle_engine(s)
+    assert g.info.fft_schedule == sched
+    rng = np.random.default_rng(6)
+    q = 1 + 0.1 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    w = 0.2 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    emw = np.exp(-w)
+    ref = o.bead_step(q, emw, bond)
+    assert rel(g.bead_step(q, emw, bond), ref) < 1e-13
+    s32 = bare_system(m, precision=PREC_FP32, fft_schedule=sched)
+    assert rel(hip_engine(s32).bead_step(q, emw, bond), ref) < 2e-5
+
+
 def test_autotuner_times_both_schedules():
     e = hip_engine(make_system(96, 100, precision=PREC_FP32))
-    print("tune ms (conv, line):", list(e.info.tune_ms), "chosen", e.info.fft_schedule, flush=True)
-    assert e.info.fft_schedule in (SCHED_CONV, SCHED_LINE)
+    print("tune ms (conv, separable):", list(e.info.tune_ms), "chosen", e.info.fft_schedule, flush=True)
+    assert e.info.fft_schedule in (SCHED_CONV, SCHED_SEPARABLE)
     tune = list(e.info.tune_ms)
     assert min(tune) > 0 and tune[0 if e.info.fft_schedule == SCHED_CONV else 1] == min(tune)
 
@@ -142,7 +161,8 @@ def test_measure_matches_oracle(which):
 
 @pytest.mark.parametrize("scheme,sched", [(SCHEME_PC_REFERENCE, SCHED_PLANE), (SCHEME_PC_REFERENCE, SCHED_LINE),
                                           (SCHEME_EM, SCHED_PLANE), (SCHEME_SEMI_IMPLICIT, SCHED_LINE),
-                                          (SCHEME_PC_REFERENCE, SCHED_CONV)])
+                                          (SCHEME_PC_REFERENCE, SCHED_CONV), (SCHEME_PC_REFERENCE, SCHED_SEPARABLE),
+                                          (SCHEME_SEMI_IMPLICIT, SCHED_SEPARABLE)])
 def test_trajectory_matches_oracle(scheme, sched):
     print(f"trajectory scheme={scheme} sched={sched}", flush=True)
     s = c1_like(stress_mask=0x7F, scheme=scheme, fft_schedule=sched)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from ftspolysol_amd.api import (FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_RHO_C, FIELD_RHO_C_ION, FIELD_RHO_C_POLY,
-                                FIELD_RHO_P, FIELD_RHO_TYPE, SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHEME_EM,
+                                FIELD_RHO_P, FIELD_RHO_TYPE, SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SCHEME_EM,
                                 SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
 from oracle_lib import oracle_engine
 from test_oracle_pins import copolymer, random_fields
@@ -36,7 +36,7 @@ def obs_close(a, b, tol=1e-10):
         assert np.abs(a[key] - b[key]).max() < tol * max(np.abs(b[key]).max(), 1.0), key
 
 
-@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_CONV, SCHED_LINE])
+@pytest.mark.parametrize("sched", [SCHED_PLANE, SCHED_CONV, SCHED_LINE, SCHED_SEPARABLE])
 def test_strong_field_densities_and_measure(sched):
     print(f"strong fields densities+measure sched={sched}", flush=True)
     s = copolymer(stress_mask=0x7F, fft_schedule=sched)

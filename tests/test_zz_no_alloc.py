@@ -65,7 +65,8 @@ def bare(m, precision):
 def test_noalloc_bead_step_matches_bead_step(precision):
     hip = Hip()
     try:
-        for m, bond, seed in (((96, 96, 96), 1.1, 5), ((32, 32, 32), 0.8, 6)):
+        # 96^3 and 32^3 on the conv schedule, then 16 x 24 x 48 on the separable one
+        for m, bond, seed in (((96, 96, 96), 1.1, 5), ((32, 32, 32), 0.8, 6), ((16, 24, 48), 0.9, 7)):
             print(f"no-alloc bead step {m} precision={precision}", flush=True)
             e = hip_engine(bare(m, precision))
             rng = np.random.default_rng(seed)
