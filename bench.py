@@ -120,7 +120,7 @@ def lrk_totals(stats):
 
 
 def run_leg(systems, label, warmup, steps, record_steps, ckpt_every=50, max_restarts=50, lrk=None,
-            lrk_threshold=1e-3, stats=True):
+            lrk_threshold=1e-3, stats=True, min_equilibration=MIN_EQUILIBRATION):
     """One leg: equilibrate max(warmup, MIN_EQUILIBRATION) untimed steps, time exactly
     `steps` steps (recording H every step), then extend the record of H untimed to
     `record_steps` steps when it is shorter.  Returns the rate of the timed steps and the
@@ -133,7 +133,7 @@ def run_leg(systems, label, warmup, steps, record_steps, ckpt_every=50, max_rest
         if lrk is not None:
             e.lrk_set_network(lrk, lrk_threshold)
     R = len(systems)
-    equil = max(warmup, MIN_EQUILIBRATION)
+    equil = max(warmup, min_equilibration)
     t0 = time.perf_counter()
     done = 0
     while done < equil:
@@ -486,7 +486,10 @@ def main():
     ap.add_argument("--lrk-threshold", type=float, default=1e-3)
     ap.add_argument("--with-fp32-pc", action="store_true", help="add the FP32 predictor-corrector batch")
     ap.add_argument("--no-debye", action="store_true", help="skip the FP32 Debye semi-implicit batch")
-    ap.add_argument("--schedule", choices=["auto", "plane", "line", "conv"], default="auto",
+    ap.add_argument("--fast-only", action="store_true", help="run the fast leg alone and print its summary (profiling runs)")
+    ap.add_argument("--equilibration", type=int, default=MIN_EQUILIBRATION,
+                    help="least untimed steps before the timed ones (lower only for short profiling runs)")
+    ap.add_argument("--schedule", choices=["auto", "plane", "line", "conv", "separable"], default="auto",
                     help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -512,7 +515,7 @@ def main():
 
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
     lrk = load_params(args.lrk)
-    sched = {"auto": 0, "plane": 1, "line": 2, "conv": 3}[args.schedule]
+    sched = {"auto": 0, "plane": 1, "line": 2, "conv": 3, "separable": 4}[args.schedule]
     fp32 = lambda scheme, dt: [make_system(**wl, dt=dt, precision=PREC_FP32, scheme=scheme, seed=0xC4 + r,
                                            fft_schedule=sched) for r in range(args.replicas)]
     plain = lambda n: [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE,
@@ -521,14 +524,20 @@ def main():
     prec = max(args.plain_record_steps, args.steps)
     W, K = args.warmup, args.steps
 
+    EQ = dict(min_equilibration=args.equilibration)
     fast = run_leg(fp32(SCHEME_SI_LEARNED, args.dt_fast), f"fast fp32 si_learned x{args.replicas}", W, K, rec, lrk=lrk,
-                   lrk_threshold=args.lrk_threshold)
-    debye = None if args.no_debye else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_fast),
-                                               f"si_debye fp32 x{args.replicas}", W, K, rec)
-    fp32_pc = run_leg(fp32(SCHEME_PC_REFERENCE, args.dt_plain), f"fp32 pc x{args.replicas}", W, K, rec) \
-        if args.with_fp32_pc else None
-    plain_b = run_leg(plain(args.plain_replicas), f"plain fp64 pc x{args.plain_replicas}", W, K, prec)
-    plain_1 = run_leg(plain(1), "plain fp64 pc x1", W, K, 0, stats=False)
+                   lrk_threshold=args.lrk_threshold, **EQ)
+    debye = None if (args.no_debye or args.fast_only) else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_fast),
+                                               f"si_debye fp32 x{args.replicas}", W, K, rec, **EQ)
+    fp32_pc = run_leg(fp32(SCHEME_PC_REFERENCE, args.dt_plain), f"fp32 pc x{args.replicas}", W, K, rec, **EQ) \
+        if args.with_fp32_pc and not args.fast_only else None
+    if args.fast_only:
+        # a short run for profilers: the fast leg alone (no statistics beyond its record)
+        print(json.dumps({"fast": leg_summary(fast), "roofline_us_per_launch_group":
+                          1e3 * fast["timing"]["sweep_ms"] / max(1, fast["timing"]["step_launches"])}))
+        return
+    plain_b = run_leg(plain(args.plain_replicas), f"plain fp64 pc x{args.plain_replicas}", W, K, prec, **EQ)
+    plain_1 = run_leg(plain(1), "plain fp64 pc x1", W, K, 0, stats=False, **EQ)
     # one system's rate; its tau_int and <H> are those of the same dynamics in the batch
     for key in ("tau_int", "tau_int_err", "window", "H_mean", "H_stderr", "record_steps", "tau_record_samples"):
         plain_1[key] = plain_b[key]
@@ -590,7 +599,7 @@ def main():
                    "plain_fp64_batched": f"FP64 pc dt={args.dt_plain}, {args.plain_replicas} replicas per launch",
                    "plain_fp64": f"FP64 pc dt={args.dt_plain}, 1 system",
                    "statistics": f"tau_int and <H> from an H record of {rec} steps (FP32 legs) / {prec} (FP64 batch) "
-                                 f"after {max(W, MIN_EQUILIBRATION)} equilibration steps; rate from the {K} timed steps",
+                                 f"after {max(W, args.equilibration)} equilibration steps; rate from the {K} timed steps",
                    "parallelism": "single GPU"},
         "equilibration_steps": fast["equilibration_steps"],
         "tau_record_samples": fast["tau_record_samples"],
