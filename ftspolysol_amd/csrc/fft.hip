@@ -7,6 +7,7 @@
 // of 16 consecutive points of the flattened (y, z) plane, so each of its row
 // reads is one contiguous 256 B (FP64) / 128 B (FP32) segment.
 #include "fft.h"
+#include "fft_reg.h"
 
 namespace fts {
 namespace {
@@ -18,6 +19,13 @@ template <int N> constexpr int plane_threads()
     return (N * N / plan_e(N)) > 1024 ? 1024 : (N * N / plan_e(N));
 }
 template <int N1, int ZC = PENCIL_ZC> constexpr int pencil_threads() { return ZC * (N1 / plan_e(N1)); }
+// PLANE_CONV_E runs the register-split transforms where the plane length allows
+// (8 threads per line), every other mode the staged Stockham transforms
+template <int N, int MODE> constexpr bool plane_split_mode() { return MODE == PLANE_CONV_E && reg::plane_split<N>() != 0; }
+template <int N, int MODE> constexpr int plane_threads_for()
+{
+    return plane_split_mode<N, MODE>() ? reg::plane_split<N>() * N : plane_threads<N>();
+}
 
 // two consecutive complex FP32 values as one 16-byte access (16-byte aligned: every
 // field starts at a multiple of an even point count)
@@ -71,13 +79,15 @@ template <typename V> __device__ __forceinline__ void st_pair(V* p, V a, V b)
 }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j0, PlaneJob<T> j1, const PlaneJob<T>* __restrict__ table, const V2<T>* __restrict__ twg)
+__global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(PlaneJob<T> j0, PlaneJob<T> j1, const PlaneJob<T>* __restrict__ table, const V2<T>* __restrict__ twg)
 {
     using V = V2<T>;
-    constexpr int NT = plane_threads<N>();
+    constexpr int NT = plane_threads_for<N, MODE>();
+    constexpr bool SPLIT = plane_split_mode<N, MODE>();
     constexpr int LS = N + 1;
     __shared__ V lds[N * LS];
     __shared__ V tw[N];
+    __shared__ double sfac[SPLIT ? 2 * N : 1]; // the split path's y and z factors of the bond kernel
     constexpr int NRED = NT % 64 == 0 ? NT / 64 : NT;
     __shared__ double red[NRED];
     __shared__ T scale_sh;
@@ -85,6 +95,10 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     const PlaneJob<T> job = pick_job<T>(j0, j1, table);
     const size_t plane = static_cast<size_t>(blockIdx.x) * (N * N);
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
+    if constexpr (SPLIT) {
+        const double* sy = job.op.sep + gridDim.x; // sep holds the m1, m2, m3 factors; gridDim.x = m1
+        for (int t = tid; t < 2 * N; t += NT) sfac[t] = sy[t];
+    }
     constexpr bool PRODUCES_Q = MODE == PLANE_INIT || MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E;
 
     // renormalisation factor from the previous bead's per-plane norms: a max, so every
@@ -165,7 +179,31 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
     if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) renorm();
     __syncthreads();
 
-    if constexpr (MODE == PLANE_CONV_E) {
+    if constexpr (SPLIT) {
+        // the plane's share of the convolution (the pencil pass has applied phi_x / M)
+        // with the split transforms, N = N1 N2: z forward on the rows leaves slot
+        // N2 k1 + k2 holding kz = k1 + N1 k2, so column c holds kz = c / N2 + N1 (c % N2);
+        // y forward, x phi_y phi_z and y inverse on the columns (the multiply between
+        // the forward and inverse halves of step C); z inverse back to natural order
+        constexpr int N2 = reg::plane_split<N>(), N1 = N / N2;
+        const int l = tid % N, p = tid / N;
+        reg::split_a<V, N, N2, -1, LS, 1, true>(lds, tw, l, p);
+        __syncthreads();
+        reg::split_c<V, N, N2, NT, LS, 1, true, false>(lds, tw, tid, reg::NoMul{});
+        __syncthreads();
+        reg::split_a<V, N, N2, -1, 1, LS, true>(lds, tw, l, p);
+        __syncthreads();
+        const double* fy = sfac;
+        const double* fz = sfac + N;
+        reg::split_c<V, N, N2, NT, 1, LS, true, true>(lds, tw, tid, [&](int c, int ky) { return fy[ky] * fz[c / N2 + N1 * (c % N2)]; });
+        __syncthreads();
+        reg::split_a<V, N, N2, +1, 1, LS, false>(lds, tw, l, p);
+        __syncthreads();
+        reg::split_c<V, N, N2, NT, LS, 1, false, true>(lds, tw, tid, reg::NoMul{});
+        __syncthreads();
+        reg::split_a<V, N, N2, +1, LS, 1, false>(lds, tw, l, p);
+        __syncthreads();
+    } else if constexpr (MODE == PLANE_CONV_E) {
         // the plane's share of the convolution: forward along z and y, x phi_y phi_z
         // (the pencil pass has applied phi_x / M), inverse below
         lds_fft<T, N, -1, N, LS, 1, true, NT>(lds, tw, tid);
@@ -189,7 +227,7 @@ __global__ void __launch_bounds__(plane_threads<N>()) plane_kernel(PlaneJob<T> j
         }
         __syncthreads();
     }
-    if constexpr (MODE == PLANE_INV || MODE == PLANE_INV_F2D || MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
+    if constexpr (MODE == PLANE_INV || MODE == PLANE_INV_F2D || MODE == PLANE_STEP || MODE == PLANE_LAST || (MODE == PLANE_CONV_E && !SPLIT)) {
         lds_fft<T, N, +1, N, LS, 1, true, NT>(lds, tw, tid); // along z
         lds_fft<T, N, +1, N, 1, LS, true, NT>(lds, tw, tid);  // along y
     }
@@ -409,7 +447,7 @@ hipError_t plane_go(const FftGeom& g, const PlaneJob<T>* jobs, const PlaneJob<T>
 {
     dim3 grid(g.m1, njobs);
     const PlaneJob<T> none{};
-    hipLaunchKernelGGL((plane_kernel<T, N, MODE>), grid, dim3(plane_threads<N>()), 0, s, jobs ? jobs[0] : none,
+    hipLaunchKernelGGL((plane_kernel<T, N, MODE>), grid, dim3(plane_threads_for<N, MODE>()), 0, s, jobs ? jobs[0] : none,
                        jobs ? jobs[njobs > 1 ? 1 : 0] : none, table, static_cast<const V2<T>*>(g.tw_yz));
     return hipGetLastError();
 }

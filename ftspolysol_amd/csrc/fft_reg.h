@@ -123,5 +123,81 @@ template <> struct LineSplit<192> { static constexpr int N1 = 24, N2 = 8; };
 template <> struct LineSplit<256> { static constexpr int N1 = 32, N2 = 8; };
 template <int N> constexpr bool reg_line_ok() { return LineSplit<N>::N2 != 0; }
 
+// ---- split transforms of the lines of a tile in LDS ------------------------------
+// Line l, element t at lds[l * RS + t * ES]; N = N1 * N2.  Step A runs on thread
+// (l, p), p < N2, over the thread's own slots N2 j + p: DFT_N1 over j in registers,
+// then (forward) the twiddle W_N^{p k1}, written back to slot N2 k1 + p.  Step C
+// runs on work items (l, k1), k1 < N1, over the contiguous slots N2 k1 + m: the
+// forward DFT_N2 over m leaves X[k1 + N1 k2] in slot N2 k1 + k2 (digit-reversed
+// order); an optional k-space multiply; the inverse DFT_N2 over k2 and the twiddle
+// W_N^{-p k1} take the values back to slot N2 k1 + p, where the inverse step A (no
+// twiddle) finishes the line in natural order.  Each step reads and writes only
+// slots that no other thread touches in that step, so a barrier separates steps
+// and none is needed inside one.
+template <typename V, int N, int N2, int SIGN, int RS, int ES, bool TWID>
+__device__ __forceinline__ void split_a(V* lds, const V* tw, int l, int p)
+{
+    constexpr int N1 = N / N2;
+    V x[N1];
+#pragma unroll
+    for (int j = 0; j < N1; ++j) x[j] = lds[l * RS + (N2 * j + p) * ES];
+    RegDft<V, N1, SIGN>::run(x);
+    if constexpr (TWID) {
+        if (p != 0) {
+#pragma unroll
+            for (int k1 = 1; k1 < N1; ++k1) {
+                V t = tw[p * k1];
+                if (SIGN > 0) t.y = -t.y;
+                x[k1] = cmul(x[k1], t);
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) lds[l * RS + (N2 * k1 + p) * ES] = x[k1];
+}
+
+// mul(l, k): the k-space factor of line l at frequency k (FWD and INV together)
+template <typename V, int N, int N2, int NT, int RS, int ES, bool FWD, bool INV, typename F>
+__device__ __forceinline__ void split_c(V* lds, const V* tw, int tid, F mul)
+{
+    using T = decltype(V::x);
+    constexpr int N1 = N / N2, ITEMS = N * N1, IT = (ITEMS + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * NT;
+        if (ITEMS % NT == 0 || w < ITEMS) {
+            const int l = w % N, k1 = w / N;
+            V z[N2];
+#pragma unroll
+            for (int m = 0; m < N2; ++m) z[m] = lds[l * RS + (N2 * k1 + m) * ES];
+            if constexpr (FWD) RegDft<V, N2, -1>::run(z);
+            if constexpr (FWD && INV) {
+#pragma unroll
+                for (int k2 = 0; k2 < N2; ++k2) z[k2] = cscale(z[k2], static_cast<T>(mul(l, k1 + N1 * k2)));
+            }
+            if constexpr (INV) {
+                RegDft<V, N2, +1>::run(z);
+                if (k1 != 0) {
+#pragma unroll
+                    for (int pp = 1; pp < N2; ++pp) {
+                        V t = tw[pp * k1];
+                        t.y = -t.y;
+                        z[pp] = cmul(z[pp], t);
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < N2; ++m) lds[l * RS + (N2 * k1 + m) * ES] = z[m];
+        }
+    }
+}
+
+struct NoMul {
+    __device__ double operator()(int, int) const { return 1.0; }
+};
+
+// the plane pass's split N = (N / 8) * 8 (0: none): 8 N threads per plane
+template <int N> constexpr int plane_split() { return (N % 8 == 0 && N >= 16 && 8 * N <= 1024) ? 8 : 0; }
+
 } // namespace reg
 } // namespace fts
