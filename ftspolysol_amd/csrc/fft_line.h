@@ -10,6 +10,14 @@ namespace fts {
 namespace line_detail {
 
 template <int N> constexpr int line_threads() { return LINE_TC * (N / plan_e(N)); }
+// the separable schedule's passes (CONV, CONV_E) run the register-split transforms
+// N = (N / 8) * 8 up to N = 96 (0: the staged Stockham transforms).  Measured on
+// the 16-line tiles: at 96 the split is faster (FP32 16-job bead step 247 -> 206 us,
+// FP64 one job 53 -> 44 us); at 256 it is slower (a bead step 582 us staged against
+// 632 us with a 16 x 16 split and 995 us with 32 x 8)
+template <int N> constexpr int line_split() { return (N % 8 == 0 && N >= 16 && N <= 96) ? 8 : 0; }
+template <int N, int MODE> constexpr bool line_split_mode() { return (MODE == LINE_CONV || MODE == LINE_CONV_E) && line_split<N>() != 0; }
+template <int N, int MODE> constexpr int line_threads_for() { return line_split_mode<N, MODE>() ? LINE_TC * line_split<N>() : line_threads<N>(); }
 
 __device__ __forceinline__ size_t line_base(int axis, int j, int m2, int m3)
 {
@@ -79,10 +87,11 @@ __device__ __forceinline__ double line_kop_value(const KOp& op, const FftGeom& g
 }
 
 template <typename T, int N, int AXIS, int MODE>
-__global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, LineJob<T> j1, const LineJob<T>* __restrict__ table, FftGeom g, const V2<T>* __restrict__ twg)
+__global__ void __launch_bounds__((line_threads_for<N, MODE>())) line_kernel(LineJob<T> j0, LineJob<T> j1, const LineJob<T>* __restrict__ table, FftGeom g, const V2<T>* __restrict__ twg)
 {
     using V = V2<T>;
-    constexpr int NT = line_threads<N>();
+    constexpr int NT = line_threads_for<N, MODE>();
+    constexpr bool SPLIT = line_split_mode<N, MODE>();
     constexpr int TC = LINE_TC;
     constexpr bool ZL = AXIS == 2; // z lines: LDS rows are lines ([c][t]); otherwise [t][c]
     constexpr int LS = ZL ? N + 1 : TC + 1;
@@ -132,12 +141,29 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
     constexpr bool CONV = MODE == LINE_CONV || MODE == LINE_CONV_E;
     constexpr bool DO_INV = MODE == LINE_INV || MODE == LINE_INV_E_FWD || MODE == LINE_INV_E;
     constexpr bool MUL = MODE == LINE_FWD_MUL_INV || MODE == LINE_FWD_MUL_INV_NOISE;
-    constexpr bool DO_FWD_FIRST = MODE == LINE_FWD || MUL || CONV;
+    constexpr bool DO_FWD_FIRST = MODE == LINE_FWD || MUL || (CONV && !SPLIT);
+    if constexpr (SPLIT) {
+        // FFT, x the line axis' factor of the separable kernel and op.c0, IFFT: split
+        // step A forward, step C (forward, multiply, inverse), step A inverse; line l
+        // of the tile at lds[l * RS + t * ES]
+        constexpr int N2 = line_split<N>();
+        constexpr int RS = ZL ? LS : 1, ES = ZL ? 1 : LS;
+        const int off = AXIS == 0 ? 0 : AXIS == 1 ? g.m1 : g.m1 + g.m2;
+        const double* f = job.op.sep + off;
+        const double c0 = job.op.c0;
+        const int l = tid % TC, p = tid / TC;
+        reg::split_a<V, N, N2, -1, RS, ES, true>(lds, tw, l, p);
+        __syncthreads();
+        reg::split_c<V, N, N2, NT, RS, ES, true, true, TC>(lds, tw, tid, [&](int, int k) { return f[k] * c0; });
+        __syncthreads();
+        reg::split_a<V, N, N2, +1, RS, ES, false>(lds, tw, l, p);
+        __syncthreads();
+    }
     if constexpr (DO_FWD_FIRST) {
         if constexpr (ZL) lds_fft<T, N, -1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, -1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
-    if constexpr (CONV) {
+    if constexpr (CONV && !SPLIT) {
         // x the line axis' factor of the separable kernel (op.sep holds the m1, m2, m3
         // factors) and op.c0
         const int off = AXIS == 0 ? 0 : AXIS == 1 ? g.m1 : g.m1 + g.m2;
@@ -178,7 +204,7 @@ __global__ void __launch_bounds__(line_threads<N>()) line_kernel(LineJob<T> j0, 
         }
         __syncthreads();
     }
-    if constexpr (DO_INV || MUL || CONV) {
+    if constexpr (DO_INV || MUL || (CONV && !SPLIT)) {
         if constexpr (ZL) lds_fft<T, N, +1, TC, LS, 1, true, NT>(lds, tw, tid);
         else lds_fft<T, N, +1, TC, 1, LS, true, NT>(lds, tw, tid);
     }
@@ -331,7 +357,7 @@ hipError_t line_go(const FftGeom& g, const void* tw, const LineJob<T>* jobs, con
     const size_t ncols = AXIS == 0 ? static_cast<size_t>(g.m2) * g.m3 : AXIS == 1 ? static_cast<size_t>(g.m1) * g.m3 : static_cast<size_t>(g.m1) * g.m2;
     dim3 grid(static_cast<unsigned>(ncols / LINE_TC), njobs);
     const LineJob<T> none{};
-    hipLaunchKernelGGL((line_kernel<T, N, AXIS, MODE>), grid, dim3(line_threads<N>()), 0, s, jobs ? jobs[0] : none,
+    hipLaunchKernelGGL((line_kernel<T, N, AXIS, MODE>), grid, dim3(line_threads_for<N, MODE>()), 0, s, jobs ? jobs[0] : none,
                        jobs ? jobs[njobs > 1 ? 1 : 0] : none, table, g, static_cast<const V2<T>*>(tw));
     return hipGetLastError();
 }

@@ -156,17 +156,18 @@ __device__ __forceinline__ void split_a(V* lds, const V* tw, int l, int p)
     for (int k1 = 0; k1 < N1; ++k1) lds[l * RS + (N2 * k1 + p) * ES] = x[k1];
 }
 
-// mul(l, k): the k-space factor of line l at frequency k (FWD and INV together)
-template <typename V, int N, int N2, int NT, int RS, int ES, bool FWD, bool INV, typename F>
+// NL lines (the tile's, N by default); mul(l, k): the k-space factor of line l at
+// frequency k (FWD and INV together)
+template <typename V, int N, int N2, int NT, int RS, int ES, bool FWD, bool INV, int NL = N, typename F>
 __device__ __forceinline__ void split_c(V* lds, const V* tw, int tid, F mul)
 {
     using T = decltype(V::x);
-    constexpr int N1 = N / N2, ITEMS = N * N1, IT = (ITEMS + NT - 1) / NT;
+    constexpr int N1 = N / N2, ITEMS = NL * N1, IT = (ITEMS + NT - 1) / NT;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int w = tid + it * NT;
         if (ITEMS % NT == 0 || w < ITEMS) {
-            const int l = w % N, k1 = w / N;
+            const int l = w % NL, k1 = w / NL;
             V z[N2];
 #pragma unroll
             for (int m = 0; m < N2; ++m) z[m] = lds[l * RS + (N2 * k1 + m) * ES];
