@@ -216,7 +216,7 @@ struct fts_engine {
     int fft32_n(int kind, const FftJob* jobs, int n); // kind: 0 apply, 1 apply from a spectrum, 2 forward
     int fft_apply_hat_n(const FftJob* jobs, int n);
     int fft_forward_n(const FftJob* jobs, int n);
-    template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst);
+    template <typename T> int fft_forward_t(const V2<T>* src, V2<T>* dst, const V2<T>* src2 = nullptr, V2<T>* dst2 = nullptr);
     int reduce_finalize(int K, int slot, double scale);
     int sum_field(const double2* a, int slot, double scale);
     int remove_mean(double2* a);

@@ -697,22 +697,25 @@ int fts_engine::fft_apply_hat(const double2* hat, double2* dst, const KOp& op)
     return FTS_OK;
 }
 
-template <typename T> int fts_engine::fft_forward_t(const V2<T>* src, V2<T>* dst)
+// forward transforms src -> dst, and src2 -> dst2 in the same launches when given
+template <typename T> int fts_engine::fft_forward_t(const V2<T>* src, V2<T>* dst, const V2<T>* src2, V2<T>* dst2)
 {
     const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
+    const int n = src2 ? 2 : 1;
     if (plane_ok) {
-        PlaneJob<T> pj{src, dst, nullptr, nullptr};
-        HIP_TRY(launch_plane<T>(g, PLANE_FWD, &pj, 1, stream));
-        PencilJob<T> xj{dst, KOp{}};
-        HIP_TRY(launch_pencil<T>(g, PENCIL_FWD, &xj, 1, stream));
+        PlaneJob<T> pj[2] = {{src, dst, nullptr, nullptr}, {src2, dst2, nullptr, nullptr}};
+        HIP_TRY(launch_plane<T>(g, PLANE_FWD, pj, n, stream));
+        PencilJob<T> xj[2] = {{dst, KOp{}}, {dst2, KOp{}}};
+        HIP_TRY(launch_pencil<T>(g, PENCIL_FWD, xj, n, stream));
         return FTS_OK;
     }
-    if (dst != src) HIP_TRY(hipMemcpyAsync(dst, src, sizeof(V2<T>) * M, hipMemcpyDeviceToDevice, stream));
     const bool d = sizeof(T) == 8;
-    LineJob<T> j{dst};
-    HIP_TRY(launch_line<T>(g, 2, LINE_FWD, d ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p), &j, 1, stream));
-    HIP_TRY(launch_line<T>(g, 1, LINE_FWD, d ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p), &j, 1, stream));
-    HIP_TRY(launch_line<T>(g, 0, LINE_FWD, d ? static_cast<const void*>(tw64_x.p) : static_cast<const void*>(tw32_x.p), &j, 1, stream));
+    // the z pass reads src and writes dst; y and x transform dst in place
+    LineJob<T> zj[2] = {{dst, nullptr, nullptr, KOp{}, nullptr, nullptr, nullptr, src}, {dst2, nullptr, nullptr, KOp{}, nullptr, nullptr, nullptr, src2}};
+    LineJob<T> j[2] = {{dst}, {dst2}};
+    HIP_TRY(launch_line<T>(g, 2, LINE_FWD, d ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p), zj, n, stream));
+    HIP_TRY(launch_line<T>(g, 1, LINE_FWD, d ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p), j, n, stream));
+    HIP_TRY(launch_line<T>(g, 0, LINE_FWD, d ? static_cast<const void*>(tw64_x.p) : static_cast<const void*>(tw32_x.p), j, n, stream));
     return FTS_OK;
 }
 
@@ -1409,9 +1412,9 @@ template <typename T> int fts_engine::stress_chain(int p)
     Cv* qb = static_cast<Cv*>(qB);
     double2* rows = stress_acc.p + stress_chain_base(p) * 7;
     for (int s = 0; s + 1 < sp.N; ++s) {
-        int rc = fft_forward_t<T>(qf + static_cast<size_t>(s) * Ms, static_cast<Cv*>(kA));
-        if (rc) return rc;
-        rc = fft_forward_t<T>(sp.mirror ? qf + static_cast<size_t>(sp.N - 2 - s) * Ms : qb + static_cast<size_t>(s + 1) * Ms, static_cast<Cv*>(kB));
+        // both spectra of bead s in one pair of launches
+        int rc = fft_forward_t<T>(qf + static_cast<size_t>(s) * Ms, static_cast<Cv*>(kA),
+                                  sp.mirror ? qf + static_cast<size_t>(sp.N - 2 - s) * Ms : qb + static_cast<size_t>(s + 1) * Ms, static_cast<Cv*>(kB));
         if (rc) return rc;
         const double* sep = sep_bond.p + static_cast<size_t>(sp.bond_cls[static_cast<size_t>(s)]) * (m1 + m2 + m3);
         if ((rc = contract_to(kB, kA, sizeof(T) == 4, sep, rows + static_cast<size_t>(s) * 7))) return rc;

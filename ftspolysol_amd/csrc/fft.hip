@@ -26,6 +26,12 @@ template <int N, int MODE> constexpr int plane_threads_for()
 {
     return plane_split_mode<N, MODE>() ? reg::plane_split<N>() * N : plane_threads<N>();
 }
+// PENCIL_CONV likewise, with 8 threads per x line, up to N1 = 96
+template <int N1, int MODE> constexpr bool pencil_split_mode() { return MODE == PENCIL_CONV && N1 % 8 == 0 && N1 >= 16 && N1 <= 96; }
+template <int N1, int MODE, int ZC> constexpr int pencil_threads_for()
+{
+    return pencil_split_mode<N1, MODE>() ? ZC * 8 : pencil_threads<N1, ZC>();
+}
 
 // two consecutive complex FP32 values as one 16-byte access (16-byte aligned: every
 // field starts at a multiple of an even point count)
@@ -324,10 +330,11 @@ __device__ __forceinline__ double kop_value(const KOp& op, const double* kvec, i
 // ZCW: (y, z) columns per workgroup; the bead-step multiplies of batches take 32
 // (256-byte row segments), everything else PENCIL_ZC
 template <typename T, int N1, int MODE, int ZCW>
-__global__ void __launch_bounds__((pencil_threads<N1, ZCW>())) pencil_kernel(PencilJob<T> j0, PencilJob<T> j1, const PencilJob<T>* __restrict__ table, FftGeom g)
+__global__ void __launch_bounds__((pencil_threads_for<N1, MODE, ZCW>())) pencil_kernel(PencilJob<T> j0, PencilJob<T> j1, const PencilJob<T>* __restrict__ table, FftGeom g)
 {
     using V = V2<T>;
-    constexpr int NT = pencil_threads<N1, ZCW>();
+    constexpr int NT = pencil_threads_for<N1, MODE, ZCW>();
+    constexpr bool SPLIT = pencil_split_mode<N1, MODE>();
     constexpr int ZC = ZCW;
     constexpr int LS = ZC + 1;
     __shared__ V lds[N1 * LS];
@@ -365,9 +372,22 @@ __global__ void __launch_bounds__((pencil_threads<N1, ZCW>())) pencil_kernel(Pen
         }
     }
     __syncthreads();
-    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_F2D || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD_MUL_INV_NOISE || MODE == PENCIL_CONV)
+    if constexpr (SPLIT) {
+        // FFT along x, x phi_x(k) c0, IFFT with the split transforms: column c of the
+        // tile is line c at lds[c + x LS]
+        const double* sxf = job.op.sep;
+        const double c0 = job.op.c0;
+        const int l = tid % ZC, p = tid / ZC;
+        reg::split_a<V, N1, 8, -1, 1, LS, true>(lds, tw, l, p);
+        __syncthreads();
+        reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, [&](int, int k) { return sxf[k] * c0; });
+        __syncthreads();
+        reg::split_a<V, N1, 8, +1, 1, LS, false>(lds, tw, l, p);
+        __syncthreads();
+    }
+    if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_F2D || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD_MUL_INV_NOISE || (MODE == PENCIL_CONV && !SPLIT))
         lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
-    if constexpr (MODE == PENCIL_CONV) {
+    if constexpr (MODE == PENCIL_CONV && !SPLIT) {
         const double* sxf = job.op.sep;
         const double c0 = job.op.c0;
 #pragma unroll 4
@@ -417,7 +437,7 @@ __global__ void __launch_bounds__((pencil_threads<N1, ZCW>())) pencil_kernel(Pen
         __syncthreads();
     }
     if constexpr (MODE == PENCIL_INV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD_MUL_INV_NOISE || MODE == PENCIL_MUL_INV || MODE == PENCIL_MUL_INV_D2F ||
-                  MODE == PENCIL_CONV)
+                  (MODE == PENCIL_CONV && !SPLIT))
         lds_fft<T, N1, +1, ZC, 1, LS, true, NT>(lds, tw, tid);
     if constexpr (MODE == PENCIL_FWD_F2D) {
 #pragma unroll 4
@@ -474,7 +494,7 @@ hipError_t pencil_launch(const FftGeom& g, const PencilJob<T>* jobs, const Penci
     const size_t tiles = static_cast<size_t>(g.m2) * g.m3 / ZC;
     dim3 grid(static_cast<unsigned>(tiles), njobs);
     const PencilJob<T> none{};
-    hipLaunchKernelGGL((pencil_kernel<T, N1, MODE, ZC>), grid, dim3(pencil_threads<N1, ZC>()), 0, s, jobs ? jobs[0] : none,
+    hipLaunchKernelGGL((pencil_kernel<T, N1, MODE, ZC>), grid, dim3(pencil_threads_for<N1, MODE, ZC>()), 0, s, jobs ? jobs[0] : none,
                        jobs ? jobs[njobs > 1 ? 1 : 0] : none, table, g);
     return hipGetLastError();
 }
