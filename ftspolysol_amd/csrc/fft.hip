@@ -93,7 +93,7 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     constexpr int LS = N + 1;
     __shared__ V lds[N * LS];
     __shared__ V tw[N];
-    __shared__ double sfac[SPLIT ? 2 * N : 1]; // the split path's y and z factors of the bond kernel
+    __shared__ T sfac[SPLIT ? 2 * N : 1]; // the split path's y and z factors of the bond kernel
     constexpr int NRED = NT % 64 == 0 ? NT / 64 : NT;
     __shared__ double red[NRED];
     __shared__ T scale_sh;
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
     if constexpr (SPLIT) {
         const double* sy = job.op.sep + gridDim.x; // sep holds the m1, m2, m3 factors; gridDim.x = m1
-        for (int t = tid; t < 2 * N; t += NT) sfac[t] = sy[t];
+        for (int t = tid; t < 2 * N; t += NT) sfac[t] = static_cast<T>(sy[t]);
     }
     constexpr bool PRODUCES_Q = MODE == PLANE_INIT || MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E;
 
@@ -144,8 +144,13 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     NormAcc<T> nrm;
 
     const V* src = (MODE == PLANE_INIT) ? job.emw : job.src;
-    // FP32: pairs of points per 16-byte access
+    // FP32: pairs of points per 16-byte access.  Pair j = tid + k NT starts at plane
+    // index 2 j; when 2 NT is a multiple of N, all pairs of a thread share one column
+    // and lie 2 NT / N rows apart (COLFIX: the slot without a division per pair)
     constexpr bool PAIRS = sizeof(T) == 4 && N % 2 == 0;
+    constexpr bool COLFIX = PAIRS && (2 * NT) % N == 0 && (N * N / 2) % NT == 0;
+    const int slot0 = (2 * tid / N) * LS + (2 * tid) % N;
+    auto pair_slot = [&](int k, int i) { return COLFIX ? slot0 + k * ((2 * NT / N) * LS) : (i / N) * LS + (i % N); };
     if constexpr (MODE == PLANE_FWD_D2F) {
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) {
@@ -154,7 +159,7 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
         }
     } else if constexpr (PAIRS) {
 #pragma unroll
-        for (int j = tid; j < N * N / 2; j += NT) {
+        for (int k = 0, j = tid; j < N * N / 2; ++k, j += NT) {
             const int i = 2 * j;
             V v0, v1;
             if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) ld_pair_stream(src + plane + i, v0, v1);
@@ -166,7 +171,7 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
                 nrm.add(v0);
                 nrm.add(v1);
             }
-            V* l = &lds[(i / N) * LS + (i % N)];
+            V* l = &lds[pair_slot(k, i)];
             l[0] = v0;
             l[1] = v1;
         }
@@ -199,8 +204,8 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
         __syncthreads();
         reg::split_a<V, N, N2, -1, 1, LS, true>(lds, tw, l, p);
         __syncthreads();
-        const double* fy = sfac;
-        const double* fz = sfac + N;
+        const T* fy = sfac;
+        const T* fz = sfac + N;
         reg::split_c<V, N, N2, NT, 1, LS, true, true>(lds, tw, tid, [&](int c, int ky) { return fy[ky] * fz[c / N2 + N1 * (c % N2)]; });
         __syncthreads();
         reg::split_a<V, N, N2, +1, 1, LS, false>(lds, tw, l, p);
@@ -239,9 +244,9 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     }
     if constexpr (PAIRS && (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E)) {
 #pragma unroll 2
-        for (int j = tid; j < N * N / 2; j += NT) {
+        for (int k = 0, j = tid; j < N * N / 2; ++k, j += NT) {
             const int i = 2 * j;
-            V* p = &lds[(i / N) * LS + (i % N)];
+            V* p = &lds[pair_slot(k, i)];
             V e0, e1;
             ld_pair(job.emw + plane + i, e0, e1);
             V v0 = cmul(p[0], e0), v1 = cmul(p[1], e1);
@@ -249,13 +254,16 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
                 v0 = cscale(v0, c);
                 v1 = cscale(v1, c);
             }
-            p[0] = v0;
-            p[1] = v1;
+            if constexpr (MODE == PLANE_STEP) {
+                // the forward transform below reads q_{s+1} back from LDS
+                p[0] = v0;
+                p[1] = v1;
+            }
             st_pair_stream(job.qout + plane + i, v0, v1);
             nrm.add(v0);
             nrm.add(v1);
         }
-        __syncthreads();
+        if constexpr (MODE == PLANE_STEP) __syncthreads();
     } else if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) {
@@ -339,12 +347,16 @@ __global__ void __launch_bounds__((pencil_threads_for<N1, MODE, ZCW>())) pencil_
     constexpr int LS = ZC + 1;
     __shared__ V lds[N1 * LS];
     __shared__ V tw[N1];
+    __shared__ T fx[SPLIT ? N1 : 1]; // the split path's phi_x(k) c0
     const int tid = threadIdx.x;
     const PencilJob<T> job = pick_job<T>(j0, j1, table);
     const size_t sx = static_cast<size_t>(g.m2) * g.m3;
     const size_t jbase = static_cast<size_t>(blockIdx.x) * ZC;
     const V* twg = static_cast<const V*>(g.tw_x);
-    for (int t = tid; t < N1; t += NT) tw[t] = twg[t];
+    for (int t = tid; t < N1; t += NT) {
+        tw[t] = twg[t];
+        if constexpr (SPLIT) fx[t] = static_cast<T>(job.op.sep[t] * job.op.c0);
+    }
     V* data = job.data;
     const V* in = job.src ? job.src : data;
     // FP32: pairs of columns per 16-byte access (ZC and sx are even)
@@ -375,12 +387,10 @@ __global__ void __launch_bounds__((pencil_threads_for<N1, MODE, ZCW>())) pencil_
     if constexpr (SPLIT) {
         // FFT along x, x phi_x(k) c0, IFFT with the split transforms: column c of the
         // tile is line c at lds[c + x LS]
-        const double* sxf = job.op.sep;
-        const double c0 = job.op.c0;
         const int l = tid % ZC, p = tid / ZC;
         reg::split_a<V, N1, 8, -1, 1, LS, true>(lds, tw, l, p);
         __syncthreads();
-        reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, [&](int, int k) { return sxf[k] * c0; });
+        reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, [&](int, int k) { return fx[k]; });
         __syncthreads();
         reg::split_a<V, N1, 8, +1, 1, LS, false>(lds, tw, l, p);
         __syncthreads();

@@ -100,11 +100,15 @@ __global__ void __launch_bounds__((line_threads_for<N, MODE>())) line_kernel(Lin
     __shared__ V lds[ZL ? TC * (N + 1) : N * (TC + 1)];
     __shared__ V tw[N];
     __shared__ double red[NRED];
+    __shared__ T fk[SPLIT ? N : 1]; // the split path's factor of the line axis times op.c0
     const int tid = threadIdx.x;
     const LineJob<T> job = pick_job<T>(j0, j1, table);
     const int col0 = blockIdx.x * TC;
     const size_t S = AXIS == 0 ? static_cast<size_t>(g.m2) * g.m3 : AXIS == 1 ? static_cast<size_t>(g.m3) : 1;
-    for (int t = tid; t < N; t += NT) tw[t] = twg[t];
+    for (int t = tid; t < N; t += NT) {
+        tw[t] = twg[t];
+        if constexpr (SPLIT) fk[t] = static_cast<T>(job.op.sep[(AXIS == 0 ? 0 : AXIS == 1 ? g.m1 : g.m1 + g.m2) + t] * job.op.c0);
+    }
 
     T c = T(1);
     if constexpr (PRODUCES_Q) {
@@ -148,13 +152,10 @@ __global__ void __launch_bounds__((line_threads_for<N, MODE>())) line_kernel(Lin
         // of the tile at lds[l * RS + t * ES]
         constexpr int N2 = line_split<N>();
         constexpr int RS = ZL ? LS : 1, ES = ZL ? 1 : LS;
-        const int off = AXIS == 0 ? 0 : AXIS == 1 ? g.m1 : g.m1 + g.m2;
-        const double* f = job.op.sep + off;
-        const double c0 = job.op.c0;
         const int l = tid % TC, p = tid / TC;
         reg::split_a<V, N, N2, -1, RS, ES, true>(lds, tw, l, p);
         __syncthreads();
-        reg::split_c<V, N, N2, NT, RS, ES, true, true, TC>(lds, tw, tid, [&](int, int k) { return f[k] * c0; });
+        reg::split_c<V, N, N2, NT, RS, ES, true, true, TC>(lds, tw, tid, [&](int, int k) { return fk[k]; });
         __syncthreads();
         reg::split_a<V, N, N2, +1, RS, ES, false>(lds, tw, l, p);
         __syncthreads();
