@@ -120,12 +120,13 @@ def lrk_totals(stats):
 
 
 def run_leg(systems, label, warmup, steps, record_steps, ckpt_every=50, max_restarts=50, lrk=None,
-            lrk_threshold=1e-3, stats=True, min_equilibration=MIN_EQUILIBRATION):
+            lrk_threshold=1e-3, stats=True, min_equilibration=MIN_EQUILIBRATION, graph_level=2):
     """One leg: equilibrate max(warmup, MIN_EQUILIBRATION) untimed steps, time exactly
     `steps` steps (recording H every step), then extend the record of H untimed to
     `record_steps` steps when it is shorter.  Returns the rate of the timed steps and the
     statistics of the record."""
     b = hip_batch(systems)
+    b.set_graph(graph_level)
     for r, (e, s) in enumerate(zip(b.replicas, systems)):
         om, ps = initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r)
         e.set_fields(om, ps)
@@ -487,6 +488,8 @@ def main():
     ap.add_argument("--with-fp32-pc", action="store_true", help="add the FP32 predictor-corrector batch")
     ap.add_argument("--no-debye", action="store_true", help="skip the FP32 Debye semi-implicit batch")
     ap.add_argument("--fast-only", action="store_true", help="run the fast leg alone and print its summary (profiling runs)")
+    ap.add_argument("--graph-level", type=int, choices=[0, 1, 2], default=2,
+                    help="batch launches: 2 replays each whole CL step as a captured graph, 1 only the sweeps, 0 none")
     ap.add_argument("--equilibration", type=int, default=MIN_EQUILIBRATION,
                     help="least untimed steps before the timed ones (lower only for short profiling runs)")
     ap.add_argument("--schedule", choices=["auto", "plane", "line", "conv", "separable"], default="auto",
@@ -524,7 +527,7 @@ def main():
     prec = max(args.plain_record_steps, args.steps)
     W, K = args.warmup, args.steps
 
-    EQ = dict(min_equilibration=args.equilibration)
+    EQ = dict(min_equilibration=args.equilibration, graph_level=args.graph_level)
     fast = run_leg(fp32(SCHEME_SI_LEARNED, args.dt_fast), f"fast fp32 si_learned x{args.replicas}", W, K, rec, lrk=lrk,
                    lrk_threshold=args.lrk_threshold, **EQ)
     debye = None if (args.no_debye or args.fast_only) else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_fast),
@@ -578,7 +581,7 @@ def main():
             return None
         return {"ratio": x["samples_per_s"] / y["samples_per_s"], "rel_err": ratio_err(x, y)}
 
-    sched_name = {1: "plane", 2: "line", 3: "conv"}
+    sched_name = {1: "plane", 2: "line", 3: "conv", 4: "separable"}
     out = {
         "metric": "decorrelated CL samples/s at 96^3 N=100, FP32+learned vs plain FP64, <H> err",
         "value": fast["samples_per_s"],
