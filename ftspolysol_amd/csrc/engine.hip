@@ -1507,8 +1507,12 @@ int fts_engine::measure_queue(const double2* om, const double2* ps, fts_observab
             if ((rc = pair(f(W_type_s, al), f(rho_hat, al), static_cast<size_t>(al)))) return rc;
         if (electro()) {
             if ((rc = pair(psi_s.p, rho_c_hat.p, static_cast<size_t>(nT)))) return rc;
+            // the ion rows contract psi~_j with the normalised ion density rho_j = (n_j/V) b_j / Q_j,
+            // not with b_j: the same sum by linearity, and b_j = e^{-i z_j psi~_j} leaves the FP32
+            // range of the FP32 form's transforms where |Im psi~_j| is large (~e^184 measured in a
+            // rare C2 excursion) while rho_j stays of the order of the density
             for (size_t j = 0; j < ions.size(); ++j)
-                if ((rc = pair(f(ion_psi, j), f(ion_b, j), static_cast<size_t>(nT) + 1 + j))) return rc;
+                if ((rc = pair(f(ion_psi, j), f(ion_rho, j), static_cast<size_t>(nT) + 1 + j))) return rc;
         }
         HIP_TRY(hipMemcpyAsync(h_acc, stress_acc.p, sizeof(double2) * stress_acc.n, hipMemcpyDeviceToHost, stream));
     }
@@ -1643,12 +1647,11 @@ int fts_engine::measure_finish(fts_observables* rec)
             sig[c].x += -v.y; // i * v
             sig[c].y += v.x;
         }
-        // ions: i (n_j z_j / (V Q_j)) [ -a_j^2/M^2 S_ab(psi~^_j, b^_j) + (delta/6) S_1/M^2 ]
+        // ions: i (n_j z_j / (V Q_j)) [ -a_j^2/M^2 S_ab(psi~^_j, b^_j) + (delta/6) S_1/M^2 ], with
+        // (n_j / (V Q_j)) b_j = rho_j contracted directly: i z_j [ ... S(psi~^_j, rho^_j) ... ]
         for (size_t j = 0; j < ions.size(); ++j) {
             const double2* Sj = fixed + (static_cast<size_t>(nT) + 1 + j) * 7;
-            const double2 inv = cinv_host(Qi[j]);
-            const double sc = static_cast<double>(ions[j].n) * ions[j].z / V;
-            const double2 pref{-inv.y * sc, inv.x * sc}; // i * inv * sc
+            const double2 pref{0.0, ions[j].z}; // i z_j
             const double aj2 = ions[j].a * ions[j].a;
             for (int c = 0; c < 6; ++c) {
                 const double d = c < 3 ? 1.0 / 6.0 : 0.0;

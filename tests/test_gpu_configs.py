@@ -126,3 +126,29 @@ def test_precision_form_64():
         assert np.abs(a["mu_ex_poly"] - b["mu_ex_poly"]).max() <= 1e-4 * max(1.0, np.abs(b["mu_ex_poly"]).max())
     for e in (g64, g32, probe):
         e.close()
+
+
+def test_fp32_stress_finite_where_ion_boltzmann_factors_leave_fp32_range():
+    """A strong imaginary psi mode (amplitude 150) makes the ions' Boltzmann factors
+    b_j = e^{-i z_j psi~_j} reach about e^150, beyond FP32 (about e^88), as in rare CL
+    excursions of C2.  The FP32 form must still give a finite stress tensor, within its
+    precision bar of the FP64 engine on the same fields (the ion rows contract the
+    normalised ion density, which stays of the order of the density)."""
+    print("strong imaginary psi mode, FP32 vs FP64 measure", flush=True)
+    mk = lambda prec: config(1, precision=prec, scheme=SCHEME_SEMI_IMPLICIT, stress_mask=0x7F, seed=0xC1)
+    s64 = mk(PREC_FP64)
+    n = s64.m[0]
+    x = np.arange(n)
+    psi = 150j * np.cos(2 * np.pi * x / n)[:, None, None] * np.ones(s64.m)
+    om = initial_fields(s64, sigma=0.05, cfg=1)[0]
+    out = []
+    for prec in (PREC_FP64, PREC_FP32):
+        e = hip_engine(mk(prec))
+        e.set_fields(om, psi)
+        out.append(e.measure())
+        e.close()
+    b, a = out
+    assert np.all(np.isfinite(b["stress"])) and np.isfinite(b["p_iso"])
+    assert np.all(np.isfinite(a["stress"])) and np.isfinite(a["p_iso"])
+    assert np.abs(a["stress"] - b["stress"]).max() <= 1e-4 * np.abs(b["stress"]).max()
+    assert abs(a["H"] - b["H"]) <= 1e-4 * abs(b["H"])
