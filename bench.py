@@ -470,6 +470,43 @@ def run_c5(args):
                       "stress_finite": bool(np.all(np.isfinite(rec["stress"])))}))
 
 
+def run_beadstep(args):
+    """The bead-step launch groups alone, at 96^3 N=100 (the headline workload): for each
+    case prec:replicas:schedule of --cases, the average duration of one bead-step launch
+    group (HIP events on the batch stream over the sweeps of `steps` CL steps after
+    `warmup`), its algorithmic bytes (jobs x read q_s + write q_{s+1}) and their rate
+    against the 8 TB/s peak.  FP64 cases step the predictor-corrector, FP32 ones the
+    semi-implicit scheme; schedules 3 = conv, 4 = separable, 0 = timed at creation."""
+    wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
+    rows = []
+    for case in args.cases.split(","):
+        prec, R, sched = case.split(":")
+        R, sched = int(R), int(sched)
+        p = PREC_FP64 if prec == "fp64" else PREC_FP32
+        scheme = SCHEME_PC_REFERENCE if prec == "fp64" else SCHEME_SEMI_IMPLICIT
+        systems = [make_system(**wl, dt=0.01, precision=p, scheme=scheme, seed=0xC4 + r, fft_schedule=sched)
+                   for r in range(R)]
+        b = hip_batch(systems)
+        for r, (e, s) in enumerate(zip(b.replicas, systems)):
+            e.set_fields(*initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r))
+        b.step(max(args.warmup, 1))
+        b.set_timing(True)
+        t0 = time.perf_counter()
+        b.step(args.steps)
+        el = time.perf_counter() - t0
+        t = b.get_timing(0)
+        used = b.info.group_schedule[0]
+        b.close()
+        us = 1e3 * t["sweep_ms"] / max(1, t["step_launches"])
+        gbs = t["algorithmic_bytes"] / max(1e-30, t["sweep_ms"] * 1e-3) / 1e9
+        rows.append({"precision": prec, "replicas": R, "schedule": used, "us_per_launch_group": us,
+                     "alg_bytes_per_launch_group": t["algorithmic_bytes"] / max(1, t["step_launches"]),
+                     "alg_GBps": gbs, "frac": gbs / HBM_PEAK_GBS, "ms_per_step": 1e3 * el / args.steps})
+        print(f"[bench beadstep] {case}: {us:.2f} us per launch group, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "bead-step launch group at 96^3 N=100", "unit": "us per launch group",
+                      "steps": args.steps, "warmup": args.warmup, "cases": rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -496,8 +533,11 @@ def main():
                     help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["c4", "c2", "c3", "c5"], default="c4",
-                    help="c4: the headline (default); c2: FP32 vs FP64 errors at 64^3; c3: 64 mixed replicas; c5: 256^3")
+    ap.add_argument("--config", choices=["c4", "c2", "c3", "c5", "beadstep"], default="c4",
+                    help="c4: the headline (default); c2: FP32 vs FP64 errors at 64^3; c3: 64 mixed replicas; c5: 256^3; "
+                         "beadstep: the bead-step launch groups alone at 96^3 (--cases)")
+    ap.add_argument("--cases", default="fp64:1:3,fp64:1:4,fp64:8:3,fp32:16:3",
+                    help="beadstep cases precision:replicas:schedule (3 conv, 4 separable, 0 timed)")
     ap.add_argument("--c2-steps", type=int, default=10000)
     ap.add_argument("--c2-every", type=int, default=10)
     ap.add_argument("--c2-replicas", type=int, default=8)
@@ -510,7 +550,7 @@ def main():
               "and runs on several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
     if args.config != "c4":
-        {"c2": run_c2, "c3": run_c3, "c5": run_c5}[args.config](args)
+        {"c2": run_c2, "c3": run_c3, "c5": run_c5, "beadstep": run_beadstep}[args.config](args)
         return
     if args.steps < 1:
         print("--steps must be at least 1", file=sys.stderr)

@@ -58,6 +58,42 @@ template <typename V> __device__ __forceinline__ void st_pair_stream(V* p, V a, 
     const fts_f4 w = {a.x, a.y, b.x, b.y};
     __builtin_nontemporal_store(w, reinterpret_cast<fts_f4*>(p));
 }
+// one complex FP64 value, the same streaming access (16 bytes)
+typedef double fts_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_stream(const double2* p)
+{
+    const fts_d2 w = __builtin_nontemporal_load(reinterpret_cast<const fts_d2*>(p));
+    return make_double2(w.x, w.y);
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v)
+{
+    const fts_d2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<fts_d2*>(p));
+}
+// one point of a bead-step input read by a split pass's first step: FP32 streams it
+// (q_s and the scratch field are read once per bead step), FP64 reads it plainly
+// (streaming measured slower there: FP64 pencil pass 13.4 -> 17.1 us for one system)
+typedef float fts_f2 __attribute__((ext_vector_type(2)));
+template <typename T> __device__ __forceinline__ V2<T> ld_point(const V2<T>* p)
+{
+    if constexpr (sizeof(T) == 4) {
+        const fts_f2 w = __builtin_nontemporal_load(reinterpret_cast<const fts_f2*>(p));
+        return make_float2(w.x, w.y);
+    } else {
+        return *p;
+    }
+}
+// one point of q_{s+1} stored by the split plane pass's last step (streaming: the
+// next bead step reads it once)
+template <typename T> __device__ __forceinline__ void st_point(V2<T>* p, V2<T> v)
+{
+    if constexpr (sizeof(T) == 4) {
+        const fts_f2 w = {v.x, v.y};
+        __builtin_nontemporal_store(w, reinterpret_cast<fts_f2*>(p));
+    } else {
+        st_stream(p, v);
+    }
+}
 
 // running max for the per-bead renormalisation: FP64 keeps max |q|^2; FP32 keeps
 // max(|Re q|, |Im q|) in FP32 (one instruction per point) and squares it at the end.
@@ -151,7 +187,16 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     constexpr bool COLFIX = PAIRS && (2 * NT) % N == 0 && (N * N / 2) % NT == 0;
     const int slot0 = (2 * tid / N) * LS + (2 * tid) % N;
     auto pair_slot = [&](int k, int i) { return COLFIX ? slot0 + k * ((2 * NT / N) * LS) : (i / N) * LS + (i % N); };
-    if constexpr (MODE == PLANE_FWD_D2F) {
+    // the split bead step's first z step reads its rows from global memory itself: lane
+    // (sr, sq) holds points z = SN2 j + sq of row sr (consecutive lanes, consecutive z)
+    constexpr int SN2 = SPLIT ? reg::plane_split<N>() : 1, SN1 = N / SN2;
+    const int sr = tid / SN2, sq = tid % SN2;
+    V xs[SPLIT ? SN1 : 1];
+    if constexpr (SPLIT) {
+        const V* row = src + plane + sr * N + sq;
+#pragma unroll
+        for (int j = 0; j < SN1; ++j) xs[j] = ld_point<T>(row + SN2 * j);
+    } else if constexpr (MODE == PLANE_FWD_D2F) {
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) {
             const double2 d = job.src64[plane + i];
@@ -178,7 +223,9 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     } else {
 #pragma unroll
         for (int i = tid; i < N * N; i += NT) {
-            V v = src[plane + i];
+            V v;
+            if constexpr (sizeof(T) == 8 && (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E)) v = ld_stream(src + plane + i);
+            else v = src[plane + i];
             if constexpr (MODE == PLANE_INIT) {
                 v = cscale(v, c);
                 job.qout[plane + i] = v;
@@ -186,6 +233,16 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
             }
             lds[(i / N) * LS + (i % N)] = v;
         }
+    }
+    // FP64 bead step: e^{-W} read into registers now, so its latency hides behind the
+    // transforms (the FP64 plane fills the LDS, so one workgroup per CU, 3 waves per SIMD,
+    // leaves the registers for it)
+    constexpr bool EPF = sizeof(T) == 8 && MODE == PLANE_CONV_E && (SPLIT || (N * N) % NT == 0);
+    constexpr int NPF = EPF ? (SPLIT ? SN1 : N * N / NT) : 1;
+    V epf[NPF];
+    if constexpr (EPF) {
+#pragma unroll
+        for (int k = 0; k < NPF; ++k) epf[k] = job.emw[plane + (SPLIT ? sr * N + SN2 * k + sq : tid + k * NT)];
     }
     if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) renorm();
     __syncthreads();
@@ -196,9 +253,10 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
         // N2 k1 + k2 holding kz = k1 + N1 k2, so column c holds kz = c / N2 + N1 (c % N2);
         // y forward, x phi_y phi_z and y inverse on the columns (the multiply between
         // the forward and inverse halves of step C); z inverse back to natural order
-        constexpr int N2 = reg::plane_split<N>(), N1 = N / N2;
+        // (the middle steps put lanes on consecutive lines, l = tid % N)
+        constexpr int N2 = SN2, N1 = SN1;
         const int l = tid % N, p = tid / N;
-        reg::split_a<V, N, N2, -1, LS, 1, true>(lds, tw, l, p);
+        reg::split_a_regs<V, N, N2, -1, LS, 1, true>(xs, lds, tw, sr, sq);
         __syncthreads();
         reg::split_c<V, N, N2, NT, LS, 1, true, false>(lds, tw, tid, reg::NoMul{});
         __syncthreads();
@@ -212,8 +270,20 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
         __syncthreads();
         reg::split_c<V, N, N2, NT, LS, 1, false, true>(lds, tw, tid, reg::NoMul{});
         __syncthreads();
-        reg::split_a<V, N, N2, +1, LS, 1, false>(lds, tw, l, p);
-        __syncthreads();
+        // the last z step ends in registers: x e^{-W}, x 2^k and the store of q_{s+1}
+        // follow without an LDS round trip
+        reg::split_a_to_regs<V, N, N2, +1, LS, 1>(lds, xs, sr, sq);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) {
+            const size_t gi = plane + sr * N + N2 * k1 + sq;
+            V e;
+            if constexpr (EPF) e = epf[k1];
+            else e = job.emw[gi];
+            V v = cmul(xs[k1], e);
+            if (job.norm_in) v = cscale(v, c);
+            st_point<T>(job.qout + gi, v);
+            nrm.add(v);
+        }
     } else if constexpr (MODE == PLANE_CONV_E) {
         // the plane's share of the convolution: forward along z and y, x phi_y phi_z
         // (the pencil pass has applied phi_x / M), inverse below
@@ -242,7 +312,9 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
         lds_fft<T, N, +1, N, LS, 1, true, NT>(lds, tw, tid); // along z
         lds_fft<T, N, +1, N, 1, LS, true, NT>(lds, tw, tid);  // along y
     }
-    if constexpr (PAIRS && (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E)) {
+    if constexpr (SPLIT) {
+        // stored above
+    } else if constexpr (PAIRS && (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E)) {
 #pragma unroll 2
         for (int k = 0, j = tid; j < N * N / 2; ++k, j += NT) {
             const int i = 2 * j;
@@ -264,6 +336,15 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
             nrm.add(v1);
         }
         if constexpr (MODE == PLANE_STEP) __syncthreads();
+    } else if constexpr (EPF) {
+#pragma unroll
+        for (int k = 0; k < NPF; ++k) {
+            const int i = tid + k * NT;
+            V v = cmul(lds[(i / N) * LS + (i % N)], epf[k]);
+            if (job.norm_in) v = cscale(v, c);
+            st_stream(job.qout + plane + i, v);
+            nrm.add(v);
+        }
     } else if constexpr (MODE == PLANE_STEP || MODE == PLANE_LAST || MODE == PLANE_CONV_E) {
 #pragma unroll 4
         for (int i = tid; i < N * N; i += NT) {
@@ -271,7 +352,8 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
             V v = cmul(*p, job.emw[plane + i]);
             if (job.norm_in) v = cscale(v, c);
             *p = v;
-            job.qout[plane + i] = v;
+            if constexpr (sizeof(T) == 8) st_stream(job.qout + plane + i, v);
+            else job.qout[plane + i] = v;
             nrm.add(v);
         }
         __syncthreads();
@@ -359,6 +441,29 @@ __global__ void __launch_bounds__((pencil_threads_for<N1, MODE, ZCW>())) pencil_
     }
     V* data = job.data;
     const V* in = job.src ? job.src : data;
+    if constexpr (SPLIT) {
+        // FFT along x, x phi_x(k) c0, IFFT with the split transforms, whose first and
+        // last steps read and write global memory themselves: lane (l, p) holds points
+        // x = 8 j + p of column l (consecutive lanes, consecutive columns: each row
+        // segment of the tile is one contiguous access); in LDS column l is the line at
+        // lds[l + x LS]
+        constexpr int NJ = N1 / 8;
+        const int l = tid % ZC, p = tid / ZC;
+        V x[NJ];
+        const V* col = in + jbase + l;
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) x[j] = ld_point<T>(col + static_cast<size_t>(8 * j + p) * sx);
+        __syncthreads(); // the twiddles and factors
+        reg::split_a_regs<V, N1, 8, -1, 1, LS, true>(x, lds, tw, l, p);
+        __syncthreads();
+        reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, [&](int, int k) { return fx[k]; });
+        __syncthreads();
+        reg::split_a_to_regs<V, N1, 8, +1, 1, LS>(lds, x, l, p);
+        V* out = data + jbase + l;
+#pragma unroll
+        for (int k1 = 0; k1 < NJ; ++k1) out[static_cast<size_t>(8 * k1 + p) * sx] = x[k1];
+        return;
+    }
     // FP32: pairs of columns per 16-byte access (ZC and sx are even)
     constexpr bool PAIRS = sizeof(T) == 4 && ZC % 2 == 0;
     if constexpr (MODE == PENCIL_MUL_INV_D2F) {
@@ -384,17 +489,6 @@ __global__ void __launch_bounds__((pencil_threads_for<N1, MODE, ZCW>())) pencil_
         }
     }
     __syncthreads();
-    if constexpr (SPLIT) {
-        // FFT along x, x phi_x(k) c0, IFFT with the split transforms: column c of the
-        // tile is line c at lds[c + x LS]
-        const int l = tid % ZC, p = tid / ZC;
-        reg::split_a<V, N1, 8, -1, 1, LS, true>(lds, tw, l, p);
-        __syncthreads();
-        reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, [&](int, int k) { return fx[k]; });
-        __syncthreads();
-        reg::split_a<V, N1, 8, +1, 1, LS, false>(lds, tw, l, p);
-        __syncthreads();
-    }
     if constexpr (MODE == PENCIL_FWD || MODE == PENCIL_FWD_F2D || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD_MUL_INV_NOISE || (MODE == PENCIL_CONV && !SPLIT))
         lds_fft<T, N1, -1, ZC, 1, LS, true, NT>(lds, tw, tid);
     if constexpr (MODE == PENCIL_CONV && !SPLIT) {

@@ -156,6 +156,41 @@ __device__ __forceinline__ void split_a(V* lds, const V* tw, int l, int p)
     for (int k1 = 0; k1 < N1; ++k1) lds[l * RS + (N2 * k1 + p) * ES] = x[k1];
 }
 
+// Step A on values already in registers: x[j] is point N2 j + p of line l (a pass
+// that reads its lines straight from global memory loads them in this order, so
+// the LDS staging round trip before the first step disappears).  Writes the step's
+// results to LDS as split_a does.
+template <typename V, int N, int N2, int SIGN, int RS, int ES, bool TWID>
+__device__ __forceinline__ void split_a_regs(V (&x)[N / N2], V* lds, const V* tw, int l, int p)
+{
+    constexpr int N1 = N / N2;
+    RegDft<V, N1, SIGN>::run(x);
+    if constexpr (TWID) {
+        if (p != 0) {
+#pragma unroll
+            for (int k1 = 1; k1 < N1; ++k1) {
+                V t = tw[p * k1];
+                if (SIGN > 0) t.y = -t.y;
+                x[k1] = cmul(x[k1], t);
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) lds[l * RS + (N2 * k1 + p) * ES] = x[k1];
+}
+
+// The last (inverse, untwiddled) step A of a pass, leaving its results in registers:
+// on return x[k1] is point N2 k1 + p of line l in natural order, for the caller to
+// store straight to global memory.
+template <typename V, int N, int N2, int SIGN, int RS, int ES>
+__device__ __forceinline__ void split_a_to_regs(const V* lds, V (&x)[N / N2], int l, int p)
+{
+    constexpr int N1 = N / N2;
+#pragma unroll
+    for (int j = 0; j < N1; ++j) x[j] = lds[l * RS + (N2 * j + p) * ES];
+    RegDft<V, N1, SIGN>::run(x);
+}
+
 // NL lines (the tile's, N by default); mul(l, k): the k-space factor of line l at
 // frequency k (FWD and INV together)
 template <typename V, int N, int N2, int NT, int RS, int ES, bool FWD, bool INV, int NL = N, typename F>
