@@ -14,9 +14,15 @@ every leg has the same seed and fields):
               kernel (FTS_SCHEME_SI_LEARNED; network trained by
               ftspolysol_amd.train_lrk on 48^3 data, lrk_c4_48cubed.npy), judged
               every step by the exact force, falling back to the Debye kernel when
-              its excess force residual passes 1e-3.  `value` is this leg.
-  si_debye    the same batch, seeds and dt with the Debye kernel only
-              (FTS_SCHEME_SEMI_IMPLICIT): what the learned kernel gains or loses.
+              its excess force residual passes 1e-3, at --dt-fast (0.02: twice
+              the plain dt; the learned kernel is there for a larger stable dt,
+              and at 0.02 its <H> bias matches the Debye scheme's at 0.01).
+              `value` is this leg.
+  si_debye    the same batch and seeds with the Debye kernel only
+              (FTS_SCHEME_SEMI_IMPLICIT) at --dt-debye (0.01, the plain dt): what
+              the learned kernel gains or loses.  A leg whose replica exhausts
+              its non-finite restarts (the Debye scheme at dt 0.04) is reported
+              as failed.
   plain_fp64_batched  FP64, the reference's predictor-corrector
               (complexLangevin.cu:669-698), a batch of --plain-replicas: the plain
               engine at its own best throughput, and the plain scheme's tau_int and
@@ -29,7 +35,7 @@ Each leg: max(warmup, 200) untimed equilibration steps; then exactly `steps` tim
 steps (bracketed by device synchronisation, recording H every step as the
 reference measures its observables every step, complexLangevin.cu:684), which set
 the leg's rate; then untimed steps that extend the record of H to --record-steps
-(2000 for the FP32 legs, --plain-record-steps 1000 for the FP64 batch) whatever
+(2000 steps for the FP32 legs and --plain-record-steps 2000 for the FP64 batch) whatever
 `steps` is.  tau_int and <H> come from that record, so they do not depend on the
 driver's --steps.  A leg whose record is shorter than 50 Sokal windows reports
 samples_per_s null.
@@ -71,7 +77,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, Engine, PREC_FP32, PREC_FP64,  # noqa: E402
+from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, Engine, FtsError, PREC_FP32, PREC_FP64,  # noqa: E402
                                 SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, hip_batch, hip_engine)
 from ftspolysol_amd.lrk import load_params  # noqa: E402
 from ftspolysol_amd.system import initial_fields, make_system  # noqa: E402
@@ -119,13 +125,35 @@ def lrk_totals(stats):
             "res_debye": [sum(x["sum_rel_residual_debye"][c] for x in stats) for c in range(2)]}
 
 
-def run_leg(systems, label, warmup, steps, record_steps, ckpt_every=50, max_restarts=50, lrk=None,
-            lrk_threshold=1e-3, stats=True, min_equilibration=MIN_EQUILIBRATION, graph_level=2):
+def run_leg(systems, label, *args, **kw):
+    """One leg (run_leg_steps); a leg whose replica exhausts its non-finite restarts is
+    reported as failed, with the reason, instead of ending the run."""
+    try:
+        out = run_leg_steps(systems, label, *args, **kw)
+    except FtsError as exc:
+        print(f"[bench] {label}: failed: {exc}", file=sys.stderr, flush=True)
+        out = {"replicas": len(systems), "failed": str(exc), "replica_steps_per_s": None, "samples_per_s": None,
+               "tau_int": None, "tau_int_err": None, "H_mean": None, "H_stderr": None, "lrk": None}
+    out["dt"] = systems[0].desc.dt
+    return out
+
+
+def run_leg_steps(systems, label, warmup, steps, record_steps, ckpt_every=50, max_restarts=50, lrk=None,
+                  lrk_threshold=1e-3, stats=True, min_equilibration=MIN_EQUILIBRATION, graph_level=2):
     """One leg: equilibrate max(warmup, MIN_EQUILIBRATION) untimed steps, time exactly
     `steps` steps (recording H every step), then extend the record of H untimed to
     `record_steps` steps when it is shorter.  Returns the rate of the timed steps and the
     statistics of the record."""
     b = hip_batch(systems)
+    try:
+        return leg_steps(b, systems, label, warmup, steps, record_steps, ckpt_every, max_restarts, lrk, lrk_threshold,
+                         stats, min_equilibration, graph_level)
+    finally:
+        b.close()
+
+
+def leg_steps(b, systems, label, warmup, steps, record_steps, ckpt_every, max_restarts, lrk, lrk_threshold, stats,
+              min_equilibration, graph_level):
     b.set_graph(graph_level)
     for r, (e, s) in enumerate(zip(b.replicas, systems)):
         om, ps = initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r)
@@ -180,7 +208,6 @@ def run_leg(systems, label, warmup, steps, record_steps, ckpt_every=50, max_rest
     final = (b.replicas[0].get_field(FIELD_OMEGA), b.replicas[0].get_field(FIELD_PSI)[0])
     sched = b.info.group_schedule[0]
     tune = list(b.info.group_tune_ms[0])
-    b.close()
     out = dict(replicas=R, replica_steps_per_s=R * steps / elapsed, ms_per_step=1e3 * elapsed / steps,
                equilibration_steps=equil, timed_steps=steps, timing=timing, elapsed=elapsed, schedule=sched,
                tune_ms=tune, restarts=int(restarts), final=final, lrk=lrk_stats)
@@ -274,12 +301,19 @@ def cpu_baseline(system, n_steps, tau_plain, scheme_name):
                       f"({el:.1f} s, own mixed-radix FFT, OpenMP); samples/s uses the GPU plain engine's tau_int"}
 
 
+def speedup(x, y):
+    """Ratio of two legs' decorrelated samples/s with its relative error."""
+    if x is None or y is None or x.get("samples_per_s") is None or y.get("samples_per_s") is None:
+        return None
+    return {"ratio": x["samples_per_s"] / y["samples_per_s"], "rel_err": ratio_err(x, y)}
+
+
 def ratio_err(x, y):
     """Relative error of x/y's decorrelated-sample ratio from the two tau_int errors."""
     return float(np.hypot(x["tau_int_err"] / x["tau_int"], y["tau_int_err"] / y["tau_int"]))
 
 
-LEG_KEYS = ("replicas", "replica_steps_per_s", "ms_per_step", "equilibration_steps", "timed_steps", "record_steps",
+LEG_KEYS = ("failed", "dt", "replicas", "replica_steps_per_s", "ms_per_step", "equilibration_steps", "timed_steps", "record_steps",
             "tau_record_samples", "record_replica_steps_per_s", "tau_int", "tau_int_err", "window", "tau_int_each",
             "samples_per_s", "H_mean", "H_stderr", "schedule", "tune_ms", "restarts")
 
@@ -515,16 +549,21 @@ def main():
                     help=f"untimed steps before the timed ones (at least {MIN_EQUILIBRATION} are taken)")
     ap.add_argument("--record-steps", type=int, default=2000,
                     help="steps of the H record of the FP32 legs (tau_int, <H>), timed steps included")
-    ap.add_argument("--plain-record-steps", type=int, default=1000, help="the same for the plain FP64 batch")
+    ap.add_argument("--plain-record-steps", type=int, default=2000, help="the same for the plain FP64 batch")
     ap.add_argument("--replicas", type=int, default=16, help="replicas in the FP32 batches")
     ap.add_argument("--plain-replicas", type=int, default=8, help="replicas in the plain FP64 batch")
-    ap.add_argument("--dt-fast", type=float, default=0.01)
+    ap.add_argument("--dt-fast", type=float, default=0.02,
+                    help="time step of the learned leg: the learned kernel is there for a larger stable dt; at 0.02 its "
+                         "<H> bias matches the Debye scheme's at 0.01, and at 0.04 it stays finite where Debye does not")
+    ap.add_argument("--dt-debye", type=float, default=0.01, help="time step of the Debye semi-implicit leg")
     ap.add_argument("--dt-plain", type=float, default=0.01)
     ap.add_argument("--lrk", default=LRK_PARAMS, help="parameters of the learned kernel (.npy)")
     ap.add_argument("--lrk-threshold", type=float, default=1e-3)
     ap.add_argument("--with-fp32-pc", action="store_true", help="add the FP32 predictor-corrector batch")
     ap.add_argument("--no-debye", action="store_true", help="skip the FP32 Debye semi-implicit batch")
     ap.add_argument("--fast-only", action="store_true", help="run the fast leg alone and print its summary (profiling runs)")
+    ap.add_argument("--learned-vs-debye", action="store_true",
+                    help="run the fast (learned) and Debye semi-implicit legs only and print their comparison")
     ap.add_argument("--graph-level", type=int, choices=[0, 1, 2], default=2,
                     help="batch launches: 2 replays each whole CL step as a captured graph, 1 only the sweeps, 0 none")
     ap.add_argument("--equilibration", type=int, default=MIN_EQUILIBRATION,
@@ -570,10 +609,16 @@ def main():
     EQ = dict(min_equilibration=args.equilibration, graph_level=args.graph_level)
     fast = run_leg(fp32(SCHEME_SI_LEARNED, args.dt_fast), f"fast fp32 si_learned x{args.replicas}", W, K, rec, lrk=lrk,
                    lrk_threshold=args.lrk_threshold, **EQ)
-    debye = None if (args.no_debye or args.fast_only) else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_fast),
+    debye = None if (args.no_debye or args.fast_only) else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_debye),
                                                f"si_debye fp32 x{args.replicas}", W, K, rec, **EQ)
     fp32_pc = run_leg(fp32(SCHEME_PC_REFERENCE, args.dt_plain), f"fp32 pc x{args.replicas}", W, K, rec, **EQ) \
         if args.with_fp32_pc and not args.fast_only else None
+    if args.learned_vs_debye:
+        print(json.dumps({"metric": "learned vs Debye semi-implicit kernel at 96^3 N=100", "dt": args.dt_fast,
+                          "dt_debye": args.dt_debye,
+                          "replicas": args.replicas, "fast": leg_summary(fast), "si_debye": leg_summary(debye),
+                          "learned_kernel": fast["lrk"], "learned_vs_debye": speedup(fast, debye)}))
+        return
     if args.fast_only:
         # a short run for profilers: the fast leg alone (no statistics beyond its record)
         print(json.dumps({"fast": leg_summary(fast), "roofline_us_per_launch_group":
@@ -583,10 +628,17 @@ def main():
     plain_1 = run_leg(plain(1), "plain fp64 pc x1", W, K, 0, stats=False, **EQ)
     # one system's rate; its tau_int and <H> are those of the same dynamics in the batch
     for key in ("tau_int", "tau_int_err", "window", "H_mean", "H_stderr", "record_steps", "tau_record_samples"):
-        plain_1[key] = plain_b[key]
+        plain_1[key] = plain_b.get(key)
     plain_1["samples_per_s"] = plain_1["replica_steps_per_s"] / (2 * plain_b["tau_int"]) \
-        if plain_b["samples_per_s"] is not None else None
+        if plain_b["samples_per_s"] is not None and plain_1["replica_steps_per_s"] is not None else None
     plain_1["tau_source"] = f"the plain FP64 batch's record ({args.plain_replicas} replicas)"
+    failed = {name: leg["failed"] for name, leg in (("fast", fast), ("plain_fp64_batched", plain_b), ("plain_fp64", plain_1))
+              if leg.get("failed")}
+    if failed:
+        print(json.dumps({"metric": "decorrelated CL samples/s at 96^3 N=100, FP32+learned vs plain FP64, <H> err",
+                          "value": None, "unit": "decorrelated CL samples/s", "n_gpus": 1, "steps": K, "warmup": W,
+                          "higher_is_better": True, "failed": failed}))
+        sys.exit(1)
 
     same = same_state_errors(wl, fast["final"])
 
@@ -613,13 +665,14 @@ def main():
     except Exception as exc:  # report the peak fraction only
         print(f"[bench] copy bandwidth not measured: {exc}", file=sys.stderr)
         copy_gbs = None
-    H_err_vs_plain = abs(fast["H_mean"] - plain_b["H_mean"]) / abs(plain_b["H_mean"])
-    H_err_sigma = np.hypot(fast["H_stderr"], plain_b["H_stderr"]) / abs(plain_b["H_mean"])
-
-    def speedup(x, y):
-        if x is None or y is None or x.get("samples_per_s") is None or y.get("samples_per_s") is None:
-            return None
-        return {"ratio": x["samples_per_s"] / y["samples_per_s"], "rel_err": ratio_err(x, y)}
+    def h_err(x):
+        """|<H>_x - <H>_plain| / <H>_plain and its combined standard error"""
+        if x is None or x.get("H_mean") is None or plain_b.get("H_mean") is None:
+            return None, None
+        return (abs(x["H_mean"] - plain_b["H_mean"]) / abs(plain_b["H_mean"]),
+                float(np.hypot(x["H_stderr"], plain_b["H_stderr"]) / abs(plain_b["H_mean"])))
+    H_err_vs_plain, H_err_sigma = h_err(fast)
+    H_err_debye, H_err_debye_sigma = h_err(debye)
 
     sched_name = {1: "plane", 2: "line", 3: "conv", 4: "separable"}
     out = {
@@ -638,7 +691,7 @@ def main():
         "config": {"workload": "96^3 L=48 N=100 z=-0.5 salt=0.05 B=0.5 lB=1 (221 chains)",
                    "fast": f"FP32 si_learned dt={args.dt_fast}, {args.replicas} replicas per launch, learned kernel "
                            f"{os.path.basename(args.lrk)}, fallback threshold {args.lrk_threshold}",
-                   "si_debye": None if debye is None else f"FP32 si (Debye kernel) dt={args.dt_fast}, {args.replicas} replicas",
+                   "si_debye": None if debye is None else f"FP32 si (Debye kernel) dt={args.dt_debye}, {args.replicas} replicas",
                    "plain_fp64_batched": f"FP64 pc dt={args.dt_plain}, {args.plain_replicas} replicas per launch",
                    "plain_fp64": f"FP64 pc dt={args.dt_plain}, 1 system",
                    "statistics": f"tau_int and <H> from an H record of {rec} steps (FP32 legs) / {prec} (FP64 batch) "
@@ -646,7 +699,7 @@ def main():
                    "parallelism": "single GPU"},
         "equilibration_steps": fast["equilibration_steps"],
         "tau_record_samples": fast["tau_record_samples"],
-        "learned_fraction": fast["lrk"]["learned_fraction"],
+        "learned_fraction": fast["lrk"]["learned_fraction"] if fast.get("lrk") else None,
         "fast": leg_summary(fast),
         "si_debye": None if debye is None else leg_summary(debye),
         "fp32_pc": None if fp32_pc is None else leg_summary(fp32_pc),
@@ -660,8 +713,10 @@ def main():
         "fp32_pc_speedup_vs_plain_fp64": speedup(fp32_pc, plain_1),
         "H_err_same_state": float(same["H"]),
         "same_state_rel_err": {k: float(v) for k, v in same.items() if k != "H_fp64"},
-        "H_err_vs_plain": float(H_err_vs_plain),
-        "H_err_sigma": float(H_err_sigma),
+        "H_err_vs_plain": H_err_vs_plain,
+        "H_err_sigma": H_err_sigma,
+        "H_err_vs_plain_si_debye": H_err_debye,
+        "H_err_sigma_si_debye": H_err_debye_sigma,
         "H_mean": {"fast": fast["H_mean"], "si_debye": None if debye is None else debye["H_mean"],
                    "plain_fp64_batched": plain_b["H_mean"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -22,12 +22,23 @@ template <int N1, int ZC = PENCIL_ZC> constexpr int pencil_threads() { return ZC
 // PLANE_CONV_E runs the register-split transforms where the plane length allows
 // (8 threads per line), every other mode the staged Stockham transforms
 template <int N, int MODE> constexpr bool plane_split_mode() { return MODE == PLANE_CONV_E && reg::plane_split<N>() != 0; }
+// the plane passes that transform a field (forward or inverse, optionally from or to
+// FP64) run the split transforms too, with the spectrum stored in natural order
+template <int N, int MODE> constexpr bool plane_field_split()
+{
+    return (MODE == PLANE_FWD || MODE == PLANE_FWD_D2F || MODE == PLANE_INV || MODE == PLANE_INV_F2D) && reg::plane_split<N>() != 0;
+}
 template <int N, int MODE> constexpr int plane_threads_for()
 {
-    return plane_split_mode<N, MODE>() ? reg::plane_split<N>() * N : plane_threads<N>();
+    return (plane_split_mode<N, MODE>() || plane_field_split<N, MODE>()) ? reg::plane_split<N>() * N : plane_threads<N>();
 }
 // PENCIL_CONV likewise, with 8 threads per x line, up to N1 = 96
-template <int N1, int MODE> constexpr bool pencil_split_mode() { return MODE == PENCIL_CONV && N1 % 8 == 0 && N1 >= 16 && N1 <= 96; }
+template <int N1, int MODE> constexpr bool pencil_split_mode()
+{
+    return (MODE == PENCIL_CONV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD || MODE == PENCIL_INV || MODE == PENCIL_MUL_INV ||
+            MODE == PENCIL_FWD_F2D || MODE == PENCIL_MUL_INV_D2F) &&
+           N1 % 8 == 0 && N1 >= 16 && N1 <= 96;
+}
 template <int N1, int MODE, int ZC> constexpr int pencil_threads_for()
 {
     return pencil_split_mode<N1, MODE>() ? ZC * 8 : pencil_threads<N1, ZC>();
@@ -137,6 +148,58 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     const PlaneJob<T> job = pick_job<T>(j0, j1, table);
     const size_t plane = static_cast<size_t>(blockIdx.x) * (N * N);
     for (int t = tid; t < N; t += NT) tw[t] = twg[t];
+    if constexpr (plane_field_split<N, MODE>()) {
+        // 2D transform of the plane, N = N1 N2, lanes as in the bead step: the outer
+        // steps on (sr, sq) = row, point N2 j + sq of it; the middle steps on lines l.
+        // After a forward step C, slot N2 k1 + k2 of a line holds frequency k1 + N1 k2,
+        // so the spectrum leaves (and an inverse's enters) LDS through slot(k)
+        constexpr int N2 = reg::plane_split<N>(), N1 = N / N2;
+        const int sr = tid / N2, sq = tid % N2;
+        const int l = tid % N, p = tid / N;
+        auto slot = [](int k) { return N2 * (k % N1) + k / N1; };
+        V x[N1];
+        if constexpr (MODE == PLANE_FWD || MODE == PLANE_FWD_D2F) {
+#pragma unroll
+            for (int j = 0; j < N1; ++j) {
+                const size_t gi = plane + sr * N + sq + N2 * j;
+                if constexpr (MODE == PLANE_FWD_D2F) {
+                    const double2 d = job.src64[gi];
+                    x[j] = cmk<V>(static_cast<T>(d.x), static_cast<T>(d.y));
+                } else {
+                    x[j] = job.src[gi];
+                }
+            }
+            __syncthreads(); // the twiddles
+            reg::split_a_regs<V, N, N2, -1, LS, 1, true>(x, lds, tw, sr, sq); // z
+            __syncthreads();
+            reg::split_c<V, N, N2, NT, LS, 1, true, false>(lds, tw, tid, reg::NoMul{});
+            __syncthreads();
+            reg::split_a<V, N, N2, -1, 1, LS, true>(lds, tw, l, p); // y
+            __syncthreads();
+            reg::split_c<V, N, N2, NT, 1, LS, true, false>(lds, tw, tid, reg::NoMul{});
+            __syncthreads();
+#pragma unroll 4
+            for (int i = tid; i < N * N; i += NT) job.dst[plane + i] = lds[slot(i / N) * LS + slot(i % N)];
+        } else {
+#pragma unroll 4
+            for (int i = tid; i < N * N; i += NT) lds[slot(i / N) * LS + slot(i % N)] = job.src[plane + i];
+            __syncthreads();
+            reg::split_c<V, N, N2, NT, 1, LS, false, true>(lds, tw, tid, reg::NoMul{}); // y
+            __syncthreads();
+            reg::split_a<V, N, N2, +1, 1, LS, false>(lds, tw, l, p);
+            __syncthreads();
+            reg::split_c<V, N, N2, NT, LS, 1, false, true>(lds, tw, tid, reg::NoMul{}); // z
+            __syncthreads();
+            reg::split_a_to_regs<V, N, N2, +1, LS, 1>(lds, x, sr, sq);
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) {
+                const size_t gi = plane + sr * N + N2 * k1 + sq;
+                if constexpr (MODE == PLANE_INV_F2D) job.dst64[gi] = make_double2(static_cast<double>(x[k1].x), static_cast<double>(x[k1].y));
+                else job.dst[gi] = x[k1];
+            }
+        }
+        return;
+    }
     if constexpr (SPLIT) {
         const double* sy = job.op.sep + gridDim.x; // sep holds the m1, m2, m3 factors; gridDim.x = m1
         for (int t = tid; t < 2 * N; t += NT) sfac[t] = static_cast<T>(sy[t]);
@@ -437,31 +500,86 @@ __global__ void __launch_bounds__((pencil_threads_for<N1, MODE, ZCW>())) pencil_
     const V* twg = static_cast<const V*>(g.tw_x);
     for (int t = tid; t < N1; t += NT) {
         tw[t] = twg[t];
-        if constexpr (SPLIT) fx[t] = static_cast<T>(job.op.sep[t] * job.op.c0);
+        if constexpr (MODE == PENCIL_CONV && SPLIT) fx[t] = static_cast<T>(job.op.sep[t] * job.op.c0);
     }
     V* data = job.data;
     const V* in = job.src ? job.src : data;
     if constexpr (SPLIT) {
-        // FFT along x, x phi_x(k) c0, IFFT with the split transforms, whose first and
-        // last steps read and write global memory themselves: lane (l, p) holds points
-        // x = 8 j + p of column l (consecutive lanes, consecutive columns: each row
-        // segment of the tile is one contiguous access); in LDS column l is the line at
-        // lds[l + x LS]
+        // Transforms along x with the split transforms (N1 = NJ x 8), whose outer steps
+        // read and write global memory themselves: lane (l, p) holds points x = 8 j + p
+        // of column l (consecutive lanes, consecutive columns: each row segment of the
+        // tile is one contiguous access); in LDS column l is the line at lds[l + x LS].
+        // A spectrum that enters or leaves in natural order goes through slot(k): after
+        // a forward step C, slot 8 k1 + k2 holds frequency k1 + NJ k2.
         constexpr int NJ = N1 / 8;
+        constexpr bool FWD = MODE == PENCIL_CONV || MODE == PENCIL_FWD_MUL_INV || MODE == PENCIL_FWD || MODE == PENCIL_FWD_F2D;
+        constexpr bool INV = MODE != PENCIL_FWD && MODE != PENCIL_FWD_F2D;
         const int l = tid % ZC, p = tid / ZC;
+        auto slot = [](int k) { return 8 * (k % NJ) + k / NJ; };
+        // the k-space kernel at frequency x of tile column lc (as the staged path)
+        const KOp op = job.op;
+        auto kv = [&](int lc, int x) -> double {
+            const int j = static_cast<int>(jbase) + lc;
+            const int y = j / g.m3, z = j % g.m3;
+            double v = op.sep ? op.sep[x] * (op.sep[g.m1 + y] * op.sep[g.m1 + g.m2 + z]) : 1.0;
+            if (op.table) v *= kop_table(op, op.table[static_cast<size_t>(x) * sx + j]);
+            if (op.c1 != 0.0 || op.c2 != 0.0) {
+                const double k[3] = {g.kvec[x], g.kvec[g.m1 + y], g.kvec[g.m1 + g.m2 + z]};
+                double poly = op.c0;
+                if (op.c1 != 0.0) poly += op.c1 * (k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
+                if (op.c2 != 0.0) poly += op.c2 * (k[op.ka] * k[op.kb]);
+                v *= poly;
+            } else {
+                v *= op.c0;
+            }
+            return v;
+        };
         V x[NJ];
-        const V* col = in + jbase + l;
+        if constexpr (FWD) {
+            const V* col = in + jbase + l;
 #pragma unroll
-        for (int j = 0; j < NJ; ++j) x[j] = ld_point<T>(col + static_cast<size_t>(8 * j + p) * sx);
-        __syncthreads(); // the twiddles and factors
-        reg::split_a_regs<V, N1, 8, -1, 1, LS, true>(x, lds, tw, l, p);
-        __syncthreads();
-        reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, [&](int, int k) { return fx[k]; });
-        __syncthreads();
-        reg::split_a_to_regs<V, N1, 8, +1, 1, LS>(lds, x, l, p);
-        V* out = data + jbase + l;
+            for (int j = 0; j < NJ; ++j) {
+                if constexpr (MODE == PENCIL_CONV) x[j] = ld_point<T>(col + static_cast<size_t>(8 * j + p) * sx);
+                else x[j] = col[static_cast<size_t>(8 * j + p) * sx];
+            }
+            __syncthreads(); // the twiddles and factors
+            reg::split_a_regs<V, N1, 8, -1, 1, LS, true>(x, lds, tw, l, p);
+            __syncthreads();
+            if constexpr (MODE == PENCIL_CONV) reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, [&](int, int k) { return fx[k]; });
+            else if constexpr (INV) reg::split_c<V, N1, 8, NT, 1, LS, true, true, ZC>(lds, tw, tid, kv);
+            else reg::split_c<V, N1, 8, NT, 1, LS, true, false, ZC>(lds, tw, tid, reg::NoMul{});
+            __syncthreads();
+        } else {
+            // a spectrum in natural order (times the kernel for the MUL modes) into the slots
+#pragma unroll 4
+            for (int i = tid; i < N1 * ZC; i += NT) {
+                const int xx = i / ZC, c = i % ZC;
+                const size_t gi = static_cast<size_t>(xx) * sx + jbase + c;
+                V v;
+                if constexpr (MODE == PENCIL_MUL_INV_D2F) v = cmk<V>(static_cast<T>(job.src64[gi].x), static_cast<T>(job.src64[gi].y));
+                else v = in[gi];
+                if constexpr (MODE != PENCIL_INV) v = cscale(v, static_cast<T>(kv(c, xx)));
+                lds[c + slot(xx) * LS] = v;
+            }
+            __syncthreads();
+            reg::split_c<V, N1, 8, NT, 1, LS, false, true, ZC>(lds, tw, tid, reg::NoMul{});
+            __syncthreads();
+        }
+        if constexpr (INV) {
+            reg::split_a_to_regs<V, N1, 8, +1, 1, LS>(lds, x, l, p);
+            V* out = data + jbase + l;
 #pragma unroll
-        for (int k1 = 0; k1 < NJ; ++k1) out[static_cast<size_t>(8 * k1 + p) * sx] = x[k1];
+            for (int k1 = 0; k1 < NJ; ++k1) out[static_cast<size_t>(8 * k1 + p) * sx] = x[k1];
+        } else {
+#pragma unroll 4
+            for (int i = tid; i < N1 * ZC; i += NT) {
+                const int xx = i / ZC, c = i % ZC;
+                const size_t gi = static_cast<size_t>(xx) * sx + jbase + c;
+                const V v = lds[c + slot(xx) * LS];
+                if constexpr (MODE == PENCIL_FWD_F2D) job.dst64[gi] = make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
+                else data[gi] = v;
+            }
+        }
         return;
     }
     // FP32: pairs of columns per 16-byte access (ZC and sx are even)
