@@ -521,6 +521,7 @@ def run_beadstep(args):
         systems = [make_system(**wl, dt=0.01, precision=p, scheme=scheme, seed=0xC4 + r, fft_schedule=sched)
                    for r in range(R)]
         b = hip_batch(systems)
+        b.set_graph(args.graph_level)
         for r, (e, s) in enumerate(zip(b.replicas, systems)):
             e.set_fields(*initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r))
         b.step(max(args.warmup, 1))
@@ -537,8 +538,12 @@ def run_beadstep(args):
                      "alg_bytes_per_launch_group": t["algorithmic_bytes"] / max(1, t["step_launches"]),
                      "alg_GBps": gbs, "frac": gbs / HBM_PEAK_GBS, "ms_per_step": 1e3 * el / args.steps})
         print(f"[bench beadstep] {case}: {us:.2f} us per launch group, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    # a device-to-device copy of 256 MiB (twice): the rate, and under rocprofv3 --pmc the
+    # calibration of FETCH_SIZE / WRITE_SIZE on a copy of known size
+    copy_gbs = copy_bandwidth(gib=0.25, reps=2)
     print(json.dumps({"metric": "bead-step launch group at 96^3 N=100", "unit": "us per launch group",
-                      "steps": args.steps, "warmup": args.warmup, "cases": rows}))
+                      "steps": args.steps, "warmup": args.warmup, "graph_level": args.graph_level, "cases": rows,
+                      "copy_GBps": copy_gbs, "copy_bytes": 1 << 28}))
 
 
 def main():

@@ -333,8 +333,10 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
         __syncthreads();
         reg::split_c<V, N, N2, NT, LS, 1, false, true>(lds, tw, tid, reg::NoMul{});
         __syncthreads();
-        // the last z step ends in registers: x e^{-W}, x 2^k and the store of q_{s+1}
-        // follow without an LDS round trip
+        // the last z step ends in registers: x e^{-W}, x 2^k.  FP64 stores q_{s+1} from
+        // there (eight lanes, one 128-byte row segment); FP32 would write 64-byte
+        // segments, which the memory counts as 1.38x the bytes (PMC), so its results go
+        // back to their own slots and leave as 16-byte pairs
         reg::split_a_to_regs<V, N, N2, +1, LS, 1>(lds, xs, sr, sq);
 #pragma unroll
         for (int k1 = 0; k1 < N1; ++k1) {
@@ -344,8 +346,18 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
             else e = job.emw[gi];
             V v = cmul(xs[k1], e);
             if (job.norm_in) v = cscale(v, c);
-            st_point<T>(job.qout + gi, v);
+            if constexpr (PAIRS) lds[sr * LS + N2 * k1 + sq] = v;
+            else st_point<T>(job.qout + gi, v);
             nrm.add(v);
+        }
+        if constexpr (PAIRS) {
+            __syncthreads();
+#pragma unroll 2
+            for (int k = 0, j = tid; j < N * N / 2; ++k, j += NT) {
+                const int i = 2 * j;
+                const V* q = &lds[pair_slot(k, i)];
+                st_pair_stream(job.qout + plane + i, q[0], q[1]);
+            }
         }
     } else if constexpr (MODE == PLANE_CONV_E) {
         // the plane's share of the convolution: forward along z and y, x phi_y phi_z
