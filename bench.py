@@ -573,7 +573,7 @@ def main():
                     help="batch launches: 2 replays each whole CL step as a captured graph, 1 only the sweeps, 0 none")
     ap.add_argument("--equilibration", type=int, default=MIN_EQUILIBRATION,
                     help="least untimed steps before the timed ones (lower only for short profiling runs)")
-    ap.add_argument("--schedule", choices=["auto", "plane", "line", "conv", "separable"], default="auto",
+    ap.add_argument("--schedule", choices=["auto", "plane", "line", "conv", "separable", "conv_mfma"], default="auto",
                     help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -581,7 +581,7 @@ def main():
                     help="c4: the headline (default); c2: FP32 vs FP64 errors at 64^3; c3: 64 mixed replicas; c5: 256^3; "
                          "beadstep: the bead-step launch groups alone at 96^3 (--cases)")
     ap.add_argument("--cases", default="fp64:1:3,fp64:1:4,fp64:8:3,fp32:16:3",
-                    help="beadstep cases precision:replicas:schedule (3 conv, 4 separable, 0 timed)")
+                    help="beadstep cases precision:replicas:schedule (3 conv, 4 separable, 5 conv on the matrix cores, 0 timed)")
     ap.add_argument("--c2-steps", type=int, default=10000)
     ap.add_argument("--c2-every", type=int, default=10)
     ap.add_argument("--c2-replicas", type=int, default=8)
@@ -602,7 +602,7 @@ def main():
 
     wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
     lrk = load_params(args.lrk)
-    sched = {"auto": 0, "plane": 1, "line": 2, "conv": 3, "separable": 4}[args.schedule]
+    sched = {"auto": 0, "plane": 1, "line": 2, "conv": 3, "separable": 4, "conv_mfma": 5}[args.schedule]
     fp32 = lambda scheme, dt: [make_system(**wl, dt=dt, precision=PREC_FP32, scheme=scheme, seed=0xC4 + r,
                                            fft_schedule=sched) for r in range(args.replicas)]
     plain = lambda n: [make_system(**wl, dt=args.dt_plain, precision=PREC_FP64, scheme=SCHEME_PC_REFERENCE,
@@ -679,7 +679,7 @@ def main():
     H_err_vs_plain, H_err_sigma = h_err(fast)
     H_err_debye, H_err_debye_sigma = h_err(debye)
 
-    sched_name = {1: "plane", 2: "line", 3: "conv", 4: "separable"}
+    sched_name = {1: "plane", 2: "line", 3: "conv", 4: "separable", 5: "conv_mfma"}
     out = {
         "metric": "decorrelated CL samples/s at 96^3 N=100, FP32+learned vs plain FP64, <H> err",
         "value": fast["samples_per_s"],

@@ -198,7 +198,7 @@ template <typename T> int build_tables(Group& g, int p, int sched, int max_steps
     auto add = [&](int kind, int mode, int axis, size_t off, size_t end) {
         tb.launches.push_back(SweepLaunch{kind, mode, axis, off, static_cast<int>(end - off)});
     };
-    if (sched == FTS_SCHED_PLANE || sched == FTS_SCHED_CONV) {
+    if (sched == FTS_SCHED_PLANE || sched_is_conv(sched)) {
         size_t o = J.plane.size();
         for (const RepView<T>& r : act) {
             const int N = r.sp->N;
@@ -209,7 +209,7 @@ template <typename T> int build_tables(Group& g, int p, int sched, int max_steps
         add(L_PLANE, PLANE_INIT, 0, o, J.plane.size());
         tb.init_launches = tb.launches.size();
     }
-    if (sched == FTS_SCHED_CONV) {
+    if (sched_is_conv(sched)) {
         // bead step s: pencil q_s -> tmp, plane tmp -> q_{s+1}; a replica whose chain
         // has ended drops out of both launches
         for (int s = 0; s < nsteps; ++s) {
@@ -220,7 +220,7 @@ template <typename T> int build_tables(Group& g, int p, int sched, int max_steps
                 J.pencil.push_back(PencilJob<T>{r.tf, r.bond_op(s), r.qf + static_cast<size_t>(s) * r.Ms});
                 if (!r.sp->mirror) J.pencil.push_back(PencilJob<T>{r.tb, r.bond_op(N - 2 - s), r.qb + static_cast<size_t>(N - 1 - s) * r.Ms});
             }
-            add(L_PENCIL, PENCIL_CONV, 0, op, J.pencil.size());
+            add(L_PENCIL, conv_pencil_mode(sched), 0, op, J.pencil.size());
             const size_t ol = J.plane.size();
             for (const RepView<T>& r : act) {
                 const int N = r.sp->N;
@@ -871,7 +871,7 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
         const int want = systems[g.ids[0]].fft_schedule;
         int rc = FTS_OK;
         if (want == FTS_SCHED_AUTO) rc = g.precision == FTS_PREC_FP32 ? group_autotune<float>(g) : group_autotune<double>(g);
-        else if ((want == FTS_SCHED_PLANE || want == FTS_SCHED_CONV) && !g.plane_ok) return fail(FTS_INVALID, "batch: the plane schedules do not fit this mesh");
+        else if ((want == FTS_SCHED_PLANE || sched_is_conv(want)) && !g.plane_ok) return fail(FTS_INVALID, "batch: the plane schedules do not fit this mesh");
         else g.sched = want;
         if (rc) return rc;
         for (fts_engine* e : g.reps) {

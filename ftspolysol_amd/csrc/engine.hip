@@ -953,7 +953,7 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
     const void* twy = sizeof(T) == 8 ? static_cast<const void*>(tw64_yz.p) : static_cast<const void*>(tw32_yz.p);
     const void* twz = sizeof(T) == 8 ? static_cast<const void*>(tw64_z.p) : static_cast<const void*>(tw32_z.p);
     hipEvent_t ev0 = nullptr;
-    if (schedule == FTS_SCHED_CONV) {
+    if (sched_is_conv(schedule)) {
         // bead step s: pencil (x factor and 1/M) q_s -> tmp, plane (y, z factors,
         // e^{-W}, renormalisation) tmp -> q_{s+1}; the first plane pass stores q_0
         // (its transformed copy in tmp is not used)
@@ -968,7 +968,7 @@ template <typename T> int fts_engine::run_sweeps(int p, int schedule, int nsteps
             const int bf = sp.bond_cls[static_cast<size_t>(s)], bb = sp.bond_cls[static_cast<size_t>(N - 2 - s)];
             const KOp of = kop_sep(sep_bond.p + static_cast<size_t>(bf) * ks, invM), ob = kop_sep(sep_bond.p + static_cast<size_t>(bb) * ks, invM);
             PencilJob<T> pj[2] = {{tf, of, qf + static_cast<size_t>(s) * Ms}, {tb, ob, qb + static_cast<size_t>(N - 1 - s) * Ms}};
-            HIP_TRY(launch_pencil<T>(g, PENCIL_CONV, pj, nj, stream));
+            HIP_TRY(launch_pencil<T>(g, conv_pencil_mode(schedule), pj, nj, stream));
             const int sf = s + 1, sb = N - 2 - s;
             PlaneJob<T> st[2] = {{tf, nullptr, qf + static_cast<size_t>(sf) * Ms, E_of(sf), nF[s % 2], nF[(s + 1) % 2], lcF + sf, of},
                                  {tb, nullptr, qb + static_cast<size_t>(sb) * Ms, E_of(sb), nB[s % 2], nB[(s + 1) % 2], lcB + sb, ob}};
@@ -1060,9 +1060,9 @@ template <typename T> int fts_engine::bead_hook(V2<T>* q, const V2<T>* emw, V2<T
 {
     const FftGeom& g = sizeof(T) == 8 ? g64 : g32;
     const bool d = sizeof(T) == 8;
-    if (sched == FTS_SCHED_CONV) {
+    if (sched_is_conv(sched)) {
         PencilJob<T> j1{q, op};
-        HIP_TRY(launch_pencil<T>(g, PENCIL_CONV, &j1, 1, stream));
+        HIP_TRY(launch_pencil<T>(g, conv_pencil_mode(sched), &j1, 1, stream));
         PlaneJob<T> j2{q, nullptr, qout, emw, nullptr, nullptr, nullptr, op};
         HIP_TRY(launch_plane<T>(g, PLANE_CONV_E, &j2, 1, stream));
         return FTS_OK;
@@ -1974,7 +1974,9 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     const bool plane_fits = fft_supported(s->m1, s->m2, s->m3, s->precision == FTS_PREC_FP32);
     if ((s->fft_schedule == FTS_SCHED_PLANE || s->fft_schedule == FTS_SCHED_CONV) && !plane_fits)
         return fail(FTS_INVALID, "the plane schedules need m2 == m3 and the m2 x m3 plane in LDS (FP64 up to 96, FP32 up to 128)");
-    if (s->fft_schedule < 0 || s->fft_schedule > 4) return fail(FTS_INVALID, "fft_schedule");
+    if (s->fft_schedule == FTS_SCHED_CONV_MFMA && !conv_mfma_supported(s->m1, s->m2, s->m3, s->precision == FTS_PREC_FP32))
+        return fail(FTS_INVALID, "the matrix-core conv schedule needs FP32, m1 in {32, 48, 64, 96} and the plane schedules' mesh");
+    if (s->fft_schedule < 0 || s->fft_schedule > 5) return fail(FTS_INVALID, "fft_schedule");
     if (s->chain_symmetry != FTS_SYMMETRY_AUTO && s->chain_symmetry != FTS_SYMMETRY_OFF) return fail(FTS_INVALID, "chain_symmetry");
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");

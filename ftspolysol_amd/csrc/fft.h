@@ -54,8 +54,10 @@ enum PlaneMode { PLANE_FWD = 0, PLANE_INV = 1, PLANE_INIT = 2, PLANE_STEP = 3, P
 // PENCIL_FWD_MUL_INV_NOISE / LINE_FWD_MUL_INV_NOISE: the multiply adds the k-space
 // noise of its KOp (the launch functions pick them when a job's op has namp != 0; the
 // Philox rounds stay out of the registers of the plain multiply)
+// PENCIL_CONV_MFMA: PENCIL_CONV as DFT-as-GEMM on the matrix cores (FP32, m1 in {32, 48, 64, 96},
+// m2 m3 a multiple of 64; the opt-in FTS_SCHED_CONV_MFMA)
 enum PencilMode { PENCIL_FWD_MUL_INV = 0, PENCIL_FWD = 1, PENCIL_INV = 2, PENCIL_MUL = 3, PENCIL_MUL_INV = 4, PENCIL_CONV = 5, PENCIL_FWD_F2D = 6,
-                  PENCIL_MUL_INV_D2F = 7, PENCIL_FWD_MUL_INV_NOISE = 8 };
+                  PENCIL_MUL_INV_D2F = 7, PENCIL_FWD_MUL_INV_NOISE = 8, PENCIL_CONV_MFMA = 9 };
 
 template <typename T> struct PlaneJob {
     const V2<T>* src;
@@ -233,6 +235,11 @@ struct FftGeom {
 
 // True when the plane and pencil kernels exist for this grid and precision.
 bool fft_supported(int m1, int m2, int m3, bool fp32);
+// True when the x pass of FTS_SCHED_CONV_MFMA exists for this grid (FP32 only)
+bool conv_mfma_supported(int m1, int m2, int m3, bool fp32);
+// the conv schedule and its matrix-core form share every pass but the x pass
+inline bool sched_is_conv(int sched) { return sched == 3 || sched == 5; }
+inline int conv_pencil_mode(int sched) { return sched == 5 ? PENCIL_CONV_MFMA : PENCIL_CONV; }
 
 // Launch one plane pass over all x-planes for njobs (1 or 2) jobs.
 template <typename T>

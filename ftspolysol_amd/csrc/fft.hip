@@ -696,6 +696,124 @@ __global__ void __launch_bounds__((pencil_threads_for<N1, MODE, ZCW>())) pencil_
     }
 }
 
+// ---------------------------------------------------------------------------
+// The x pass of the conv schedule as DFT-as-GEMM on the matrix cores (FP32, the
+// opt-in FTS_SCHED_CONV_MFMA): a length N1 = 16 R line is split n = R n1 + n2,
+// k = k1 + 16 k2.  The DFT-16 over n1 runs on v_mfma_f32_16x16x4f32 (a 16 x 16
+// complex matrix times 16 columns of the tile, as four real products per complex
+// one), the twiddle e^{-2 pi i n2 k1 / N1} and the DFT-R over n2 in registers; then
+// x phi_x(k) c0 and the same steps inverse.  One wave holds 16 columns in registers
+// from load to store (no LDS).  Operand layout of the instruction: A[i][kk] from lane
+// i + 16 kk, B[kk][j] from lane j + 16 kk, D[4 q + r][j] in register r of lane
+// j + 16 q; chunk s of the sum over n1 (forward) takes n1 = 4 s + kk, and chunk s of
+// the sum over k1 (inverse) takes k1 = 4 kk + s, the register s a lane holds.
+// ---------------------------------------------------------------------------
+constexpr int MFMA_WAVES = 4; // waves (16 columns each) per workgroup
+
+__device__ __forceinline__ fts_f4 mfma4(float a, float b, fts_f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+template <int N1>
+__global__ void __launch_bounds__(64 * MFMA_WAVES) __attribute__((amdgpu_waves_per_eu(3))) pencil_mfma_kernel(PencilJob<float> j0, PencilJob<float> j1, const PencilJob<float>* __restrict__ table, FftGeom g)
+{
+    constexpr int R = N1 / 16;
+    const PencilJob<float> job = pick_job<float>(j0, j1, table);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = lane >> 4, c = lane & 15;
+    const size_t sx = static_cast<size_t>(g.m2) * g.m3;
+    const size_t col = static_cast<size_t>(blockIdx.x) * (16 * MFMA_WAVES) + wave * 16 + c;
+    const float2* in = job.src ? job.src : job.data;
+    const float2* twg = static_cast<const float2*>(g.tw_x); // e^{-2 pi i t / N1}, t < N1
+    // DFT-16 entries this lane feeds: forward F[c][4 s + q], inverse conj F[c][4 q + s],
+    // F[k][n] = e^{-2 pi i k n / 16} = twg[((k n) mod 16) R]
+    float far[4], fai[4], iar[4], iai[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const float2 wf = twg[((c * (4 * s + q)) & 15) * R];
+        const float2 wi = twg[((c * (4 * q + s)) & 15) * R];
+        far[s] = wf.x;
+        fai[s] = wf.y;
+        iar[s] = wi.x;
+        iai[s] = -wi.y;
+    }
+    // forward DFT-16 over n1 for each n2: D = sum_s A_s B_s, complex as four real products
+    // (the four points of one n2 are loaded just before its products: fewer live registers)
+    fts_f4 dr[R], di[R];
+#pragma unroll
+    for (int n2 = 0; n2 < R; ++n2) {
+        float2 xv[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) xv[s] = ld_point<float>(in + static_cast<size_t>(R * (4 * s + q) + n2) * sx + col);
+        fts_f4 a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            a = mfma4(far[s], xv[s].x, a);
+            a = mfma4(-fai[s], xv[s].y, a);
+            b = mfma4(far[s], xv[s].y, b);
+            b = mfma4(fai[s], xv[s].x, b);
+        }
+        dr[n2] = a;
+        di[n2] = b;
+    }
+    // register r holds k1 = 4 q + r: twiddle, DFT-R over n2, x phi_x c0, inverse DFT-R, twiddle
+    const double* sep = job.op.sep;
+    const double c0 = job.op.c0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k1 = 4 * q + r;
+        float2 y[R];
+#pragma unroll
+        for (int n2 = 0; n2 < R; ++n2) {
+            y[n2] = make_float2(dr[n2][r], di[n2][r]);
+            if (n2 != 0 && k1 != 0) y[n2] = cmul(y[n2], twg[n2 * k1]);
+        }
+        Dft<float2, R, -1>::run(y);
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) y[k2] = cscale(y[k2], static_cast<float>(sep[k1 + 16 * k2] * c0));
+        Dft<float2, R, +1>::run(y);
+#pragma unroll
+        for (int n2 = 0; n2 < R; ++n2) {
+            if (n2 != 0 && k1 != 0) {
+                float2 t = twg[n2 * k1];
+                t.y = -t.y;
+                y[n2] = cmul(y[n2], t);
+            }
+            dr[n2][r] = y[n2].x;
+            di[n2][r] = y[n2].y;
+        }
+    }
+    // inverse DFT-16 over k1 for each n2; lane holds x[R (4 q + r) + n2] of its column
+    float2* out = job.data;
+#pragma unroll
+    for (int n2 = 0; n2 < R; ++n2) {
+        fts_f4 a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            a = mfma4(iar[s], dr[n2][s], a);
+            a = mfma4(-iai[s], di[n2][s], a);
+            b = mfma4(iar[s], di[n2][s], b);
+            b = mfma4(iai[s], dr[n2][s], b);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[static_cast<size_t>(R * (4 * q + r) + n2) * sx + col] = make_float2(a[r], b[r]);
+    }
+}
+
+template <typename T, int N1>
+hipError_t pencil_mfma_go(const FftGeom& g, const PencilJob<T>* jobs, const PencilJob<T>* table, int njobs, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 4 && (N1 == 32 || N1 == 48 || N1 == 64 || N1 == 96)) {
+        const size_t cols = static_cast<size_t>(g.m2) * g.m3;
+        if (cols % (16 * MFMA_WAVES) != 0) return hipErrorInvalidValue;
+        dim3 grid(static_cast<unsigned>(cols / (16 * MFMA_WAVES)), njobs);
+        const PencilJob<T> none{};
+        hipLaunchKernelGGL((pencil_mfma_kernel<N1>), grid, dim3(64 * MFMA_WAVES), 0, s, jobs ? jobs[0] : none, jobs ? jobs[njobs > 1 ? 1 : 0] : none,
+                           table, g);
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
 template <typename T, int N, int MODE>
 hipError_t plane_go(const FftGeom& g, const PlaneJob<T>* jobs, const PlaneJob<T>* table, int njobs, hipStream_t s)
 {
@@ -758,6 +876,7 @@ hipError_t pencil_mode(const FftGeom& g, int mode, const PencilJob<T>* jobs, con
     case PENCIL_FWD_F2D: return pencil_go<T, N1, PENCIL_FWD_F2D>(g, jobs, table, njobs, s);
     case PENCIL_MUL_INV_D2F: return pencil_go<T, N1, PENCIL_MUL_INV_D2F>(g, jobs, table, njobs, s);
     case PENCIL_FWD_MUL_INV_NOISE: return pencil_go<T, N1, PENCIL_FWD_MUL_INV_NOISE>(g, jobs, table, njobs, s);
+    case PENCIL_CONV_MFMA: return pencil_mfma_go<T, N1>(g, jobs, table, njobs, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -778,6 +897,12 @@ bool line_supported(int m1, int m2, int m3)
     auto ok = [](int n) { return n == 192 || n == 256 || in_sizes(n); };
     if (!ok(m1) || !ok(m2) || !ok(m3)) return false;
     return (m2 * m3) % LINE_TC == 0 && (m1 * m3) % LINE_TC == 0 && (m1 * m2) % LINE_TC == 0;
+}
+
+bool conv_mfma_supported(int m1, int m2, int m3, bool fp32)
+{
+    return fp32 && fft_supported(m1, m2, m3, fp32) && (m1 == 32 || m1 == 48 || m1 == 64 || m1 == 96) &&
+           (static_cast<size_t>(m2) * m3) % (16 * MFMA_WAVES) == 0;
 }
 
 bool fft_supported(int m1, int m2, int m3, bool fp32)

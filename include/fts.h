@@ -61,6 +61,8 @@ extern "C" {
                              store q_{s+1}); separable bond kernels; needs m2 == m3, plane in LDS */
 #define FTS_SCHED_SEPARABLE 4 /* three line passes, one bond-kernel factor each: x (FFT of q_s, x phi_x/M, IFFT),
                                  z (FFT, x phi_z, IFFT), y (FFT, x phi_y, IFFT, x e^{-W}, store q_{s+1}); any supported lengths */
+#define FTS_SCHED_CONV_MFMA 5 /* FTS_SCHED_CONV with its x pass as DFT-as-GEMM on the matrix cores (DFT-16 stage on
+                                 v_mfma_f32_16x16x4f32); FP32, m1 in {32, 48, 64, 96}; used only when asked for */
 
 /* stress_mask bits: components in the reference's fixed order (stress.cu:56-61) */
 #define FTS_STRESS_XX 0x01
@@ -173,7 +175,7 @@ typedef struct fts_info {
     int n_segment_classes; /* distinct (type, charge) bead classes: e^{-W} fields kept per species */
     int precision;
     int scheme;
-    int fft_schedule; /* FTS_SCHED_PLANE, _LINE, _CONV or _SEPARABLE in use */
+    int fft_schedule; /* FTS_SCHED_PLANE, _LINE, _CONV, _SEPARABLE or _CONV_MFMA in use */
     double volume;
     double rho0;
     double short_range_self_pressure; /* param.cu:135-149 */

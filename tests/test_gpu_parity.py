@@ -8,7 +8,7 @@ engine 1e-4 relative to FP64 for H, mu and stress, 2e-5 for a bead step.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
+from ftspolysol_amd.api import (SCHED_CONV, SCHED_CONV_MFMA, FtsError, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
                                 FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
                                 SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
 from ftspolysol_amd.system import System, initial_fields, make_system
@@ -82,6 +82,25 @@ def test_bead_step_line_schedules_match_oracle(m, bond, sched):
     assert rel(g.bead_step(q, emw, bond), ref) < 1e-13
     s32 = bare_system(m, precision=PREC_FP32, fft_schedule=sched)
     assert rel(hip_engine(s32).bead_step(q, emw, bond), ref) < 2e-5
+
+
+@pytest.mark.parametrize("m,bond", [((32, 32, 32), 1.3), ((96, 96, 96), 1.0), ((48, 24, 24), 0.7), ((64, 32, 32), 1.1)])
+def test_bead_step_conv_mfma_matches_oracle(m, bond):
+    """The conv schedule with its x pass as DFT-as-GEMM on the matrix cores (FP32 only),
+    against the FP64 oracle at the FP32 bar; FP64 asks for it are refused."""
+    print(f"bead step conv_mfma {m} b={bond}", flush=True)
+    s = bare_system(m, fft_schedule=SCHED_CONV)
+    o = oracle_engine(s)
+    rng = np.random.default_rng(7)
+    q = 1 + 0.1 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    w = 0.2 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    emw = np.exp(-w)
+    ref = o.bead_step(q, emw, bond)
+    g32 = hip_engine(bare_system(m, precision=PREC_FP32, fft_schedule=SCHED_CONV_MFMA))
+    assert g32.info.fft_schedule == SCHED_CONV_MFMA
+    assert rel(g32.bead_step(q, emw, bond), ref) < 2e-5
+    with pytest.raises(FtsError):
+        hip_engine(bare_system(m, fft_schedule=SCHED_CONV_MFMA))
 
 
 def test_autotuner_times_both_schedules():
