@@ -139,7 +139,7 @@ def run_leg(systems, label, *args, **kw):
 
 
 def run_leg_steps(systems, label, warmup, steps, record_steps, ckpt_every=50, max_restarts=50, lrk=None,
-                  lrk_threshold=1e-3, stats=True, min_equilibration=MIN_EQUILIBRATION, graph_level=2):
+                  lrk_threshold=1e-3, stats=True, min_equilibration=MIN_EQUILIBRATION, graph_level=2, streams=-1):
     """One leg: equilibrate max(warmup, MIN_EQUILIBRATION) untimed steps, time exactly
     `steps` steps (recording H every step), then extend the record of H untimed to
     `record_steps` steps when it is shorter.  Returns the rate of the timed steps and the
@@ -147,14 +147,16 @@ def run_leg_steps(systems, label, warmup, steps, record_steps, ckpt_every=50, ma
     b = hip_batch(systems)
     try:
         return leg_steps(b, systems, label, warmup, steps, record_steps, ckpt_every, max_restarts, lrk, lrk_threshold,
-                         stats, min_equilibration, graph_level)
+                         stats, min_equilibration, graph_level, streams)
     finally:
         b.close()
 
 
 def leg_steps(b, systems, label, warmup, steps, record_steps, ckpt_every, max_restarts, lrk, lrk_threshold, stats,
-              min_equilibration, graph_level):
+              min_equilibration, graph_level, streams=-1):
     b.set_graph(graph_level)
+    if streams >= 0:
+        b.set_streams(streams)
     for r, (e, s) in enumerate(zip(b.replicas, systems)):
         om, ps = initial_fields(s, sigma=1e-3, seed=0x5EED0004 + r)
         e.set_fields(om, ps)
@@ -569,6 +571,8 @@ def main():
     ap.add_argument("--fast-only", action="store_true", help="run the fast leg alone and print its summary (profiling runs)")
     ap.add_argument("--learned-vs-debye", action="store_true",
                     help="run the fast (learned) and Debye semi-implicit legs only and print their comparison")
+    ap.add_argument("--streams", type=int, default=-1,
+                    help="side streams per replica group for the replicas' own work (-1: the library's default)")
     ap.add_argument("--graph-level", type=int, choices=[0, 1, 2], default=2,
                     help="batch launches: 2 replays each whole CL step as a captured graph, 1 only the sweeps, 0 none")
     ap.add_argument("--equilibration", type=int, default=MIN_EQUILIBRATION,
@@ -611,7 +615,7 @@ def main():
     prec = max(args.plain_record_steps, args.steps)
     W, K = args.warmup, args.steps
 
-    EQ = dict(min_equilibration=args.equilibration, graph_level=args.graph_level)
+    EQ = dict(min_equilibration=args.equilibration, graph_level=args.graph_level, streams=args.streams)
     fast = run_leg(fp32(SCHEME_SI_LEARNED, args.dt_fast), f"fast fp32 si_learned x{args.replicas}", W, K, rec, lrk=lrk,
                    lrk_threshold=args.lrk_threshold, **EQ)
     debye = None if (args.no_debye or args.fast_only) else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_debye),

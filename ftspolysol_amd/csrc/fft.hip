@@ -300,7 +300,7 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
     // FP64 bead step: e^{-W} read into registers now, so its latency hides behind the
     // transforms (the FP64 plane fills the LDS, so one workgroup per CU, 3 waves per SIMD,
     // leaves the registers for it)
-    constexpr bool EPF = sizeof(T) == 8 && MODE == PLANE_CONV_E && (SPLIT || (N * N) % NT == 0);
+    constexpr bool EPF = MODE == PLANE_CONV_E && (SPLIT || (sizeof(T) == 8 && (N * N) % NT == 0));
     constexpr int NPF = EPF ? (SPLIT ? SN1 : N * N / NT) : 1;
     V epf[NPF];
     if constexpr (EPF) {
