@@ -14,15 +14,14 @@ every leg has the same seed and fields):
               kernel (FTS_SCHEME_SI_LEARNED; network trained by
               ftspolysol_amd.train_lrk on 48^3 data, lrk_c4_48cubed.npy), judged
               every step by the exact force, falling back to the Debye kernel when
-              its excess force residual passes 1e-3, at --dt-fast (0.02: twice
-              the plain dt; the learned kernel is there for a larger stable dt,
-              and at 0.02 its <H> bias matches the Debye scheme's at 0.01).
-              `value` is this leg.
+              its excess force residual passes 1e-3, at --dt-fast (0.01, the
+              plain dt).  `value` is this leg.
   si_debye    the same batch and seeds with the Debye kernel only
-              (FTS_SCHEME_SEMI_IMPLICIT) at --dt-debye (0.01, the plain dt): what
-              the learned kernel gains or loses.  A leg whose replica exhausts
-              its non-finite restarts (the Debye scheme at dt 0.04) is reported
-              as failed.
+              (FTS_SCHEME_SEMI_IMPLICIT) at --dt-debye (0.01): what the learned
+              kernel gains or loses.  --learned-vs-debye runs these two legs alone
+              (at larger dt the learned kernel's gain is stability and a smaller
+              <H> bias); a leg whose replica exhausts its non-finite restarts (the
+              Debye scheme at dt 0.04) is reported as failed.
   plain_fp64_batched  FP64, the reference's predictor-corrector
               (complexLangevin.cu:669-698), a batch of --plain-replicas: the plain
               engine at its own best throughput, and the plain scheme's tau_int and
@@ -558,10 +557,10 @@ def main():
                     help="steps of the H record of the FP32 legs (tau_int, <H>), timed steps included")
     ap.add_argument("--plain-record-steps", type=int, default=2000, help="the same for the plain FP64 batch")
     ap.add_argument("--replicas", type=int, default=16, help="replicas in the FP32 batches")
-    ap.add_argument("--plain-replicas", type=int, default=8, help="replicas in the plain FP64 batch")
-    ap.add_argument("--dt-fast", type=float, default=0.02,
-                    help="time step of the learned leg: the learned kernel is there for a larger stable dt; at 0.02 its "
-                         "<H> bias matches the Debye scheme's at 0.01, and at 0.04 it stays finite where Debye does not")
+    ap.add_argument("--plain-replicas", type=int, default=16, help="replicas in the plain FP64 batch")
+    ap.add_argument("--dt-fast", type=float, default=0.01,
+                    help="time step of the learned leg (the learned kernel keeps dt 0.04 finite where the Debye scheme "
+                         "diverges, and lowers the <H> bias at dt >= 0.02; see --learned-vs-debye)")
     ap.add_argument("--dt-debye", type=float, default=0.01, help="time step of the Debye semi-implicit leg")
     ap.add_argument("--dt-plain", type=float, default=0.01)
     ap.add_argument("--lrk", default=LRK_PARAMS, help="parameters of the learned kernel (.npy)")

@@ -1,11 +1,13 @@
 // fft.hip -- plane and pencil FFT passes for gfx950 (see fft.h).
 //
-// Both passes stage their tile in LDS and run the Stockham line FFT of
-// fts_device.h on it, so a pass costs one HBM read and one HBM write of the
-// field.  The plane pass holds a whole m2 x m3 plane (96 x 97 complex doubles
-// = 149 KB of gfx950's 160 KB LDS at 96^3 FP64); the pencil pass holds m1 rows
-// of 16 consecutive points of the flattened (y, z) plane, so each of its row
-// reads is one contiguous 256 B (FP64) / 128 B (FP32) segment.
+// Both passes hold their tile in LDS, so a pass costs one HBM read and one HBM
+// write of the field.  The plane pass holds a whole m2 x m3 plane (96 x 97
+// complex doubles = 149 KB of gfx950's 160 KB LDS at 96^3 FP64); the pencil pass
+// holds m1 rows of 16 or 32 consecutive points of the flattened (y, z) plane, so
+// each of its row reads is one contiguous segment.  The bead-step and field
+// transforms run the register-split transforms of fft_reg.h, whose first and last
+// steps read and write global memory directly; the other modes run the staged
+// Stockham line FFT of fts_device.h.
 #include "fft.h"
 #include "fft_reg.h"
 
@@ -297,9 +299,10 @@ __global__ void __launch_bounds__((plane_threads_for<N, MODE>())) plane_kernel(P
             lds[(i / N) * LS + (i % N)] = v;
         }
     }
-    // FP64 bead step: e^{-W} read into registers now, so its latency hides behind the
-    // transforms (the FP64 plane fills the LDS, so one workgroup per CU, 3 waves per SIMD,
-    // leaves the registers for it)
+    // bead step: e^{-W} read into registers now, so its latency hides behind the
+    // transforms.  The FP64 plane fills the LDS (one workgroup per CU, 3 waves per SIMD),
+    // which leaves the registers for it; the FP32 split pass stays at 79 VGPRs, within
+    // its 6 waves per SIMD (measured 115.8 -> 108.8 us per 16-job group at 96^3)
     constexpr bool EPF = MODE == PLANE_CONV_E && (SPLIT || (sizeof(T) == 8 && (N * N) % NT == 0));
     constexpr int NPF = EPF ? (SPLIT ? SN1 : N * N / NT) : 1;
     V epf[NPF];
@@ -713,6 +716,7 @@ constexpr int MFMA_WAVES = 4; // waves (16 columns each) per workgroup
 __device__ __forceinline__ fts_f4 mfma4(float a, float b, fts_f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
 template <int N1>
+// (3 waves per SIMD: 138 VGPRs; the compiler's own choice was 2, and 4 spills)
 __global__ void __launch_bounds__(64 * MFMA_WAVES) __attribute__((amdgpu_waves_per_eu(3))) pencil_mfma_kernel(PencilJob<float> j0, PencilJob<float> j1, const PencilJob<float>* __restrict__ table, FftGeom g)
 {
     constexpr int R = N1 / 16;
