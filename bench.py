@@ -486,10 +486,20 @@ def run_c5(args):
     print("[bench c5] warmed up", file=sys.stderr, flush=True)
     e.set_timing(True)
     t0 = time.perf_counter()
-    recs = e.step(args.steps, record_every=args.c5_stress_every)
+    recs = []
+    done = 0
+    chunk = max(args.c5_stress_every, 250 // args.c5_stress_every * args.c5_stress_every)
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
+        recs += e.step(k, record_every=args.c5_stress_every)
+        done += k
+        if args.steps > chunk:
+            print(f"[bench c5] {done}/{args.steps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     el = time.perf_counter() - t0
     t = e.get_timing()
     e.set_timing(False)
+    info = e.get_info()
+    lrk_stats = e.lrk_get_stats()
     print(f"[bench c5] {args.steps} timed steps in {el:.2f} s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     rec = e.measure()
@@ -502,7 +512,11 @@ def run_c5(args):
                                  "stress_every": args.c5_stress_every},
                       "bead_step_us": us, "bead_step_alg_GBps": t["algorithmic_bytes"] / max(1e-30, t["sweep_ms"] * 1e-3) / 1e9,
                       "stress_records": len(recs), "measure_with_stress_s": measure_s,
-                      "stress_finite": bool(np.all(np.isfinite(rec["stress"])))}))
+                      "stress_finite": bool(np.all(np.isfinite(rec["stress"]))),
+                      "records_finite": bool(all(np.isfinite(x["H"]) and np.all(np.isfinite(x["stress"])) for x in recs)),
+                      "restarts": int(info.restarts), "learned_steps": int(lrk_stats["learned_steps"]),
+                      "judged_steps": int(lrk_stats["judged"]),
+                      "H_records": [float(x["H"].real) for x in recs[:: max(1, len(recs) // 20)]]}))
 
 
 def run_beadstep(args):

@@ -143,3 +143,46 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120, cwd=ROOT)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and not r.stdout.strip()
+
+
+def test_bench_reports_a_failed_leg(monkeypatch):
+    """A leg whose replica exhausts its non-finite restarts (the library's FTS_NONFINITE
+    error out of fts_batch_step) is reported with the reason and no statistics, and the
+    batch is released; the run goes on.  A stand-in batch raises the error the binding
+    raises, so no device is touched."""
+    import bench
+    from ftspolysol_amd.api import FTS_NONFINITE, FtsError
+    from ftspolysol_amd.system import make_system
+
+    closed = []
+
+    class Replica:
+        def set_fields(self, *a):
+            pass
+
+        def set_restart(self, *a):
+            pass
+
+    class FailingBatch:
+        def __init__(self, systems):
+            self.replicas = [Replica() for _ in systems]
+
+        def set_graph(self, level):
+            pass
+
+        def step(self, n, record_every=0):
+            raise FtsError(FTS_NONFINITE, "replica 1: updated fields have non-finite values at step 252; "
+                                          "maximum non-finite restarts (50) exceeded")
+
+        def close(self):
+            closed.append(True)
+
+    monkeypatch.setattr(bench, "hip_batch", FailingBatch)
+    systems = [make_system(16, 4, dt=0.04, seed=r) for r in range(2)]
+    leg = bench.run_leg(systems, "stand-in", 5, 20, 100)
+    assert "maximum non-finite restarts" in leg["failed"]
+    assert leg["samples_per_s"] is None and leg["tau_int"] is None and leg["H_mean"] is None
+    assert leg["dt"] == 0.04 and leg["replicas"] == 2
+    assert closed == [True]
+    assert bench.speedup(leg, {"samples_per_s": 1.0, "tau_int": 1.0, "tau_int_err": 0.1}) is None
+    assert set(bench.leg_summary(leg)) >= {"failed", "dt", "replicas"}
