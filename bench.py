@@ -83,6 +83,7 @@ from ftspolysol_amd.system import initial_fields, make_system  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 LRK_PARAMS = os.path.join(ROOT, "ftspolysol_amd", "lrk_c4_48cubed.npy")
+LRK_PARAMS_C5 = os.path.join(ROOT, "ftspolysol_amd", "lrk_c5_48cubed.npy")
 
 
 def autocorr(x):
@@ -479,7 +480,7 @@ def run_c5(args):
     from ftspolysol_amd.system import config
     s = config(5, precision=PREC_FP32, scheme=SCHEME_SI_LEARNED, stress_mask=0x7F, seed=0xC5)
     e = hip_engine(s)
-    e.lrk_set_network(load_params(args.lrk), args.lrk_threshold)
+    e.lrk_set_network(load_params(args.lrk_c5), args.lrk_threshold)
     e.set_fields(*initial_fields(s, sigma=1e-3, cfg=5))
     e.set_restart(50, 20)
     e.step(max(args.warmup, 5))
@@ -509,7 +510,7 @@ def run_c5(args):
     print(json.dumps({"metric": "C5 large system: CL steps/s at 256^3 N=200", "value": args.steps / el, "unit": "CL steps/s",
                       "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "ms_per_step": 1e3 * el / args.steps,
                       "config": {"workload": "256^3 L=128 N=200 z=-1 on odd beads salt=0.02 B=1 lB=0.7; FP32 + learned SI",
-                                 "stress_every": args.c5_stress_every},
+                                 "stress_every": args.c5_stress_every, "learned_kernel": os.path.basename(args.lrk_c5)},
                       "bead_step_us": us, "bead_step_alg_GBps": t["algorithmic_bytes"] / max(1e-30, t["sweep_ms"] * 1e-3) / 1e9,
                       "stress_records": len(recs), "measure_with_stress_s": measure_s,
                       "stress_finite": bool(np.all(np.isfinite(rec["stress"]))),
@@ -578,6 +579,8 @@ def main():
     ap.add_argument("--dt-debye", type=float, default=0.01, help="time step of the Debye semi-implicit leg")
     ap.add_argument("--dt-plain", type=float, default=0.01)
     ap.add_argument("--lrk", default=LRK_PARAMS, help="parameters of the learned kernel (.npy)")
+    ap.add_argument("--lrk-c5", default=LRK_PARAMS_C5 if os.path.exists(LRK_PARAMS_C5) else LRK_PARAMS,
+                    help="parameters of the learned kernel for --config c5 (trained on C5's chemistry)")
     ap.add_argument("--lrk-threshold", type=float, default=1e-3)
     ap.add_argument("--with-fp32-pc", action="store_true", help="add the FP32 predictor-corrector batch")
     ap.add_argument("--no-debye", action="store_true", help="skip the FP32 Debye semi-implicit batch")

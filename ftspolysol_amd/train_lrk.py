@@ -22,17 +22,17 @@ import numpy as np
 
 from .api import LRK_RECORD, SCHEME_SEMI_IMPLICIT, PREC_FP32, hip_batch, hip_lrk
 from .lrk import init_params, network_r, relative_residuals, save_params
-from .system import initial_fields, make_system
+from .system import CONFIGS, initial_fields, make_system
 
 
 def generate_records(n=48, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, z=-0.5, dt=0.01, trajectories=200, steps=1000,
-                     burn=100, precision=PREC_FP32, seed0=0x7A1E0000, group=50, chunk=100, log=sys.stderr):
+                     burn=100, precision=PREC_FP32, seed0=0x7A1E0000, group=50, chunk=100, log=sys.stderr, charge="uniform"):
     """Records of `trajectories` plain semi-implicit runs (groups of `group` replicas per batch)."""
     out = []
     t0 = time.perf_counter()
     for g0 in range(0, trajectories, group):
         R = min(group, trajectories - g0)
-        systems = [make_system(n, N, rho=rho, salt=salt, B=B, lB=lB, z=z, dt=dt, scheme=SCHEME_SEMI_IMPLICIT,
+        systems = [make_system(n, N, rho=rho, salt=salt, B=B, lB=lB, charge=charge, z=z, dt=dt, scheme=SCHEME_SEMI_IMPLICIT,
                                precision=precision, seed=seed0 + g0 + r) for r in range(R)]
         b = hip_batch(systems)
         try:
@@ -92,9 +92,12 @@ def train(records, epochs=20, batch_records=64, lr0=3e-3, lr1=1e-4, seed=1, val_
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", required=True)
+    ap.add_argument("--config", type=int, choices=[4, 5], default=4,
+                    help="chemistry of the training systems: 4 (N 100, charge -0.5 per bead, salt 0.05, B 0.5, lB 1) or 5 "
+                         "(N 200, charge -1 on every second bead, salt 0.02, B 1, lB 0.7); --N and --salt override")
     ap.add_argument("--grid", type=int, default=48)
-    ap.add_argument("--N", type=int, default=100)
-    ap.add_argument("--salt", type=float, default=0.05)
+    ap.add_argument("--N", type=int, default=None)
+    ap.add_argument("--salt", type=float, default=None)
     ap.add_argument("--dt", type=float, default=0.01)
     ap.add_argument("--trajectories", type=int, default=200)
     ap.add_argument("--steps", type=int, default=1000)
@@ -107,12 +110,17 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
-    rec = generate_records(n=args.grid, N=args.N, salt=args.salt, dt=args.dt, trajectories=args.trajectories,
-                           steps=args.steps, burn=args.burn, group=args.group)
+    chem = {k: CONFIGS[args.config][k] for k in ("N", "rho", "salt", "B", "lB", "charge", "z")}
+    if args.N is not None:
+        chem["N"] = args.N
+    if args.salt is not None:
+        chem["salt"] = args.salt
+    rec = generate_records(n=args.grid, dt=args.dt, trajectories=args.trajectories, steps=args.steps, burn=args.burn,
+                           group=args.group, **chem)
     t_gen = time.perf_counter() - t0
     P, report = train(rec, epochs=args.epochs, batch_records=args.batch_records, lr0=args.lr0, lr1=args.lr1,
                       seed=args.seed)
-    report.update({"grid": args.grid, "N": args.N, "salt": args.salt, "dt": args.dt, "trajectories": args.trajectories,
+    report.update({"config": args.config, "grid": args.grid, **chem, "dt": args.dt, "trajectories": args.trajectories,
                    "steps": args.steps, "burn": args.burn, "records": int(len(rec)), "generate_seconds": t_gen,
                    "data": "plain semi-implicit (Debye kernel) CL, FP32 propagators, synthetic initial fields"})
     save_params(args.out, P, report)
