@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 FTS_OK, FTS_NONFINITE, FTS_INVALID, FTS_DEVICE_ERROR = 0, 1, 2, 3
-SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED = 0, 1, 2, 3
+SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, SCHEME_SIPC, SCHEME_SIPC_LEARNED = 0, 1, 2, 3, 4, 5
 PREC_FP64, PREC_FP32 = 0, 1
 SCHED_AUTO, SCHED_PLANE, SCHED_LINE, SCHED_CONV, SCHED_SEPARABLE, SCHED_CONV_MFMA = 0, 1, 2, 3, 4, 5
 SYMMETRY_AUTO, SYMMETRY_OFF = 0, 1

@@ -562,7 +562,8 @@ template <typename T> int group_step_queue(Group& g, int scheme, const std::vect
         fts_engine* e = g.reps[r];
         int c;
         if (scheme == FTS_SCHEME_PC_REFERENCE) c = e->pc_correct(rec[r] != nullptr, rec[r]);
-        else c = e->em_update(rec[r] != nullptr, rec[r], scheme == FTS_SCHEME_SEMI_IMPLICIT || scheme == FTS_SCHEME_SI_LEARNED);
+        else if (scheme_two_stage(scheme)) c = e->sipc_correct(rec[r] != nullptr, rec[r]);
+        else c = e->em_update(rec[r] != nullptr, rec[r], e->si_scheme());
         if (c) return c;
         ++e->step_counter;
         if (e->drift_stats)
@@ -570,9 +571,10 @@ template <typename T> int group_step_queue(Group& g, int scheme, const std::vect
                                    hipMemcpyDeviceToHost, e->stream));
         return e->finite_queue(g.hflag + r);
     };
-    if (scheme == FTS_SCHEME_PC_REFERENCE) {
+    if (scheme_two_stage(scheme)) {
+        const bool pc = scheme == FTS_SCHEME_PC_REFERENCE;
         if ((rc = group_densities<T>(g, false, mask)) || (rc = group_queue_Q(g, 0, mask))) return rc;
-        if ((rc = group_each(g, mask, [&](size_t r) { return g.reps[r]->pc_predict(); }))) return rc;
+        if ((rc = group_each(g, mask, [&](size_t r) { return pc ? g.reps[r]->pc_predict() : g.reps[r]->sipc_predict(); }))) return rc;
         if ((rc = group_densities<T>(g, true, mask)) || (rc = group_queue_Q(g, 1, mask))) return rc;
         if ((rc = group_each(g, mask, update))) return rc;
     } else {
@@ -594,7 +596,7 @@ int group_step_check(Group& g, int scheme, const std::vector<fts_observables*>& 
                      std::vector<std::string>& why)
 {
     int rc;
-    const bool pc = scheme == FTS_SCHEME_PC_REFERENCE;
+    const bool pc = scheme_two_stage(scheme);
     if (g.timing) resolve_stamps(g);
     for (size_t r = 0; r < g.reps.size(); ++r) {
         if (!mask[r]) continue;

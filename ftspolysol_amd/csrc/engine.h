@@ -277,10 +277,16 @@ struct fts_engine {
     size_t pin_bytes = 0;
     int step_pc(bool record, fts_observables* rec);
     int step_em(bool record, fts_observables* rec, bool si);
+    int step_sipc(bool record, fts_observables* rec);
     // the parts of a CL step between its density computations
     int pc_predict();                                          // after the densities at (w, psi)
     int pc_correct(bool record, fts_observables* rec);         // after the densities at (w_bar, psi_bar)
     int em_update(bool record, fts_observables* rec, bool si); // after the densities at (w, psi)
+    int sipc_predict();                                        // after the densities at (w, psi)
+    int sipc_correct(bool record, fts_observables* rec);       // after the densities at (w_bar, psi_bar)
+    // the semi-implicit predictor-corrector's per-step fields: the predictor's increments
+    // w_bar - w (omega modes, then psi), kept for the corrector's (1 - P2) share
+    DevBuf<double2> sipc_d, sipc_dpsi;
     int finish_step();
     // drift statistics (complexLangevin.cu:318-324, 466-497): with drift_stats set, every
     // step queues max over sites of |lam_l/2 (u_l + u_bar_l)| per mode and |lam_psi/2
@@ -322,10 +328,18 @@ struct fts_engine {
     long long lrk_read_idx = 0;
     bool lrk_ready = false, lrk_have_prev = false;
     double lrk_threshold = 1.0e-3;
-    bool lrk_active() const { return scheme == FTS_SCHEME_SI_LEARNED || lrk_capacity > 0; }
+    bool lrk_active() const { return learned_scheme() || lrk_capacity > 0; }
+    bool learned_scheme() const { return scheme == FTS_SCHEME_SI_LEARNED || scheme == FTS_SCHEME_SIPC_LEARNED; }
+    bool si_scheme() const { return scheme == FTS_SCHEME_SEMI_IMPLICIT || scheme == FTS_SCHEME_SIPC || learned_scheme(); }
     int lrk_setup();
     int lrk_step();
 };
+
+// Schemes whose step computes the densities twice (at w and at the predictor w_bar)
+inline bool scheme_two_stage(int scheme)
+{
+    return scheme == FTS_SCHEME_PC_REFERENCE || scheme == FTS_SCHEME_SIPC || scheme == FTS_SCHEME_SIPC_LEARNED;
+}
 
 // Builds an engine for system s.  shared: a stream the engine uses instead of
 // creating its own (a replica batch's); tune: run the schedule autotuner when

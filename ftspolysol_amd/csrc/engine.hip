@@ -37,7 +37,7 @@ void fts_engine::release()
     tw32_z.release();
     for (DevBuf<double2>* b : {&omega, &omega_bar, &omega_s, &drift, &psi, &psi_bar, &psi_s, &dpsi, &dpsi_bar, &W_type_s, &rho_hat,
                                &rho_type, &rho_c_hat, &rho_c, &rho_c_poly, &rho_c_ion, &rho_p, &ion_psi, &ion_b, &ion_rho, &t1, &t2,
-                               &t3, &psi_hat, &mode_coef, &partials, &scal})
+                               &t3, &psi_hat, &mode_coef, &partials, &scal, &sipc_d, &sipc_dpsi})
         b->release();
     flag.release();
     ckpt_omega.release();
@@ -507,6 +507,10 @@ int fts_engine::build_si_tables()
     HIP_TRY(si_A_psi.alloc(Ap.size()));
     if (!Aw.empty()) HIP_TRY(hipMemcpy(si_A_omega.p, Aw.data(), sizeof(double) * Aw.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(si_A_psi.p, Ap.data(), sizeof(double) * Ap.size(), hipMemcpyHostToDevice));
+    if (scheme_two_stage(scheme)) {
+        HIP_TRY(sipc_d.alloc(static_cast<size_t>(n_omega) * M));
+        HIP_TRY(sipc_dpsi.alloc(static_cast<size_t>(M)));
+    }
     return FTS_OK;
 }
 
@@ -601,7 +605,7 @@ int fts_engine::lrk_setup()
 
 // One judged step: called by the semi-implicit update once u(w_t) is known (lrk_u after
 // k_mode0_drift, dpsi after psi_drift).  Judges the previous step, writes its record,
-// decides this step's kernel and (SI_LEARNED) rebuilds the P tables the update applies.
+// decides this step's kernel and (the learned schemes) rebuilds the P tables the update applies.
 int fts_engine::lrk_step()
 {
     int rc;
@@ -642,8 +646,8 @@ int fts_engine::lrk_step()
                                f1 ? lrk_hat[1][0][cur].p : nullptr, f1 ? lrk_hat[1][0][prv].p : nullptr, f1 ? lrk_hat[1][1][cur].p : nullptr,
                                f1 ? lrk_hat[1][1][prv].p : nullptr, s32 ? 1 : 0, lrk_chunk_sums.p, stream));
     HIP_TRY(lrk_judge(lrk_state.p, lrk_have_prev ? 1 : 0, lrk_bin_chunk.p, lrk_chunk_sums.p, lrk_threshold,
-                      scheme == FTS_SCHEME_SI_LEARNED ? 1 : 0, lrk_ring.p, lrk_capacity, stream));
-    if (scheme == FTS_SCHEME_SI_LEARNED)
+                      learned_scheme() ? 1 : 0, lrk_ring.p, lrk_capacity, stream));
+    if (learned_scheme())
         HIP_TRY(lrk_ptable(lrk_state.p, lrk_bin.p, f0 ? si_A_omega.p : nullptr, f1 ? si_A_psi.p : nullptr,
                            f0 ? dt * lambda_omega[0] : 0.0, dt * lambda_psi, M, f0 ? si_table_omega.p : nullptr, f1 ? si_table_psi.p : nullptr,
                            stream));
@@ -1862,6 +1866,142 @@ int fts_engine::em_update(bool record, fts_observables* rec, bool si)
     return FTS_OK;
 }
 
+// Semi-implicit predictor-corrector (FTS_SCHEME_SIPC / _SIPC_LEARNED; no reference
+// counterpart), with P = 1 / (1 + a), a = dt lam A(k) from this step's tables, and
+// P2 = 2 P / (1 + P) = 1 / (1 + a/2):
+//   predictor  d = S_P[-dt lam u(w)] + S_Pn[eta],  w_bar = w + d        (the semi-implicit step)
+//   corrector  w <- w + S_P2[-(dt lam / 2)(u(w) + u(w_bar))] + S_P2[eta] + S_{1-P2}[d]
+// with one k-space draw of eta for both.  For a linear drift u = A w this is
+// Crank-Nicolson, w <- (1 - a/2)/(1 + a/2) w + P2 eta, whose stationary variance is 1 / A
+// at any dt.  The record is taken at the predictor state (as the predictor-corrector's,
+// complexLangevin.cu:680-684).
+int fts_engine::step_sipc(bool record, fts_observables* rec)
+{
+    int rc = compute_densities(omega.p, psi.p);
+    if (rc || (rc = check_Q("Q")) || (rc = sipc_predict())) return rc;
+    if ((rc = compute_densities(omega_bar.p, psi_bar.p)) || (rc = check_Q("Qbar"))) return rc;
+    if ((rc = sipc_correct(record, rec))) return rc;
+    return finish_step();
+}
+
+// predictor, after the densities at (w, psi): drift and dpsi keep -dt lam u(w), sipc_d and
+// sipc_dpsi the increments d
+int fts_engine::sipc_predict()
+{
+    if (psi_hat_of == psi_bar.p) psi_hat_of = nullptr; // psi_bar is rewritten below
+    const size_t Ms = static_cast<size_t>(M);
+    const double invM = 1.0 / static_cast<double>(M), rM = 1.0 / std::sqrt(static_cast<double>(M));
+    int rc;
+    if ((rc = key_queue())) return rc; // the step's noise key (the corrector draws the same eta)
+    const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
+    if ((rc = psi_drift(psi.p, rho_c.p, dpsi.p))) return rc;
+    if (drift_stats && (rc = drift_queue(false))) return rc;
+    if (lrk_active() && (rc = lrk_step())) return rc;
+    if (n_omega > 0) {
+        hipLaunchKernelGGL(k_omega_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, rho_type.p, nkey.p, M, 0, omega.p, drift.p, 0);
+        HIP_TRY(hipGetLastError());
+    }
+    if (electro()) {
+        hipLaunchKernelGGL(k_psi_increment, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, dpsi.p, -dt * lambda_psi, sigma_psi,
+                           scal.p + SC_ETA_MEAN, nkey.p, M, dpsi.p, 0);
+        HIP_TRY(hipGetLastError());
+    }
+    FftJob jb[FTS_MAX_TYPES + 1];
+    int nj = 0;
+    for (int l = 0; l < n_omega; ++l) {
+        KOp op{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0};
+        op.nkey = nkey.p;
+        op.nfield = SI_NOISE_FIELD + 2u * static_cast<unsigned>(l);
+        op.namp = mc.sigma[l] * rM;
+        jb[nj++] = FftJob{f(drift, l), f(sipc_d, l), op};
+    }
+    if (electro()) {
+        KOp op{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0};
+        op.nkey = nkey.p;
+        op.nfield = SI_NOISE_FIELD_PSI;
+        op.namp = sigma_psi * rM;
+        op.nzero = 1;
+        jb[nj++] = FftJob{dpsi.p, sipc_dpsi.p, op};
+    }
+    if ((rc = fft_apply_n(jb, nj))) return rc;
+    if (n_omega > 0) {
+        const size_t n = static_cast<size_t>(n_omega) * Ms;
+        hipLaunchKernelGGL(k_add_out, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream, omega.p, sipc_d.p, n, omega_bar.p);
+        HIP_TRY(hipGetLastError());
+    }
+    if (electro()) {
+        hipLaunchKernelGGL(k_add_out, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, sipc_dpsi.p, Ms, psi_bar.p);
+        HIP_TRY(hipGetLastError());
+        if ((rc = remove_mean(psi_bar.p))) return rc;
+    } else if ((rc = zero_async(psi_bar.p, Ms))) {
+        return rc;
+    }
+    return FTS_OK;
+}
+
+// corrector, after the densities at (w_bar, psi_bar); the record is taken at the predictor state
+int fts_engine::sipc_correct(bool record, fts_observables* rec)
+{
+    const size_t Ms = static_cast<size_t>(M);
+    const double invM = 1.0 / static_cast<double>(M), rM = 1.0 / std::sqrt(static_cast<double>(M));
+    int rc;
+    const double sigma_psi = electro() ? (noise ? 1.0 : 0.0) * std::sqrt(2.0 * lambda_psi * dt / dV) : 0.0;
+    if ((rc = psi_drift(psi_bar.p, rho_c.p, dpsi_bar.p))) return rc;
+    if (record && (rc = measure(omega_bar.p, psi_bar.p, rec))) return rc;
+    // r = (-dt lam u(w) - dt lam u(w_bar)) / 2 in place of -dt lam u(w)
+    if (n_omega > 0) {
+        hipLaunchKernelGGL(k_omega_midpoint, dim3(ew_blocks(M)), dim3(EW_THREADS), 0, stream, mc, omega_bar.p, rho_type.p, M, drift.p);
+        HIP_TRY(hipGetLastError());
+    }
+    if (electro()) {
+        hipLaunchKernelGGL(k_midpoint, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, dpsi_bar.p, -dt * lambda_psi, Ms, dpsi.p);
+        HIP_TRY(hipGetLastError());
+    }
+    // S_P2[r] + S_P2[eta] and S_{1-P2}[d], every field's in the same launches
+    FftJob jb[2 * (FTS_MAX_TYPES + 1)];
+    int nj = 0;
+    for (int l = 0; l < n_omega; ++l) {
+        KOp op{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0};
+        op.table_form = KOP_CN;
+        op.nkey = nkey.p;
+        op.nfield = SI_NOISE_FIELD + 2u * static_cast<unsigned>(l);
+        op.namp = mc.sigma[l] * rM;
+        op.ncn = 1;
+        jb[nj++] = FftJob{f(drift, l), f(drift, l), op};
+        KOp od{nullptr, si_table_omega.p + l * Ms, invM, 0.0, 0.0, 0, 0};
+        od.table_form = KOP_CNR;
+        jb[nj++] = FftJob{f(sipc_d, l), f(sipc_d, l), od};
+    }
+    if (electro()) {
+        KOp op{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0};
+        op.table_form = KOP_CN;
+        op.nkey = nkey.p;
+        op.nfield = SI_NOISE_FIELD_PSI;
+        op.namp = sigma_psi * rM;
+        op.nzero = 1;
+        op.ncn = 1;
+        jb[nj++] = FftJob{dpsi.p, dpsi.p, op};
+        KOp od{nullptr, si_table_psi.p, invM, 0.0, 0.0, 0, 0};
+        od.table_form = KOP_CNR;
+        jb[nj++] = FftJob{sipc_dpsi.p, sipc_dpsi.p, od};
+    }
+    if ((rc = fft_apply_n(jb, nj))) return rc;
+    if (n_omega > 0) {
+        const size_t n = static_cast<size_t>(n_omega) * Ms;
+        hipLaunchKernelGGL(k_axpy2, dim3(ew_blocks(n)), dim3(EW_THREADS), 0, stream, omega.p, drift.p, sipc_d.p, n);
+        HIP_TRY(hipGetLastError());
+    }
+    psi_hat_of = nullptr; // psi changes below
+    if (electro()) {
+        hipLaunchKernelGGL(k_axpy2, dim3(ew_blocks(Ms)), dim3(EW_THREADS), 0, stream, psi.p, dpsi.p, sipc_dpsi.p, Ms);
+        HIP_TRY(hipGetLastError());
+        if ((rc = remove_mean(psi.p))) return rc;
+    } else if ((rc = zero_async(psi.p, Ms))) {
+        return rc;
+    }
+    return FTS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Checkpoint and non-finite restart
 // ---------------------------------------------------------------------------
@@ -1981,7 +2121,7 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
     if (s->n_species <= 0 || s->n_species > FTS_MAX_SPECIES || !s->species) return fail(FTS_INVALID, "polymer species_count");
-    if (s->scheme < 0 || s->scheme > 3) return fail(FTS_INVALID, "scheme");
+    if (s->scheme < 0 || s->scheme > 5) return fail(FTS_INVALID, "scheme");
     fts_engine* e = new fts_engine();
     if (shared) {
         e->stream = shared;
@@ -2105,9 +2245,8 @@ int engine_create(const fts_system* s, hipStream_t shared, bool tune, fts_engine
     }
     if ((rc = noalloc_setup(&e->noalloc_scratch))) return bail(rc);
     if ((rc = e->setup_device())) return bail(rc);
-    const bool semi = e->scheme == FTS_SCHEME_SEMI_IMPLICIT || e->scheme == FTS_SCHEME_SI_LEARNED;
-    if (semi && (rc = e->build_si_tables())) return bail(rc);
-    if (e->scheme == FTS_SCHEME_SI_LEARNED && (rc = e->lrk_setup())) return bail(rc);
+    if (e->si_scheme() && (rc = e->build_si_tables())) return bail(rc);
+    if (e->learned_scheme() && (rc = e->lrk_setup())) return bail(rc);
     if (tune && s->fft_schedule == FTS_SCHED_AUTO && (rc = e->autotune_schedule())) return bail(rc);
     *out = e;
     return FTS_OK;
@@ -2341,9 +2480,9 @@ int fts_step(fts_engine* h, int n, int record_every, fts_observables* records, i
         const long long step = h->step_counter + 1; // 1-based step label (complexLangevin.cu:206)
         const bool record = record_every > 0 && step % record_every == 0 && records && nrec < capacity;
         fts_observables* rec = record ? &records[nrec] : nullptr;
-        rc = h->scheme == FTS_SCHEME_PC_REFERENCE
-                 ? h->step_pc(record, rec)
-                 : h->step_em(record, rec, h->scheme == FTS_SCHEME_SEMI_IMPLICIT || h->scheme == FTS_SCHEME_SI_LEARNED);
+        rc = h->scheme == FTS_SCHEME_PC_REFERENCE ? h->step_pc(record, rec)
+             : scheme_two_stage(h->scheme)        ? h->step_sipc(record, rec)
+                                                  : h->step_em(record, rec, h->si_scheme());
         if (record) {
             rec->step = step;
             ++nrec;
@@ -2560,7 +2699,7 @@ int fts_lrk_set_recording(fts_engine* h, int capacity)
 {
     if (!h || capacity < 0) return fail(FTS_INVALID, "lrk_set_recording: bad argument");
     ++h->config_epoch;
-    if (capacity > 0 && h->scheme != FTS_SCHEME_SEMI_IMPLICIT && h->scheme != FTS_SCHEME_SI_LEARNED)
+    if (capacity > 0 && !h->si_scheme())
         return fail(FTS_INVALID, "training records need a semi-implicit scheme");
     int rc;
     if (capacity > 0 && (rc = h->lrk_setup())) return rc;

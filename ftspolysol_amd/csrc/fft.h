@@ -29,21 +29,28 @@ struct KOp {
     const double* table;
     double c0, c1, c2;
     int ka, kb;
-    int table_noise; // multiply by sqrt(t (1 + t) / 2) instead of the table value t
+    int table_form; // the factor taken from the table value t (kop_table)
     // k-space noise of a semi-implicit step (namp != 0): the multiply pass adds
-    // namp sqrt(t (1 + t) / 2) g(k), g the spectrum of real white noise of unit
-    // variance per site divided by sqrt(M) (kop_noise); nzero drops k = 0.  nkey: the
-    // device noise key {seed, step} of the step (read when the pass runs, so a
-    // replayed graph draws the noise of the step it replays)
+    // namp f(t) g(k), g the spectrum of real white noise of unit variance per site
+    // divided by sqrt(M) (kop_noise), f(t) = sqrt(t (1 + t) / 2), or 2 t / (1 + t) when
+    // ncn; nzero drops k = 0.  nkey: the device noise key {seed, step} of the step (read
+    // when the pass runs, so a replayed graph draws the noise of the step it replays)
     const unsigned long long* nkey;
     unsigned nfield;
     double namp;
     int nzero;
+    int ncn;
 };
 
-// the table factor of a KOp: t, or sqrt(t (1 + t) / 2) (the noise factor that goes with a
-// semi-implicit table t = 1 / (1 + a))
-__host__ __device__ __forceinline__ double kop_table(const KOp& op, double t) { return op.table_noise ? sqrt(0.5 * t * (1.0 + t)) : t; }
+// Factors of a semi-implicit table t = P = 1 / (1 + a), a = dt lam A(k):
+//   KOP_T   t                        the semi-implicit step's filter
+//   KOP_CN  2 t / (1 + t) = 1/(1 + a/2)   the predictor-corrector's corrector (Crank-Nicolson)
+//   KOP_CNR (1 - t) / (1 + t) = 1 - KOP_CN  its share of the predictor's increment
+enum KopForm { KOP_T = 0, KOP_CN = 2, KOP_CNR = 3 };
+__host__ __device__ __forceinline__ double kop_table(const KOp& op, double t)
+{
+    return op.table_form == KOP_CN ? 2.0 * t / (1.0 + t) : op.table_form == KOP_CNR ? (1.0 - t) / (1.0 + t) : t;
+}
 
 // PLANE_FWD_D2F / PLANE_INV_F2D: PLANE_FWD reading src64, PLANE_INV writing dst64 (complex
 // FP64 fields transformed in the pass's precision)
@@ -127,11 +134,12 @@ __device__ __forceinline__ KOp pick_kop(KOp a, KOp b, bool f)
     k.c2 = f ? a.c2 : b.c2;
     k.ka = f ? a.ka : b.ka;
     k.kb = f ? a.kb : b.kb;
-    k.table_noise = f ? a.table_noise : b.table_noise;
+    k.table_form = f ? a.table_form : b.table_form;
     k.nkey = f ? a.nkey : b.nkey;
     k.nfield = f ? a.nfield : b.nfield;
     k.namp = f ? a.namp : b.namp;
     k.nzero = f ? a.nzero : b.nzero;
+    k.ncn = f ? a.ncn : b.ncn;
     return k;
 }
 
@@ -151,7 +159,7 @@ __device__ __forceinline__ double2 kop_noise(const KOp& op, int m1, int m2, int 
     const size_t pi = (static_cast<size_t>((m1 - x) % m1) * m2 + (m2 - y) % m2) * m3 + (m3 - z) % m3;
     if (op.nzero && gi == 0) return r;
     const double t = op.table[gi];
-    const double amp = op.namp * sqrt(0.5 * t * (1.0 + t));
+    const double amp = op.namp * (op.ncn ? 2.0 * t / (1.0 + t) : sqrt(0.5 * t * (1.0 + t)));
     const double2 ab = cl_normal_pair<F32>(op.nkey[0], op.nkey[1], op.nfield, static_cast<uint32_t>(gi < pi ? gi : pi));
     if (pi == gi) {
         r.x = amp * ab.x;

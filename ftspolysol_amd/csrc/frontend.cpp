@@ -296,6 +296,8 @@ int fts_deck::apply(const std::string& sec, const std::string& key, const std::s
             else if (t == "em" || t == "euler_maruyama") scheme = FTS_SCHEME_EM;
             else if (t == "si" || t == "semi_implicit") scheme = FTS_SCHEME_SEMI_IMPLICIT;
             else if (t == "si_learned" || t == "learned") scheme = FTS_SCHEME_SI_LEARNED;
+            else if (t == "sipc" || t == "si_pc") scheme = FTS_SCHEME_SIPC;
+            else if (t == "sipc_learned") scheme = FTS_SCHEME_SIPC_LEARNED;
             else ok = false;
         } else if (key == "precision") {
             std::string t = unquote(v);

@@ -498,6 +498,41 @@ __global__ void k_psi_increment(const double2* __restrict__ dpsi, double c1, dou
     r[i] = v;
 }
 
+// out = a + b
+__global__ void k_add_out(const double2* __restrict__ a, const double2* __restrict__ b, size_t n, double2* __restrict__ out)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * EW_THREADS + threadIdx.x;
+    if (i < n) out[i] = cadd(a[i], b[i]);
+}
+
+// a = (a + b) + c
+__global__ void k_axpy2(double2* __restrict__ a, const double2* __restrict__ b, const double2* __restrict__ c, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * EW_THREADS + threadIdx.x;
+    if (i < n) a[i] = cadd(cadd(a[i], b[i]), c[i]);
+}
+
+// r = (r + c u) / 2: the trapezoid of the semi-implicit predictor-corrector, r holding
+// -dt lam u(w) and u = u(w_bar), c = -dt lam
+__global__ void k_midpoint(const double2* __restrict__ u, double c, size_t n, double2* __restrict__ r)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * EW_THREADS + threadIdx.x;
+    if (i < n) r[i] = cscale(cadd(r[i], cscale(u[i], c)), 0.5);
+}
+
+// the same for every omega mode, u(w_bar) from the densities at the predictor
+__global__ void k_omega_midpoint(ModeCoefs mc, const double2* __restrict__ om_bar, const double2* __restrict__ rho_bar, int M,
+                                 double2* __restrict__ r)
+{
+    const int i = blockIdx.x * EW_THREADS + threadIdx.x;
+    if (i >= M) return;
+    for (int l = 0; l < mc.nO; ++l) {
+        const size_t o = static_cast<size_t>(l) * M + i;
+        const double2 ub = mode_drift(mc, l, om_bar, rho_bar, M, i);
+        r[o] = cscale(cadd(r[o], cscale(ub, -mc.lam_dt[l])), 0.5);
+    }
+}
+
 // a += b (count fields)
 __global__ void k_axpy1(double2* __restrict__ a, const double2* __restrict__ b, size_t n)
 {

@@ -36,6 +36,8 @@ extern "C" {
 #define FTS_SCHEME_EM 1           /* plain explicit Euler-Maruyama (no reference counterpart) */
 #define FTS_SCHEME_SEMI_IMPLICIT 2 /* k-space Debye-preconditioned semi-implicit EM (no reference counterpart) */
 #define FTS_SCHEME_SI_LEARNED 3    /* semi-implicit EM with the learned linear-response kernel (no reference counterpart) */
+#define FTS_SCHEME_SIPC 4          /* semi-implicit predictor-corrector, Debye kernel (no reference counterpart; see below) */
+#define FTS_SCHEME_SIPC_LEARNED 5  /* the same with the learned linear-response kernel (no reference counterpart) */
 
 #define FTS_PREC_FP64 0 /* complex FP64 everywhere, as the reference (common.h:23) */
 #define FTS_PREC_FP32 1 /* complex FP32 propagators and FFTs (the FP64 fields are transformed in FP32 on the plane
@@ -453,7 +455,20 @@ int fts_rf_write(const char* path, const int* m, const double* L, int n_fields, 
  * (moments at its start, E, S1, S2), and the loss of a record is
  *     sum_c sum_b (S2_{c,b} / E_c) (e^{r_{c,b}} - S1_{c,b} / S2_{c,b})^2
  *     = sum_c (R_c(learned) - min_r R_c) / E_c.
- * The batch loss is the mean over its records. */
+ * The batch loss is the mean over its records.
+ *
+ * The semi-implicit predictor-corrector (FTS_SCHEME_SIPC with A = A_D,
+ * FTS_SCHEME_SIPC_LEARNED with the learned A, judged as above) takes the
+ * semi-implicit step above as its predictor, w_bar = w + S_P[-dt lam u(w)] + S_Pn[eta],
+ * and then, with the densities at w_bar,
+ *     w <- w + S_P2[-(dt lam / 2) (u(w) + u(w_bar))] + S_P2[eta] + S_{1-P2}[w_bar - w],
+ *     P2(k) = 1 / (1 + dt lam A(k) / 2) = 2 P / (1 + P),
+ * with the same eta (the same k-space draw).  For a linear drift u = A w this is the
+ * trapezoidal (Crank-Nicolson) step w <- (1 - a/2) / (1 + a/2) w + P2 eta, a = dt lam A:
+ * second order, stable at every dt, and the stationary variance of every mode is
+ * exactly 1 / A.  psi likewise (k = 0 dropped, mean removed).
+ * Records are taken at the predictor state with its Q, as the reference's
+ * predictor-corrector takes them. */
 #define FTS_LRK_FEATURES 7
 #define FTS_LRK_IN 8
 #define FTS_LRK_HIDDEN 64
@@ -495,8 +510,9 @@ typedef struct fts_lrk_stats {
  * threshold 1e-3. */
 int fts_lrk_set_network(fts_engine* h, const float* params, double threshold);
 /* Write a training record for every judged step into a device ring of `capacity`
- * records (0 stops recording and frees it).  Needs FTS_SCHEME_SEMI_IMPLICIT or
- * FTS_SCHEME_SI_LEARNED, and 1 <= n_omega + (lB > 0) with n_omega <= 1. */
+ * records (0 stops recording and frees it).  Needs one of the semi-implicit schemes
+ * (FTS_SCHEME_SEMI_IMPLICIT, _SI_LEARNED, _SIPC, _SIPC_LEARNED), and
+ * 1 <= n_omega + (lB > 0) with n_omega <= 1. */
 int fts_lrk_set_recording(fts_engine* h, int capacity);
 /* Copy the records written since the last call (at most max_records, the oldest
  * first; older ones are lost if the ring wrapped) and empty the ring. */

@@ -170,7 +170,9 @@ struct fts_engine {
     fts_lrk_stats lrk_st{};
     int lrk_capacity = 0;
     std::deque<std::vector<double>> lrk_ring;
-    bool lrk_active() const { return scheme == FTS_SCHEME_SI_LEARNED || lrk_capacity > 0; }
+    bool learned_scheme() const { return scheme == FTS_SCHEME_SI_LEARNED || scheme == FTS_SCHEME_SIPC_LEARNED; }
+    bool si_scheme() const { return scheme == FTS_SCHEME_SEMI_IMPLICIT || scheme == FTS_SCHEME_SIPC || learned_scheme(); }
+    bool lrk_active() const { return learned_scheme() || lrk_capacity > 0; }
     void lrk_build_bins();
     void lrk_moments(double* m);
     void lrk_step(std::vector<double> (&mult)[2]);
@@ -336,6 +338,8 @@ struct fts_engine {
         }
     }
     int step_em(bool record, fts_observables* rec, bool semi_implicit);
+    int step_sipc(bool record, fts_observables* rec);
+    void add_kspace_noise(Cx* w, double sigma, unsigned field, bool drop_zero, const std::vector<double>& P, bool cn);
 };
 
 // ---- interaction modes: interactionMode.cu:9-298 --------------------------
@@ -734,7 +738,7 @@ void fts_engine::lrk_step(std::vector<double> (&mult)[2])
             ++lrk_st.records;
         }
     }
-    if (scheme == FTS_SCHEME_SI_LEARNED) {
+    if (learned_scheme()) {
         if (use) ++lrk_st.learned_steps;
         else ++lrk_st.fallback_steps;
         if (use)
@@ -1305,31 +1309,7 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     // real-space eta to the drift.
     const bool split = semi_implicit && noise;
     std::vector<double> P(static_cast<size_t>(M));
-    std::vector<Cx> nz(split ? static_cast<size_t>(M) : 0);
-    auto add_noise = [&](Cx* w, double sigma, unsigned field, bool drop_zero) {
-        const double amp0 = sigma / std::sqrt(static_cast<double>(M));
-        for (int x = 0; x < m1; ++x)
-            for (int y = 0; y < m2; ++y)
-                for (int z = 0; z < m3; ++z) {
-                    const int gi = (x * m2 + y) * m3 + z;
-                    const int pi = (((m1 - x) % m1) * m2 + (m2 - y) % m2) * m3 + (m3 - z) % m3;
-                    const double t = P[static_cast<size_t>(gi)];
-                    const double amp = amp0 * std::sqrt(0.5 * t * (1.0 + t));
-                    Cx v{0.0, 0.0};
-                    if (drop_zero && gi == 0) {
-                    } else if (pi == gi) {
-                        v.x = amp * oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field, static_cast<uint32_t>(gi));
-                    } else {
-                        double ab[2];
-                        oracle::cl_normal_pair(seed, static_cast<uint64_t>(step_counter), field, static_cast<uint32_t>(std::min(gi, pi)), ab);
-                        const double h = amp * 0.70710678118654752440;
-                        v = Cx{h * ab[0], gi < pi ? h * ab[1] : -h * ab[1]};
-                    }
-                    nz[static_cast<size_t>(gi)] = v;
-                }
-        fft->run(nz.data(), +1);
-        axpy(w, nz.data(), 1.0);
-    };
+    auto add_noise = [&](Cx* w, double sigma, unsigned field, bool drop_zero) { add_kspace_noise(w, sigma, field, drop_zero, P, false); };
     for (int l = 0; l < n_omega; ++l) {
         const double lam = lambda_omega[static_cast<size_t>(l)];
         Cx* w = fld(omega, l);
@@ -1372,6 +1352,115 @@ int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
     return FTS_OK;
 }
 
+// The k-space noise of a semi-implicit step added to w: IFFT[sigma sqrt(M) f(P) g(k)] / M
+// with g the Hermitian spectrum of include/fts.h (one Philox draw per pair {k, -k}; field
+// `field`, the step's counter), f(P) = sqrt(P (1 + P) / 2) for the semi-implicit step and
+// f(P) = P2 = 2 P / (1 + P) for the predictor-corrector's corrector (cn); k = 0 dropped
+// when drop_zero.
+void fts_engine::add_kspace_noise(Cx* w, double sigma, unsigned field, bool drop_zero, const std::vector<double>& P, bool cn)
+{
+    std::vector<Cx> nz(static_cast<size_t>(M));
+    const double amp0 = sigma / std::sqrt(static_cast<double>(M));
+    for (int x = 0; x < m1; ++x)
+        for (int y = 0; y < m2; ++y)
+            for (int z = 0; z < m3; ++z) {
+                const int gi = (x * m2 + y) * m3 + z;
+                const int pi = (((m1 - x) % m1) * m2 + (m2 - y) % m2) * m3 + (m3 - z) % m3;
+                const double t = P[static_cast<size_t>(gi)];
+                const double amp = amp0 * (cn ? 2.0 * t / (1.0 + t) : std::sqrt(0.5 * t * (1.0 + t)));
+                Cx v{0.0, 0.0};
+                if (drop_zero && gi == 0) {
+                } else if (pi == gi) {
+                    v.x = amp * oracle::cl_normal(seed, static_cast<uint64_t>(step_counter), field, static_cast<uint32_t>(gi));
+                } else {
+                    double ab[2];
+                    oracle::cl_normal_pair(seed, static_cast<uint64_t>(step_counter), field, static_cast<uint32_t>(std::min(gi, pi)), ab);
+                    const double h = amp * 0.70710678118654752440;
+                    v = Cx{h * ab[0], gi < pi ? h * ab[1] : -h * ab[1]};
+                }
+                nz[static_cast<size_t>(gi)] = v;
+            }
+    fft->run(nz.data(), +1);
+    axpy(w, nz.data(), 1.0);
+}
+
+// Semi-implicit predictor-corrector (FTS_SCHEME_SIPC / _SIPC_LEARNED, no reference
+// counterpart; include/fts.h states it):
+//   predictor  w_bar = w + S_P[-dt lam u(w)] + S_Pn[eta]                      (the SI step)
+//   corrector  w    <- w + S_P2[-(dt lam / 2)(u(w) + u(w_bar))] + S_P2[eta] + S_{1-P2}[w_bar - w]
+// P = 1 / (1 + a), P2 = 2 P / (1 + P) = 1 / (1 + a / 2), a = dt lam A(k) (A_D, times the
+// learned factor when the judge keeps it), one k-space draw of eta for both.  The record
+// is taken at the predictor state with its Q (as step_pc, complexLangevin.cu:680-684).
+int fts_engine::step_sipc(bool record, fts_observables* rec)
+{
+    std::vector<Cx> Qp = compute_densities(omega.data(), psi.data());
+    for (const Cx& q : Qp)
+        if (!finite(q)) return fail(FTS_NONFINITE, "non-finite Q");
+    compute_drifts(omega.data(), drift.data());
+    compute_psi_drift(psi.data(), rho_c.data(), dpsi.data());
+    drift_statistics(false);
+    std::vector<double> mult[2];
+    if (lrk_active()) lrk_step(mult);
+    const size_t Ms = static_cast<size_t>(M);
+    // per field: P, -dt lam u(w) (u0) and the predictor's increment (d)
+    const int nf = n_omega + (electrostatics() ? 1 : 0);
+    std::vector<std::vector<double>> P(static_cast<size_t>(nf), std::vector<double>(Ms));
+    std::vector<Field> u0(static_cast<size_t>(nf), Field(Ms)), d(static_cast<size_t>(nf), Field(Ms));
+    for (int f = 0; f < nf; ++f) {
+        const bool is_psi = f == n_omega;
+        const double lam = is_psi ? lambda_psi : lambda_omega[static_cast<size_t>(f)];
+        const Cx* u = is_psi ? dpsi.data() : fld(drift, f);
+        for (size_t i = 0; i < Ms; ++i) {
+            const double A = is_psi ? A_psi[i] * (!mult[1].empty() ? mult[1][i] : 1.0)
+                                    : A_omega[static_cast<size_t>(f) * Ms + i] * (f == 0 && !mult[0].empty() ? mult[0][i] : 1.0);
+            P[static_cast<size_t>(f)][i] = 1.0 / (1.0 + dt * lam * A);
+            u0[static_cast<size_t>(f)][i] = cscale(u[i], -dt * lam);
+        }
+        apply_kernel(u0[static_cast<size_t>(f)].data(), d[static_cast<size_t>(f)].data(), P[static_cast<size_t>(f)].data());
+        if (noise)
+            add_kspace_noise(d[static_cast<size_t>(f)].data(), std::sqrt(2.0 * lam * dt / dV), is_psi ? 0x500u : 0x400u + 2u * static_cast<unsigned>(f),
+                             is_psi, P[static_cast<size_t>(f)], false);
+    }
+    for (int l = 0; l < n_omega; ++l) {
+        copy(fld(omega_bar, l), fld(omega, l));
+        axpy(fld(omega_bar, l), d[static_cast<size_t>(l)].data(), 1.0);
+    }
+    copy(psi_bar.data(), psi.data());
+    if (electrostatics()) axpy(psi_bar.data(), d[static_cast<size_t>(n_omega)].data(), 1.0);
+    else clear(psi_bar.data());
+    remove_psi_zero_mode(psi_bar.data());
+    Qp = compute_densities(omega_bar.data(), psi_bar.data());
+    for (const Cx& q : Qp)
+        if (!finite(q)) return fail(FTS_NONFINITE, "non-finite Qbar");
+    compute_psi_drift(psi_bar.data(), rho_c.data(), dpsi_bar.data());
+    if (record) measure(Qp, omega_bar.data(), psi_bar.data(), rec);
+    compute_drifts(omega_bar.data(), drift_bar.data());
+    std::vector<double> P2(Ms), Q2(Ms);
+    Field r(Ms);
+    for (int f = 0; f < nf; ++f) {
+        const bool is_psi = f == n_omega;
+        const double lam = is_psi ? lambda_psi : lambda_omega[static_cast<size_t>(f)];
+        const Cx* ub = is_psi ? dpsi_bar.data() : fld(drift_bar, f);
+        for (size_t i = 0; i < Ms; ++i) {
+            const double t = P[static_cast<size_t>(f)][i];
+            P2[i] = 2.0 * t / (1.0 + t);
+            Q2[i] = (1.0 - t) / (1.0 + t);
+            r[i] = cscale(cadd(u0[static_cast<size_t>(f)][i], cscale(ub[i], -dt * lam)), 0.5);
+        }
+        Cx* w = is_psi ? psi.data() : fld(omega, f);
+        apply_kernel(r.data(), r.data(), P2.data());
+        if (noise) add_kspace_noise(r.data(), std::sqrt(2.0 * lam * dt / dV), is_psi ? 0x500u : 0x400u + 2u * static_cast<unsigned>(f), is_psi, P[static_cast<size_t>(f)], true);
+        apply_kernel(d[static_cast<size_t>(f)].data(), d[static_cast<size_t>(f)].data(), Q2.data());
+        axpy(w, r.data(), 1.0);
+        axpy(w, d[static_cast<size_t>(f)].data(), 1.0);
+    }
+    remove_psi_zero_mode(psi.data());
+    ++step_counter;
+    inject();
+    if (!fields_finite()) return fail(FTS_NONFINITE, "updated fields have non-finite values");
+    return FTS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI (prefix fto_)
 // ---------------------------------------------------------------------------
@@ -1403,7 +1492,7 @@ int fto_create(const fts_system* s, fts_engine** out)
     if (s->a <= 0.0 || s->dL <= 0.0 || s->lB < 0.0) return fail(FTS_INVALID, "a, dL, or lB");
     if (s->dt < 0.0 || s->lambda_psi < 0.0) return fail(FTS_INVALID, "dt or lambdaPsi");
     if (s->n_species <= 0 || s->n_species > FTS_MAX_SPECIES || !s->species) return fail(FTS_INVALID, "polymer species_count");
-    if (s->scheme < 0 || s->scheme > 3) return fail(FTS_INVALID, "scheme");
+    if (s->scheme < 0 || s->scheme > 5) return fail(FTS_INVALID, "scheme");
     fts_engine* e = new fts_engine();
     e->m1 = s->m1;
     e->m2 = s->m2;
@@ -1482,11 +1571,11 @@ int fto_create(const fts_system* s, fts_engine** out)
     }
     e->fft = new oracle::Fft3d(e->m1, e->m2, e->m3);
     e->build_kernels();
-    if (e->scheme == FTS_SCHEME_SI_LEARNED && e->n_omega > 1) {
+    if (e->learned_scheme() && e->n_omega > 1) {
         delete e;
         return fail(FTS_INVALID, "the learned kernel covers one interaction mode and psi (n_omega <= 1)");
     }
-    if (e->scheme == FTS_SCHEME_SEMI_IMPLICIT || e->scheme == FTS_SCHEME_SI_LEARNED) {
+    if (e->si_scheme()) {
         e->build_si_kernels();
         e->lrk_build_bins();
     }
@@ -1633,6 +1722,7 @@ int fto_step(fts_engine* h, int n, int record_every, fts_observables* records, i
         fts_observables* rec = record ? &records[nrec] : nullptr;
         int rc;
         if (h->scheme == FTS_SCHEME_PC_REFERENCE) rc = h->step_pc(record, rec);
+        else if (h->scheme == FTS_SCHEME_SIPC || h->scheme == FTS_SCHEME_SIPC_LEARNED) rc = h->step_sipc(record, rec);
         else rc = h->step_em(record, rec, h->scheme == FTS_SCHEME_SEMI_IMPLICIT || h->scheme == FTS_SCHEME_SI_LEARNED);
         if (record) {
             rec->step = step;
@@ -1795,7 +1885,7 @@ int fto_lrk_set_network(fts_engine* h, const float* params, double threshold)
 int fto_lrk_set_recording(fts_engine* h, int capacity)
 {
     if (!h || capacity < 0) return fail(FTS_INVALID, "bad argument");
-    if (capacity > 0 && h->scheme != FTS_SCHEME_SEMI_IMPLICIT && h->scheme != FTS_SCHEME_SI_LEARNED)
+    if (capacity > 0 && !h->si_scheme())
         return fail(FTS_INVALID, "training records need a semi-implicit scheme");
     if (capacity > 0 && (h->n_omega > 1 || (h->n_omega == 0 && !h->electrostatics())))
         return fail(FTS_INVALID, "training records need n_omega <= 1 and at least one field");

@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, PREC_FP32, PREC_FP64, SCHED_CONV, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SCHEME_EM,
-                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SYMMETRY_OFF, hip_batch, hip_engine)
+                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SIPC, SYMMETRY_OFF, hip_batch,
+                                hip_engine)
 from ftspolysol_amd.system import initial_fields, make_system
 
 pytestmark = pytest.mark.gpu
@@ -46,7 +47,8 @@ def start_fields(engines, systems):
                                                     (SCHEME_EM, PREC_FP64, SCHED_CONV),
                                                     (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_CONV),
                                                     (SCHEME_SEMI_IMPLICIT, PREC_FP64, SCHED_SEPARABLE),
-                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_SEPARABLE)])
+                                                    (SCHEME_PC_REFERENCE, PREC_FP32, SCHED_SEPARABLE),
+                                                    (SCHEME_SIPC, PREC_FP32, SCHED_CONV)])
 def test_batch_matches_single_engines(scheme, precision, sched):
     print(f"batch scheme={scheme} precision={precision} sched={sched}", flush=True)
     systems = replica_systems(scheme, precision, sched)
@@ -104,7 +106,7 @@ def test_batch_headline_autotuned():
 
 @pytest.mark.parametrize("scheme,sched", [(SCHEME_PC_REFERENCE, SCHED_PLANE), (SCHEME_PC_REFERENCE, SCHED_LINE),
                                           (SCHEME_SEMI_IMPLICIT, SCHED_CONV), (SCHEME_EM, SCHED_CONV),
-                                          (SCHEME_EM, SCHED_SEPARABLE)])
+                                          (SCHEME_EM, SCHED_SEPARABLE), (SCHEME_SIPC, SCHED_CONV)])
 def test_graph_replays_match_eager_launches(scheme, sched):
     """Whole-step graphs (level 2), sweep graphs (1) and launches issued one by one (0):
     several step calls with records on every step, on none and on every second step (so

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, LRK_FEATURES, LRK_NPARAMS, LRK_OFF, LrkNet, SCHEME_SEMI_IMPLICIT,
-                                SCHEME_SI_LEARNED, hip_batch, hip_engine, hip_lrk)
+                                SCHEME_SI_LEARNED, SCHEME_SIPC_LEARNED, hip_batch, hip_engine, hip_lrk)
 from ftspolysol_amd.lrk import init_params, network_r, relative_residuals
 from oracle_lib import oracle_engine, oracle_library
 from test_lrk import charged_system, random_params, start_fields, synthetic_records
@@ -96,6 +96,21 @@ def test_learned_trajectory_matches_oracle():
     recg, reco = g.lrk_get_records(32), o.lrk_get_records(32)
     assert recg.shape == reco.shape == (4, reco.shape[1])
     np.testing.assert_allclose(recg, reco, rtol=1e-7, atol=1e-9 * np.abs(reco).max())
+
+
+def test_learned_sipc_trajectory_matches_oracle():
+    """The semi-implicit predictor-corrector with the learned kernel: the judge runs once
+    per step on (w, u(w)) at the state the step starts from, and the corrector applies
+    the same tables in their Crank-Nicolson forms."""
+    (g, rg), (o, ro) = learned_pair(small_network(31), 1e300, 5, scheme=SCHEME_SIPC_LEARNED)
+    assert len(rg) == len(ro) == 5
+    for a, b in zip(rg, ro):
+        assert abs(a["H"] - b["H"]) <= 1e-8 * abs(b["H"])
+    for fid in (FIELD_OMEGA, FIELD_PSI):
+        assert rel(g.get_field(fid), o.get_field(fid)) < 1e-8
+    sg, so = g.lrk_get_stats(), o.lrk_get_stats()
+    assert (sg["learned_steps"], sg["fallback_steps"], sg["judged"]) == (so["learned_steps"], so["fallback_steps"], so["judged"])
+    assert sg["learned_steps"] == 4
 
 
 def test_debye_records_match_oracle():

@@ -10,7 +10,7 @@ import pytest
 
 from ftspolysol_amd.api import (SCHED_CONV, SCHED_CONV_MFMA, FtsError, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
                                 FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
-                                SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, hip_engine)
+                                SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SIPC, hip_engine)
 from ftspolysol_amd.system import System, initial_fields, make_system
 from oracle_lib import oracle_engine
 
@@ -181,7 +181,8 @@ def test_measure_matches_oracle(which):
 @pytest.mark.parametrize("scheme,sched", [(SCHEME_PC_REFERENCE, SCHED_PLANE), (SCHEME_PC_REFERENCE, SCHED_LINE),
                                           (SCHEME_EM, SCHED_PLANE), (SCHEME_SEMI_IMPLICIT, SCHED_LINE),
                                           (SCHEME_PC_REFERENCE, SCHED_CONV), (SCHEME_PC_REFERENCE, SCHED_SEPARABLE),
-                                          (SCHEME_SEMI_IMPLICIT, SCHED_SEPARABLE)])
+                                          (SCHEME_SEMI_IMPLICIT, SCHED_SEPARABLE), (SCHEME_SIPC, SCHED_PLANE),
+                                          (SCHEME_SIPC, SCHED_LINE)])
 def test_trajectory_matches_oracle(scheme, sched):
     print(f"trajectory scheme={scheme} sched={sched}", flush=True)
     s = c1_like(stress_mask=0x7F, scheme=scheme, fft_schedule=sched)

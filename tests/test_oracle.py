@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C, FIELD_RHO_P, FIELD_ION_RHO, SCHEME_EM,
-                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT)
+                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SIPC)
 from ftspolysol_amd.system import System, config, make_system
 from oracle_lib import oracle_engine, oracle_library
 
@@ -246,7 +246,7 @@ def run_deterministic(scheme, dt, T, om, ps):
     return e.get_field(FIELD_OMEGA)
 
 
-@pytest.mark.parametrize("scheme,order", [(SCHEME_PC_REFERENCE, 2), (SCHEME_EM, 1)])
+@pytest.mark.parametrize("scheme,order", [(SCHEME_PC_REFERENCE, 2), (SCHEME_EM, 1), (SCHEME_SIPC, 2)])
 def test_time_step_convergence_order(scheme, order):
     rng = np.random.default_rng(12)
     om = 0.3 * (rng.standard_normal((1, 8, 8, 8)) + 1j * rng.standard_normal((1, 8, 8, 8)))
@@ -259,10 +259,11 @@ def test_time_step_convergence_order(scheme, order):
     assert abs(np.log2(ratio) - order) < 0.25, ratio
 
 
-@pytest.mark.parametrize("scheme", [SCHEME_EM, SCHEME_SEMI_IMPLICIT])
+@pytest.mark.parametrize("scheme", [SCHEME_EM, SCHEME_SEMI_IMPLICIT, SCHEME_SIPC])
 def test_linear_mode_damping(scheme):
     """Around the homogeneous fixed point omega0 = -i xi, one deterministic step
-    multiplies a weak mode k0 by 1 - dt lam A(k0) (EM) or 1/(1 + dt lam A(k0)) (SI),
+    multiplies a weak mode k0 by 1 - dt lam A(k0) (EM), 1/(1 + dt lam A(k0)) (SI) or
+    (1 - a/2)/(1 + a/2), a = dt lam A(k0) (SIPC: the Crank-Nicolson factor),
     A(k) = rho0/xi + Gamma(k)^2 (n/V) sum_{ss'} phi^{|s-s'|} from the chain's linear response."""
     m, N, n, xi, dt = (16, 16, 16), 10, 6, 0.05, 0.01
     s = homopolymer(m, N, n, xi=xi, dt=dt, scheme=scheme, noise=False, lambda_omega=(1.0,))
@@ -279,12 +280,14 @@ def test_linear_mode_damping(scheme):
     V = e.info.volume
     rho0 = n * N / V
     A = rho0 / xi + np.exp(-k0 ** 2) * (n / V) * debye_pair_sum(N, np.exp(-k0 ** 2 / 6))
-    expected = 1 - dt * A if scheme == SCHEME_EM else 1 / (1 + dt * A)
+    a = dt * A
+    expected = {SCHEME_EM: 1 - a, SCHEME_SEMI_IMPLICIT: 1 / (1 + a), SCHEME_SIPC: (1 - a / 2) / (1 + a / 2)}[scheme]
     assert abs(amp - expected) < 1e-4, (amp, expected)
 
 
-def si_noise_spectrum(seed, step, field, m, P, drop_zero):
-    """g(k) Pn(k) of the semi-implicit noise, restated from its definition: the pair
+def si_noise_spectrum(seed, step, field, m, P, drop_zero, corrector=False):
+    """g(k) Pn(k) of the semi-implicit noise, restated from its definition (g(k) P2(k),
+    P2 = 2 P / (1 + P), for the predictor-corrector's corrector): the pair
     {k, -k} takes the two Box-Muller normals a, b of one Philox draw (field `field`,
     site = the pair's smaller grid index), g = (a + i b) / sqrt 2 at the smaller index and
     its conjugate at the other; a self-conjugate k takes a alone; Pn = sqrt(P (1 + P) / 2)."""
@@ -314,7 +317,7 @@ def si_noise_spectrum(seed, step, field, m, P, drop_zero):
                     assert abs(a - g(seed, step, field, min(gi, pi))) <= 1e-15 * max(1.0, abs(a))   # cl_normal's value
                     v = (a + 1j * (b if gi < pi else -b)) / np.sqrt(2)
                 out[x, y, z] = v
-    return out * np.sqrt(P * (1 + P) / 2)
+    return out * (2 * P / (1 + P) if corrector else np.sqrt(P * (1 + P) / 2))
 
 
 def test_semi_implicit_noise_factor():
@@ -362,3 +365,42 @@ def test_semi_implicit_noise_factor():
         if drop:
             gk.flat[0] = 1.0
         assert abs(np.mean(np.abs(gk) ** 2) - 1) < 5 * np.sqrt(2 / M), field
+
+
+def test_sipc_linear_field_is_crank_nicolson():
+    """psi of a neutral chain without ions has the exactly linear drift A_p psi,
+    A_p(k) = k^2 / (4 pi lB).  For it one step of the semi-implicit predictor-corrector
+    is, mode by mode, psi <- (1 - a/2)/(1 + a/2) psi + P2 eta, a = dt lam A_p,
+    P2 = 2 P / (1 + P) = 1 / (1 + a/2): the predictor drops out (its coefficient
+    1 - P2 - P2 a/2 vanishes), so the predictor's own noise leaves no trace, and the
+    stationary variance P2^2 2 lam dt / (1 - c^2) is exactly 1 / A_p.  Checked to round-off
+    on both parts: the noise-off step against the closed-form factor, and the noise-on
+    minus noise-off increment against sigma sqrt(M) IFFT[P2 g] with g restated from its
+    definition and the exported Philox generator (k = 0 dropped).  A step that reused the
+    predictor's noise factor, forgot the trapezoid's half, or kept a share of the predictor
+    fails one of the two."""
+    m, N, n, xi, dt, lB, lam_psi, seed = (16, 16, 16), 10, 6, 0.05, 0.2, 1.0, 0.7, 77
+    rng = np.random.default_rng(13)
+    om0 = np.full((1,) + m, -1j * xi) + 1e-3 * (rng.standard_normal((1,) + m) + 1j * rng.standard_normal((1,) + m))
+    ps0 = 0.05 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    ps0 -= ps0.mean()
+
+    def step(noise):
+        e = oracle_engine(homopolymer(m, N, n, xi=xi, lB=lB, dt=dt, scheme=SCHEME_SIPC, noise=noise, seed=seed,
+                                      lambda_omega=(1.0,), lambda_psi=lam_psi))
+        e.set_fields(om0, ps0)
+        e.step(1)
+        return e.get_field(FIELD_PSI)[0], e.info.volume
+
+    (p1, V), (p0, _) = step(True), step(False)
+    kx, ky, kz = kvecs(m, 0.5)
+    A = (kx ** 2 + ky ** 2 + kz ** 2) / (4 * np.pi * lB)
+    a = dt * lam_psi * A
+    M = np.prod(m)
+    expected0 = np.fft.ifftn((1 - a / 2) / (1 + a / 2) * np.fft.fftn(ps0))
+    assert np.abs(p0 - expected0).max() < 1e-12 * np.abs(ps0).max()
+    P = 1 / (1 + a)
+    sigma = np.sqrt(2 * lam_psi * dt / (V / M))
+    expected = sigma * np.sqrt(M) * np.fft.ifftn(si_noise_spectrum(seed, 0, 0x500, m, P, True, corrector=True))
+    scale = np.abs(expected).max()
+    assert np.abs(p1 - p0 - expected).max() < 1e-10 * scale
