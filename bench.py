@@ -10,18 +10,18 @@ bead-step launch, each replica with its own seed and initial fields; replica 0 o
 every leg has the same seed and fields):
   fast        FP32 + learned: R replicas with complex-FP32 propagators and FFTs,
               per-bead renormalisation, FP64 reductions and field accumulation,
-              stepping the semi-implicit scheme with the learned linear-response
-              kernel (FTS_SCHEME_SI_LEARNED; network trained by
+              stepping the semi-implicit predictor-corrector with the learned
+              linear-response kernel (FTS_SCHEME_SIPC_LEARNED; --fast-scheme si_learned
+              for the one-stage semi-implicit step; network trained by
               ftspolysol_amd.train_lrk on 48^3 data, lrk_c4_48cubed.npy), judged
               every step by the exact force, falling back to the Debye kernel when
               its excess force residual passes 1e-3, at --dt-fast (0.01, the
               plain dt).  `value` is this leg.
-  si_debye    the same batch and seeds with the Debye kernel only
-              (FTS_SCHEME_SEMI_IMPLICIT) at --dt-debye (0.01): what the learned
-              kernel gains or loses.  --learned-vs-debye runs these two legs alone
-              (at larger dt the learned kernel's gain is stability and a smaller
-              <H> bias); a leg whose replica exhausts its non-finite restarts (the
-              Debye scheme at dt 0.04) is reported as failed.
+  si_debye    the same batch, scheme and seeds with the Debye kernel only
+              (FTS_SCHEME_SIPC, or FTS_SCHEME_SEMI_IMPLICIT) at --dt-debye (0.01): what
+              the learned kernel gains or loses.  --learned-vs-debye runs these two legs
+              alone; a leg whose replica exhausts its non-finite restarts is reported as
+              failed.
   plain_fp64_batched  FP64, the reference's predictor-corrector
               (complexLangevin.cu:669-698), a batch of --plain-replicas: the plain
               engine at its own best throughput, and the plain scheme's tau_int and
@@ -36,11 +36,11 @@ reference measures its observables every step, complexLangevin.cu:684), which se
 the leg's rate; then untimed steps that extend the record of H to --record-steps
 (2000 steps for the FP32 legs and --plain-record-steps 2000 for the FP64 batch) whatever
 `steps` is.  tau_int and <H> come from that record, so they do not depend on the
-driver's --steps.  A leg whose record is shorter than 50 Sokal windows reports
-samples_per_s null.
+driver's --steps.  A leg whose record is shorter than 20 Sokal windows, or whose
+tau has a Sokal error above 10 %, reports samples_per_s null.
 
 Decorrelated samples/s = (replica CL steps/s) / (2 tau_int), tau_int by Sokal's
-automatic window (c = 6) on the replica-averaged normalised autocorrelation of Re H,
+automatic window (c = 6) on the pooled autocorrelation of Re H over the replicas,
 with Sokal's error tau sqrt(2 (2W + 1) / n).  Every replica keeps a device checkpoint
 every 50 steps; a replica whose step turns non-finite is restored from it with a new
 noise key (the repeated steps count in the elapsed time); restarts are reported.
@@ -77,7 +77,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, Engine, FtsError, PREC_FP32, PREC_FP64,  # noqa: E402
-                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, hip_batch, hip_engine)
+                                SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SI_LEARNED, SCHEME_SIPC, SCHEME_SIPC_LEARNED,
+                                hip_batch, hip_engine)
 from ftspolysol_amd.lrk import load_params  # noqa: E402
 from ftspolysol_amd.system import initial_fields, make_system  # noqa: E402
 
@@ -86,33 +87,43 @@ LRK_PARAMS = os.path.join(ROOT, "ftspolysol_amd", "lrk_c4_48cubed.npy")
 LRK_PARAMS_C5 = os.path.join(ROOT, "ftspolysol_amd", "lrk_c5_48cubed.npy")
 
 
-def autocorr(x):
-    x = np.asarray(x, dtype=np.float64)
-    n = x.size
-    y = x - x.mean()
-    f = np.fft.rfft(y, 2 * n)
-    acf = np.fft.irfft(f * np.conj(f))[:n]
-    return acf / acf[0] if acf[0] > 0 else None
-
-
 def tau_int(x, c=6.0):
     """Integrated autocorrelation time (in samples) with Sokal's automatic window.
-    x: one series, or a list of equally long series whose normalised
-    autocorrelations are averaged first."""
+    x: one series, or a list of equally long series of the same stationary process
+    (the replicas of a leg).  The autocorrelation is pooled: covariances about the grand
+    mean summed over the series and normalised by the pooled variance.  (Normalising each
+    series by its own variance and averaging lets a series that holds one large excursion
+    of H count as nearly white, so that rare slow excursions, which carry most of the
+    variance of H here, drop out of tau: 2-3x too small on these records.)  The pooled
+    estimate agrees with block averaging (tau_block)."""
     series = [np.asarray(x, dtype=np.float64)] if np.ndim(x) == 1 else [np.asarray(v, dtype=np.float64) for v in x]
     n = series[0].size
     if n < 8:
         return float("nan"), 0
-    rhos = [r for r in (autocorr(v) for v in series) if r is not None]
-    if not rhos:
+    X = np.stack(series)
+    Y = X - X.mean()
+    f = np.fft.rfft(Y, 2 * n, axis=1)
+    cov = np.fft.irfft(f * np.conj(f), axis=1)[:, :n].sum(axis=0) / (X.shape[0] * (n - np.arange(n)))
+    if not cov[0] > 0:
         return 0.5, 0
-    rho = np.mean(rhos, axis=0)
+    rho = cov / cov[0]
     tau = 0.5
     for w in range(1, n):
         tau += rho[w]
         if w >= c * tau:
             return float(tau), w
     return float(tau), n - 1
+
+
+def tau_block(x, block=200):
+    """tau_int from block means: Var(block mean) B / (2 Var), blocks of `block` samples of
+    every series (a check of tau_int that assumes no window)."""
+    X = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    nb = X.shape[1] // block
+    if nb < 2 or not X.var() > 0:
+        return None
+    bm = X[:, :nb * block].reshape(X.shape[0], nb, block).mean(axis=2)
+    return float(bm.var(ddof=1) * block / X.var() / 2)
 
 
 MIN_EQUILIBRATION = 200
@@ -221,13 +232,17 @@ def leg_steps(b, systems, label, warmup, steps, record_steps, ckpt_every, max_re
     # Sokal's estimate of the statistical error of tau_int: var = 2 (2W + 1) / n tau^2,
     # n the number of samples pooled
     tau_err = float(tau * np.sqrt(2.0 * (2 * window + 1) / H.size))
-    valid = n >= 50 * max(window, 1)
+    # Sokal's relative error of tau over the pooled samples; samples/s needs it under 10 %
+    # and a record of at least 20 windows
+    valid = n >= 20 * max(window, 1) and tau_err <= 0.1 * tau
     out.update(record_steps=n, tau_record_samples=int(H.size), record_replica_steps_per_s=R * (n - steps) / record_s
                if n > steps and record_s > 0 else None,
                tau_int=tau, window=window, tau_int_err=tau_err,
                tau_int_each=[round(float(tau_int(H.real[r:r + 1])[0]), 3) for r in range(R)],
+               tau_block=tau_block(H.real, 200),
                samples_per_s=(out["replica_steps_per_s"] / (2 * tau)) if valid else None,
-               H_mean=float(H.real.mean()), H_imag_mean=float(H.imag.mean()),
+               H_mean=float(H.real.mean()), H_imag_mean=float(H.imag.mean()), H_std=float(H.real.std()),
+               H_median=float(np.median(H.real)),
                H_stderr=float(H.real.std(ddof=1) * np.sqrt(2 * tau / H.size)))
     return out
 
@@ -317,7 +332,7 @@ def ratio_err(x, y):
 
 LEG_KEYS = ("failed", "dt", "replicas", "replica_steps_per_s", "ms_per_step", "equilibration_steps", "timed_steps", "record_steps",
             "tau_record_samples", "record_replica_steps_per_s", "tau_int", "tau_int_err", "window", "tau_int_each",
-            "samples_per_s", "H_mean", "H_stderr", "schedule", "tune_ms", "restarts")
+            "tau_block", "samples_per_s", "H_mean", "H_stderr", "H_std", "H_median", "schedule", "tune_ms", "restarts")
 
 
 def leg_summary(leg):
@@ -585,6 +600,8 @@ def main():
     ap.add_argument("--with-fp32-pc", action="store_true", help="add the FP32 predictor-corrector batch")
     ap.add_argument("--no-debye", action="store_true", help="skip the FP32 Debye semi-implicit batch")
     ap.add_argument("--fast-only", action="store_true", help="run the fast leg alone and print its summary (profiling runs)")
+    ap.add_argument("--fast-scheme", choices=["sipc_learned", "si_learned"], default="sipc_learned",
+                    help="scheme of the fast leg (its Debye leg takes the same scheme with the Debye kernel)")
     ap.add_argument("--learned-vs-debye", action="store_true",
                     help="run the fast (learned) and Debye semi-implicit legs only and print their comparison")
     ap.add_argument("--streams", type=int, default=-1,
@@ -632,14 +649,17 @@ def main():
     W, K = args.warmup, args.steps
 
     EQ = dict(min_equilibration=args.equilibration, graph_level=args.graph_level, streams=args.streams)
-    fast = run_leg(fp32(SCHEME_SI_LEARNED, args.dt_fast), f"fast fp32 si_learned x{args.replicas}", W, K, rec, lrk=lrk,
+    sipc = args.fast_scheme == "sipc_learned"
+    fast_scheme, debye_scheme = (SCHEME_SIPC_LEARNED, SCHEME_SIPC) if sipc else (SCHEME_SI_LEARNED, SCHEME_SEMI_IMPLICIT)
+    debye_name = "sipc" if sipc else "si"
+    fast = run_leg(fp32(fast_scheme, args.dt_fast), f"fast fp32 {args.fast_scheme} x{args.replicas}", W, K, rec, lrk=lrk,
                    lrk_threshold=args.lrk_threshold, **EQ)
-    debye = None if (args.no_debye or args.fast_only) else run_leg(fp32(SCHEME_SEMI_IMPLICIT, args.dt_debye),
-                                               f"si_debye fp32 x{args.replicas}", W, K, rec, **EQ)
+    debye = None if (args.no_debye or args.fast_only) else run_leg(fp32(debye_scheme, args.dt_debye),
+                                               f"{debye_name} (Debye) fp32 x{args.replicas}", W, K, rec, **EQ)
     fp32_pc = run_leg(fp32(SCHEME_PC_REFERENCE, args.dt_plain), f"fp32 pc x{args.replicas}", W, K, rec, **EQ) \
         if args.with_fp32_pc and not args.fast_only else None
     if args.learned_vs_debye:
-        print(json.dumps({"metric": "learned vs Debye semi-implicit kernel at 96^3 N=100", "dt": args.dt_fast,
+        print(json.dumps({"metric": "learned vs Debye kernel at 96^3 N=100", "scheme": args.fast_scheme, "dt": args.dt_fast,
                           "dt_debye": args.dt_debye,
                           "replicas": args.replicas, "fast": leg_summary(fast), "si_debye": leg_summary(debye),
                           "learned_kernel": fast["lrk"], "learned_vs_debye": speedup(fast, debye)}))
@@ -714,9 +734,9 @@ def main():
         "dtype": "c64 propagators/FFT, f64 reductions",
         "data": "synthetic",
         "config": {"workload": "96^3 L=48 N=100 z=-0.5 salt=0.05 B=0.5 lB=1 (221 chains)",
-                   "fast": f"FP32 si_learned dt={args.dt_fast}, {args.replicas} replicas per launch, learned kernel "
+                   "fast": f"FP32 {args.fast_scheme} dt={args.dt_fast}, {args.replicas} replicas per launch, learned kernel "
                            f"{os.path.basename(args.lrk)}, fallback threshold {args.lrk_threshold}",
-                   "si_debye": None if debye is None else f"FP32 si (Debye kernel) dt={args.dt_debye}, {args.replicas} replicas",
+                   "si_debye": None if debye is None else f"FP32 {debye_name} (Debye kernel) dt={args.dt_debye}, {args.replicas} replicas",
                    "plain_fp64_batched": f"FP64 pc dt={args.dt_plain}, {args.plain_replicas} replicas per launch",
                    "plain_fp64": f"FP64 pc dt={args.dt_plain}, 1 system",
                    "statistics": f"tau_int and <H> from an H record of {rec} steps (FP32 legs) / {prec} (FP64 batch) "

@@ -91,7 +91,7 @@ struct Group {
     std::vector<std::unique_ptr<StepGraph>> step_graphs;
     double tune_ms[2] = {0.0, 0.0};
     std::vector<std::unique_ptr<SweepTables>> tables; // full sweeps, per species index and schedule
-    double2* hQ = nullptr;                             // pinned [2][reps][FTS_MAX_SPECIES]: Q, Q at (w_bar, psi_bar)
+    double2* hQ = nullptr;                             // pinned [2][reps][Q_SLOTS]: Q, Q at (w_bar, psi_bar)
     int* hflag = nullptr;                              // pinned [reps]
     double2* hdrift = nullptr;                         // pinned [reps][FTS_MAX_TYPES + 1]: drift statistics of the step
     // The work of single replicas (smoothing FFTs, ions, contractions, updates)
@@ -530,22 +530,18 @@ template <typename T> int group_densities(Group& g, bool bar, const std::vector<
     return group_densities<T>(g, bar, mask, [](size_t) { return FTS_OK; });
 }
 
-bool finite_c(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
-
 // queue the copy of the replicas' Q into slot k of g.hQ (read after a sync)
 int group_queue_Q(Group& g, int k, const std::vector<char>& mask)
 {
     int rc;
     for (size_t r = 0; r < g.reps.size(); ++r)
-        if (mask[r] && (rc = g.reps[r]->q_queue(g.hQ + (static_cast<size_t>(k) * g.reps.size() + r) * FTS_MAX_SPECIES))) return rc;
+        if (mask[r] && (rc = g.reps[r]->q_queue(g.hQ + (static_cast<size_t>(k) * g.reps.size() + r) * Q_SLOTS))) return rc;
     return FTS_OK;
 }
 
 bool group_Q_finite(const Group& g, int k, size_t r)
 {
-    for (size_t p = 0; p < g.reps[r]->species.size(); ++p)
-        if (!finite_c(g.hQ[(static_cast<size_t>(k) * g.reps.size() + r) * FTS_MAX_SPECIES + p])) return false;
-    return true;
+    return g.reps[r]->partition_finite(g.hQ + (static_cast<size_t>(k) * g.reps.size() + r) * Q_SLOTS);
 }
 
 // One CL step of the replicas in mask; rec[r] non-null where replica r records.
@@ -580,7 +576,7 @@ template <typename T> int group_step_queue(Group& g, int scheme, const std::vect
     } else {
         // one fork for each replica's densities, Q copy and update
         rc = group_densities<T>(g, false, mask, [&](size_t r) {
-            int c = g.reps[r]->q_queue(g.hQ + r * FTS_MAX_SPECIES);
+            int c = g.reps[r]->q_queue(g.hQ + r * Q_SLOTS);
             return c ? c : update(r);
         });
         if (rc) return rc;
@@ -856,7 +852,7 @@ int fts_batch_create(const fts_system* systems, int n, fts_batch** out)
     }
     for (auto& gp : b->groups) {
         Group& g = *gp;
-        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hQ), sizeof(double2) * FTS_MAX_SPECIES * 2 * g.reps.size()));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hQ), sizeof(double2) * Q_SLOTS * 2 * g.reps.size()));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hflag), sizeof(int) * g.reps.size()));
         HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&g.hdrift), sizeof(double2) * (FTS_MAX_TYPES + 1) * g.reps.size()));
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&g.dstamps), sizeof(unsigned long long) * MAX_STAMPS));

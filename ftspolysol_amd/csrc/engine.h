@@ -99,6 +99,7 @@ struct HostIon {
 enum {
     SC_Q = 0,        // [8] polymer partition functions (of the stored, possibly shifted, propagators)
     SC_QION = 8,     // [8] ion partition functions
+    Q_SLOTS = 16,    // SC_Q and SC_QION: the partition functions a step checks
     SC_PSI_MEAN = 16,
     SC_ETA_MEAN = 17,
     SC_DRIFT_MEAN = 18,
@@ -242,8 +243,9 @@ struct fts_engine {
     int densities_end();
     const double2* dens_psi = nullptr; // the psi field densities_begin was given
     int check_Q(const char* what);
+    bool partition_finite(const double2* Q) const; // Q: Q_SLOTS values from SC_Q
     int finite_queue(int* host_out);      // queue the field finiteness check; *host_out valid after a stream sync
-    int q_queue(double2* host_out);       // queue the copy of Q[n_species]; valid after a stream sync
+    int q_queue(double2* host_out);       // queue the copy of the Q_SLOTS partition functions; valid after a stream sync
     int psi_drift(const double2* ps, const double2* rho_src, double2* out);
     int check_finite(bool& ok);
     // measure = measure_queue (device work and copies into pinned memory), a stream

@@ -1369,7 +1369,7 @@ int fts_engine::finite_queue(int* host_out)
 
 int fts_engine::q_queue(double2* host_out)
 {
-    HIP_TRY(hipMemcpyAsync(host_out, scal.p + SC_Q, sizeof(double2) * species.size(), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(host_out, scal.p + SC_Q, sizeof(double2) * Q_SLOTS, hipMemcpyDeviceToHost, stream));
     return FTS_OK;
 }
 
@@ -1549,7 +1549,8 @@ int fts_engine::measure_finish(fts_observables* rec)
         for (int s = 0; s < sp.N; ++s) total += sp.cls_shift[static_cast<size_t>(sp.cls_of_bead[static_cast<size_t>(s)])];
         q_shift_total[p] = total;
     }
-    auto neglog = [](double2 z) { return double2{-std::log(std::sqrt(z.x * z.x + z.y * z.y)), -std::atan2(z.y, z.x)}; };
+    // |z| by hypot: finite for every finite z (the squares overflow from |z| ~ 1e154)
+    auto neglog = [](double2 z) { return double2{-std::log(std::hypot(z.x, z.y)), -std::atan2(z.y, z.x)}; };
     double2 h{0.0, 0.0};
     for (size_t p = 0; p < nsp; ++p) {
         double2 mu = neglog(Q[p]);
@@ -1685,14 +1686,25 @@ namespace {
 bool finite2(double2 z) { return std::isfinite(z.x) && std::isfinite(z.y); }
 } // namespace
 
+// A step whose polymer or ion partition functions are not finite fails (an ion Q that
+// overflows leaves finite fields but zero ion densities and H = -inf for the step)
 int fts_engine::check_Q(const char* what)
 {
-    double2 Q[FTS_MAX_SPECIES];
-    int rc = read_scal(SC_Q, static_cast<int>(species.size()), Q);
+    double2 Q[Q_SLOTS];
+    int rc = read_scal(SC_Q, Q_SLOTS, Q);
     if (rc) return rc;
-    for (size_t p = 0; p < species.size(); ++p)
-        if (!finite2(Q[p])) return fail(FTS_NONFINITE, std::string("non-finite ") + what + " at step " + std::to_string(step_counter + 1));
+    if (!partition_finite(Q)) return fail(FTS_NONFINITE, std::string("non-finite ") + what + " at step " + std::to_string(step_counter + 1));
     return FTS_OK;
+}
+
+bool fts_engine::partition_finite(const double2* Q) const
+{
+    for (size_t p = 0; p < species.size(); ++p)
+        if (!finite2(Q[SC_Q + p])) return false;
+    if (electro())
+        for (size_t j = 0; j < ions.size(); ++j)
+            if (!finite2(Q[SC_QION + j])) return false;
+    return true;
 }
 
 // run_step (complexLangevin.cu:669-698): predictor-corrector with the same noise in both halves

@@ -77,10 +77,11 @@ inline Cx cexp(Cx z)
     const double r = std::exp(z.x);
     return Cx{r * std::cos(z.y), r * std::sin(z.y)};
 }
+// -log z; |z| by hypot, which stays finite for every finite z (the squares overflow from
+// |z| ~ 1e154, which a hot spot can give an ion partition function)
 inline Cx cneglog(Cx z)
 {
-    const double r = std::sqrt(z.x * z.x + z.y * z.y);
-    return Cx{-std::log(r), -std::atan2(z.y, z.x)};
+    return Cx{-std::log(std::hypot(z.x, z.y)), -std::atan2(z.y, z.x)};
 }
 inline bool finite(Cx z) { return std::isfinite(z.x) && std::isfinite(z.y); }
 
@@ -214,6 +215,16 @@ struct fts_engine {
     Field ion_psi, ion_boltzmann, ion_charge, ion_rho;
     Field obs_a, obs_b, sa, sb, sc, dfield, dpsi_s;
     std::vector<Cx> Q_ion;
+    // the step's check: polymer Qs (Qp) and the ion Qs of the same densities finite (an
+    // ion Q that overflows leaves finite fields, zero ion densities and H = -inf)
+    bool partition_finite(const std::vector<Cx>& Qp) const
+    {
+        for (const Cx& q : Qp)
+            if (!finite(q)) return false;
+        for (const Cx& q : Q_ion)
+            if (!finite(q)) return false;
+        return true;
+    }
     int last_species = 0;
 
     ~fts_engine() { delete fft; }
@@ -1231,8 +1242,7 @@ bool fts_engine::fields_finite() const
 int fts_engine::step_pc(bool record, fts_observables* rec)
 {
     std::vector<Cx> Qp = compute_densities(omega.data(), psi.data());
-    for (const Cx& q : Qp)
-        if (!finite(q)) return fail(FTS_NONFINITE, "non-finite Q");
+    if (!partition_finite(Qp)) return fail(FTS_NONFINITE, "non-finite Q");
     copy(rho_c_old.data(), rho_c.data());
     generate_noises(step_counter);
     compute_drifts(omega.data(), drift.data());
@@ -1254,8 +1264,7 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
     }
     remove_psi_zero_mode(psi_bar.data());
     Qp = compute_densities(omega_bar.data(), psi_bar.data());
-    for (const Cx& q : Qp)
-        if (!finite(q)) return fail(FTS_NONFINITE, "non-finite Qbar");
+    if (!partition_finite(Qp)) return fail(FTS_NONFINITE, "non-finite Qbar");
     compute_psi_drift(psi_bar.data(), rho_c.data(), dpsi_bar.data());
     if (record) measure(Qp, omega_bar.data(), psi_bar.data(), rec);
     compute_drifts(omega_bar.data(), drift_bar.data());
@@ -1289,8 +1298,7 @@ int fts_engine::step_pc(bool record, fts_observables* rec)
 int fts_engine::step_em(bool record, fts_observables* rec, bool semi_implicit)
 {
     std::vector<Cx> Qp = compute_densities(omega.data(), psi.data());
-    for (const Cx& q : Qp)
-        if (!finite(q)) return fail(FTS_NONFINITE, "non-finite Q");
+    if (!partition_finite(Qp)) return fail(FTS_NONFINITE, "non-finite Q");
     if (record) measure(Qp, omega.data(), psi.data(), rec);
     generate_noises(step_counter);
     compute_drifts(omega.data(), drift.data());
@@ -1394,8 +1402,7 @@ void fts_engine::add_kspace_noise(Cx* w, double sigma, unsigned field, bool drop
 int fts_engine::step_sipc(bool record, fts_observables* rec)
 {
     std::vector<Cx> Qp = compute_densities(omega.data(), psi.data());
-    for (const Cx& q : Qp)
-        if (!finite(q)) return fail(FTS_NONFINITE, "non-finite Q");
+    if (!partition_finite(Qp)) return fail(FTS_NONFINITE, "non-finite Q");
     compute_drifts(omega.data(), drift.data());
     compute_psi_drift(psi.data(), rho_c.data(), dpsi.data());
     drift_statistics(false);
@@ -1430,8 +1437,7 @@ int fts_engine::step_sipc(bool record, fts_observables* rec)
     else clear(psi_bar.data());
     remove_psi_zero_mode(psi_bar.data());
     Qp = compute_densities(omega_bar.data(), psi_bar.data());
-    for (const Cx& q : Qp)
-        if (!finite(q)) return fail(FTS_NONFINITE, "non-finite Qbar");
+    if (!partition_finite(Qp)) return fail(FTS_NONFINITE, "non-finite Qbar");
     compute_psi_drift(psi_bar.data(), rho_c.data(), dpsi_bar.data());
     if (record) measure(Qp, omega_bar.data(), psi_bar.data(), rec);
     compute_drifts(omega_bar.data(), drift_bar.data());

@@ -134,6 +134,29 @@ def test_tau_int_ar1():
         assert abs(tau - exact) < 0.08 * exact + 0.02, (rho, tau, exact)
 
 
+def test_tau_int_pooled_over_replicas_with_rare_excursions():
+    """Replicas of white noise plus rare large excursions that decay as 0.9^t (events at
+    rate 1e-4 per step, height 134: a shot-noise process holding 90 % of the variance).
+    Its tau_int is exact: (1 * 0.5 + v_s tau_s) / (1 + v_s), v_s = p A^2 / (1 - phi^2),
+    tau_s = (1 + phi) / (2 (1 - phi)) = 8.64.  Most replicas hold no event, so a tau that
+    averages per-replica normalised autocorrelations comes out near 2.5; the pooled one
+    and block averaging must find the exact value."""
+    import bench
+    rng = np.random.default_rng(4)
+    R, N, p, A, phi = 64, 4000, 1e-4, 134.0, 0.9
+    x = rng.standard_normal((R, N))
+    jumps = (rng.random((R, N)) < p) * A
+    s = np.zeros((R, N))
+    for i in range(1, N):
+        s[:, i] = phi * s[:, i - 1] + jumps[:, i]
+    x += s
+    vs, ts = p * A * A / (1 - phi ** 2), 0.5 * (1 + phi) / (1 - phi)
+    exact = (0.5 + vs * ts) / (1 + vs)
+    tau, _ = bench.tau_int(list(x))
+    assert abs(tau - exact) < 0.1 * exact, (tau, exact)
+    assert abs(bench.tau_block(x, 200) - exact) < 0.15 * exact
+
+
 def test_bench_refuses_more_than_one_gpu():
     # the engine runs a system (or a replica batch) on one GPU; bench.py says so for --gpus 2
     # and exits before it touches a device

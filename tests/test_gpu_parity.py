@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from ftspolysol_amd.api import (SCHED_CONV, SCHED_CONV_MFMA, FtsError, SCHED_LINE, SCHED_PLANE, SCHED_SEPARABLE, SYMMETRY_OFF, FIELD_ION_RHO, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C,
-                                FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32,
+                                FIELD_RHO_C_ION, FIELD_RHO_C_POLY, FIELD_RHO_P, FIELD_RHO_TYPE, PREC_FP32, PREC_FP64,
                                 SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SIPC, hip_engine)
 from ftspolysol_amd.system import System, initial_fields, make_system
 from oracle_lib import oracle_engine
@@ -306,3 +306,28 @@ def test_headline_measure_matches_oracle_64():
     set_random_fields([g, o], 1, s.m, 0.05, 43)
     r = obs_rel(g.measure(), o.measure())
     assert max(r.values()) < 1e-10, r
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32])
+def test_ion_hot_spot_matches_oracle(precision):
+    """A psi hot spot that drives an ion partition function to 4e198 (|Q|^2 beyond FP64):
+    mu = -log Q and H stay finite and match the oracle; a higher spot whose ion Q
+    overflows fails the step as a non-finite Q does."""
+    from test_oracle import ion_hot_spot_state
+    print(f"ion hot spot precision={precision}", flush=True)
+    s, om, ps = ion_hot_spot_state(SCHEME_PC_REFERENCE, 800.0, precision=precision)
+    g, o = hip_engine(s), oracle_engine(s)
+    for e in (g, o):
+        e.set_fields(om, ps)
+    rg, ro = g.measure(), o.measure()
+    tol = 1e-10 if precision == PREC_FP64 else 1e-5
+    assert np.isfinite(rg["H"])
+    for j in range(2):
+        assert abs(rg["mu_ex_ion"][j] - ro["mu_ex_ion"][j]) <= tol * abs(ro["mu_ex_ion"][j]), j
+    assert abs(rg["H"] - ro["H"]) <= tol * abs(ro["H"])
+    s2, om2, ps2 = ion_hot_spot_state(SCHEME_PC_REFERENCE, 3000.0, precision=precision)
+    g2 = hip_engine(s2)
+    g2.set_fields(om2, ps2)
+    with pytest.raises(FtsError) as err:
+        g2.step(1, record_every=1)
+    assert "non-finite Q" in str(err.value)

@@ -22,7 +22,7 @@ import os
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import (FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C, FIELD_RHO_P, FIELD_ION_RHO, SCHEME_EM,
+from ftspolysol_amd.api import (FTS_NONFINITE, FtsError, FIELD_OMEGA, FIELD_PSI, FIELD_QB, FIELD_QF, FIELD_RHO_C, FIELD_RHO_P, FIELD_ION_RHO, SCHEME_EM,
                                 SCHEME_PC_REFERENCE, SCHEME_SEMI_IMPLICIT, SCHEME_SIPC)
 from ftspolysol_amd.system import System, config, make_system
 from oracle_lib import oracle_engine, oracle_library
@@ -404,3 +404,44 @@ def test_sipc_linear_field_is_crank_nicolson():
     expected = sigma * np.sqrt(M) * np.fft.ifftn(si_noise_spectrum(seed, 0, 0x500, m, P, True, corrector=True))
     scale = np.abs(expected).max()
     assert np.abs(p1 - p0 - expected).max() < 1e-10 * scale
+
+
+def ion_hot_spot_state(scheme, height, **kw):
+    """A neutral chain with salt ions and psi = i height on a 6^3 block: the ion
+    partition functions grow as exp(z Im psi_smeared) there while the polymer's Q (blind
+    to psi) stays 1."""
+    m = (16, 16, 16)
+    s = homopolymer(m, 8, 20, lB=1.0, charge=0.0, salt=40, dt=0.01, scheme=scheme, noise=False, **kw)
+    om = np.zeros((1,) + m, dtype=complex)
+    ps = np.zeros(m, dtype=complex)
+    ps[5:11, 5:11, 5:11] = 1j * height
+    return s, om, ps
+
+
+def test_chemical_potential_of_a_large_partition_function():
+    """mu = -log Q with |Q| = 4e198: |Q|^2 is above the FP64 range, so |Q| must come
+    from hypot, not from the square root of the sum of squares (which gave mu = -inf and
+    H = -inf for a finite state)."""
+    s, om, ps = ion_hot_spot_state(SCHEME_PC_REFERENCE, 800.0)
+    e = oracle_engine(s)
+    e.set_fields(om, ps)
+    r = e.measure()
+    Q = r["Q_ion"][0]
+    assert np.isfinite(Q) and abs(Q) > 1e190
+    assert abs(r["mu_ex_ion"][0] - (-np.log(abs(Q)) - 1j * np.angle(Q))) <= 1e-12 * abs(np.log(abs(Q)))
+    assert np.isfinite(r["H"])
+
+
+@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SIPC])
+def test_ion_partition_overflow_fails_the_step(scheme):
+    """An ion Q that is not finite fails the step (as a polymer Q does): without the
+    check the step would go on with zero ion densities."""
+    s, om, ps = ion_hot_spot_state(scheme, 3000.0)
+    e = oracle_engine(s)
+    e.set_fields(om, ps)
+    with pytest.raises(FtsError) as err:
+        e.step(1, record_every=1)
+    assert err.value.code == FTS_NONFINITE and "non-finite Q" in str(err.value)
+    e.set_fields(om, ps / 10)
+    r = e.step(1, record_every=1)
+    assert np.isfinite(r[0]["H"])
