@@ -535,6 +535,34 @@ def run_c5(args):
                       "H_records": [float(x["H"].real) for x in recs[:: max(1, len(recs) // 20)]]}))
 
 
+STUDY_SCHEMES = {"pc": SCHEME_PC_REFERENCE, "si": SCHEME_SEMI_IMPLICIT, "si_learned": SCHEME_SI_LEARNED,
+                 "sipc": SCHEME_SIPC, "sipc_learned": SCHEME_SIPC_LEARNED}
+
+
+def run_dtstudy(args):
+    """Time-step study of the headline workload: for each leg scheme:precision:dt of
+    --legs, a batch of --replicas replicas (the headline's seeds and initial fields)
+    equilibrated max(warmup, --equilibration) steps, timed over `steps` steps and
+    recorded over --record-steps steps.  One JSON line per leg: the rate, tau_int (pooled
+    over the replicas, with the block-average check), <H> with its standard error, the
+    median and spread of H (the spread holds the excursions of H), samples/s, restarts
+    and the learned kernel's use."""
+    wl = dict(n=96, N=100, rho=0.2, salt=0.05, B=0.5, lB=1.0, charge="uniform", z=-0.5)
+    lrk = load_params(args.lrk)
+    EQ = dict(min_equilibration=args.equilibration, graph_level=args.graph_level, streams=args.streams)
+    for spec in args.legs.split(","):
+        name, prec, dt = spec.split(":")
+        scheme = STUDY_SCHEMES[name]
+        learned = scheme in (SCHEME_SI_LEARNED, SCHEME_SIPC_LEARNED)
+        systems = [make_system(**wl, dt=float(dt), precision=PREC_FP32 if prec == "fp32" else PREC_FP64, scheme=scheme,
+                               seed=0xC4 + r) for r in range(args.replicas)]
+        leg = run_leg(systems, f"{name} {prec} dt={dt} x{args.replicas}", args.warmup, args.steps,
+                      max(args.record_steps, args.steps), lrk=lrk if learned else None, lrk_threshold=args.lrk_threshold, **EQ)
+        print(json.dumps({"workload": "96^3 L=48 N=100 z=-0.5 salt=0.05 B=0.5 lB=1 (221 chains)", "scheme": name,
+                          "precision": prec, **leg_summary(leg), "learned_fraction": leg["lrk"]["learned_fraction"]
+                          if leg.get("lrk") else None}), flush=True)
+
+
 def run_beadstep(args):
     """The bead-step launch groups alone, at 96^3 N=100 (the headline workload): for each
     case prec:replicas:schedule of --cases, the average duration of one bead-step launch
@@ -614,9 +642,12 @@ def main():
                     help="bead-step schedule of the FP32 batches (auto: timed at creation)")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["c4", "c2", "c3", "c5", "beadstep"], default="c4",
+    ap.add_argument("--config", choices=["c4", "c2", "c3", "c5", "beadstep", "dtstudy"], default="c4",
                     help="c4: the headline (default); c2: FP32 vs FP64 errors at 64^3; c3: 64 mixed replicas; c5: 256^3; "
-                         "beadstep: the bead-step launch groups alone at 96^3 (--cases)")
+                         "beadstep: the bead-step launch groups alone at 96^3 (--cases); dtstudy: the headline workload "
+                         "per scheme, precision and dt (--legs)")
+    ap.add_argument("--legs", default="pc:fp64:0.01,sipc_learned:fp32:0.01",
+                    help="dtstudy legs scheme:precision:dt (schemes pc, si, si_learned, sipc, sipc_learned)")
     ap.add_argument("--cases", default="fp64:1:3,fp64:1:4,fp64:8:3,fp32:16:3",
                     help="beadstep cases precision:replicas:schedule (3 conv, 4 separable, 5 conv on the matrix cores, 0 timed)")
     ap.add_argument("--c2-steps", type=int, default=10000)
@@ -631,7 +662,7 @@ def main():
               "and runs on several GPUs are not supported", file=sys.stderr)
         sys.exit(2)
     if args.config != "c4":
-        {"c2": run_c2, "c3": run_c3, "c5": run_c5, "beadstep": run_beadstep}[args.config](args)
+        {"c2": run_c2, "c3": run_c3, "c5": run_c5, "beadstep": run_beadstep, "dtstudy": run_dtstudy}[args.config](args)
         return
     if args.steps < 1:
         print("--steps must be at least 1", file=sys.stderr)

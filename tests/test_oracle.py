@@ -445,3 +445,51 @@ def test_ion_partition_overflow_fails_the_step(scheme):
     e.set_fields(om, ps / 10)
     r = e.step(1, record_every=1)
     assert np.isfinite(r[0]["H"])
+
+
+def test_sipc_record_is_the_predictor_state():
+    """The semi-implicit predictor-corrector records at its predictor state
+    psi_bar = S_P[psi] + sigma sqrt(M) IFFT[Pn g] (the semi-implicit step, noise factor
+    Pn = sqrt(P (1 + P) / 2)), as the reference's predictor-corrector records at its
+    predictor.  For a neutral chain without ions psi decouples from omega, so two runs
+    that differ only in psi's initial field record H values whose difference is the
+    electrostatic term (1/8 pi lB) <psi_bar S_{k^2} psi_bar> of the two predicted fields,
+    restated here in numpy with g from the exported Philox generator.  The cross term
+    between S_P psi and the noise fixes the predictor's noise factor, which the linear
+    update itself does not see."""
+    m, N, n, xi, dt, lB, lam_psi, seed = (16, 16, 16), 10, 6, 0.05, 0.2, 1.0, 0.7, 91
+    rng = np.random.default_rng(14)
+    om0 = np.full((1,) + m, -1j * xi) + 1e-3 * (rng.standard_normal((1,) + m) + 1j * rng.standard_normal((1,) + m))
+    psis = []
+    for _ in range(2):
+        p = 0.05 * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+        psis.append(p - p.mean())
+    H = []
+    for ps in psis:
+        e = oracle_engine(homopolymer(m, N, n, xi=xi, lB=lB, dt=dt, scheme=SCHEME_SIPC, noise=True, seed=seed,
+                                      lambda_omega=(1.0,), lambda_psi=lam_psi))
+        e.set_fields(om0, ps)
+        H.append(e.step(1, record_every=1)[0]["H"])
+        V = e.info.volume
+    kx, ky, kz = kvecs(m, 0.5)
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    P = 1 / (1 + dt * lam_psi * k2 / (4 * np.pi * lB))
+    M = np.prod(m)
+    sigma = np.sqrt(2 * lam_psi * dt / (V / M))
+    noise = sigma * np.sqrt(M) * np.fft.ifftn(si_noise_spectrum(seed, 0, 0x500, m, P, True))
+
+    def h_psi(ps):
+        pb = np.fft.ifftn(P * np.fft.fftn(ps)) + noise
+        return np.mean(pb * np.fft.ifftn(k2 * np.fft.fftn(pb))) / (8 * np.pi * lB)
+
+    expected = h_psi(psis[1]) - h_psi(psis[0])
+    assert abs((H[1] - H[0]) - expected) < 1e-10 * max(abs(expected), 1e-3), (H[1] - H[0], expected)
+    # the same with the corrector's factor P2 on the predictor's noise differs by far more
+    # than the tolerance: the check sees the factor
+    noise_p2 = sigma * np.sqrt(M) * np.fft.ifftn(si_noise_spectrum(seed, 0, 0x500, m, P, True, corrector=True))
+
+    def h_psi_p2(ps):
+        pb = np.fft.ifftn(P * np.fft.fftn(ps)) + noise_p2
+        return np.mean(pb * np.fft.ifftn(k2 * np.fft.fftn(pb))) / (8 * np.pi * lB)
+
+    assert abs((h_psi_p2(psis[1]) - h_psi_p2(psis[0])) - expected) > 1e-4 * abs(expected)
