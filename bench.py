@@ -115,6 +115,18 @@ def tau_int(x, c=6.0):
     return float(tau), n - 1
 
 
+def tau_jackknife(x):
+    """Standard error of tau_int (pooled) from the delete-one-replica jackknife: it sees
+    how much of tau rests on the excursions of a few replicas, which Sokal's error (a
+    property of the window alone) does not.  None for fewer than 4 replicas."""
+    X = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    R = X.shape[0]
+    if R < 4:
+        return None
+    t = np.array([tau_int(list(np.delete(X, r, axis=0)))[0] for r in range(R)])
+    return float(np.sqrt((R - 1) / R * np.sum((t - t.mean()) ** 2)))
+
+
 def tau_block(x, block=200):
     """tau_int from block means: Var(block mean) B / (2 Var), blocks of `block` samples of
     every series (a check of tau_int that assumes no window)."""
@@ -231,13 +243,15 @@ def leg_steps(b, systems, label, warmup, steps, record_steps, ckpt_every, max_re
     n = H.shape[1]
     # Sokal's estimate of the statistical error of tau_int: var = 2 (2W + 1) / n tau^2,
     # n the number of samples pooled
-    tau_err = float(tau * np.sqrt(2.0 * (2 * window + 1) / H.size))
-    # Sokal's relative error of tau over the pooled samples; samples/s needs it under 10 %
-    # and a record of at least 20 windows
-    valid = n >= 20 * max(window, 1) and tau_err <= 0.1 * tau
+    tau_err_sokal = float(tau * np.sqrt(2.0 * (2 * window + 1) / H.size))
+    jk = tau_jackknife(H.real)
+    tau_err = max(tau_err_sokal, jk) if jk is not None else tau_err_sokal
+    # samples/s needs a record of at least 20 Sokal windows and Sokal's relative error of
+    # tau under 10 %; tau_int_err is the larger of Sokal's and the replica jackknife's
+    valid = n >= 20 * max(window, 1) and tau_err_sokal <= 0.1 * tau
     out.update(record_steps=n, tau_record_samples=int(H.size), record_replica_steps_per_s=R * (n - steps) / record_s
                if n > steps and record_s > 0 else None,
-               tau_int=tau, window=window, tau_int_err=tau_err,
+               tau_int=tau, window=window, tau_int_err=tau_err, tau_int_err_sokal=tau_err_sokal, tau_int_err_jackknife=jk,
                tau_int_each=[round(float(tau_int(H.real[r:r + 1])[0]), 3) for r in range(R)],
                tau_block=tau_block(H.real, 200),
                samples_per_s=(out["replica_steps_per_s"] / (2 * tau)) if valid else None,
@@ -331,7 +345,8 @@ def ratio_err(x, y):
 
 
 LEG_KEYS = ("failed", "dt", "replicas", "replica_steps_per_s", "ms_per_step", "equilibration_steps", "timed_steps", "record_steps",
-            "tau_record_samples", "record_replica_steps_per_s", "tau_int", "tau_int_err", "window", "tau_int_each",
+            "tau_record_samples", "record_replica_steps_per_s", "tau_int", "tau_int_err", "tau_int_err_sokal",
+            "tau_int_err_jackknife", "window", "tau_int_each",
             "tau_block", "samples_per_s", "H_mean", "H_stderr", "H_std", "H_median", "schedule", "tune_ms", "restarts")
 
 

@@ -157,6 +157,28 @@ def test_tau_int_pooled_over_replicas_with_rare_excursions():
     assert abs(bench.tau_block(x, 200) - exact) < 0.15 * exact
 
 
+def test_tau_jackknife_matches_the_spread_over_realisations():
+    """The replica-jackknife error of tau_int against the spread of tau_int over 30
+    independent realisations of the rare-excursion process above (32 replicas x 2000
+    samples, about 6 excursions per realisation): the mean jackknife error must agree
+    with the empirical standard deviation within 30 %."""
+    import bench
+    taus, errs = [], []
+    for seed in range(100, 130):
+        rng = np.random.default_rng(seed)
+        R, N, p, A, phi = 32, 2000, 1e-4, 134.0, 0.9
+        x = rng.standard_normal((R, N))
+        jumps = (rng.random((R, N)) < p) * A
+        s = np.zeros((R, N))
+        for i in range(1, N):
+            s[:, i] = phi * s[:, i - 1] + jumps[:, i]
+        x += s
+        taus.append(bench.tau_int(list(x))[0])
+        errs.append(bench.tau_jackknife(x))
+    spread = np.std(taus)
+    assert abs(np.mean(errs) - spread) < 0.3 * spread, (np.mean(errs), spread)
+
+
 def test_bench_refuses_more_than_one_gpu():
     # the engine runs a system (or a replica batch) on one GPU; bench.py says so for --gpus 2
     # and exits before it touches a device
