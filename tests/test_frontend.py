@@ -208,6 +208,31 @@ xi = 1.0
     assert info.n_omega == 1 and info.gamma_is_imag[0] == 1.0
 
 
+@pytest.mark.parametrize("value, scheme", [("pc", 0), ("predictor_corrector", 0), ("em", 1), ("si", 2), ("semi_implicit", 2),
+                                           ("si_learned", 3), ("learned", 3), ("sipc", 4), ("si_pc", 4), ("SIPC_Learned", 5)])
+def test_deck_scheme_values(tmp_path, value, scheme):
+    text = """[grid]
+m1 = 8
+m2 = 8
+m3 = 8
+a = 1
+dL = 0.5
+[polymer]
+species_count = 1
+sequence_file = {seq}
+[polymer.0]
+name = p
+num_chains = 4
+N = 2
+[interactions]
+xi = 1.0
+[dynamics]
+scheme = %s
+""" % value
+    deck = Deck(load_hip_library(), "fts_", write_deck(tmp_path, text, "[polymer.0]\nmonomers = A A\ncharges = 0 0\n"))
+    assert deck.system.desc.scheme == scheme
+
+
 @pytest.mark.parametrize("bad, msg", [
     ("[grid]\nm1 = 8\nbogus = 1\n", "unknown parameter [grid] bogus"),
     ("[grid]\nm1 = eight\n", "cannot parse [grid] m1"),
