@@ -786,7 +786,10 @@ def main():
                    "plain_fp64_batched": f"FP64 pc dt={args.dt_plain}, {args.plain_replicas} replicas per launch",
                    "plain_fp64": f"FP64 pc dt={args.dt_plain}, 1 system",
                    "statistics": f"tau_int and <H> from an H record of {rec} steps (FP32 legs) / {prec} (FP64 batch) "
-                                 f"after {max(W, args.equilibration)} equilibration steps; rate from the {K} timed steps",
+                                 f"after {max(W, args.equilibration)} equilibration steps; rate from the {K} timed steps; "
+                                 "tau_int pooled over the replicas, its error the larger of Sokal's and the replica "
+                                 "jackknife's; rare long excursions of H set tau_int, so records this short give lower "
+                                 "bounds (bench.py --config dtstudy --record-steps 10000 measures longer records)",
                    "parallelism": "single GPU"},
         "equilibration_steps": fast["equilibration_steps"],
         "tau_record_samples": fast["tau_record_samples"],
