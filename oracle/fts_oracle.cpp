@@ -10,9 +10,9 @@
 //   - observables, H, mu, p_iso       observable.cu:53-317
 //   - stress tensor                   stress.cu:106-321
 //   - drifts, noise, PC step          complexLangevin.cu:255-368, 669-698
-// and adds the two schemes the reference lacks (plain Euler-Maruyama and the
-// Debye-preconditioned semi-implicit update), with the same formulas the engine
-// uses.  Noise is Philox4x32-10 (philox_host.h) instead of cuRAND, the FFT is
+// and adds the schemes the reference lacks (plain Euler-Maruyama, the semi-implicit
+// update and the semi-implicit predictor-corrector, each with the Debye or the learned
+// kernel), with the same formulas the engine uses.  Noise is Philox4x32-10 (philox_host.h) instead of cuRAND, the FFT is
 // oracle_fft.h instead of cuFFT, and reductions are long-double block sums.
 //
 // It exports the engine's C ABI (include/fts.h) under the prefix fto_ so that a
@@ -22,7 +22,10 @@
 // fixture; closed forms at w = psi = 0 and at uniform fields; the propagator
 // invariant <qF_s qB_s e^{W_s}> = Q; exact density means; p_iso = tr(sigma)/3;
 // M = 1 / M = 2 mode closed forms; the Heun step's second-order convergence;
-// the discrete-chain linear response for the semi-implicit kernel.
+// the discrete-chain linear response for the semi-implicit kernel; the semi-implicit
+// steps' damping factors and noise spectra; the predictor-corrector's Crank-Nicolson
+// limit, its record at the predictor state and its second order; the electrostatics at
+// psi != 0 and the stress as a strain derivative (tests/test_oracle_pins.py).
 #include "../include/fts.h"
 #include "lrk_mlp.h"
 #include "oracle_fft.h"
