@@ -13,7 +13,7 @@ rows of a batch's replicas against single engines of the same systems.
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, SCHEME_EM, SCHEME_PC_REFERENCE, hip_batch, hip_engine
+from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SIPC, hip_batch, hip_engine
 from ftspolysol_amd.system import initial_fields, make_system
 from oracle_lib import oracle_engine
 
@@ -69,7 +69,7 @@ def test_oracle_drift_rows_follow_steps_and_restarts():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM])
+@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SIPC])
 def test_engine_drift_rows_match_oracle(scheme):
     print(f"drift rows scheme={scheme}", flush=True)
     s = small(scheme)
@@ -85,7 +85,7 @@ def test_engine_drift_rows_match_oracle(scheme):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM])
+@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SIPC])
 def test_batch_drift_rows_match_single_engines(scheme):
     """Replicas of a batch collect the same rows as single engines of their systems."""
     print(f"batch drift rows scheme={scheme}", flush=True)

@@ -11,7 +11,7 @@ GPU: the engine (single and in a batch) against the oracle on the same injection
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, SCHEME_EM, SCHEME_PC_REFERENCE, FtsError, hip_batch, hip_engine
+from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SIPC, FtsError, hip_batch, hip_engine
 from ftspolysol_amd.system import initial_fields, make_system
 from oracle_lib import oracle_engine
 
@@ -31,7 +31,7 @@ def H_of(recs):
     return np.array([r["H"] for r in recs])
 
 
-@pytest.mark.parametrize("scheme", [SCHEME_EM, SCHEME_PC_REFERENCE])
+@pytest.mark.parametrize("scheme", [SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SIPC])
 def test_oracle_restart_semantics(scheme):
     s = small(scheme)
     a = oracle_engine(s)
@@ -78,7 +78,7 @@ def test_oracle_restart_cap_and_off():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scheme", [SCHEME_EM, SCHEME_PC_REFERENCE])
+@pytest.mark.parametrize("scheme", [SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SIPC])
 def test_engine_restart_matches_oracle(scheme):
     print(f"restart engine scheme={scheme}", flush=True)
     s = small(scheme)

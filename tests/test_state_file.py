@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, FTS_INVALID, SCHEME_EM, SCHEME_PC_REFERENCE, FtsError, hip_engine
+from ftspolysol_amd.api import FIELD_OMEGA, FIELD_PSI, FTS_INVALID, SCHEME_EM, SCHEME_PC_REFERENCE, SCHEME_SIPC, FtsError, hip_engine
 from ftspolysol_amd.system import initial_fields, make_system
 from oracle_lib import oracle_engine
 
@@ -25,7 +25,7 @@ def snapshot(e):
     return e.get_field(FIELD_OMEGA), e.get_field(FIELD_PSI)
 
 
-@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM])
+@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SIPC])
 def test_oracle_resume_is_exact(scheme, tmp_path):
     s = small(scheme)
     a = oracle_engine(s)
@@ -69,7 +69,7 @@ def test_oracle_refuses_bad_state_files(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM])
+@pytest.mark.parametrize("scheme", [SCHEME_PC_REFERENCE, SCHEME_EM, SCHEME_SIPC])
 def test_engine_resume_is_exact_and_reads_oracle_states(scheme, tmp_path):
     print(f"state file scheme={scheme}", flush=True)
     s = small(scheme)
