@@ -35,8 +35,8 @@ steps (bracketed by device synchronisation, recording H every step as the
 reference measures its observables every step, complexLangevin.cu:684), which set
 the leg's rate; then untimed steps that extend the record of H to --record-steps
 (2000 steps for the FP32 legs and --plain-record-steps 2000 for the FP64 batch) whatever
-`steps` is.  tau_int and <H> come from that record, so they do not depend on the
-driver's --steps.  A leg whose record is shorter than 20 Sokal windows, or whose
+`steps` is.  tau_int and <H> come from that record, so they do not depend on
+--steps.  A leg whose record is shorter than 20 Sokal windows, or whose
 tau has a Sokal error above 10 %, reports samples_per_s null.
 
 Decorrelated samples/s = (replica CL steps/s) / (2 tau_int), tau_int by Sokal's
