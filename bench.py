@@ -550,6 +550,35 @@ def run_c5(args):
                       "H_records": [float(x["H"].real) for x in recs[:: max(1, len(recs) // 20)]]}))
 
 
+def long_record_estimate(fast, plain_1, plain_b, fast_scheme, dt_fast, dt_plain):
+    """This run's rates with the tau_int of the longest committed records of the same
+    legs (profiles/long_record_96cubed_N100.jsonl, bench.py --config dtstudy): rare long
+    excursions of H set tau_int, and the run's own 2000-step records bound it from below.
+    None when no committed record matches the legs."""
+    path = os.path.join(ROOT, "profiles", "long_record_96cubed_N100.jsonl")
+    if not os.path.exists(path):
+        return None
+    best = {}
+    for line in open(path):
+        d = json.loads(line)
+        key = (d.get("scheme"), d.get("precision"), d.get("dt"))
+        if d.get("tau_int") and (key not in best or d["record_steps"] > best[key]["record_steps"]):
+            best[key] = d
+    f, p = best.get((fast_scheme, "fp32", dt_fast)), best.get(("pc", "fp64", dt_plain))
+    if f is None or p is None or not fast.get("replica_steps_per_s") or not plain_1.get("replica_steps_per_s"):
+        return None
+    fast_s = fast["replica_steps_per_s"] / (2 * f["tau_int"])
+    plain_s = plain_1["replica_steps_per_s"] / (2 * p["tau_int"])
+    plain_bs = plain_b["replica_steps_per_s"] / (2 * p["tau_int"]) if plain_b.get("replica_steps_per_s") else None
+    rel = float(np.hypot(f["tau_int_err"] / f["tau_int"], p["tau_int_err"] / p["tau_int"]))
+    return {"source": "profiles/long_record_96cubed_N100.jsonl",
+            "record_steps": {"fast": f["record_steps"], "plain_fp64": p["record_steps"]},
+            "tau_int": {"fast": f["tau_int"], "plain_fp64": p["tau_int"]},
+            "samples_per_s": {"fast": fast_s, "plain_fp64": plain_s, "plain_fp64_batched": plain_bs},
+            "speedup_vs_plain_fp64": {"ratio": fast_s / plain_s, "rel_err": rel},
+            "speedup_vs_plain_fp64_batched": None if plain_bs is None else {"ratio": fast_s / plain_bs, "rel_err": rel}}
+
+
 STUDY_SCHEMES = {"pc": SCHEME_PC_REFERENCE, "si": SCHEME_SEMI_IMPLICIT, "si_learned": SCHEME_SI_LEARNED,
                  "sipc": SCHEME_SIPC, "sipc_learned": SCHEME_SIPC_LEARNED}
 
@@ -824,6 +853,7 @@ def main():
                      "frac_of_copy": None if not copy_gbs else achieved / copy_gbs,
                      "traffic_rate": None if traffic is None else traffic / (us * 1e-6) / 1e9},
     }
+    out["long_record_estimate"] = long_record_estimate(fast, plain_1, plain_b, args.fast_scheme, args.dt_fast, args.dt_plain)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(plain(1)[0], args.cpu_steps, plain_b["tau_int"], "PC")
     print(json.dumps(out))

@@ -179,6 +179,20 @@ def test_tau_jackknife_matches_the_spread_over_realisations():
     assert abs(np.mean(errs) - spread) < 0.3 * spread, (np.mean(errs), spread)
 
 
+def test_long_record_estimate_reads_the_committed_records():
+    """The bench line's long_record_estimate: this run's rates over twice the tau_int of
+    the longest committed record of each leg."""
+    import bench
+    est = bench.long_record_estimate({"replica_steps_per_s": 400.0}, {"replica_steps_per_s": 100.0},
+                                     {"replica_steps_per_s": 200.0}, "sipc_learned", 0.01, 0.01)
+    assert est is not None and est["record_steps"]["fast"] >= 10000
+    tf, tp = est["tau_int"]["fast"], est["tau_int"]["plain_fp64"]
+    assert abs(est["samples_per_s"]["fast"] - 400.0 / (2 * tf)) < 1e-12
+    assert abs(est["speedup_vs_plain_fp64"]["ratio"] - 4.0 * tp / tf) < 1e-12
+    assert abs(est["speedup_vs_plain_fp64_batched"]["ratio"] - 2.0 * tp / tf) < 1e-12
+    assert bench.long_record_estimate({"replica_steps_per_s": 1.0}, {"replica_steps_per_s": 1.0}, {}, "sipc_learned", 0.5, 0.01) is None
+
+
 def test_bench_refuses_more_than_one_gpu():
     # the engine runs a system (or a replica batch) on one GPU; bench.py says so for --gpus 2
     # and exits before it touches a device
