@@ -571,7 +571,17 @@ def long_record_estimate(fast, plain_1, plain_b, fast_scheme, dt_fast, dt_plain)
     plain_s = plain_1["replica_steps_per_s"] / (2 * p["tau_int"])
     plain_bs = plain_b["replica_steps_per_s"] / (2 * p["tau_int"]) if plain_b.get("replica_steps_per_s") else None
     rel = float(np.hypot(f["tau_int_err"] / f["tau_int"], p["tau_int_err"] / p["tau_int"]))
+    pc32 = best.get(("pc", "fp32", dt_plain))
+    fp32_pc = None
+    if pc32 is not None:
+        # the FP32 predictor-corrector batch is not a leg of this run: its committed rate
+        s32 = pc32["replica_steps_per_s"] / (2 * pc32["tau_int"])
+        fp32_pc = {"record_steps": pc32["record_steps"], "replica_steps_per_s": pc32["replica_steps_per_s"],
+                   "tau_int": pc32["tau_int"], "samples_per_s": s32,
+                   "speedup_vs_plain_fp64": {"ratio": s32 / plain_s, "rel_err": float(
+                       np.hypot(pc32["tau_int_err"] / pc32["tau_int"], p["tau_int_err"] / p["tau_int"]))}}
     return {"source": "profiles/long_record_96cubed_N100.jsonl",
+            "fp32_pc_committed": fp32_pc,
             "record_steps": {"fast": f["record_steps"], "plain_fp64": p["record_steps"]},
             "tau_int": {"fast": f["tau_int"], "plain_fp64": p["tau_int"]},
             "samples_per_s": {"fast": fast_s, "plain_fp64": plain_s, "plain_fp64_batched": plain_bs},

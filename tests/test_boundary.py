@@ -190,6 +190,8 @@ def test_long_record_estimate_reads_the_committed_records():
     assert abs(est["samples_per_s"]["fast"] - 400.0 / (2 * tf)) < 1e-12
     assert abs(est["speedup_vs_plain_fp64"]["ratio"] - 4.0 * tp / tf) < 1e-12
     assert abs(est["speedup_vs_plain_fp64_batched"]["ratio"] - 2.0 * tp / tf) < 1e-12
+    pc = est["fp32_pc_committed"]
+    assert pc is not None and abs(pc["speedup_vs_plain_fp64"]["ratio"] - pc["samples_per_s"] / est["samples_per_s"]["plain_fp64"]) < 1e-12
     assert bench.long_record_estimate({"replica_steps_per_s": 1.0}, {"replica_steps_per_s": 1.0}, {}, "sipc_learned", 0.5, 0.01) is None
 
 
