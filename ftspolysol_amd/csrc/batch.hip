@@ -592,15 +592,15 @@ int group_step_check(Group& g, int scheme, const std::vector<fts_observables*>& 
                      std::vector<std::string>& why)
 {
     int rc;
-    const bool pc = scheme_two_stage(scheme);
+    const bool two_stage = scheme_two_stage(scheme); // Q at the predicted fields too
     if (g.timing) resolve_stamps(g);
     for (size_t r = 0; r < g.reps.size(); ++r) {
         if (!mask[r]) continue;
         const std::string at = " at step " + std::to_string(g.reps[r]->step_counter);
-        const bool q_ok = group_Q_finite(g, 0, r) && (!pc || group_Q_finite(g, 1, r));
+        const bool q_ok = group_Q_finite(g, 0, r) && (!two_stage || group_Q_finite(g, 1, r));
         if (!q_ok) --g.reps[r]->step_counter; // a step that fails at Q is not counted (as fts_step)
         if (!group_Q_finite(g, 0, r)) why[r] = "non-finite Q" + at;
-        else if (pc && !group_Q_finite(g, 1, r)) why[r] = "non-finite Qbar" + at;
+        else if (two_stage && !group_Q_finite(g, 1, r)) why[r] = "non-finite Qbar" + at;
         else if (!g.hflag[r]) why[r] = "updated fields have non-finite values" + at;
         else if (rec[r] && (rc = g.reps[r]->measure_finish(rec[r]))) return rc;
     }
