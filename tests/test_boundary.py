@@ -6,7 +6,8 @@
   touches a device (the reference returns false with "error: ..." on stderr,
   errHandle.cu:5-9);
 - the ctypes structures have the C layout of the header;
-- bench.py's integrated autocorrelation time on AR(1) series with known tau.
+- bench.py's integrated autocorrelation time on AR(1) series with known tau, its
+  pooled and jackknife forms on a rare-excursion process, and the shape of its line.
 """
 import ctypes
 import os
@@ -247,3 +248,99 @@ def test_bench_reports_a_failed_leg(monkeypatch):
     assert closed == [True]
     assert bench.speedup(leg, {"samples_per_s": 1.0, "tau_int": 1.0, "tau_int_err": 0.1}) is None
     assert set(bench.leg_summary(leg)) >= {"failed", "dt", "replicas"}
+
+
+def test_bench_line_contract(monkeypatch, capsys):
+    """The headline line's shape, on the CPU: stand-in batches return records of H from
+    an AR(1) process (so every leg has a finite tau_int) and timing counters; the FP32/FP64
+    same-state comparison and the CPU baseline are stubbed.  The one JSON line must carry
+    the keys a reader of the line relies on, with the types they need."""
+    import json
+    import sys
+    import bench
+
+    class EInfo:
+        restarts = 0
+
+    class Replica:
+        def __init__(self, r):
+            self.rng = np.random.default_rng(100 + r)
+            self.x = 0.0
+            self.judged = 0
+
+        def set_fields(self, *a):
+            pass
+
+        def set_restart(self, *a):
+            pass
+
+        def lrk_set_network(self, *a):
+            pass
+
+        def get_info(self):
+            return EInfo()
+
+        def lrk_get_stats(self):
+            j = self.judged
+            return {"judged": j, "learned_steps": j, "fallback_steps": 0, "sum_rel_residual_learned": [0.8 * j, 0.8 * j],
+                    "sum_rel_residual_debye": [0.9 * j, 0.9 * j]}
+
+        def get_field(self, fid):
+            return np.zeros((1, 4, 4, 4), dtype=complex)
+
+        def record(self):
+            self.x = 0.6 * self.x + self.rng.standard_normal()
+            self.judged += 1
+            return {"H": complex(7.9 + 0.3 * self.x, 0.0)}
+
+    class BInfo:
+        group_schedule = [3]
+        group_tune_ms = [[1.0, 2.0]]
+
+    class Batch:
+        def __init__(self, systems):
+            self.replicas = [Replica(r) for r in range(len(systems))]
+            self.info = BInfo()
+            self.steps = 0
+
+        def set_graph(self, level):
+            pass
+
+        def set_streams(self, n):
+            pass
+
+        def set_timing(self, on):
+            self.steps = 0
+
+        def get_timing(self, g):
+            return {"sweep_ms": 0.1 * self.steps, "step_launches": 99 * self.steps, "algorithmic_bytes": 2.2e8 * 99 * self.steps,
+                    "sweeps": self.steps, "bead_steps": 99 * self.steps}
+
+        def step(self, n, record_every=0):
+            self.steps += n
+            return [[rep.record() for _ in range(n)] if record_every else [] for rep in self.replicas]
+
+        def close(self):
+            pass
+
+    monkeypatch.setattr(bench, "hip_batch", Batch)
+    monkeypatch.setattr(bench, "same_state_errors", lambda wl, f: {"H": 1e-8, "mu_ex_poly": 1e-7, "stress": 1e-7, "p_iso": 1e-7,
+                                                                     "H_fp64": 7.9})
+    monkeypatch.setattr(bench, "cpu_baseline", lambda *a: {"value": 0.1, "unit": "decorrelated CL samples/s", "cores": 1,
+                                                           "kind": "port", "sample": "stand-in"})
+    monkeypatch.setattr(sys, "argv", ["bench.py", "--steps", "20", "--warmup", "5", "--equilibration", "10", "--record-steps", "2000",
+                                      "--plain-record-steps", "2000", "--replicas", "4", "--plain-replicas", "4"])
+    bench.main()
+    line = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better", "scaling",
+                "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
+        assert key in line, key
+    assert line["n_gpus"] == 1 and line["steps"] == 20 and line["warmup"] == 5 and line["higher_is_better"] is True
+    assert isinstance(line["value"], float) and line["value"] > 0 and line["vs_baseline"] is None
+    assert "workload" in line["config"]
+    r = line["roofline"]
+    assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == 8000.0
+    assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-12
+    assert set(line["cpu_baseline"]) >= {"value", "unit", "cores", "kind", "sample"}
+    for leg in ("fast", "si_debye", "plain_fp64_batched", "plain_fp64"):
+        assert line[leg]["tau_int"] > 0 and line[leg]["tau_int_err"] > 0, leg
